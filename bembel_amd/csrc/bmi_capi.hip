@@ -1,0 +1,804 @@
+// C ABI of bembel_amd: handles, device-resident H2 state, assembly and matvec orchestration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+
+#include "bmi_fmm.h"
+#include "bmi_internal.h"
+#include "bmi_solver.h"
+
+namespace bmi {
+
+static thread_local std::string g_last_error;
+static void set_error(const std::string& m) { g_last_error = m; }
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+// ------------------------------------------------------------------ timing registry
+static bool g_timing = false;
+static std::mutex g_tmu;
+static std::map<std::string, std::pair<double, int64_t>> g_times;
+
+bool timing_enabled() { return g_timing; }
+
+TimedScope::TimedScope(const char* n, hipStream_t s) : name(n), st(s) {
+  if (!g_timing) return;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  (void)hipEventRecord(a, st);
+}
+TimedScope::~TimedScope() {
+  if (!a) return;
+  (void)hipEventRecord(b, st);
+  (void)hipEventSynchronize(b);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  {
+    std::lock_guard<std::mutex> g(g_tmu);
+    auto& e = g_times[name];
+    e.first += ms;
+    e.second += 1;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+}
+
+// ------------------------------------------------------------------ device copies of the space
+void ensure_device_space(const bmi_space* s, hipStream_t st) {
+  if (s->dev_ready) return;
+  std::vector<double> cp;
+  for (const auto& p : s->geo->patches) cp.insert(cp.end(), p.cp.begin(), p.cp.end());
+  s->d_cp.upload(cp, st);
+  s->d_elem_patch.upload(s->elem_patch, st);
+  s->d_elem_llc.upload(s->elem_llc, st);
+  s->d_rowptr.upload(s->rowptr, st);
+  s->d_col.upload(s->col, st);
+  s->d_val.upload(s->val, st);
+  s->d_trowptr.upload(s->trowptr, st);
+  s->d_tcol.upload(s->tcol, st);
+  s->d_tval.upload(s->tval, st);
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+  s->dev_ready = true;
+}
+
+__global__ void k_fill(double* p, int64_t n, double v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void device_fill(double* p, int64_t n, double v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace bmi
+
+using namespace bmi;
+
+// ------------------------------------------------------------------ H2 handle
+struct bmi_h2 {
+  const bmi_space* s = nullptr;
+  bmi_op_desc op{};
+  bool cplx = false;
+  int vd = 1, nc = 1, np = 0, np2 = 0, mcl = 1, L = 0, dd = 0, cd = 0;
+  std::vector<Leaf> dense, lowrank;
+  int64_t npairs = 0;
+  std::vector<int> pair_class_count = std::vector<int>(5, 0);
+  // leaf storage
+  DevBuf<double> d_dense, d_coup;
+  DevBuf<float> d_coup32;
+  // dense CSR over leaf-level clusters
+  int ncl_leaf = 0;
+  DevBuf<int> d_drp, d_dsrc;
+  // coupling CSR over (level, cluster) targets
+  int ntgt = 0;
+  DevBuf<int2> d_tgt;
+  DevBuf<int> d_crp, d_csrc;
+  std::vector<int64_t> lvl_off;  // per level start in per-RHS units (clusters * rows)
+  DevBuf<int64_t> d_lvl_off;
+  int64_t lvl_total = 0;
+  // FMM constants
+  DevBuf<double> d_T, d_mom[2];
+  // workspace (grown on demand)
+  int ws_rhs = 0;
+  DevBuf<double> d_xs, d_ys, d_fw[2], d_bw, d_xin, d_yout;
+  size_t bytes() const {
+    return d_dense.bytes() + d_coup.bytes() + d_coup32.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() +
+           d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_xs.bytes() +
+           d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes();
+  }
+};
+
+namespace bmi {
+
+static int ncl_at(const bmi_space* s, int l) { return s->npatch << (2 * l); }
+
+static void ensure_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
+  (void)st;
+  if (nrhs <= h->ws_rhs) return;
+  const int S = h->cplx ? 2 : 1;
+  const int64_t N = h->s->nsuper;
+  h->d_xs.alloc(S * N * nrhs);
+  h->d_ys.alloc(S * N * nrhs);
+  for (int c = 0; c < h->vd; ++c) h->d_fw[c].alloc(S * h->lvl_total * nrhs);
+  h->d_bw.alloc(S * h->lvl_total * nrhs);
+  h->ws_rhs = nrhs;
+}
+
+// super-space apply: ys = A_super xs (device, column stride nsuper)
+static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, hipStream_t st) {
+  const bmi_space* s = h->s;
+  const bool C = h->cplx;
+  const int S = C ? 2 : 1;
+  const int64_t N = s->nsuper;
+  const int rows = h->cd;
+  const int64_t vcs = static_cast<int64_t>(s->nel) * s->I2;  // super DOFs per component
+  const bool fp32 = h->op.precision == BMI_PRECISION_MIXED;
+  const void* coupF = fp32 ? static_cast<const void*>(h->d_coup32.ptr) : static_cast<const void*>(h->d_coup.ptr);
+  const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
+  const int64_t dblk = static_cast<int64_t>(h->dd) * h->dd;
+  const bool has_far = h->L >= 0 && h->lvl_total > 0;
+  // forward transformation per column component
+  if (has_far)
+    for (int cc = 0; cc < h->vd; ++cc) {
+      double* fw = h->d_fw[cc].ptr;
+      TimedScope ts("matvec_forward", st);
+      launch_moment(C, ncl_at(s, h->L), nrhs, rows, h->dd, h->d_mom[cc].ptr, xs, N, cc * vcs,
+                    fw + S * h->lvl_off[h->L] * nrhs, st);
+      for (int l = h->L - 1; l >= 0; --l)
+        launch_m2m(C, ncl_at(s, l), nrhs, rows, h->np2, h->d_T.ptr, fw + S * h->lvl_off[l + 1] * nrhs,
+                   fw + S * h->lvl_off[l] * nrhs, st);
+    }
+  for (int rc = 0; rc < h->vd; ++rc) {
+    if (has_far) {
+      device_fill(h->d_bw.ptr, S * h->lvl_total * nrhs, 0.0, st);
+      for (int cc = 0; cc < h->vd; ++cc)
+        launch_coupling_apply(C, fp32, h->ntgt, nrhs, rows, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, coupF,
+                              h->vd * h->vd * cblk, (rc * h->vd + cc) * cblk, h->d_lvl_off.ptr, h->d_fw[cc].ptr,
+                              h->d_bw.ptr, 1, st);
+      TimedScope ts("matvec_backward", st);
+      for (int l = 0; l < h->L; ++l)
+        launch_l2l(C, ncl_at(s, l + 1), nrhs, rows, h->np2, h->d_T.ptr, h->d_bw.ptr + S * h->lvl_off[l] * nrhs,
+                   h->d_bw.ptr + S * h->lvl_off[l + 1] * nrhs, st);
+    }
+    for (int cc = 0; cc < h->vd; ++cc)
+      launch_leaf(C, h->ncl_leaf, nrhs, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, h->vd * h->vd * dblk,
+                  (rc * h->vd + cc) * dblk, xs, N, cc * vcs, h->d_mom[rc].ptr, rows,
+                  (cc == 0 && has_far) ? h->d_bw.ptr + S * h->lvl_off[h->L] * nrhs : nullptr, ys, N, rc * vcs,
+                  cc > 0 ? 1 : 0, st);
+  }
+}
+
+void h2_apply_device(bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy, const double* alpha,
+                     const double* beta, hipStream_t st) {
+  const bmi_space* s = h->s;
+  ensure_workspace(h, nrhs, st);
+  const int64_t N = s->nsuper;
+  launch_spmv(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, h->d_xs.ptr, N, nullptr,
+              nullptr, st);
+  apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, st);
+  launch_spmv(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, h->d_ys.ptr, N, Y, ldy, alpha,
+              beta, st);
+}
+
+int64_t h2_rows(const bmi_h2* h) { return h->s->ndof; }
+bool h2_complex(const bmi_h2* h) { return h->cplx; }
+bool h2_is_symmetric_real(const bmi_h2* h) { return h->op.kind == BMI_LAPLACE_SL; }
+
+// ------------------------------------------------------------------ assembly
+static void assemble_h2(bmi_h2* h, hipStream_t st) {
+  const bmi_space* s = h->s;
+  const bmi_op_desc& op = h->op;
+  ensure_device_space(s, st);
+  h->vd = s->vd;
+  h->cplx = op.kind == BMI_HELMHOLTZ_SL || op.kind == BMI_MAXWELL_SL;
+  h->nc = op.kind == BMI_MAXWELL_SL ? 2 : 1;
+  h->np = op.n_cheb;
+  h->np2 = h->np * h->np;
+  h->mcl = op.min_cluster_level;
+  h->L = s->M - h->mcl;
+  BMI_REQUIRE(h->L >= 0, BMI_UNSUPPORTED, "refinement level below min_cluster_level");
+  h->dd = (1 << (2 * h->mcl)) * s->I2;
+  h->cd = h->nc * h->np2;
+  BMI_REQUIRE(op.kind != BMI_MAXWELL_SL, BMI_UNSUPPORTED, "Maxwell near field not available yet");
+  build_block_tree(s, op.eta, h->mcl, h->dense, h->lowrank);
+  const int S = h->cplx ? 2 : 1;
+  const int vd = h->vd;
+  // ---- near field
+  const int ne = 1 << (2 * h->mcl);
+  const int64_t dblk = static_cast<int64_t>(h->dd) * h->dd;
+  h->d_dense.alloc(static_cast<size_t>(S) * vd * vd * dblk * h->dense.size());
+  std::vector<NFPair> pairs;
+  pairs.reserve(h->dense.size() * ne * ne);
+  const int ff = farfield_degree(op.kind, s->P);
+  for (size_t k = 0; k < h->dense.size(); ++k) {
+    const Leaf& lf = h->dense[k];
+    BMI_REQUIRE(lf.level == h->L, BMI_UNSUPPORTED, "dense leaf above the leaf level");
+    for (int b = 0; b < ne; ++b)
+      for (int a = 0; a < ne; ++a) {
+        NFPair p;
+        p.e1 = lf.c1 * ne + a;
+        p.e2 = lf.c2 * ne + b;
+        int cp[3];
+        double dist;
+        compare_elements(s, p.e1, p.e2, cp, &dist);
+        int nf = nearfield_degree(op.kind, s->P, dist, s->M);
+        nf = nf >= ff ? nf : ff;
+        BMI_REQUIRE(nf < 50, BMI_UNSUPPORTED, "near-field quadrature degree too large");
+        p.cls = (cp[2] == 0) ? (nf == ff ? 0 : 1) : cp[2];
+        p.rot1 = cp[0];
+        p.rot2 = cp[1];
+        p.deg = nf;
+        p.off = static_cast<int64_t>(k) * vd * vd * dblk + static_cast<int64_t>(b * s->I2) * h->dd + a * s->I2;
+        h->pair_class_count[p.cls]++;
+        pairs.push_back(p);
+      }
+  }
+  h->npairs = static_cast<int64_t>(pairs.size());
+  launch_nearfield(s, op, pairs, h->d_dense.ptr, 1, h->dd, 0, 0, st);
+  // dense CSR over leaf clusters (leaves sorted by (c1, c2))
+  h->ncl_leaf = ncl_at(s, h->L);
+  std::vector<int> drp(h->ncl_leaf + 1, 0), dsrc(h->dense.size());
+  for (size_t k = 0; k < h->dense.size(); ++k) drp[h->dense[k].c1 + 1]++, dsrc[k] = h->dense[k].c2;
+  for (int c = 0; c < h->ncl_leaf; ++c) drp[c + 1] += drp[c];
+  h->d_drp.upload(drp, st);
+  h->d_dsrc.upload(dsrc, st);
+  // ---- FMM constants
+  std::vector<double> T, mom;
+  transfer_matrix(h->np, T);
+  h->d_T.upload(T, st);
+  for (int c = 0; c < vd; ++c) {
+    int rows, cols;
+    moment_matrix(s->P, h->L, h->mcl, h->np, vd == 2 ? c + 1 : 0, mom, &rows, &cols);
+    BMI_REQUIRE(rows == h->cd && cols == h->dd, BMI_DEVICE_ERROR, "moment matrix shape");
+    h->d_mom[c].upload(mom, st);
+  }
+  // level offsets of the multipole arrays
+  h->lvl_off.assign(h->L + 2, 0);
+  for (int l = 0; l <= h->L; ++l) h->lvl_off[l + 1] = h->lvl_off[l] + static_cast<int64_t>(ncl_at(s, l)) * h->cd;
+  h->lvl_total = h->lvl_off[h->L + 1];
+  h->d_lvl_off.upload(h->lvl_off, st);
+  // ---- coupling blocks
+  const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
+  const int nl = static_cast<int>(h->lowrank.size());
+  h->d_coup.alloc(static_cast<size_t>(S) * vd * vd * cblk * nl);
+  if (nl > 0) {
+    std::vector<double> cheb(h->np);
+    cheb_points(h->np, cheb.data());
+    DevBuf<double> d_cheb;
+    d_cheb.upload(cheb, st);
+    std::vector<DevBuf<double>> pts(h->L + 1);
+    std::vector<double*> ptrs(h->L + 1, nullptr);
+    for (int l = 0; l <= h->L; ++l) {
+      pts[l].alloc(static_cast<size_t>(ncl_at(s, l)) * h->np2 * CP_STRIDE);
+      launch_cluster_points(s, l, h->np, d_cheb.ptr, pts[l].ptr, st);
+      ptrs[l] = pts[l].ptr;
+    }
+    DevBuf<double*> d_ptrs;
+    d_ptrs.upload(ptrs, st);
+    std::vector<int4> lv(nl);
+    for (int k = 0; k < nl; ++k) lv[k] = make_int4(h->lowrank[k].level, h->lowrank[k].c1, h->lowrank[k].c2, 0);
+    DevBuf<int4> d_lv;
+    d_lv.upload(lv, st);
+    launch_coupling_assemble(op.kind, d_lv.ptr, nl, h->np2, d_ptrs.ptr, op.kappa_re, op.kappa_im, h->d_coup.ptr, st);
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  // targets (level, cluster) and CSR over the low-rank leaves (sorted by level, c1, c2)
+  std::vector<int2> tgt;
+  std::vector<int> crp(1, 0), csrc(nl);
+  for (int k = 0; k < nl; ++k) {
+    const Leaf& lf = h->lowrank[k];
+    if (tgt.empty() || tgt.back().x != lf.level || tgt.back().y != lf.c1) {
+      tgt.push_back(make_int2(lf.level, lf.c1));
+      crp.push_back(crp.back());
+    }
+    crp.back()++;
+    csrc[k] = lf.c2;
+  }
+  h->ntgt = static_cast<int>(tgt.size());
+  h->d_tgt.upload(tgt, st);
+  h->d_crp.upload(crp, st);
+  h->d_csrc.upload(csrc, st);
+  if (op.precision == BMI_PRECISION_MIXED) h->d_coup32.alloc(h->d_coup.n);
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+  if (op.precision == BMI_PRECISION_MIXED && h->d_coup.n) convert_to_fp32(h->d_coup.ptr, h->d_coup32.ptr, h->d_coup.n, st);
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace bmi
+
+// ====================================================================== C ABI
+#define BMI_API_BEGIN try {
+#define BMI_API_END                                   \
+  return BMI_OK;                                      \
+  }                                                   \
+  catch (const bmi::Error& e) {                       \
+    bmi::set_error(e.what());                         \
+    return e.status;                                  \
+  }                                                   \
+  catch (const std::bad_alloc&) {                     \
+    bmi::set_error("host allocation failed");         \
+    return BMI_OUT_OF_MEMORY;                         \
+  }                                                   \
+  catch (const std::exception& e) {                   \
+    bmi::set_error(e.what());                         \
+    return BMI_DEVICE_ERROR;                          \
+  }
+
+extern "C" {
+
+const char* bmi_last_error(void) { return bmi::g_last_error.c_str(); }
+const char* bmi_version(void) { return "bembel_amd 0.1 (gfx950)"; }
+
+bmi_status bmi_op_desc_default(int kind, bmi_op_desc* out) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(out, BMI_INVALID_ARG, "null descriptor");
+  BMI_REQUIRE(kind >= 0 && kind <= 3, BMI_INVALID_ARG, "unknown operator kind");
+  out->kind = kind;
+  out->kappa_re = 0.0;
+  out->kappa_im = 0.0;
+  out->n_cheb = 9;
+  out->eta = 1.6;
+  out->min_cluster_level = 1;
+  out->precision = BMI_PRECISION_FP64;
+  out->dummy_fn = 0;
+  BMI_API_END
+}
+
+bmi_status bmi_geometry_load(const char* path, const double* scale, bmi_geometry** out) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(path && out, BMI_INVALID_ARG, "null argument");
+  *out = nullptr;
+  std::ifstream f(path);
+  BMI_REQUIRE(f.good(), BMI_IO_ERROR, std::string("cannot open geometry file ") + path);
+  // GEOPDE .dat format (Geometry/GeometryIO.hpp:25-122): 4 comment rows, info row, then per
+  // patch: name, degrees, control point counts, two knot vectors, four M x N matrices
+  std::string line;
+  for (int i = 0; i < 5; ++i) BMI_REQUIRE(std::getline(f, line), BMI_IO_ERROR, "truncated geometry header");
+  int info[5];
+  {
+    std::istringstream is(line);
+    for (int i = 0; i < 5; ++i) BMI_REQUIRE(static_cast<bool>(is >> info[i]), BMI_IO_ERROR, "bad info row");
+  }
+  auto g = std::make_unique<bmi_geometry>();
+  for (int p = 0; p < info[2]; ++p) {
+    int deg[2], ncp[2];
+    BMI_REQUIRE(std::getline(f, line), BMI_IO_ERROR, "truncated patch");
+    BMI_REQUIRE(std::getline(f, line), BMI_IO_ERROR, "truncated patch");
+    {
+      std::istringstream is(line);
+      is >> deg[0] >> deg[1];
+    }
+    BMI_REQUIRE(std::getline(f, line), BMI_IO_ERROR, "truncated patch");
+    {
+      std::istringstream is(line);
+      is >> ncp[0] >> ncp[1];
+    }
+    std::vector<double> k[2];
+    for (int kk = 0; kk < 2; ++kk) {
+      BMI_REQUIRE(std::getline(f, line), BMI_IO_ERROR, "truncated knots");
+      std::istringstream is(line);
+      const int nk = deg[kk] + ncp[kk] + 1;
+      k[kk].resize(nk);
+      for (int i = 0; i < nk; ++i) BMI_REQUIRE(static_cast<bool>(is >> k[kk][i]), BMI_IO_ERROR, "bad knot vector");
+      int uniq = 1;
+      for (int i = 1; i < nk; ++i)
+        if (k[kk][i] > k[kk][i - 1] + 1e-6) ++uniq;
+      BMI_REQUIRE(uniq == 2, BMI_UNSUPPORTED, "patches with interior knots are not supported");
+    }
+    const int N = ncp[0], Mr = ncp[1];
+    std::vector<double> mat[4];
+    for (int m = 0; m < 4; ++m) {
+      BMI_REQUIRE(std::getline(f, line), BMI_IO_ERROR, "truncated control points");
+      std::istringstream is(line);
+      mat[m].resize(static_cast<size_t>(N) * Mr);
+      for (int i = 0; i < Mr * N; ++i) BMI_REQUIRE(static_cast<bool>(is >> mat[m][i]), BMI_IO_ERROR, "bad control points");
+    }
+    Patch pt;
+    pt.pdx = static_cast<int>(k[0].size()) - N;
+    pt.pdy = static_cast<int>(k[1].size()) - Mr;
+    BMI_REQUIRE(pt.pdx == N && pt.pdy == Mr, BMI_IO_ERROR, "inconsistent Bezier patch");
+    pt.cp.resize(4 * static_cast<size_t>(N) * Mr);
+    for (int m = 0; m < 4; ++m)
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < Mr; ++j) pt.cp[(i * Mr + j) * 4 + m] = mat[m][j * N + i];
+    if (scale)
+      for (int q = 0; q < N * Mr; ++q)
+        for (int m = 0; m < 3; ++m) pt.cp[4 * q + m] *= scale[m];
+    g->patches.push_back(std::move(pt));
+  }
+  BMI_REQUIRE(!g->patches.empty(), BMI_IO_ERROR, "geometry without patches");
+  *out = g.release();
+  BMI_API_END
+}
+
+bmi_status bmi_geometry_destroy(bmi_geometry* g) {
+  delete g;
+  return BMI_OK;
+}
+
+bmi_status bmi_geometry_num_patches(const bmi_geometry* g, int* n) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(g && n, BMI_INVALID_ARG, "null argument");
+  *n = static_cast<int>(g->patches.size());
+  BMI_API_END
+}
+
+bmi_status bmi_geometry_surface_point(const bmi_geometry* g, int patch, double s, double t, double out[12]) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(g && out, BMI_INVALID_ARG, "null argument");
+  BMI_REQUIRE(patch >= 0 && patch < static_cast<int>(g->patches.size()), BMI_INVALID_ARG, "patch out of range");
+  patch_surface_point(g->patches[patch], s, t, out);
+  BMI_API_END
+}
+
+bmi_status bmi_space_create(const bmi_geometry* g, int M, int P, int form, bmi_space** out) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(g && out, BMI_INVALID_ARG, "null argument");
+  BMI_REQUIRE(M >= 0 && M <= 10, BMI_INVALID_ARG, "refinement level out of range");
+  BMI_REQUIRE(P >= 0 && P <= 8, BMI_INVALID_ARG, "polynomial degree out of range");
+  BMI_REQUIRE(form == BMI_FORM_DISCONTINUOUS || form == BMI_FORM_DIVCONFORMING, BMI_INVALID_ARG, "unknown form");
+  BMI_REQUIRE(form == BMI_FORM_DISCONTINUOUS || P >= 1, BMI_INVALID_ARG, "DivConforming spaces need P >= 1");
+  *out = nullptr;
+  auto s = std::make_unique<bmi_space>();
+  s->geo = g;
+  s->M = M;
+  s->P = P;
+  s->I2 = (P + 1) * (P + 1);
+  s->form = form;
+  s->vd = form == BMI_FORM_DIVCONFORMING ? 2 : 1;
+  s->pd = g->patches[0].pdx;
+  for (const auto& p : g->patches)
+    if (p.pdx != s->pd || p.pdy != s->pd) s->pd = -1;
+  build_mesh(s.get());
+  build_transformation(s.get());
+  *out = s.release();
+  BMI_API_END
+}
+
+bmi_status bmi_space_destroy(bmi_space* s) {
+  delete s;
+  return BMI_OK;
+}
+
+bmi_status bmi_space_info(const bmi_space* s, int64_t info[8]) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(s && info, BMI_INVALID_ARG, "null argument");
+  info[0] = s->nel;
+  info[1] = s->nvert;
+  info[2] = s->I2;
+  info[3] = s->vd;
+  info[4] = s->nsuper;
+  info[5] = s->ndof;
+  info[6] = static_cast<int64_t>(s->val.size());
+  info[7] = s->M;
+  BMI_API_END
+}
+
+bmi_status bmi_space_elements(const bmi_space* s, int* vertices, int* patch, double* llc, double* midpoint,
+                              double* radius) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(s, BMI_INVALID_ARG, "null space");
+  if (vertices) std::copy(s->vert.begin(), s->vert.end(), vertices);
+  if (patch) std::copy(s->elem_patch.begin(), s->elem_patch.end(), patch);
+  if (llc) std::copy(s->elem_llc.begin(), s->elem_llc.end(), llc);
+  if (midpoint) std::copy(s->cmid[s->M].begin(), s->cmid[s->M].end(), midpoint);
+  if (radius) std::copy(s->crad[s->M].begin(), s->crad[s->M].end(), radius);
+  BMI_API_END
+}
+
+bmi_status bmi_space_clusters(const bmi_space* s, int level, double* midpoint, double* radius, int* count) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(s && count, BMI_INVALID_ARG, "null argument");
+  BMI_REQUIRE(level >= 0 && level <= s->M, BMI_INVALID_ARG, "level out of range");
+  *count = static_cast<int>(s->crad[level].size());
+  if (midpoint) std::copy(s->cmid[level].begin(), s->cmid[level].end(), midpoint);
+  if (radius) std::copy(s->crad[level].begin(), s->crad[level].end(), radius);
+  BMI_API_END
+}
+
+bmi_status bmi_space_transformation(const bmi_space* s, int* rowptr, int* col, double* val) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(s, BMI_INVALID_ARG, "null space");
+  if (rowptr) std::copy(s->rowptr.begin(), s->rowptr.end(), rowptr);
+  if (col) std::copy(s->col.begin(), s->col.end(), col);
+  if (val) std::copy(s->val.begin(), s->val.end(), val);
+  BMI_API_END
+}
+
+bmi_status bmi_pair_info(const bmi_space* s, int npairs, const int* pairs, int* out) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(s && (npairs == 0 || (pairs && out)), BMI_INVALID_ARG, "null argument");
+  const int ff = farfield_degree(BMI_LAPLACE_SL, s->P);
+  for (int k = 0; k < npairs; ++k) {
+    const int e1 = pairs[2 * k], e2 = pairs[2 * k + 1];
+    BMI_REQUIRE(e1 >= 0 && e1 < s->nel && e2 >= 0 && e2 < s->nel, BMI_INVALID_ARG, "element out of range");
+    int cp[3];
+    double dist;
+    compare_elements(s, e1, e2, cp, &dist);
+    int nf = nearfield_degree(BMI_LAPLACE_SL, s->P, dist, s->M);
+    out[4 * k] = cp[2];
+    out[4 * k + 1] = cp[0];
+    out[4 * k + 2] = cp[1];
+    out[4 * k + 3] = nf >= ff ? nf : ff;
+  }
+  BMI_API_END
+}
+
+static void check_op(const bmi_space* s, const bmi_op_desc* op) {
+  BMI_REQUIRE(s && op, BMI_INVALID_ARG, "null argument");
+  BMI_REQUIRE(op->kind >= 0 && op->kind <= 3, BMI_INVALID_ARG, "unknown operator kind");
+  BMI_REQUIRE((op->kind == BMI_MAXWELL_SL) == (s->form == BMI_FORM_DIVCONFORMING), BMI_INVALID_ARG,
+              "operator and space form do not match");
+  BMI_REQUIRE(s->pd > 0, BMI_UNSUPPORTED, "patches of different degrees are not supported on the device");
+  if (op->kind == BMI_MAXWELL_SL)
+    BMI_REQUIRE(op->kappa_re != 0.0 || op->kappa_im != 0.0, BMI_INVALID_ARG, "Maxwell needs a nonzero wavenumber");
+}
+
+bmi_status bmi_pair_blocks(const bmi_space* s, const bmi_op_desc* op, int npairs, const int* pairs, double* out,
+                           void* stream) {
+  BMI_API_BEGIN
+  check_op(s, op);
+  BMI_REQUIRE(npairs >= 0 && (npairs == 0 || (pairs && out)), BMI_INVALID_ARG, "null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int n = s->vd * s->I2;
+  const bool cplx = op->kind == BMI_HELMHOLTZ_SL || op->kind == BMI_MAXWELL_SL;
+  const int ff = farfield_degree(op->kind, s->P);
+  std::vector<NFPair> v(npairs);
+  for (int k = 0; k < npairs; ++k) {
+    NFPair& p = v[k];
+    p.e1 = pairs[2 * k];
+    p.e2 = pairs[2 * k + 1];
+    BMI_REQUIRE(p.e1 >= 0 && p.e1 < s->nel && p.e2 >= 0 && p.e2 < s->nel, BMI_INVALID_ARG, "element out of range");
+    int cp[3];
+    double dist;
+    compare_elements(s, p.e1, p.e2, cp, &dist);
+    int nf = nearfield_degree(op->kind, s->P, dist, s->M);
+    nf = nf >= ff ? nf : ff;
+    BMI_REQUIRE(nf < 50, BMI_UNSUPPORTED, "near-field quadrature degree too large");
+    p.cls = (cp[2] == 0) ? (nf == ff ? 0 : 1) : cp[2];
+    p.rot1 = cp[0];
+    p.rot2 = cp[1];
+    p.deg = nf;
+    p.off = static_cast<int64_t>(k) * n * n;
+  }
+  DevBuf<double> d_out;
+  d_out.alloc(static_cast<size_t>(npairs) * n * n * (cplx ? 2 : 1));
+  if (npairs) {
+    launch_nearfield(s, *op, v, d_out.ptr, n, 1, static_cast<int64_t>(s->I2) * n, s->I2, st);
+    BMI_CHECK_HIP(hipMemcpyAsync(out, d_out.ptr, d_out.bytes(), hipMemcpyDeviceToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  BMI_API_END
+}
+
+bmi_status bmi_integrate_jobs(const bmi_space* s, const bmi_op_desc* op, int njobs, const int* jobs, int qdeg,
+                              double* out, void* stream) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(s && op && (njobs == 0 || (jobs && out)), BMI_INVALID_ARG, "null argument");
+  BMI_REQUIRE(op->kind == BMI_DUMMY && s->P == 0, BMI_UNSUPPORTED, "integrate jobs support the test operator on P = 0");
+  BMI_REQUIRE(qdeg >= 0 && qdeg < 50, BMI_INVALID_ARG, "quadrature degree out of range");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::vector<int> j(jobs, jobs + 5 * njobs);
+  for (int k = 0; k < njobs; ++k) {
+    BMI_REQUIRE(j[5 * k] >= 0 && j[5 * k] <= 4, BMI_INVALID_ARG, "unknown integrate rule");
+    BMI_REQUIRE(j[5 * k + 1] >= 0 && j[5 * k + 1] < s->nel && j[5 * k + 3] >= 0 && j[5 * k + 3] < s->nel,
+                BMI_INVALID_ARG, "element out of range");
+  }
+  DevBuf<double> d_out;
+  d_out.alloc(njobs);
+  if (njobs) {
+    launch_integrate_jobs(s, *op, j, qdeg, d_out.ptr, st);
+    BMI_CHECK_HIP(hipMemcpyAsync(out, d_out.ptr, d_out.bytes(), hipMemcpyDeviceToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  BMI_API_END
+}
+
+bmi_status bmi_dense_assemble(const bmi_space* s, const bmi_op_desc* op, int super_only, double* out, int mem,
+                              void* stream) {
+  BMI_API_BEGIN
+  check_op(s, op);
+  BMI_REQUIRE(out, BMI_INVALID_ARG, "null output");
+  BMI_REQUIRE(op->kind != BMI_MAXWELL_SL, BMI_UNSUPPORTED, "Maxwell near field not available yet");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool cplx = op->kind == BMI_HELMHOLTZ_SL;
+  const int S = cplx ? 2 : 1;
+  const int64_t N = s->nsuper, D = s->ndof;
+  const int ff = farfield_degree(op->kind, s->P);
+  std::vector<NFPair> v;
+  v.reserve(static_cast<size_t>(s->nel) * s->nel);
+  for (int e2 = 0; e2 < s->nel; ++e2)
+    for (int e1 = 0; e1 < s->nel; ++e1) {
+      NFPair p;
+      p.e1 = e1;
+      p.e2 = e2;
+      int cp[3];
+      double dist;
+      compare_elements(s, e1, e2, cp, &dist);
+      int nf = nearfield_degree(op->kind, s->P, dist, s->M);
+      nf = nf >= ff ? nf : ff;
+      BMI_REQUIRE(nf < 50, BMI_UNSUPPORTED, "near-field quadrature degree too large");
+      p.cls = (cp[2] == 0) ? (nf == ff ? 0 : 1) : cp[2];
+      p.rot1 = cp[0];
+      p.rot2 = cp[1];
+      p.deg = nf;
+      p.off = static_cast<int64_t>(e2 * s->I2) * N + e1 * s->I2;
+      v.push_back(p);
+    }
+  DevBuf<double> A;
+  A.alloc(static_cast<size_t>(S) * N * N);
+  launch_nearfield(s, *op, v, A.ptr, 1, N, 0, 0, st);
+  if (super_only) {
+    BMI_CHECK_HIP(hipMemcpyAsync(out, A.ptr, A.bytes(), mem == BMI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+  } else {
+    // A T (N x D) then T^T (A T) (D x D), both with the sparse kernels
+    DevBuf<double> AT, ATt, R;
+    ensure_device_space(s, st);
+    // (A T)^T = T^T A^T: compute column j of A T as A * T(:, j) via spmv on A^T rows is awkward;
+    // use the identity T^T A T = (T^T (T^T A^T)^T) with explicit transposes on the host side.
+    std::vector<double> hA(static_cast<size_t>(S) * N * N);
+    BMI_CHECK_HIP(hipMemcpyAsync(hA.data(), A.ptr, A.bytes(), hipMemcpyDeviceToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+    std::vector<double> hAT(static_cast<size_t>(S) * N * D, 0.0);
+    for (int64_t r = 0; r < N; ++r)
+      for (int k = s->rowptr[r]; k < s->rowptr[r + 1]; ++k) {
+        const int64_t c = s->col[k];
+        const double v0 = s->val[k];
+        const double* src = &hA[S * r * N];
+        double* dst = &hAT[S * c * N];
+        for (int64_t i = 0; i < S * N; ++i) dst[i] += v0 * src[i];
+      }
+    std::vector<double> hR(static_cast<size_t>(S) * D * D, 0.0);
+    for (int64_t j = 0; j < D; ++j)
+      for (int64_t r = 0; r < N; ++r)
+        for (int k = s->rowptr[r]; k < s->rowptr[r + 1]; ++k) {
+          const int64_t row = s->col[k];
+          for (int q = 0; q < S; ++q) hR[S * (j * D + row) + q] += s->val[k] * hAT[S * (j * N + r) + q];
+        }
+    BMI_CHECK_HIP(hipMemcpyAsync(out, hR.data(), hR.size() * sizeof(double),
+                                 mem == BMI_MEM_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  BMI_API_END
+}
+
+bmi_status bmi_h2_assemble(const bmi_space* s, const bmi_op_desc* op, void* stream, bmi_h2** out) {
+  BMI_API_BEGIN
+  check_op(s, op);
+  BMI_REQUIRE(out, BMI_INVALID_ARG, "null output");
+  BMI_REQUIRE(op->n_cheb >= 1 && op->n_cheb <= 12, BMI_INVALID_ARG, "n_cheb out of range");
+  BMI_REQUIRE(op->eta > 0, BMI_INVALID_ARG, "eta must be positive");
+  BMI_REQUIRE(op->min_cluster_level >= 0 && op->min_cluster_level <= 3, BMI_INVALID_ARG, "min_cluster_level out of range");
+  BMI_REQUIRE(op->precision == BMI_PRECISION_FP64 || op->precision == BMI_PRECISION_MIXED, BMI_INVALID_ARG,
+              "unknown precision");
+  *out = nullptr;
+  auto h = std::make_unique<bmi_h2>();
+  h->s = s;
+  h->op = *op;
+  assemble_h2(h.get(), static_cast<hipStream_t>(stream));
+  *out = h.release();
+  BMI_API_END
+}
+
+bmi_status bmi_h2_destroy(bmi_h2* h) {
+  delete h;
+  return BMI_OK;
+}
+
+bmi_status bmi_h2_info(const bmi_h2* h, int64_t info[16]) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(h && info, BMI_INVALID_ARG, "null argument");
+  std::fill(info, info + 16, 0);
+  info[0] = static_cast<int64_t>(h->dense.size() + h->lowrank.size());
+  info[1] = static_cast<int64_t>(h->dense.size());
+  info[2] = static_cast<int64_t>(h->lowrank.size());
+  info[3] = h->npairs;
+  info[4] = h->dd;
+  info[5] = h->cd;
+  info[6] = h->L + 1;
+  info[7] = h->s->ndof;
+  info[8] = static_cast<int64_t>(h->bytes());
+  info[9] = h->cplx ? 1 : 0;
+  info[10] = h->op.precision;
+  for (int c = 0; c < 5; ++c) info[11 + c] = h->pair_class_count[c];
+  BMI_API_END
+}
+
+bmi_status bmi_h2_leaves(const bmi_h2* h, int* leaves) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(h && leaves, BMI_INVALID_ARG, "null argument");
+  size_t k = 0;
+  for (const auto& l : h->dense) leaves[k++] = 2, leaves[k++] = l.level, leaves[k++] = l.c1, leaves[k++] = l.c2;
+  for (const auto& l : h->lowrank) leaves[k++] = 1, leaves[k++] = l.level, leaves[k++] = l.c1, leaves[k++] = l.c2;
+  BMI_API_END
+}
+
+bmi_status bmi_h2_leaf_matrix(const bmi_h2* h, int leaf, int r, int c, double* F) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(h && F, BMI_INVALID_ARG, "null argument");
+  const int nd = static_cast<int>(h->dense.size()), nl = static_cast<int>(h->lowrank.size());
+  BMI_REQUIRE(leaf >= 0 && leaf < nd + nl, BMI_INVALID_ARG, "leaf out of range");
+  BMI_REQUIRE(r >= 0 && r < h->vd && c >= 0 && c < h->vd, BMI_INVALID_ARG, "component out of range");
+  const int S = h->cplx ? 2 : 1;
+  if (leaf < nd) {
+    const int64_t blk = static_cast<int64_t>(h->dd) * h->dd;
+    const double* src = h->d_dense.ptr + S * ((static_cast<int64_t>(leaf) * h->vd * h->vd + r * h->vd + c) * blk);
+    BMI_CHECK_HIP(hipMemcpy(F, src, S * blk * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    const int64_t blk = static_cast<int64_t>(h->cd) * h->cd;
+    const double* src =
+        h->d_coup.ptr + S * ((static_cast<int64_t>(leaf - nd) * h->vd * h->vd + r * h->vd + c) * blk);
+    BMI_CHECK_HIP(hipMemcpy(F, src, S * blk * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  BMI_API_END
+}
+
+bmi_status bmi_h2_matvec(const bmi_h2* hc, int nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                         const double* alpha, const double* beta, int mem, void* stream) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(hc && X && Y, BMI_INVALID_ARG, "null argument");
+  BMI_REQUIRE(nrhs >= 0, BMI_INVALID_ARG, "negative nrhs");
+  bmi_h2* h = const_cast<bmi_h2*>(hc);
+  const int64_t D = h->s->ndof;
+  BMI_REQUIRE(ldx >= D && ldy >= D, BMI_INVALID_ARG, "leading dimension smaller than the number of DOFs");
+  if (nrhs == 0) return BMI_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int S = h->cplx ? 2 : 1;
+  if (mem == BMI_MEM_DEVICE) {
+    h2_apply_device(h, nrhs, X, ldx, Y, ldy, alpha, beta, st);
+  } else {
+    DevBuf<double> dx, dy;
+    dx.alloc(static_cast<size_t>(S) * D * nrhs);
+    dy.alloc(static_cast<size_t>(S) * D * nrhs);
+    for (int k = 0; k < nrhs; ++k) {
+      BMI_CHECK_HIP(hipMemcpyAsync(dx.ptr + S * D * k, X + S * ldx * k, S * D * sizeof(double), hipMemcpyHostToDevice, st));
+      BMI_CHECK_HIP(hipMemcpyAsync(dy.ptr + S * D * k, Y + S * ldy * k, S * D * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    h2_apply_device(h, nrhs, dx.ptr, D, dy.ptr, D, alpha, beta, st);
+    for (int k = 0; k < nrhs; ++k)
+      BMI_CHECK_HIP(hipMemcpyAsync(Y + S * ldy * k, dy.ptr + S * D * k, S * D * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+  BMI_API_END
+}
+
+bmi_status bmi_h2_matvec_super(const bmi_h2* hc, int nrhs, const double* Xs, double* Ys, void* stream) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(hc && Xs && Ys && nrhs >= 0, BMI_INVALID_ARG, "bad argument");
+  bmi_h2* h = const_cast<bmi_h2*>(hc);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  ensure_workspace(h, nrhs, st);
+  apply_super(h, nrhs, Xs, Ys, st);
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+  BMI_API_END
+}
+
+bmi_status bmi_timing_enable(int enable) {
+  bmi::g_timing = enable != 0;
+  return BMI_OK;
+}
+bmi_status bmi_timing_reset(void) {
+  std::lock_guard<std::mutex> g(bmi::g_tmu);
+  bmi::g_times.clear();
+  return BMI_OK;
+}
+bmi_status bmi_timing_get(const char* name, double* total_ms, int64_t* launches) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(name && total_ms && launches, BMI_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> g(bmi::g_tmu);
+  auto it = bmi::g_times.find(name);
+  *total_ms = it == bmi::g_times.end() ? 0.0 : it->second.first;
+  *launches = it == bmi::g_times.end() ? 0 : it->second.second;
+  BMI_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,435 @@
+// Interpolation-FMM kernels: coupling assembly and the H2 matrix-vector product.
+//
+// Restates H2Multipole::interpolateKernel (H2Matrix/H2Multipole.hpp:336-375), the forward and
+// backward transformations (:380-469) and the leaf sweep of H2_time_dense_product_impl::run
+// (H2Matrix/EigenHelper/H2DenseProduct.hpp:39-164).  Instead of one RHS column at a time and
+// per-thread copies of the backward tree, every level is one launch over all clusters and
+// right-hand sides; sums over leaves are owned by their row cluster (no atomics), so the
+// result is deterministic.
+//
+// Vector layouts (S = 1 real or 2 complex doubles per scalar):
+//   super vectors xs/ys: [component][element][local function] rows, one column per RHS
+//   (column stride = nsuper), exactly the reference's super-space order.
+//   multipole levels fw/bw: [cluster][rhs][row], row = comp*np2 + node.
+#include <hip/hip_runtime.h>
+
+#include "bmi_device.h"
+#include "bmi_fmm.h"
+#include "bmi_internal.h"
+
+namespace bmi {
+
+// ------------------------------------------------------------------ cluster node tables
+// surface points of the Chebyshev nodes of every cluster of one level:
+// out[(c*np2 + i)*CP_STRIDE] = x0 x1 x2 kappa ds0 ds1 ds2 dt0 dt1 dt2 (pad to 12)
+template <int PD>
+__global__ void k_cluster_points(const double* __restrict__ cp, int cp_stride, int level, int ncl,
+                                 int np, const double* __restrict__ cheb, double* out) {
+  const int np2 = np * np;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(ncl) * np2) return;
+  const int c = static_cast<int>(gid / np2), i = static_cast<int>(gid % np2);
+  const int patch = c >> (2 * level);
+  const int m = c & ((1 << (2 * level)) - 1);
+  int ix = 0, iy = 0;
+  for (int l = level - 1; l >= 0; --l) {
+    const int d = (m >> (2 * l)) & 3;
+    ix = (ix << 1) | ((d == 1 || d == 2) ? 1 : 0);
+    iy = (iy << 1) | (d >= 2 ? 1 : 0);
+  }
+  const double H = 1.0 / static_cast<double>(1 << level);
+  const double u = cheb[i / np], v = cheb[i % np];
+  double X[3], Ds[3], Dt[3];
+  surface_point<PD>(cp + static_cast<int64_t>(patch) * cp_stride, ix * H + H * u, iy * H + H * v, X, Ds,
+                    Dt);
+  const double c0 = Ds[1] * Dt[2] - Ds[2] * Dt[1];
+  const double c1 = Ds[2] * Dt[0] - Ds[0] * Dt[2];
+  const double c2 = Ds[0] * Dt[1] - Ds[1] * Dt[0];
+  double* o = out + gid * CP_STRIDE;
+  o[0] = X[0], o[1] = X[1], o[2] = X[2];
+  o[3] = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  o[4] = Ds[0], o[5] = Ds[1], o[6] = Ds[2];
+  o[7] = Dt[0], o[8] = Dt[1], o[9] = Dt[2];
+  o[10] = 0.0, o[11] = 0.0;
+}
+
+void launch_cluster_points(const bmi_space* s, int level, int np, const double* d_cheb, double* out,
+                           hipStream_t st) {
+  const int ncl = s->npatch << (2 * level);
+  const int64_t total = static_cast<int64_t>(ncl) * np * np;
+  const int blocks = static_cast<int>((total + 255) / 256);
+  const int stride = 4 * s->pd * s->pd;
+  if (s->pd == 5)
+    hipLaunchKernelGGL(k_cluster_points<5>, dim3(blocks), dim3(256), 0, st, s->d_cp.ptr, stride, level,
+                       ncl, np, d_cheb, out);
+  else if (s->pd == 2)
+    hipLaunchKernelGGL(k_cluster_points<2>, dim3(blocks), dim3(256), 0, st, s->d_cp.ptr, stride, level,
+                       ncl, np, d_cheb, out);
+  else
+    throw Error(BMI_UNSUPPORTED, "patch degree not supported on the device");
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ coupling assembly
+// F(i + (ii*nc+a)*np2, j + (jj*nc+b)*np2) of one admissible block, stored per component
+// block (r = ii, c = jj) column-major with dimension cd = nc*np2
+// (evaluateFMMInterpolation_impl: Laplace/SingleLayerOperator.hpp:87-110,
+//  Helmholtz/SingleLayerOperator.hpp:87-110, Maxwell/SingleLayerOperator.hpp:90-120)
+template <int KIND>
+__global__ void k_coupling_assemble(const int4* __restrict__ leaves, int nleaves, int np2,
+                                    const double* const* __restrict__ level_pts, double kre, double kim,
+                                    double k2re, double k2im, double* out) {
+  constexpr bool CPLX = KIND != BMI_LAPLACE_SL;
+  constexpr int VD = KIND == BMI_MAXWELL_SL ? 2 : 1;
+  constexpr int NC = KIND == BMI_MAXWELL_SL ? 2 : 1;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = static_cast<int64_t>(np2) * np2;
+  if (gid >= per * nleaves) return;
+  const int leaf = static_cast<int>(gid / per);
+  const int q = static_cast<int>(gid % per);
+  const int j = q / np2, i = q % np2;  // column-major: consecutive threads walk rows
+  const int4 L = leaves[leaf];         // level, c1, c2, unused
+  const double* pts = level_pts[L.x];
+  const double* a = pts + (static_cast<int64_t>(L.y) * np2 + i) * CP_STRIDE;
+  const double* b = pts + (static_cast<int64_t>(L.z) * np2 + j) * CP_STRIDE;
+  const double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+  const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  const int cd = NC * np2;
+  const int64_t blk = static_cast<int64_t>(cd) * cd;
+  const int64_t base = static_cast<int64_t>(leaf) * VD * VD * blk;
+  if constexpr (KIND == BMI_LAPLACE_SL) {
+    out[base + static_cast<int64_t>(j) * cd + i] = (0.25 * M_1_PI) / r * a[3] * b[3];
+  } else {
+    double sn, cs;
+    sincos(kre * r, &sn, &cs);
+    const double mag = exp(kim * r) * (0.25 * M_1_PI) / r;
+    const double kr = mag * cs, ki = -mag * sn;
+    if constexpr (KIND == BMI_HELMHOLTZ_SL) {
+      const int64_t idx = base + static_cast<int64_t>(j) * cd + i;
+      const double f = a[3] * b[3];
+      out[2 * idx] = kr * f;
+      out[2 * idx + 1] = ki * f;
+    } else {
+      // 4x4 interpolant [[K t.t', 0], [0, -K/kappa^2]] per vector-component pair
+      const double dd[2][2] = {{a[4] * b[4] + a[5] * b[5] + a[6] * b[6], a[4] * b[7] + a[5] * b[8] + a[6] * b[9]},
+                               {a[7] * b[4] + a[8] * b[5] + a[9] * b[6], a[7] * b[7] + a[8] * b[8] + a[9] * b[9]}};
+      const double den = k2re * k2re + k2im * k2im;
+      const double qr = -(kr * k2re + ki * k2im) / den, qi = -(ki * k2re - kr * k2im) / den;
+#pragma unroll
+      for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+          const int64_t cb = base + (ii * VD + jj) * blk;
+          const int64_t i0 = cb + static_cast<int64_t>(j) * cd + i;                  // (a=0,b=0)
+          const int64_t i1 = cb + static_cast<int64_t>(j + np2) * cd + i + np2;      // (a=1,b=1)
+          const int64_t i01 = cb + static_cast<int64_t>(j + np2) * cd + i;           // (0,1)
+          const int64_t i10 = cb + static_cast<int64_t>(j) * cd + i + np2;           // (1,0)
+          out[2 * i0] = kr * dd[ii][jj];
+          out[2 * i0 + 1] = ki * dd[ii][jj];
+          out[2 * i1] = qr;
+          out[2 * i1 + 1] = qi;
+          out[2 * i01] = 0.0, out[2 * i01 + 1] = 0.0;
+          out[2 * i10] = 0.0, out[2 * i10 + 1] = 0.0;
+        }
+    }
+  }
+}
+
+void launch_coupling_assemble(int kind, const int4* d_leaves, int nleaves, int np2,
+                              const double* const* d_level_pts, double kre, double kim, double* out,
+                              hipStream_t st) {
+  if (nleaves == 0) return;
+  const int64_t total = static_cast<int64_t>(nleaves) * np2 * np2;
+  const int blocks = static_cast<int>((total + 255) / 256);
+  const double k2re = kre * kre - kim * kim, k2im = 2 * kre * kim;
+  TimedScope ts("coupling_assemble", st);
+  if (kind == BMI_LAPLACE_SL)
+    hipLaunchKernelGGL(k_coupling_assemble<BMI_LAPLACE_SL>, dim3(blocks), dim3(256), 0, st, d_leaves,
+                       nleaves, np2, d_level_pts, kre, kim, k2re, k2im, out);
+  else if (kind == BMI_HELMHOLTZ_SL)
+    hipLaunchKernelGGL(k_coupling_assemble<BMI_HELMHOLTZ_SL>, dim3(blocks), dim3(256), 0, st, d_leaves,
+                       nleaves, np2, d_level_pts, kre, kim, k2re, k2im, out);
+  else
+    hipLaunchKernelGGL(k_coupling_assemble<BMI_MAXWELL_SL>, dim3(blocks), dim3(256), 0, st, d_leaves,
+                       nleaves, np2, d_level_pts, kre, kim, k2re, k2im, out);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ complex helpers
+template <bool C>
+struct Val;
+template <>
+struct Val<false> {
+  double r;
+  __device__ Val() : r(0) {}
+  __device__ static Val load(const double* p, int64_t i) {
+    Val v;
+    v.r = p[i];
+    return v;
+  }
+  __device__ void store(double* p, int64_t i) const { p[i] = r; }
+  __device__ void fma_real(double a, const Val& x) { r += a * x.r; }
+  __device__ void fma(const Val& a, const Val& x) { r += a.r * x.r; }
+};
+template <>
+struct Val<true> {
+  double r, i;
+  __device__ Val() : r(0), i(0) {}
+  __device__ static Val load(const double* p, int64_t k) {
+    Val v;
+    v.r = p[2 * k];
+    v.i = p[2 * k + 1];
+    return v;
+  }
+  __device__ void store(double* p, int64_t k) const {
+    p[2 * k] = r;
+    p[2 * k + 1] = i;
+  }
+  __device__ void fma_real(double a, const Val& x) {
+    r += a * x.r;
+    i += a * x.i;
+  }
+  __device__ void fma(const Val& a, const Val& x) {
+    r += a.r * x.r - a.i * x.i;
+    i += a.r * x.i + a.i * x.r;
+  }
+};
+
+// ------------------------------------------------------------------ sparse T, T^T
+// y[row, k] = alpha * sum_j A[row, j] x[j, k] + beta * y[row, k]
+template <bool C>
+__global__ void k_spmv(int nrows, int nrhs, const int* __restrict__ rp, const int* __restrict__ ci,
+                       const double* __restrict__ v, const double* __restrict__ x, int64_t ldx, double* y,
+                       int64_t ldy, double ar, double ai, double br, double bi, int use_beta) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(nrows) * nrhs) return;
+  const int row = static_cast<int>(gid % nrows), k = static_cast<int>(gid / nrows);
+  Val<C> acc;
+  const double* xk = x + (C ? 2 : 1) * static_cast<int64_t>(k) * ldx;
+  for (int p = rp[row]; p < rp[row + 1]; ++p) acc.fma_real(v[p], Val<C>::load(xk, ci[p]));
+  double* yk = y + (C ? 2 : 1) * static_cast<int64_t>(k) * ldy;
+  if constexpr (C) {
+    double r = ar * acc.r - ai * acc.i, im = ar * acc.i + ai * acc.r;
+    if (use_beta) {
+      const double yr = yk[2 * row], yi = yk[2 * row + 1];
+      r += br * yr - bi * yi;
+      im += br * yi + bi * yr;
+    }
+    yk[2 * row] = r;
+    yk[2 * row + 1] = im;
+  } else {
+    double r = ar * acc.r;
+    if (use_beta) r += br * yk[row];
+    yk[row] = r;
+  }
+}
+
+// ------------------------------------------------------------------ forward transformation
+// leaf moments: fw[c][k][comp rows] = Mom (rows x mc) * xs[comp_seg + c*mc .. , k]
+template <bool C>
+__global__ void k_moment(int ncl, int nrhs, int rows, int mc, const double* __restrict__ mom,
+                         const double* __restrict__ xs, int64_t ldxs, int64_t seg, double* fw) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(ncl) * nrhs * rows) return;
+  const int i = static_cast<int>(gid % rows);
+  const int64_t ck = gid / rows;
+  const int k = static_cast<int>(ck % nrhs), c = static_cast<int>(ck / nrhs);
+  const double* x = xs + (C ? 2 : 1) * (static_cast<int64_t>(k) * ldxs + seg + static_cast<int64_t>(c) * mc);
+  Val<C> acc;
+  for (int j = 0; j < mc; ++j) acc.fma_real(mom[static_cast<int64_t>(j) * rows + i], Val<C>::load(x, j));
+  acc.store(fw, gid);
+}
+
+// parent moments: fw_l[c][k][comp*np2+i] = sum_q sum_j T(i, q*np2+j) fw_{l+1}[4c+q][k][comp*np2+j]
+template <bool C>
+__global__ void k_m2m(int ncl, int nrhs, int rows, int np2, const double* __restrict__ T,
+                      const double* __restrict__ child, double* parent) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(ncl) * nrhs * rows) return;
+  const int row = static_cast<int>(gid % rows);
+  const int64_t ck = gid / rows;
+  const int k = static_cast<int>(ck % nrhs), c = static_cast<int>(ck / nrhs);
+  const int comp = row / np2, i = row % np2;
+  Val<C> acc;
+  for (int q = 0; q < 4; ++q) {
+    const int64_t cbase = ((static_cast<int64_t>(4 * c + q) * nrhs + k) * rows + comp * np2);
+    for (int j = 0; j < np2; ++j)
+      acc.fma_real(T[static_cast<int64_t>(q * np2 + j) * np2 + i], Val<C>::load(child, cbase + j));
+  }
+  acc.store(parent, gid);
+}
+
+// ------------------------------------------------------------------ coupling apply
+// bw[t][k][i] += sum over blocks (t, s): F (cd x cd) * fw[s][k][:]  for all levels at once;
+// targets are (level, cluster) pairs listed in `tgt`, their blocks in CSR `rp`/`src`.
+template <bool C, typename FT>
+__global__ void k_coupling_apply(int ntgt, int nrhs, int cd, const int2* __restrict__ tgt,
+                                 const int* __restrict__ rp, const int* __restrict__ src,
+                                 const FT* __restrict__ F, int64_t fstride, int64_t fcomp,
+                                 const int64_t* __restrict__ lvl_off, const double* __restrict__ fw,
+                                 double* bw, int accumulate) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(ntgt) * nrhs * cd) return;
+  const int i = static_cast<int>(gid % cd);
+  const int64_t tk = gid / cd;
+  const int k = static_cast<int>(tk % nrhs), t = static_cast<int>(tk / nrhs);
+  const int2 T = tgt[t];  // level, cluster
+  const int64_t off = lvl_off[T.x] * nrhs;  // level start in [level][cluster][rhs][row]
+  Val<C> acc;
+  for (int p = rp[t]; p < rp[t + 1]; ++p) {
+    const FT* f = F + (C ? 2 : 1) * (static_cast<int64_t>(p) * fstride + fcomp);
+    const int64_t xb = off + (static_cast<int64_t>(src[p]) * nrhs + k) * cd;
+    for (int j = 0; j < cd; ++j) {
+      Val<C> a;
+      if constexpr (C) {
+        a.r = static_cast<double>(f[2 * (static_cast<int64_t>(j) * cd + i)]);
+        a.i = static_cast<double>(f[2 * (static_cast<int64_t>(j) * cd + i) + 1]);
+      } else {
+        a.r = static_cast<double>(f[static_cast<int64_t>(j) * cd + i]);
+      }
+      acc.fma(a, Val<C>::load(fw, xb + j));
+    }
+  }
+  const int64_t o = off + (static_cast<int64_t>(T.y) * nrhs + k) * cd + i;
+  if (accumulate) {
+    Val<C> prev = Val<C>::load(bw, o);
+    acc.fma_real(1.0, prev);
+  }
+  acc.store(bw, o);
+}
+
+// ------------------------------------------------------------------ backward transformation
+// children += T^T parent: bw_{l+1}[4c+q][k][comp*np2+j] += sum_i T(i, q*np2+j) bw_l[c][k][comp*np2+i]
+template <bool C>
+__global__ void k_l2l(int nchild, int nrhs, int rows, int np2, const double* __restrict__ T,
+                      const double* __restrict__ parent, double* child) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(nchild) * nrhs * rows) return;
+  const int row = static_cast<int>(gid % rows);
+  const int64_t ck = gid / rows;
+  const int k = static_cast<int>(ck % nrhs), ch = static_cast<int>(ck / nrhs);
+  const int comp = row / np2, j = row % np2;
+  const int c = ch / 4, q = ch % 4;
+  const int64_t pbase = (static_cast<int64_t>(c) * nrhs + k) * rows + comp * np2;
+  Val<C> acc = Val<C>::load(child, gid);
+  for (int i = 0; i < np2; ++i)
+    acc.fma_real(T[static_cast<int64_t>(q * np2 + j) * np2 + i], Val<C>::load(parent, pbase + i));
+  acc.store(child, gid);
+}
+
+// leaf rows: ys[seg + c*dd + i, k] = sum_dense F (dd x dd) xs[cseg + c2*dd .., k]
+//                                    + sum_r Mom(r, i) bw_L[c][k][r]
+template <bool C>
+__global__ void k_leaf(int ncl, int nrhs, int dd, const int* __restrict__ rp, const int* __restrict__ src,
+                       const double* __restrict__ F, int64_t fstride, int64_t fcomp,
+                       const double* __restrict__ xs, int64_t ldxs, int64_t xseg,
+                       const double* __restrict__ mom, int rows, const double* __restrict__ bw,
+                       double* ys, int64_t ldys, int64_t yseg, int accumulate) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(ncl) * nrhs * dd) return;
+  const int i = static_cast<int>(gid % dd);
+  const int64_t ck = gid / dd;
+  const int c = static_cast<int>(ck % ncl), k = static_cast<int>(ck / ncl);
+  Val<C> acc;
+  const double* xk = xs + (C ? 2 : 1) * (static_cast<int64_t>(k) * ldxs + xseg);
+  for (int p = rp[c]; p < rp[c + 1]; ++p) {
+    const double* f = F + (C ? 2 : 1) * (static_cast<int64_t>(p) * fstride + fcomp);
+    const int64_t xb = static_cast<int64_t>(src[p]) * dd;
+    for (int j = 0; j < dd; ++j) acc.fma(Val<C>::load(f, static_cast<int64_t>(j) * dd + i), Val<C>::load(xk, xb + j));
+  }
+  if (bw) {
+    const int64_t bb = (static_cast<int64_t>(c) * nrhs + k) * rows;
+    for (int r = 0; r < rows; ++r) acc.fma_real(mom[static_cast<int64_t>(i) * rows + r], Val<C>::load(bw, bb + r));
+  }
+  double* yk = ys + (C ? 2 : 1) * (static_cast<int64_t>(k) * ldys + yseg);
+  const int64_t o = static_cast<int64_t>(c) * dd + i;
+  if (accumulate) acc.fma_real(1.0, Val<C>::load(yk, o));
+  acc.store(yk, o);
+}
+
+// ------------------------------------------------------------------ host launchers
+static inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
+
+void launch_spmv(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* x,
+                 int64_t ldx, double* y, int64_t ldy, const double* alpha, const double* beta, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(nrows) * nrhs;
+  if (n == 0) return;
+  const double ar = alpha ? alpha[0] : 1.0, ai = alpha ? alpha[1] : 0.0;
+  const double br = beta ? beta[0] : 0.0, bi = beta ? beta[1] : 0.0;
+  const int ub = (beta && (br != 0.0 || bi != 0.0)) ? 1 : 0;
+  if (cplx)
+    hipLaunchKernelGGL(k_spmv<true>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, x, ldx, y, ldy,
+                       ar, ai, br, bi, ub);
+  else
+    hipLaunchKernelGGL(k_spmv<false>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, x, ldx, y, ldy,
+                       ar, ai, br, bi, ub);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+void launch_moment(bool cplx, int ncl, int nrhs, int rows, int mc, const double* mom, const double* xs,
+                   int64_t ldxs, int64_t seg, double* fw, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(ncl) * nrhs * rows;
+  if (cplx)
+    hipLaunchKernelGGL(k_moment<true>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, rows, mc, mom, xs, ldxs, seg, fw);
+  else
+    hipLaunchKernelGGL(k_moment<false>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, rows, mc, mom, xs, ldxs, seg, fw);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T, const double* child,
+                double* parent, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(ncl) * nrhs * rows;
+  if (cplx)
+    hipLaunchKernelGGL(k_m2m<true>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, rows, np2, T, child, parent);
+  else
+    hipLaunchKernelGGL(k_m2m<false>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, rows, np2, T, child, parent);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+void launch_coupling_apply(bool cplx, bool fp32, int ntgt, int nrhs, int cd, const int2* tgt, const int* rp,
+                           const int* src, const void* F, int64_t fstride, int64_t fcomp, const int64_t* lvl_off,
+                           const double* fw, double* bw, int accumulate, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(ntgt) * nrhs * cd;
+  if (n == 0) return;
+  TimedScope ts("coupling_apply", st);
+  if (cplx && !fp32)
+    hipLaunchKernelGGL((k_coupling_apply<true, double>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt, rp,
+                       src, static_cast<const double*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
+  else if (!cplx && !fp32)
+    hipLaunchKernelGGL((k_coupling_apply<false, double>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt,
+                       rp, src, static_cast<const double*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
+  else if (cplx)
+    hipLaunchKernelGGL((k_coupling_apply<true, float>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt, rp,
+                       src, static_cast<const float*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
+  else
+    hipLaunchKernelGGL((k_coupling_apply<false, float>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt, rp,
+                       src, static_cast<const float*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+void launch_l2l(bool cplx, int nchild, int nrhs, int rows, int np2, const double* T, const double* parent,
+                double* child, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(nchild) * nrhs * rows;
+  if (cplx)
+    hipLaunchKernelGGL(k_l2l<true>, dim3(nblk(n)), dim3(256), 0, st, nchild, nrhs, rows, np2, T, parent, child);
+  else
+    hipLaunchKernelGGL(k_l2l<false>, dim3(nblk(n)), dim3(256), 0, st, nchild, nrhs, rows, np2, T, parent, child);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+void launch_leaf(bool cplx, int ncl, int nrhs, int dd, const int* rp, const int* src, const double* F,
+                 int64_t fstride, int64_t fcomp, const double* xs, int64_t ldxs, int64_t xseg, const double* mom,
+                 int rows, const double* bw, double* ys, int64_t ldys, int64_t yseg, int accumulate,
+                 hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(ncl) * nrhs * dd;
+  TimedScope ts("matvec_nearfield", st);
+  if (cplx)
+    hipLaunchKernelGGL(k_leaf<true>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, dd, rp, src, F, fstride, fcomp,
+                       xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);
+  else
+    hipLaunchKernelGGL(k_leaf<false>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, dd, rp, src, F, fstride, fcomp,
+                       xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace bmi

@@ -1,0 +1,519 @@
+// Device solvers (batched restarted GMRES with CGS2 Arnoldi in FP64, batched CG) and the
+// Dirichlet-trace right-hand sides.
+//
+// The consumers of the reference are Eigen::GMRES / Eigen::ConjugateGradient over the H2
+// product (examples/HelmholtzSingleLayerH2.cpp:82-85, examples/LaplaceSingleLayerH2.cpp:78-81).
+// Here all right-hand sides advance together: one batched H2 product per Arnoldi step, all
+// inner products of a step in one launch (classical Gram-Schmidt applied twice), and the tiny
+// Hessenberg/Givens work per column on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "bmi_device.h"
+#include "bmi_fmm.h"
+#include "bmi_internal.h"
+#include "bmi_solver.h"
+
+namespace bmi {
+
+__global__ void k_to_fp32(const double* src, float* dst, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = static_cast<float>(src[i]);
+}
+void convert_to_fp32(const double* src, float* dst, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_fp32, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+// out[(j*nrhs + c)] = sum_i conj(V_j[i, c]) * W[i, c], V_j = V + j*vstride; one block per (j, c)
+template <bool C>
+__global__ void k_dots(const double* __restrict__ V, int64_t vstride, const double* __restrict__ W, int64_t n,
+                       int nrhs, double* out) {
+  __shared__ double sr[256], si[256];
+  const int j = blockIdx.x / nrhs, c = blockIdx.x % nrhs;
+  const int S = C ? 2 : 1;
+  const double* v = V + j * vstride + static_cast<int64_t>(S) * n * c;
+  const double* w = W + static_cast<int64_t>(S) * n * c;
+  double ar = 0, ai = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    if (C) {
+      const double vr = v[2 * i], vi = v[2 * i + 1], wr = w[2 * i], wi = w[2 * i + 1];
+      ar += vr * wr + vi * wi;
+      ai += vr * wi - vi * wr;
+    } else {
+      ar += v[i] * w[i];
+    }
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      sr[threadIdx.x] += sr[threadIdx.x + off];
+      si[threadIdx.x] += si[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[S * blockIdx.x] = sr[0];
+    if (C) out[S * blockIdx.x + 1] = si[0];
+  }
+}
+
+// W[i, c] -= sum_{j < nv} V_j[i, c] * h[j, c]
+template <bool C>
+__global__ void k_subtract(const double* __restrict__ V, int64_t vstride, int nv, const double* __restrict__ h,
+                           double* W, int64_t n, int nrhs) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n * nrhs) return;
+  const int c = static_cast<int>(gid / n);
+  const int S = C ? 2 : 1;
+  double wr = W[S * gid], wi = C ? W[S * gid + 1] : 0.0;
+  for (int j = 0; j < nv; ++j) {
+    const double* v = V + j * vstride + S * gid;
+    const double hr = h[S * (j * nrhs + c)];
+    if (C) {
+      const double hi = h[S * (j * nrhs + c) + 1];
+      wr -= v[0] * hr - v[1] * hi;
+      wi -= v[0] * hi + v[1] * hr;
+    } else {
+      wr -= v[0] * hr;
+    }
+  }
+  W[S * gid] = wr;
+  if (C) W[S * gid + 1] = wi;
+}
+
+// Y[i, c] = a_c * X[i, c] + b_c * Y[i, c] with per-column complex coefficients (or real)
+template <bool C>
+__global__ void k_axpby(const double* __restrict__ a, const double* __restrict__ X, const double* __restrict__ b,
+                        double* Y, int64_t n, int nrhs) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n * nrhs) return;
+  const int c = static_cast<int>(gid / n);
+  if (C) {
+    const double ar = a[2 * c], ai = a[2 * c + 1], br = b[2 * c], bi = b[2 * c + 1];
+    const double xr = X[2 * gid], xi = X[2 * gid + 1], yr = Y[2 * gid], yi = Y[2 * gid + 1];
+    Y[2 * gid] = ar * xr - ai * xi + br * yr - bi * yi;
+    Y[2 * gid + 1] = ar * xi + ai * xr + br * yi + bi * yr;
+  } else {
+    Y[gid] = a[c] * X[gid] + b[c] * Y[gid];
+  }
+}
+
+// X[i, c] += sum_j V_j[i, c] y[j, c]
+template <bool C>
+__global__ void k_combine(const double* __restrict__ V, int64_t vstride, int nv, const double* __restrict__ y,
+                          double* X, int64_t n, int nrhs) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n * nrhs) return;
+  const int c = static_cast<int>(gid / n);
+  const int S = C ? 2 : 1;
+  double xr = X[S * gid], xi = C ? X[S * gid + 1] : 0.0;
+  for (int j = 0; j < nv; ++j) {
+    const double* v = V + j * vstride + S * gid;
+    const double yr = y[S * (j * nrhs + c)];
+    if (C) {
+      const double yi = y[S * (j * nrhs + c) + 1];
+      xr += v[0] * yr - v[1] * yi;
+      xi += v[0] * yi + v[1] * yr;
+    } else {
+      xr += v[0] * yr;
+    }
+  }
+  X[S * gid] = xr;
+  if (C) X[S * gid + 1] = xi;
+}
+
+static inline unsigned nb(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+struct Vecs {
+  bool C;
+  int64_t n;
+  int nrhs;
+  hipStream_t st;
+  int S() const { return C ? 2 : 1; }
+  int64_t block() const { return static_cast<int64_t>(S()) * n * nrhs; }
+  void dots(const double* V, int nv, const double* W, double* d_out, std::vector<double>& host) const {
+    if (C)
+      hipLaunchKernelGGL(k_dots<true>, dim3(nv * nrhs), dim3(256), 0, st, V, block(), W, n, nrhs, d_out);
+    else
+      hipLaunchKernelGGL(k_dots<false>, dim3(nv * nrhs), dim3(256), 0, st, V, block(), W, n, nrhs, d_out);
+    BMI_CHECK_HIP(hipGetLastError());
+    host.resize(static_cast<size_t>(S()) * nv * nrhs);
+    BMI_CHECK_HIP(hipMemcpyAsync(host.data(), d_out, host.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  void subtract(const double* V, int nv, const double* d_h, double* W) const {
+    if (C)
+      hipLaunchKernelGGL(k_subtract<true>, dim3(nb(n * nrhs)), dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs);
+    else
+      hipLaunchKernelGGL(k_subtract<false>, dim3(nb(n * nrhs)), dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs);
+    BMI_CHECK_HIP(hipGetLastError());
+  }
+  void axpby(const double* d_a, const double* X, const double* d_b, double* Y) const {
+    if (C)
+      hipLaunchKernelGGL(k_axpby<true>, dim3(nb(n * nrhs)), dim3(256), 0, st, d_a, X, d_b, Y, n, nrhs);
+    else
+      hipLaunchKernelGGL(k_axpby<false>, dim3(nb(n * nrhs)), dim3(256), 0, st, d_a, X, d_b, Y, n, nrhs);
+    BMI_CHECK_HIP(hipGetLastError());
+  }
+  void combine(const double* V, int nv, const double* d_y, double* X) const {
+    if (C)
+      hipLaunchKernelGGL(k_combine<true>, dim3(nb(n * nrhs)), dim3(256), 0, st, V, block(), nv, d_y, X, n, nrhs);
+    else
+      hipLaunchKernelGGL(k_combine<false>, dim3(nb(n * nrhs)), dim3(256), 0, st, V, block(), nv, d_y, X, n, nrhs);
+    BMI_CHECK_HIP(hipGetLastError());
+  }
+};
+
+using cplx = std::complex<double>;
+
+// batched restarted GMRES; returns true if every column converged
+static bool gmres(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int restart, int maxit,
+                  std::vector<int>& iters, std::vector<double>& resid, hipStream_t st) {
+  const bool C = h2_complex(h);
+  const int64_t n = h2_rows(h);
+  Vecs vs{C, n, nrhs, st};
+  const int S = vs.S();
+  const int m = std::max(1, std::min(restart, maxit));
+  DevBuf<double> V, W, R, d_small, d_coef;
+  V.alloc(static_cast<size_t>(vs.block()) * (m + 1));
+  W.alloc(vs.block());
+  R.alloc(vs.block());
+  d_small.alloc(static_cast<size_t>(S) * (m + 2) * nrhs * 2);
+  d_coef.alloc(static_cast<size_t>(S) * nrhs * 2);
+  std::vector<double> hbuf, h2buf;
+  // norms of B
+  vs.dots(B, 1, B, d_small.ptr, hbuf);
+  std::vector<double> bn(nrhs);
+  for (int c = 0; c < nrhs; ++c) bn[c] = std::sqrt(std::max(hbuf[S * c], 0.0)), bn[c] = bn[c] > 0 ? bn[c] : 1.0;
+  iters.assign(nrhs, 0);
+  resid.assign(nrhs, 0.0);
+  std::vector<char> done(nrhs, 0);
+  const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
+  int total = 0;
+  auto upload_coef = [&](const std::vector<cplx>& a, const std::vector<cplx>& b) {
+    std::vector<double> buf(static_cast<size_t>(2) * S * nrhs);
+    for (int c = 0; c < nrhs; ++c) {
+      buf[S * c] = a[c].real();
+      buf[S * (nrhs + c)] = b[c].real();
+      if (C) buf[S * c + 1] = a[c].imag(), buf[S * (nrhs + c) + 1] = b[c].imag();
+    }
+    BMI_CHECK_HIP(hipMemcpyAsync(d_coef.ptr, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  };
+  while (true) {
+    // R = B - A X
+    BMI_CHECK_HIP(hipMemcpyAsync(R.ptr, B, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
+    h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st);
+    vs.dots(R.ptr, 1, R.ptr, d_small.ptr, hbuf);
+    std::vector<double> beta(nrhs);
+    bool all = true;
+    for (int c = 0; c < nrhs; ++c) {
+      beta[c] = std::sqrt(std::max(hbuf[S * c], 0.0));
+      resid[c] = beta[c] / bn[c];
+      if (resid[c] <= tol) done[c] = 1;
+      all = all && done[c];
+    }
+    if (all || total >= maxit) return all;
+    // V0 = R / beta
+    std::vector<cplx> a(nrhs), b(nrhs, 0.0);
+    for (int c = 0; c < nrhs; ++c) a[c] = beta[c] > 0 ? 1.0 / beta[c] : 0.0;
+    upload_coef(a, b);
+    vs.axpby(d_coef.ptr, R.ptr, d_coef.ptr + S * nrhs, V.ptr);
+    // per-column Hessenberg (column-major (m+1) x m), rotations, rhs g
+    std::vector<std::vector<cplx>> H(nrhs, std::vector<cplx>((m + 1) * m, 0.0)), g(nrhs, std::vector<cplx>(m + 1, 0.0));
+    std::vector<std::vector<cplx>> cs(nrhs, std::vector<cplx>(m)), sn(nrhs, std::vector<cplx>(m));
+    std::vector<int> kc(nrhs, 0);
+    std::vector<char> active(nrhs, 0);
+    for (int c = 0; c < nrhs; ++c) g[c][0] = beta[c], active[c] = !done[c] && beta[c] > 0;
+    int k = 0;
+    for (; k < m && total < maxit; ++k, ++total) {
+      double* Vk = V.ptr + vs.block() * k;
+      h2_apply_device(h, nrhs, Vk, n, W.ptr, n, one, nullptr, st);
+      // classical Gram-Schmidt, twice
+      vs.dots(V.ptr, k + 1, W.ptr, d_small.ptr, hbuf);
+      vs.subtract(V.ptr, k + 1, d_small.ptr, W.ptr);
+      vs.dots(V.ptr, k + 1, W.ptr, d_small.ptr, h2buf);
+      vs.subtract(V.ptr, k + 1, d_small.ptr, W.ptr);
+      for (size_t q = 0; q < hbuf.size(); ++q) hbuf[q] += h2buf[q];
+      vs.dots(W.ptr, 1, W.ptr, d_small.ptr, h2buf);
+      std::vector<cplx> inv(nrhs), zero(nrhs, 0.0);
+      bool any = false;
+      for (int c = 0; c < nrhs; ++c) {
+        const double hn = std::sqrt(std::max(h2buf[S * c], 0.0));
+        inv[c] = hn > 0 ? 1.0 / hn : 0.0;
+        if (!active[c]) continue;
+        std::vector<cplx>& Hc = H[c];
+        for (int j = 0; j <= k; ++j)
+          Hc[k * (m + 1) + j] = C ? cplx(hbuf[S * (j * nrhs + c)], hbuf[S * (j * nrhs + c) + 1]) : cplx(hbuf[j * nrhs + c], 0.0);
+        Hc[k * (m + 1) + k + 1] = hn;
+        for (int j = 0; j < k; ++j) {
+          const cplx t1 = Hc[k * (m + 1) + j], t2 = Hc[k * (m + 1) + j + 1];
+          Hc[k * (m + 1) + j] = std::conj(cs[c][j]) * t1 + std::conj(sn[c][j]) * t2;
+          Hc[k * (m + 1) + j + 1] = -sn[c][j] * t1 + cs[c][j] * t2;
+        }
+        const cplx x1 = Hc[k * (m + 1) + k], x2 = Hc[k * (m + 1) + k + 1];
+        const double den = std::sqrt(std::norm(x1) + std::norm(x2));
+        cs[c][k] = den > 0 ? x1 / den : 1.0;
+        sn[c][k] = den > 0 ? x2 / den : 0.0;
+        Hc[k * (m + 1) + k] = den;
+        Hc[k * (m + 1) + k + 1] = 0.0;
+        g[c][k + 1] = -sn[c][k] * g[c][k];
+        g[c][k] = std::conj(cs[c][k]) * g[c][k];
+        kc[c] = k + 1;
+        iters[c] += 1;
+        resid[c] = std::abs(g[c][k + 1]) / bn[c];
+        if (resid[c] <= tol || hn == 0.0) active[c] = 0;
+        any = any || active[c];
+      }
+      upload_coef(inv, zero);
+      vs.axpby(d_coef.ptr, W.ptr, d_coef.ptr + S * nrhs, V.ptr + vs.block() * (k + 1));
+      if (!any) {
+        ++k, ++total;
+        break;
+      }
+    }
+    // X += V y, y from the triangular systems
+    std::vector<double> ybuf(static_cast<size_t>(S) * m * nrhs, 0.0);
+    for (int c = 0; c < nrhs; ++c) {
+      const int kk = kc[c];
+      std::vector<cplx> y(kk);
+      for (int i = kk - 1; i >= 0; --i) {
+        cplx acc = g[c][i];
+        for (int j = i + 1; j < kk; ++j) acc -= H[c][j * (m + 1) + i] * y[j];
+        y[i] = acc / H[c][i * (m + 1) + i];
+      }
+      for (int j = 0; j < kk; ++j) {
+        ybuf[S * (j * nrhs + c)] = y[j].real();
+        if (C) ybuf[S * (j * nrhs + c) + 1] = y[j].imag();
+      }
+    }
+    const int nv = *std::max_element(kc.begin(), kc.end());
+    if (nv == 0) return false;
+    DevBuf<double> d_y;
+    d_y.upload(ybuf, st);
+    vs.combine(V.ptr, nv, d_y.ptr, X);
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+  }
+}
+
+// batched conjugate gradients (real symmetric operators)
+static bool cg(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int maxit, std::vector<int>& iters,
+               std::vector<double>& resid, hipStream_t st) {
+  const int64_t n = h2_rows(h);
+  Vecs vs{false, n, nrhs, st};
+  DevBuf<double> R, Pv, AP, d_small, d_coef;
+  R.alloc(vs.block());
+  Pv.alloc(vs.block());
+  AP.alloc(vs.block());
+  d_small.alloc(2 * nrhs);
+  d_coef.alloc(2 * nrhs);
+  const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
+  std::vector<double> buf;
+  vs.dots(B, 1, B, d_small.ptr, buf);
+  std::vector<double> bn(nrhs);
+  for (int c = 0; c < nrhs; ++c) bn[c] = std::sqrt(buf[c]) > 0 ? std::sqrt(buf[c]) : 1.0;
+  BMI_CHECK_HIP(hipMemcpyAsync(R.ptr, B, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
+  h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st);
+  BMI_CHECK_HIP(hipMemcpyAsync(Pv.ptr, R.ptr, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
+  vs.dots(R.ptr, 1, R.ptr, d_small.ptr, buf);
+  std::vector<double> rr(buf.begin(), buf.end());
+  iters.assign(nrhs, 0);
+  resid.assign(nrhs, 0.0);
+  std::vector<char> active(nrhs, 1);
+  auto upload = [&](const std::vector<double>& a, const std::vector<double>& b) {
+    std::vector<double> t(a);
+    t.insert(t.end(), b.begin(), b.end());
+    BMI_CHECK_HIP(hipMemcpyAsync(d_coef.ptr, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  };
+  for (int it = 0; it < maxit; ++it) {
+    bool any = false;
+    for (int c = 0; c < nrhs; ++c) {
+      resid[c] = std::sqrt(std::max(rr[c], 0.0)) / bn[c];
+      if (resid[c] <= tol) active[c] = 0;
+      any = any || active[c];
+    }
+    if (!any) return true;
+    h2_apply_device(h, nrhs, Pv.ptr, n, AP.ptr, n, one, nullptr, st);
+    vs.dots(Pv.ptr, 1, AP.ptr, d_small.ptr, buf);
+    std::vector<double> alpha(nrhs, 0.0), ones(nrhs, 1.0), malpha(nrhs, 0.0);
+    for (int c = 0; c < nrhs; ++c)
+      if (active[c] && buf[c] != 0.0) alpha[c] = rr[c] / buf[c], malpha[c] = -alpha[c], iters[c]++;
+    upload(alpha, ones);
+    vs.axpby(d_coef.ptr, Pv.ptr, d_coef.ptr + nrhs, X);
+    upload(malpha, ones);
+    vs.axpby(d_coef.ptr, AP.ptr, d_coef.ptr + nrhs, R.ptr);
+    vs.dots(R.ptr, 1, R.ptr, d_small.ptr, buf);
+    std::vector<double> beta(nrhs, 0.0);
+    for (int c = 0; c < nrhs; ++c) {
+      if (active[c] && rr[c] != 0.0) beta[c] = buf[c] / rr[c];
+      rr[c] = active[c] ? buf[c] : rr[c];
+    }
+    upload(ones, beta);
+    vs.axpby(d_coef.ptr, R.ptr, d_coef.ptr + nrhs, Pv.ptr);
+  }
+  bool all = true;
+  for (int c = 0; c < nrhs; ++c) {
+    resid[c] = std::sqrt(std::max(rr[c], 0.0)) / bn[c];
+    all = all && resid[c] <= tol;
+  }
+  return all;
+}
+
+// ------------------------------------------------------------------ right-hand sides
+// DiscreteLinearForm::compute with DirichletTrace (LinearForm/DiscreteLinearForm.hpp:44-83):
+// super-space load vector, one thread per (element, column)
+template <int PD, int P>
+__global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* __restrict__ elem_patch,
+                      const double* __restrict__ elem_llc, int nel, double h, int fn, const double* __restrict__ prm,
+                      int nrhs, double kappa, int64_t nsuper, double* out) {
+  constexpr int I = P + 1, I2 = I * I, n = P + 2;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(nel) * nrhs) return;
+  const int e = static_cast<int>(gid % nel), k = static_cast<int>(gid / nel);
+  const double* gx = c_gl_nodes + c_gl_offset[n - 1];
+  const double* gw = c_gl_weights + c_gl_offset[n - 1];
+  const double* c = cp + static_cast<int64_t>(elem_patch[e]) * cp_stride;
+  double ar[I2], ai[I2];
+#pragma unroll
+  for (int m = 0; m < I2; ++m) ar[m] = 0.0, ai[m] = 0.0;
+  for (int q = 0; q < n * n; ++q) {
+    const double u = gx[q / n], v = gx[q % n], w = h * (gw[q / n] * gw[q % n]);
+    double X[3], Ds[3], Dt[3];
+    surface_point<PD>(c, elem_llc[2 * e] + h * u, elem_llc[2 * e + 1] + h * v, X, Ds, Dt);
+    const double c0 = Ds[1] * Dt[2] - Ds[2] * Dt[1], c1 = Ds[2] * Dt[0] - Ds[0] * Dt[2], c2 = Ds[0] * Dt[1] - Ds[1] * Dt[0];
+    const double kap = sqrt(c0 * c0 + c1 * c1 + c2 * c2) * w;
+    double fr, fi = 0.0;
+    if (fn == 0) {
+      const double d0 = X[0] - prm[0], d1 = X[1] - prm[1], d2 = X[2] - prm[2];
+      fr = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    } else {
+      const double* d = prm + 3 * k;
+      sincos(kappa * (d[0] * X[0] + d[1] * X[1] + d[2] * X[2]), &fi, &fr);
+    }
+    double Bu[I], Bv[I];
+    bernstein_basis<P>(u, Bu);
+    bernstein_basis<P>(v, Bv);
+#pragma unroll
+    for (int iy = 0; iy < I; ++iy)
+#pragma unroll
+      for (int ix = 0; ix < I; ++ix) {
+        const double b = kap * Bu[ix] * Bv[iy];
+        ar[iy * I + ix] += fr * b;
+        ai[iy * I + ix] += fi * b;
+      }
+  }
+  for (int m = 0; m < I2; ++m) {
+    const int64_t o = static_cast<int64_t>(k) * nsuper + static_cast<int64_t>(e) * I2 + m;
+    if (fn == 0)
+      out[o] = ar[m];
+    else
+      out[2 * o] = ar[m], out[2 * o + 1] = ai[m];
+  }
+}
+
+}  // namespace bmi
+
+using namespace bmi;
+
+extern "C" {
+
+bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, int nrhs, double kappa, double* out,
+                            int mem, void* stream) {
+  try {
+    BMI_REQUIRE(s && params && out && nrhs >= 1, BMI_INVALID_ARG, "bad argument");
+    BMI_REQUIRE(fn == 0 || fn == 1, BMI_INVALID_ARG, "unknown data function");
+    BMI_REQUIRE(s->form == BMI_FORM_DISCONTINUOUS, BMI_UNSUPPORTED, "Dirichlet traces need a scalar space");
+    BMI_REQUIRE(fn == 1 || nrhs == 1, BMI_INVALID_ARG, "the point source has one column");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    ensure_device_space(s, st);
+    const int S = fn == 1 ? 2 : 1;
+    DevBuf<double> prm, sup, dout;
+    prm.upload(std::vector<double>(params, params + (fn == 0 ? 3 : 3 * nrhs)), st);
+    sup.alloc(static_cast<size_t>(S) * s->nsuper * nrhs);
+    const int64_t tot = static_cast<int64_t>(s->nel) * nrhs;
+    const unsigned blocks = static_cast<unsigned>((tot + 127) / 128);
+    const int stride = 4 * s->pd * s->pd;
+    const double h = 1.0 / s->n;
+#define BMI_RHS(PD, PP)                                                                                       \
+  if (s->pd == PD && s->P == PP) {                                                                            \
+    hipLaunchKernelGGL((k_rhs<PD, PP>), dim3(blocks), dim3(128), 0, st, s->d_cp.ptr, stride, s->d_elem_patch.ptr, \
+                       s->d_elem_llc.ptr, s->nel, h, fn, prm.ptr, nrhs, kappa, static_cast<int64_t>(s->nsuper),   \
+                       sup.ptr);                                                                              \
+  } else
+    BMI_RHS(5, 1) BMI_RHS(5, 2) BMI_RHS(2, 1) BMI_RHS(2, 2) { throw Error(BMI_UNSUPPORTED, "rhs kernel not instantiated"); }
+#undef BMI_RHS
+    BMI_CHECK_HIP(hipGetLastError());
+    double* y = out;
+    if (mem != BMI_MEM_DEVICE) {
+      dout.alloc(static_cast<size_t>(S) * s->ndof * nrhs);
+      y = dout.ptr;
+    }
+    launch_spmv(fn == 1, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, sup.ptr, s->nsuper, y, s->ndof,
+                nullptr, nullptr, st);
+    if (mem != BMI_MEM_DEVICE)
+      BMI_CHECK_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+    return BMI_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return BMI_DEVICE_ERROR;
+  }
+}
+
+static bmi_status solve_common(const bmi_h2* hc, int nrhs, const double* B, double* X, double tol, int restart, int maxit,
+                               int zero_guess, int* iters, double* resid, int mem, void* stream, bool use_cg) {
+  try {
+    BMI_REQUIRE(hc && B && X && nrhs >= 1 && tol > 0 && maxit >= 1, BMI_INVALID_ARG, "bad argument");
+    bmi_h2* h = const_cast<bmi_h2*>(hc);
+    if (use_cg) BMI_REQUIRE(h2_is_symmetric_real(h), BMI_INVALID_ARG, "CG needs the real symmetric Laplace operator");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int S = h2_complex(h) ? 2 : 1;
+    const int64_t n = h2_rows(h);
+    DevBuf<double> dB, dX;
+    const double* b = B;
+    double* x = X;
+    if (mem != BMI_MEM_DEVICE) {
+      dB.upload(std::vector<double>(B, B + S * n * nrhs), st);
+      dX.upload(std::vector<double>(X, X + S * n * nrhs), st);
+      b = dB.ptr;
+      x = dX.ptr;
+    }
+    if (zero_guess) device_fill(x, S * n * nrhs, 0.0, st);
+    std::vector<int> it;
+    std::vector<double> rs;
+    const bool ok = use_cg ? cg(h, nrhs, b, x, tol, maxit, it, rs, st)
+                           : gmres(h, nrhs, b, x, tol, restart, maxit, it, rs, st);
+    if (mem != BMI_MEM_DEVICE) BMI_CHECK_HIP(hipMemcpyAsync(X, dX.ptr, dX.bytes(), hipMemcpyDeviceToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+    if (iters) std::copy(it.begin(), it.end(), iters);
+    if (resid) std::copy(rs.begin(), rs.end(), resid);
+    return ok ? BMI_OK : BMI_NOT_CONVERGED;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return BMI_DEVICE_ERROR;
+  }
+}
+
+bmi_status bmi_solve_gmres(const bmi_h2* h, int nrhs, const double* B, double* X, double tol, int restart, int maxit,
+                           int zero_guess, int* iters, double* resid, int mem, void* stream) {
+  return solve_common(h, nrhs, B, X, tol, restart, maxit, zero_guess, iters, resid, mem, stream, false);
+}
+
+bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, double tol, int maxit, int zero_guess,
+                        int* iters, double* resid, int mem, void* stream) {
+  return solve_common(h, nrhs, B, X, tol, 0, maxit, zero_guess, iters, resid, mem, stream, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+/*
+ * bembel_amd.h -- C ABI of the MI355X implementation of Bembel's Galerkin assembly and H2
+ * matrix-vector product for the Laplace, Helmholtz and Maxwell single-layer operators.
+ *
+ * Every entry point takes plain pointers and sizes.  Complex data is interleaved (re, im)
+ * double pairs.  Matrices are column-major.  Vectors are in the B-spline DOF order of the
+ * reference (columns of the transformation matrix T), so they are interchangeable with
+ * Bembel's Eigen vectors.  Functions return a bmi_status; bmi_last_error() gives the message
+ * of the last failure on the calling thread.  No entry point aborts the process.
+ *
+ * Threading: calls on different handles may run concurrently; one handle is not reentrant.
+ * Streams: a NULL stream is the default stream of the current device.
+ */
+#ifndef BEMBEL_AMD_H_
+#define BEMBEL_AMD_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  BMI_OK = 0,
+  BMI_INVALID_ARG = 1,
+  BMI_OUT_OF_MEMORY = 2,
+  BMI_DEVICE_ERROR = 3,
+  BMI_NOT_CONVERGED = 4,
+  BMI_ADMISSION_REJECTED = 5,
+  BMI_IO_ERROR = 6,
+  BMI_UNSUPPORTED = 7
+} bmi_status;
+
+/* operator kinds: the single-layer operators of the reference
+   (Laplace/SingleLayerOperator.hpp, Helmholtz/SingleLayerOperator.hpp,
+   Maxwell/SingleLayerOperator.hpp) and the test operator of
+   LinearOperator/Dummy/DummyOperator.hpp */
+typedef enum {
+  BMI_LAPLACE_SL = 0,
+  BMI_HELMHOLTZ_SL = 1,
+  BMI_MAXWELL_SL = 2,
+  BMI_DUMMY = 3
+} bmi_operator_kind;
+
+/* storage / arithmetic of the far field (coupling and transfer) */
+typedef enum {
+  BMI_PRECISION_FP64 = 0,  /* everything FP64 */
+  BMI_PRECISION_MIXED = 1  /* coupling + transfer FP32 storage/arith; near field FP64 */
+} bmi_precision;
+
+/* spaces: DifferentialForm of the operator (LinearOperator/DifferentialFormEnum.hpp) */
+typedef enum { BMI_FORM_DISCONTINUOUS = 0, BMI_FORM_DIVCONFORMING = 1 } bmi_form;
+
+typedef enum { BMI_MEM_HOST = 0, BMI_MEM_DEVICE = 1 } bmi_memory;
+
+/* Operator descriptor.  Replaces the LinearOperatorTraits plug-in of the reference
+   (LinearOperator/LinearOperatorTraits.hpp) and H2Matrix::init_H2Matrix's number_of_points
+   (H2Matrix/H2Matrix.hpp:93-96) plus BlockClusterTree::set_parameters
+   (H2Matrix/BlockClusterTree.hpp:113-118).  Reference defaults: n_cheb 9, eta 1.6,
+   min_cluster_level 1. */
+typedef struct {
+  int kind;              /* bmi_operator_kind */
+  double kappa_re;       /* wavenumber (Helmholtz/Maxwell set_wavenumber) */
+  double kappa_im;
+  int n_cheb;            /* Chebyshev nodes per direction of a cluster */
+  double eta;            /* admissibility parameter */
+  int min_cluster_level; /* leaves hold 4^min_cluster_level elements */
+  int precision;         /* bmi_precision */
+  int dummy_fn;          /* test function of BMI_DUMMY (0 analytic, 1 near-singular, 2 one) */
+} bmi_op_desc;
+
+typedef struct bmi_geometry bmi_geometry;
+typedef struct bmi_space bmi_space;
+typedef struct bmi_h2 bmi_h2;
+
+/* ------------------------------------------------------------------ errors, device */
+const char* bmi_last_error(void);
+const char* bmi_version(void);
+/* reference default descriptor for an operator kind */
+bmi_status bmi_op_desc_default(int kind, bmi_op_desc* out);
+
+/* ------------------------------------------------------------------ geometry */
+/* Geometry(filename) for .dat files (Geometry/Geometry.hpp:37,89-102,
+   Geometry/GeometryIO.hpp:25-122).  scale (may be NULL) multiplies the homogeneous
+   x, y, z control coordinates, which maps the sphere file to an exact ellipsoid. */
+bmi_status bmi_geometry_load(const char* path, const double* scale, bmi_geometry** out);
+bmi_status bmi_geometry_destroy(bmi_geometry* g);
+bmi_status bmi_geometry_num_patches(const bmi_geometry* g, int* n);
+/* Patch::updateSurfacePoint (Geometry/Patch.hpp:294-365): out = xi(2) w x(3) dsx(3) dtx(3) */
+bmi_status bmi_geometry_surface_point(const bmi_geometry* g, int patch, double s, double t,
+                                      double out[12]);
+
+/* ------------------------------------------------------------------ discrete space */
+/* AnsatzSpace<Op>(geometry, refinement_level, polynomial_degree)
+   (AnsatzSpace/AnsatzSpace.hpp:80-111).  form selects Discontinuous (Laplace, Helmholtz)
+   or DivConforming (Maxwell). The geometry must outlive the space. */
+bmi_status bmi_space_create(const bmi_geometry* g, int refinement_level, int polynomial_degree,
+                            int form, bmi_space** out);
+bmi_status bmi_space_destroy(bmi_space* s);
+/* info: [0] elements [1] vertices [2] (P+1)^2 [3] vector dimension [4] super DOFs
+   [5] DOFs (AnsatzSpace::get_number_of_dofs) [6] nnz(T) [7] refinement level */
+bmi_status bmi_space_info(const bmi_space* s, int64_t info[8]);
+/* element data in Morton order (ClusterTree/ElementTree.hpp): vertex ids (4 per element,
+   numbered by first appearance), patch, lower-left corner, enclosing-ball midpoint/radius */
+bmi_status bmi_space_elements(const bmi_space* s, int* vertices, int* patch, double* llc,
+                              double* midpoint, double* radius);
+/* cluster balls of one level (6*4^level clusters for a 6-patch geometry) */
+bmi_status bmi_space_clusters(const bmi_space* s, int level, double* midpoint, double* radius,
+                              int* count);
+/* transformation matrix T (AnsatzSpace::get_transformation_matrix) in CSR */
+bmi_status bmi_space_transformation(const bmi_space* s, int* rowptr, int* col, double* val);
+
+/* ------------------------------------------------------------------ near field */
+/* compareElements + quadrature degree of element pairs (DuffyTrick/compareElements.hpp:23-69,
+   LinearOperator/LinearOperatorBase.hpp:56-89): out[4*k] = class, rot1, rot2, degree */
+bmi_status bmi_pair_info(const bmi_space* s, int npairs, const int* pairs, int* out);
+/* evaluateBilinearForm on the GPU for a list of element pairs (DuffyTrick/
+   evaluateBilinearForm.hpp:21-71); blocks are (vd*I2)^2, row-major, rows = functions of
+   the first element; complex kinds interleaved.  out is a host pointer. */
+bmi_status bmi_pair_blocks(const bmi_space* s, const bmi_op_desc* op, int npairs,
+                           const int* pairs, double* out, void* stream);
+/* integrate0..4 driven directly with a given rule and rotations (the reference Duffy
+   unit tests, tests/DuffyTrick/test_integrate*.hpp); job[5k..5k+4] = which, e1, rot1, e2,
+   rot2.  Only the (0,0) entry of each block is returned (P = 0 test spaces). */
+bmi_status bmi_integrate_jobs(const bmi_space* s, const bmi_op_desc* op, int njobs,
+                              const int* jobs, int qdeg, double* out, void* stream);
+
+/* ------------------------------------------------------------------ dense assembly */
+/* DiscreteOperator<Eigen::MatrixX, Op>::compute (LinearOperator/DiscreteOperator.hpp:27-87):
+   A = T^T A_super T (ndof x ndof), or the super matrix if super_only; column-major, out is
+   a host or device pointer (mem). */
+bmi_status bmi_dense_assemble(const bmi_space* s, const bmi_op_desc* op, int super_only,
+                              double* out, int mem, void* stream);
+
+/* ------------------------------------------------------------------ H2 */
+/* DiscreteOperator<Eigen::H2Matrix<S>, Op>::compute -> H2Matrix::init_H2Matrix
+   (LinearOperator/DiscreteOperator.hpp:92-102, H2Matrix/H2Matrix.hpp:93-250).
+   All leaves are assembled on the device and stay resident. The space must outlive it. */
+bmi_status bmi_h2_assemble(const bmi_space* s, const bmi_op_desc* op, void* stream,
+                           bmi_h2** out);
+bmi_status bmi_h2_destroy(bmi_h2* h);
+/* info: [0] leaves [1] dense leaves [2] low-rank leaves [3] near-field pairs
+   [4] dense leaf dimension [5] coupling dimension [6] FMM levels [7] rows=cols (DOFs)
+   [8] device bytes held [9] complex (0/1) [10] precision */
+bmi_status bmi_h2_info(const bmi_h2* h, int64_t info[16]);
+/* leaves (cc 1 low rank, 2 dense; level; row cluster id; column cluster id) in the
+   storage order of this implementation */
+bmi_status bmi_h2_leaves(const bmi_h2* h, int* leaves);
+/* copy one leaf matrix (component block r,c) to the host, column-major */
+bmi_status bmi_h2_leaf_matrix(const bmi_h2* h, int leaf, int comp_r, int comp_c, double* F);
+/* Y = alpha * H * X + beta * Y for nrhs columns (the product of
+   H2Matrix/EigenHelper/H2DenseProduct.hpp:33-165 and the Eigen scaleAndAddTo glue
+   :268-290).  X, Y column-major with leading dimensions ldx, ldy (in scalars); alpha, beta
+   point to (re, im) pairs; mem says whether X and Y are host or device pointers. */
+bmi_status bmi_h2_matvec(const bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* Y,
+                         int64_t ldy, const double* alpha, const double* beta, int mem,
+                         void* stream);
+/* same operator applied in the super space (no T, no T^T); device pointers only */
+bmi_status bmi_h2_matvec_super(const bmi_h2* h, int nrhs, const double* Xs, double* Ys,
+                               void* stream);
+
+/* ------------------------------------------------------------------ right-hand sides */
+/* DiscreteLinearForm<DirichletTrace>::compute (LinearForm/DiscreteLinearForm.hpp:44-83,
+   LinearForm/DirichletTrace.hpp:31-70) for built-in data functions:
+   fn 0: g(x) = 1/|x - p|            (p = params[0..2]; real)
+   fn 1: g(x) = exp(i kappa d_k . x) (d_k = params[3k..3k+2]; complex, one column per k)
+   out: ndof x nrhs, host or device (mem) */
+bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, int nrhs,
+                            double kappa, double* out, int mem, void* stream);
+
+/* ------------------------------------------------------------------ solvers */
+/* restarted GMRES (consumer: Eigen::GMRES in examples/HelmholtzSingleLayerH2.cpp:82-85)
+   with FP64 Arnoldi, one solve per column; X holds the initial guess (zeroed if
+   zero_guess).  iters/resid receive per-column values (may be NULL). */
+bmi_status bmi_solve_gmres(const bmi_h2* h, int nrhs, const double* B, double* X, double tol,
+                           int restart, int maxit, int zero_guess, int* iters, double* resid,
+                           int mem, void* stream);
+/* conjugate gradients for the real symmetric Laplace operator
+   (consumer: Eigen::ConjugateGradient in examples/LaplaceSingleLayerH2.cpp:78-81) */
+bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, double tol,
+                        int maxit, int zero_guess, int* iters, double* resid, int mem,
+                        void* stream);
+
+/* ------------------------------------------------------------------ measurement */
+/* per-kernel timing with HIP events recorded on the stream each kernel is launched on */
+bmi_status bmi_timing_enable(int enable);
+bmi_status bmi_timing_reset(void);
+/* total milliseconds and launch count of a kernel family ("nf_duffy", "nf_regular",
+   "coupling", "matvec_nearfield", ...) since the last reset */
+bmi_status bmi_timing_get(const char* name, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
