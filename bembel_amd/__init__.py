@@ -82,6 +82,7 @@ def library():
         "bmi_h2_info": (C.c_int, [vp, i64p]),
         "bmi_h2_leaves": (C.c_int, [vp, ip]),
         "bmi_h2_leaf_matrix": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
+        "bmi_h2_export_leaves": (C.c_int, [vp, dp]),
         "bmi_h2_matvec": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, C.c_int, vp]),
         "bmi_h2_matvec_super": (C.c_int, [vp, C.c_int, dp, dp, vp]),
         "bmi_rhs_assemble": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp, C.c_int, vp]),
@@ -411,6 +412,17 @@ class H2Matrix:
         if cplx:
             return out.view(np.complex128).reshape(d, d).T.copy()
         return out.reshape(d, d).T.copy()
+
+    def export_leaves(self):
+        """(keys, data): keys as in leaves(), data = all leaf matrices back to back (float64,
+        complex entries interleaved)."""
+        vd = self.space.vector_dimension
+        S = 2 if self.scalar == np.complex128 else 1
+        n = (self.info["dense"] * self.info["dense_dim"] ** 2 +
+             self.info["lowrank"] * self.info["coupling_dim"] ** 2) * vd * vd * S
+        data = np.empty(n)
+        _check(library().bmi_h2_export_leaves(self._h, _dp(data)))
+        return self.leaves(), data
 
     def matvec(self, x, alpha=1.0, stream=None):
         """alpha * H x for a vector or a (n x nrhs) matrix (numpy on host or torch on device)."""

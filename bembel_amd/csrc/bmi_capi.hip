@@ -742,6 +742,14 @@ bmi_status bmi_h2_leaf_matrix(const bmi_h2* h, int leaf, int r, int c, double* F
   BMI_API_END
 }
 
+bmi_status bmi_h2_export_leaves(const bmi_h2* h, double* data) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(h && data, BMI_INVALID_ARG, "null argument");
+  if (h->d_dense.n) BMI_CHECK_HIP(hipMemcpy(data, h->d_dense.ptr, h->d_dense.bytes(), hipMemcpyDeviceToHost));
+  if (h->d_coup.n) BMI_CHECK_HIP(hipMemcpy(data + h->d_dense.n, h->d_coup.ptr, h->d_coup.bytes(), hipMemcpyDeviceToHost));
+  BMI_API_END
+}
+
 bmi_status bmi_h2_matvec(const bmi_h2* hc, int nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy,
                          const double* alpha, const double* beta, int mem, void* stream) {
   BMI_API_BEGIN

@@ -1,0 +1,278 @@
+#!/usr/bin/env python3
+"""Benchmark of the H2 hot path on one MI355X.
+
+Workload: Laplace single-layer operator on the 6-patch unit sphere, refinement level 6
+(24,576 elements), B-spline degree 2 (26,136 DOFs), H2 with 4x4 Chebyshev nodes per cluster,
+eta = 1.6.  A "step" is one H2 matrix-vector product (1 RHS) with the vectors resident in HBM;
+`value` is DOF/s = DOFs / step time.  The line also carries the H2 assembly time, CG solve
+latency (p50/p99, FP64 and mixed precision, with the mixed solution's error against FP64), a
+roofline object for the dominant matvec kernel (near-field leaf sweep, HBM-bound) and for the
+near-field quadrature (FP64-bound), and the CPU baseline (the oracle's OpenMP matvec on the host
+cores, on the same leaves).
+
+usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a), q))
+
+
+def nearfield_flops(classes_deg, P, pd):
+    """algorithmic FP64 flops of the Duffy pairs: evaluations * F_e + surface points * F_g"""
+    Fe = 2 * (P + 1) ** 4 + 20
+    Fg = 24 * pd * pd + 40
+    tot = 0.0
+    for cls, n, cnt in classes_deg:
+        q = float(n) ** 4
+        if cls == 2:
+            e, g = 8 * q, 8 * q
+        elif cls == 3:
+            e, g = 6 * q, 12 * q
+        else:
+            e, g = 4 * q, 4 * n * n + 2 * q
+        tot += cnt * (e * Fe + g * Fg)
+    return tot
+
+
+def run_solves(B, torch, H, space, op, npts, b, nsolves):
+    """CG to 1e-10 in FP64 and with the mixed-precision far field; latencies and error"""
+    cg = B.ConjugateGradient()
+    cg.set_tolerance(1e-10)
+    cg.set_max_iterations(2000)
+    cg.compute(H)
+    lat64 = []
+    for _ in range(nsolves):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u64 = cg.solve(b)
+        torch.cuda.synchronize()
+        lat64.append(time.perf_counter() - t0)
+    it64 = int(cg.iterations()[0])
+    Hm = B.H2Matrix(space, op, n_cheb=npts, precision=B.MIXED)
+    cgm = B.ConjugateGradient()
+    cgm.set_tolerance(1e-10)
+    cgm.set_max_iterations(2000)
+    cgm.compute(Hm)
+    latm = []
+    for _ in range(nsolves):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        um = cgm.solve(b)
+        torch.cuda.synchronize()
+        latm.append(time.perf_counter() - t0)
+    rel_err_mixed = float(torch.linalg.norm(um - u64) / torch.linalg.norm(u64))
+    log("CG FP64: %d it, p50 %.1f ms; mixed: %d it, p50 %.1f ms, rel err %.2e" % (
+        it64, 1e3 * pct(lat64, 50), int(cgm.iterations()[0]), 1e3 * pct(latm, 50), rel_err_mixed))
+
+    return lat64, latm, it64, rel_err_mixed
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--solves", type=int, default=5)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--level", type=int, default=6)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run one plain device copy of this many MB first (counter calibration)")
+    ap.add_argument("--no-breakdown", action="store_true", help="skip the per-kernel assembly breakdown")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs the H2 path on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import bembel_amd as B
+    torch.cuda.set_device(0)
+    torch.cuda.init()
+    M, P, npts = a.level, 2, 4
+    geo = B.Geometry()
+    op = B.LaplaceSingleLayerOperator()
+    t0 = time.perf_counter()
+    space = B.AnsatzSpace(geo, M, P, op)
+    t_space = time.perf_counter() - t0
+    n = space.number_of_dofs
+    log("space: %d elements, %d dofs (%.2f s)" % (space.number_of_elements, n, t_space))
+
+    if a.copy_mb:
+        src = torch.ones(a.copy_mb * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    H = B.H2Matrix(space, op, n_cheb=npts)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    info = H.info
+    log("H2 assembly %.3f s: %d dense leaves, %d low-rank leaves, %d near-field pairs, classes %s, %.2f GB" % (
+        t_asm, info["dense"], info["lowrank"], info["pairs"], info["pair_classes"], info["device_bytes"] / 1e9))
+
+    # per-kernel breakdown of the assembly (second assembly, events around every launch)
+    duffy_ms = reg_ms = cpl_ms = 0.0
+    duffy_n = reg_n = cpl_n = 0
+    if not a.no_breakdown:
+        B.timing.reset()
+        B.timing.enable(True)
+        H2b = B.H2Matrix(space, op, n_cheb=npts)
+        B.timing.enable(False)
+        duffy_ms, duffy_n = B.timing.get("nf_duffy")
+        reg_ms, reg_n = B.timing.get("nf_regular")
+        cpl_ms, cpl_n = B.timing.get("coupling_assemble")
+        del H2b
+    log("assembly kernels: nf_duffy %.1f ms (%d launches), nf_regular %.1f ms (%d), coupling %.1f ms" % (
+        duffy_ms, duffy_n, reg_ms, reg_n, cpl_ms))
+
+    # ---------------------------------------------------------------- matvec (the step)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1234)
+    x = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) - 0.5
+    for _ in range(a.warmup):
+        y = H.matvec(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        y = H.matvec(x)
+    torch.cuda.synchronize()
+    t_mv = (time.perf_counter() - t0) / a.steps
+    value = n / t_mv
+    log("matvec %.3f ms -> %.3e DOF/s" % (t_mv * 1e3, value))
+
+    # kernel timing pass for the roofline (HIP events on the launch stream, inside the library)
+    B.timing.reset()
+    B.timing.enable(True)
+    for _ in range(max(3, a.steps // 4)):
+        H.matvec(x)
+    B.timing.enable(False)
+    nf_ms, nf_n = B.timing.get("matvec_nearfield")
+    cp_ms, cp_n = B.timing.get("coupling_apply")
+    nf_avg = nf_ms / max(nf_n, 1)
+    dd = info["dense_dim"]
+    nsuper = space.number_of_super_dofs
+    rows = info["coupling_dim"]
+    ncl_leaf = space.number_of_elements // 4
+    nf_bytes = info["dense"] * dd * dd * 8 + 2 * nsuper * 8 + ncl_leaf * rows * 8 + rows * dd * 8
+    achieved = nf_bytes / (nf_avg * 1e-3) / 1e9
+    log("near-field sweep %.3f ms/launch, %.1f MB -> %.0f GB/s; coupling %.3f ms/launch" % (
+        nf_avg, nf_bytes / 1e6, achieved, cp_ms / max(cp_n, 1)))
+
+    # near-field quadrature roofline (FP64)
+    pairs_by_class = info["pair_classes"]
+    deg_touch = int(0.5 * ((4 + 2 * P) * M * math.log(2.0)) / (2 * math.log(2.0)))
+    ntouch = deg_touch + 1
+    fl = nearfield_flops([(2, ntouch, pairs_by_class[2]), (3, ntouch, pairs_by_class[3]),
+                          (4, ntouch, pairs_by_class[4])], P, 5)
+    duffy_tflops = fl / (duffy_ms * 1e-3) / 1e12 if duffy_ms > 0 else 0.0
+
+    # ---------------------------------------------------------------- solves
+    rng = np.random.default_rng(2024)
+    d = rng.normal(size=3)
+    x0 = 0.3 * rng.uniform() ** (1 / 3) * d / np.linalg.norm(d)
+    b = B.DiscreteLinearForm(B.DirichletTrace(x0), space).compute(
+        device=True).get_discrete_linear_form()
+    lat64, latm, it64, rel_err_mixed = [float("nan")], [float("nan")], 0, float("nan")
+    if a.solves > 0:
+        lat64, latm, it64, rel_err_mixed = run_solves(B, torch, H, space, op, npts, b, a.solves)
+
+    # ---------------------------------------------------------------- CPU baseline
+    cpu = None
+    if not a.no_cpu_baseline:
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib as O
+            threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+            og = O.Geometry()
+            osp = O.Space(og, M, P)
+            oh = O.H2(osp, O.LAPLACE, 0j, npts, assemble=False)
+            keys, data = H.export_leaves()
+            matched = oh.import_leaves(keys, data)
+            del data
+            assert matched == len(keys), (matched, len(keys))
+            t_cpu = oh.time_matvec(1, 3, threads)
+            cpu = {"value": n / t_cpu, "unit": "DOF/s", "cores": threads, "kind": "port",
+                   "sample": "oracle OpenMP H2 matvec of the same 24,576-element operator (same leaves), "
+                             "3 matvecs, %.3f s each" % t_cpu}
+            # CPU assembly estimate from a random sample of near-field pairs and coupling blocks
+            leaves = keys
+            dense = leaves[leaves[:, 0] == 2]
+            lr_idx = np.nonzero(leaves[:, 0] == 1)[0]
+            ns = 256
+            pick = rng.choice(len(dense), ns, replace=False)
+            pairs = np.array([(4 * c1 + i, 4 * c2 + j) for (_, _, c1, c2) in dense[pick] for i in range(4)
+                              for j in range(4)], dtype=np.int32)
+            ol = oh.leaves()
+            okey = {tuple(r): i for i, r in enumerate(ol.tolist())}
+            blocks = np.array([okey[tuple(leaves[k])] for k in rng.choice(lr_idx, 2000, replace=False)],
+                              dtype=np.int32)
+            t_pairs = oh.time_assembly_sample(pairs, [], threads)
+            t_blocks = oh.time_assembly_sample(np.zeros((0, 2)), blocks, threads)
+            cpu_asm = t_pairs * len(dense) / ns + t_blocks * len(lr_idx) / len(blocks)
+            cpu["assembly_s_estimate"] = cpu_asm
+            cpu["assembly_sample"] = "%d near-field element pairs (%.1f s) + %d coupling blocks (%.2f s), scaled" % (
+                len(pairs), t_pairs, len(blocks), t_blocks)
+            log("CPU baseline: matvec %.3f s (%d threads), assembly estimate %.1f s" % (t_cpu, threads, cpu_asm))
+        except Exception as e:  # the CPU leg must not hide the GPU result
+            log("CPU baseline failed:", repr(e))
+            cpu = None
+
+    out = {
+        "metric": "H2 assembly s + matvec DOF/s at 1e-6 rel err vs FP64; p50/p99 solve latency",
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_mv * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded point-source Dirichlet data, random input vectors), unit-sphere geometry file",
+        "config": {"workload": "laplace_single_layer_h2_sphere_M%d_P%d_np%d" % (M, P, npts),
+                   "elements": space.number_of_elements, "dofs": n, "polynomial_degree": P, "n_cheb": npts,
+                   "eta": 1.6, "min_cluster_level": 1, "nrhs": 1},
+        "h2_assembly_s": t_asm,
+        "space_setup_s": t_space,
+        "assembly_kernels_ms": {"nf_duffy": duffy_ms, "nf_regular": reg_ms, "coupling_assemble": cpl_ms},
+        "h2": {k: info[k] for k in ("dense", "lowrank", "pairs", "pair_classes", "device_bytes")},
+        "solve": {"solver": "CG", "tol": 1e-10, "fp64_iterations": it64, "fp64_p50_s": pct(lat64, 50),
+                  "fp64_p99_s": pct(lat64, 99), "mixed_p50_s": pct(latm, 50), "mixed_p99_s": pct(latm, 99),
+                  "mixed_rel_err_vs_fp64": rel_err_mixed, "solves": a.solves},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_leaf (near-field leaf sweep of the matvec)", "bytes_per_launch": nf_bytes,
+                     "avg_launch_ms": nf_avg},
+        "roofline_assembly": {"bound": "fp64", "achieved": duffy_tflops, "peak": FP64_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": duffy_tflops / FP64_PEAK_TFLOPS,
+                              "kernel": "k_nearfield identical/edge/vertex (Duffy)", "flops": fl},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -147,6 +147,10 @@ bmi_status bmi_h2_info(const bmi_h2* h, int64_t info[16]);
 bmi_status bmi_h2_leaves(const bmi_h2* h, int* leaves);
 /* copy one leaf matrix (component block r,c) to the host, column-major */
 bmi_status bmi_h2_leaf_matrix(const bmi_h2* h, int leaf, int comp_r, int comp_c, double* F);
+/* all leaf matrices to the host in the order of bmi_h2_leaves: per leaf vd*vd component
+   blocks (row component major), each column-major; dense leaves first.  data holds
+   (dense * dd^2 + lowrank * cd^2) * vd^2 scalars (doubles, or interleaved complex). */
+bmi_status bmi_h2_export_leaves(const bmi_h2* h, double* data);
 /* Y = alpha * H * X + beta * Y for nrhs columns (the product of
    H2Matrix/EigenHelper/H2DenseProduct.hpp:33-165 and the Eigen scaleAndAddTo glue
    :268-290).  X, Y column-major with leading dimensions ldx, ldy (in scalars); alpha, beta

@@ -777,3 +777,52 @@ void or_forward_transform(int n2, int nc, const double* mom, int mom_cols, const
     ncl = nn;
   }
 }
+
+/* bulk import of leaf matrices in the order of `keys` (cc, level, id1, id2 per leaf); the
+   data holds vd*vd component blocks per leaf, column-major, back to back.  Used by the CPU
+   baseline to time the matvec on leaves assembled elsewhere. Returns leaves matched. */
+int or_h2_import_leaves(or_h2* h, int n, const int* keys, const double* data) {
+  int matched = 0;
+  int m = h->cplx ? 2 : 1;
+  /* index of oracle leaves by (level, id1, id2, cc) through a sort-free hash */
+  long cap = 1;
+  while (cap < 4L * h->nleaves) cap <<= 1;
+  long* tab = (long*)malloc(sizeof(long) * cap);
+  for (long i = 0; i < cap; ++i) tab[i] = -1;
+  for (int i = 0; i < h->nleaves; ++i) {
+    const or_leaf* l = &h->leaves[i];
+    unsigned long k = ((unsigned long)l->level * 1000003UL + (unsigned long)l->id1) * 1000033UL + (unsigned long)l->id2;
+    long s = (long)(k & (cap - 1));
+    while (tab[s] >= 0) s = (s + 1) & (cap - 1);
+    tab[s] = i;
+  }
+  size_t off = 0;
+  for (int q = 0; q < n; ++q) {
+    int cc = keys[4 * q], lv = keys[4 * q + 1], a = keys[4 * q + 2], b = keys[4 * q + 3];
+    unsigned long k = ((unsigned long)lv * 1000003UL + (unsigned long)a) * 1000033UL + (unsigned long)b;
+    long s = (long)(k & (cap - 1));
+    int found = -1;
+    while (tab[s] >= 0) {
+      const or_leaf* l = &h->leaves[tab[s]];
+      if (l->level == lv && l->id1 == a && l->id2 == b && l->cc == cc) {
+        found = (int)tab[s];
+        break;
+      }
+      s = (s + 1) & (cap - 1);
+    }
+    int d = cc == 2 ? cluster_nel(h->s, lv) * h->s->I2 : h->nc * h->np2;
+    size_t blk = (size_t)m * d * d;
+    if (found >= 0) {
+      for (int r = 0; r < h->vd; ++r)
+        for (int c = 0; c < h->vd; ++c) {
+          double** dst = &h->F[found * h->vd * h->vd + r * h->vd + c];
+          if (!*dst) *dst = (double*)malloc(sizeof(double) * blk);
+          memcpy(*dst, data + off + (size_t)(r * h->vd + c) * blk, sizeof(double) * blk);
+        }
+      ++matched;
+    }
+    off += (size_t)h->vd * h->vd * blk;
+  }
+  free(tab);
+  return matched;
+}

@@ -60,6 +60,8 @@ def lib():
         L.or_h2_leaves.argtypes = [vp, ip]
         L.or_h2_leaf_matrix.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.or_h2_set_leaf_matrix.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+        L.or_h2_import_leaves.restype = C.c_int
+        L.or_h2_import_leaves.argtypes = [vp, C.c_int, ip, dp]
         L.or_h2_matvec.argtypes = [vp, C.c_int, dp, dp]
         L.or_h2_matvec_super.argtypes = [vp, C.c_int, dp, dp]
         L.or_rhs.argtypes = [vp, C.c_int, dp, C.c_double, dp]
@@ -293,6 +295,19 @@ class H2:
         else:
             a = np.ascontiguousarray(F.T.reshape(-1))
         lib().or_h2_set_leaf_matrix(self.h, i, r, c, _dp(a))
+
+    def import_leaves(self, keys, data):
+        k = np.ascontiguousarray(keys, dtype=np.int32)
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        return lib().or_h2_import_leaves(self.h, len(k), _ip(k), _dp(d))
+
+    def time_matvec(self, nrhs=1, reps=3, threads=0):
+        return lib().or_time_matvec(self.h, nrhs, reps, threads)
+
+    def time_assembly_sample(self, pairs, blocks, threads=0):
+        p = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        b = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1)
+        return lib().or_time_assembly_sample(self.h, len(p) // 2, _ip(p), len(b), _ip(b), threads)
 
     def matvec(self, x):
         x = np.asarray(x)
