@@ -53,6 +53,19 @@ def build_product(jobs=8):
     return LIB
 
 
+def build_examples():
+    """C++ program written against the reference-style facade (include/bembel_amd/Bembel.hpp)"""
+    os.makedirs(BUILD, exist_ok=True)
+    out = os.path.join(BUILD, "facade_example")
+    cmd = ["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "facade_example.cpp"), "-L" + HERE, "-lbembel_amd",
+           "-Wl,-rpath,$ORIGIN/..", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("facade example build failed:\n" + r.stderr[-4000:])
+    return out
+
+
 def build_oracle():
     odir = os.path.join(ROOT, "oracle")
     r = subprocess.run(["make", "-C", odir, "-j4"], capture_output=True, text=True)
@@ -67,6 +80,7 @@ def main(argv=None):
     ap.add_argument("--no-oracle", action="store_true")
     a = ap.parse_args(argv)
     print("built", build_product(a.jobs))
+    print("built", build_examples())
     if not a.no_oracle:
         print("built", build_oracle())
 
