@@ -20,7 +20,7 @@ import numpy as np
 __all__ = [
     "BembelError", "Geometry", "AnsatzSpace", "LaplaceSingleLayerOperator", "HelmholtzSingleLayerOperator",
     "MaxwellSingleLayerOperator", "DummyOperator", "DiscreteOperator", "H2Matrix", "DenseMatrix",
-    "DiscreteLinearForm", "DirichletTrace", "PlaneWaves", "GMRES", "ConjugateGradient", "library", "timing",
+    "DiscreteLinearForm", "DirichletTrace", "PlaneWaves", "RotatedTangentialTrace", "GMRES", "ConjugateGradient", "library", "timing",
     "SPHERE_DAT", "FP64", "MIXED",
 ]
 
@@ -85,6 +85,7 @@ def library():
         "bmi_h2_export_leaves": (C.c_int, [vp, dp]),
         "bmi_h2_matvec": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, C.c_int, vp]),
         "bmi_h2_matvec_super": (C.c_int, [vp, C.c_int, dp, dp, vp]),
+        "bmi_h2_use_graphs": (C.c_int, [vp, C.c_int]),
         "bmi_rhs_assemble": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp, C.c_int, vp]),
         "bmi_solve_gmres": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, ip, dp,
                                       C.c_int, vp]),
@@ -443,6 +444,10 @@ class H2Matrix:
     def __matmul__(self, x):
         return self.matvec(x)
 
+    def use_graphs(self, enable=True):
+        """Replay the product from captured HIP graphs (default) or launch kernel by kernel."""
+        _check(library().bmi_h2_use_graphs(self._h, 1 if enable else 0))
+
     def get_dense(self):
         """Dense matrix by applying H to the identity (H2Matrix::get_dense, H2Matrix.hpp:254-264)."""
         n = self.rows()
@@ -514,8 +519,25 @@ class PlaneWaves:
         self.kappa = float(kappa)
 
 
+class RotatedTangentialTrace:
+    """Rotated tangential trace of Hertz dipole fields for the EFIE right-hand side
+    (LinearForm/RotatedTangentialTrace.hpp with the examples/Data.hpp:76-90 Dipole); one column per
+    (position, moment) pair; DivConforming spaces."""
+
+    def __init__(self, positions, moments, kappa):
+        p = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+        m = np.asarray(moments, dtype=np.float64).reshape(-1, 3)
+        if p.shape != m.shape:
+            raise BembelError(INVALID_ARG, "one moment per dipole position")
+        self.fn = 2
+        self.params = np.ascontiguousarray(np.concatenate([p, m], axis=1))
+        self.nrhs = p.shape[0]
+        self.kappa = float(kappa)
+
+
 class DiscreteLinearForm:
-    """DiscreteLinearForm<DirichletTrace>(ansatz_space) (LinearForm/DiscreteLinearForm.hpp:44-83)."""
+    """DiscreteLinearForm<DirichletTrace | RotatedTangentialTrace>(ansatz_space)
+    (LinearForm/DiscreteLinearForm.hpp:44-83)."""
 
     def __init__(self, data, space):
         self.data = data
@@ -524,7 +546,7 @@ class DiscreteLinearForm:
 
     def compute(self, stream=None, device=False):
         n = self.space.number_of_dofs
-        cplx = self.data.fn == 1
+        cplx = self.data.fn != 0
         S = 2 if cplx else 1
         nrhs = self.data.nrhs
         if device:

@@ -109,6 +109,19 @@ struct bmi_h2 {
   // workspace (grown on demand)
   int ws_rhs = 0;
   DevBuf<double> d_xs, d_ys, d_fw[2], d_bw, d_xin, d_yout;
+  // super-space apply captured once per RHS count and replayed (workspace pointers are fixed
+  // until the workspace grows, which drops the graphs)
+  bool use_graphs = true;
+  hipStream_t cap_stream = nullptr;
+  std::map<int, hipGraphExec_t> graphs;
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
+  ~bmi_h2() {
+    drop_graphs();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+  }
   size_t bytes() const {
     return d_dense.bytes() + d_coup.bytes() + d_coup32.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() +
            d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_xs.bytes() +
@@ -123,6 +136,7 @@ static int ncl_at(const bmi_space* s, int l) { return s->npatch << (2 * l); }
 static void ensure_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
   (void)st;
   if (nrhs <= h->ws_rhs) return;
+  h->drop_graphs();
   const int S = h->cplx ? 2 : 1;
   const int64_t N = h->s->nsuper;
   h->d_xs.alloc(S * N * nrhs);
@@ -176,6 +190,36 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, hipSt
   }
 }
 
+// ys = A_super xs on the workspace, through a captured graph when possible.  The graph holds
+// only kernel nodes (buffers are cleared by a fill kernel, not a memset node); in-library
+// timing needs host synchronisation per kernel and therefore runs eagerly.
+static void apply_super_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
+  if (!h->use_graphs || timing_enabled()) {
+    apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, st);
+    return;
+  }
+  auto it = h->graphs.find(nrhs);
+  if (it == h->graphs.end()) {
+    if (!h->cap_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    BMI_CHECK_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+    try {
+      apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->cap_stream);
+    } catch (...) {
+      (void)hipStreamEndCapture(h->cap_stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    BMI_CHECK_HIP(hipStreamEndCapture(h->cap_stream, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    BMI_CHECK_HIP(e);
+    it = h->graphs.emplace(nrhs, ex).first;
+  }
+  BMI_CHECK_HIP(hipGraphLaunch(it->second, st));
+}
+
 void h2_apply_device(bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy, const double* alpha,
                      const double* beta, hipStream_t st) {
   const bmi_space* s = h->s;
@@ -183,7 +227,7 @@ void h2_apply_device(bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* 
   const int64_t N = s->nsuper;
   launch_spmv(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, h->d_xs.ptr, N, nullptr,
               nullptr, st);
-  apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, st);
+  apply_super_workspace(h, nrhs, st);
   launch_spmv(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, h->d_ys.ptr, N, Y, ldy, alpha,
               beta, st);
 }
@@ -207,7 +251,6 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   BMI_REQUIRE(h->L >= 0, BMI_UNSUPPORTED, "refinement level below min_cluster_level");
   h->dd = (1 << (2 * h->mcl)) * s->I2;
   h->cd = h->nc * h->np2;
-  BMI_REQUIRE(op.kind != BMI_MAXWELL_SL, BMI_UNSUPPORTED, "Maxwell near field not available yet");
   build_block_tree(s, op.eta, h->mcl, h->dense, h->lowrank);
   const int S = h->cplx ? 2 : 1;
   const int vd = h->vd;
@@ -242,7 +285,8 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
       }
   }
   h->npairs = static_cast<int64_t>(pairs.size());
-  launch_nearfield(s, op, pairs, h->d_dense.ptr, 1, h->dd, 0, 0, st);
+  // component block (rc, cc) of leaf k starts at (k*vd*vd + rc*vd + cc) * dblk
+  launch_nearfield(s, op, pairs, h->d_dense.ptr, 1, h->dd, vd * dblk, dblk, st);
   // dense CSR over leaf clusters (leaves sorted by (c1, c2))
   h->ncl_leaf = ncl_at(s, h->L);
   std::vector<int> drp(h->ncl_leaf + 1, 0), dsrc(h->dense.size());
@@ -607,9 +651,8 @@ bmi_status bmi_dense_assemble(const bmi_space* s, const bmi_op_desc* op, int sup
   BMI_API_BEGIN
   check_op(s, op);
   BMI_REQUIRE(out, BMI_INVALID_ARG, "null output");
-  BMI_REQUIRE(op->kind != BMI_MAXWELL_SL, BMI_UNSUPPORTED, "Maxwell near field not available yet");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const bool cplx = op->kind == BMI_HELMHOLTZ_SL;
+  const bool cplx = op->kind == BMI_HELMHOLTZ_SL || op->kind == BMI_MAXWELL_SL;
   const int S = cplx ? 2 : 1;
   const int64_t N = s->nsuper, D = s->ndof;
   const int ff = farfield_degree(op->kind, s->P);
@@ -635,7 +678,9 @@ bmi_status bmi_dense_assemble(const bmi_space* s, const bmi_op_desc* op, int sup
     }
   DevBuf<double> A;
   A.alloc(static_cast<size_t>(S) * N * N);
-  launch_nearfield(s, *op, v, A.ptr, 1, N, 0, 0, st);
+  // super-space rows and columns are component-major: component c starts at c * nel * I2
+  const int64_t vcs = static_cast<int64_t>(s->nel) * s->I2;
+  launch_nearfield(s, *op, v, A.ptr, 1, N, vcs, vcs * N, st);
   if (super_only) {
     BMI_CHECK_HIP(hipMemcpyAsync(out, A.ptr, A.bytes(), mem == BMI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     BMI_CHECK_HIP(hipStreamSynchronize(st));
@@ -762,7 +807,10 @@ bmi_status bmi_h2_matvec(const bmi_h2* hc, int nrhs, const double* X, int64_t ld
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int S = h->cplx ? 2 : 1;
   if (mem == BMI_MEM_DEVICE) {
+    // device operands: enqueued on the stream, returns without waiting
     h2_apply_device(h, nrhs, X, ldx, Y, ldy, alpha, beta, st);
+    BMI_CHECK_HIP(hipGetLastError());
+    return BMI_OK;
   } else {
     DevBuf<double> dx, dy;
     dx.alloc(static_cast<size_t>(S) * D * nrhs);
@@ -787,6 +835,14 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* hc, int nrhs, const double* Xs, dou
   ensure_workspace(h, nrhs, st);
   apply_super(h, nrhs, Xs, Ys, st);
   BMI_CHECK_HIP(hipStreamSynchronize(st));
+  BMI_API_END
+}
+
+bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(h, BMI_INVALID_ARG, "null handle");
+  h->use_graphs = enable != 0;
+  if (!h->use_graphs) h->drop_graphs();
   BMI_API_END
 }
 

@@ -13,6 +13,8 @@
 //   multipole levels fw/bw: [cluster][rhs][row], row = comp*np2 + node.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bmi_device.h"
 #include "bmi_fmm.h"
 #include "bmi_internal.h"
@@ -347,6 +349,162 @@ __global__ void k_leaf(int ncl, int nrhs, int dd, const int* __restrict__ rp, co
   acc.store(yk, o);
 }
 
+// ------------------------------------------------------------------ wave-per-cluster sweeps
+// The two HBM-bound sweeps of the matvec stream every stored leaf exactly once.  One 64-lane
+// wave owns one row cluster (near field) or one target cluster (coupling): lane i owns row i,
+// issues all DD (CD) column loads of a block before its first multiply-add, so each wave keeps a
+// whole block in flight, and reads the source segment of the input, which is wave-uniform, via
+// scalar loads.  RT right-hand sides share each block load.
+
+// near field: ys[seg + c*DD + i, k] = sum_blocks F (DD x DD) xs[c2*DD + .., k] + Mom^T bw_L[c][k]
+template <bool C, int DD, int RT>
+__global__ __launch_bounds__(256) void k_leaf_w(int ncl, int nrhs, const int* __restrict__ rp,
+                                                const int* __restrict__ src, const double* __restrict__ F,
+                                                int64_t fstride, int64_t fcomp, const double* __restrict__ xs,
+                                                int64_t ldxs, int64_t xseg, const double* __restrict__ mom, int rows,
+                                                const double* __restrict__ bw, double* __restrict__ ys, int64_t ldys,
+                                                int64_t yseg, int accumulate) {
+  constexpr int S = C ? 2 : 1;
+  const int c = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6));
+  if (c >= ncl) return;
+  const int lane = threadIdx.x & 63;
+  const int i = lane < DD ? lane : 0;  // spare lanes shadow row 0 and store nothing
+  const int p0 = rp[c], p1 = rp[c + 1];
+  for (int k0 = 0; k0 < nrhs; k0 += RT) {
+    double ar[RT], ai[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) ar[r] = 0.0, ai[r] = 0.0;
+    for (int p = p0; p < p1; ++p) {
+      const int s2 = src[p];
+      const double* f = F + S * (static_cast<int64_t>(p) * fstride + fcomp);
+      double fr[DD], fi[C ? DD : 1];
+#pragma unroll
+      for (int j = 0; j < DD; ++j) {
+        if constexpr (C) {
+          const double2 v = *reinterpret_cast<const double2*>(f + 2 * (j * DD + i));
+          fr[j] = v.x, fi[j] = v.y;
+        } else {
+          fr[j] = f[j * DD + i];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        const int k = k0 + r;
+        if (k >= nrhs) break;
+        const double* x = xs + S * (static_cast<int64_t>(k) * ldxs + xseg + static_cast<int64_t>(s2) * DD);
+#pragma unroll
+        for (int j = 0; j < DD; ++j) {
+          if constexpr (C) {
+            const double xr = x[2 * j], xi = x[2 * j + 1];
+            ar[r] += fr[j] * xr - fi[j] * xi;
+            ai[r] += fr[j] * xi + fi[j] * xr;
+          } else {
+            ar[r] += fr[j] * x[j];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+      const int k = k0 + r;
+      if (k >= nrhs) break;
+      if (bw) {
+        const double* b = bw + S * ((static_cast<int64_t>(c) * nrhs + k) * rows);
+        const double* m = mom + static_cast<int64_t>(i) * rows;
+        for (int q = 0; q < rows; ++q) {
+          ar[r] += m[q] * b[S * q];
+          if constexpr (C) ai[r] += m[q] * b[S * q + 1];
+        }
+      }
+      if (lane < DD) {
+        double* y = ys + S * (static_cast<int64_t>(k) * ldys + yseg + static_cast<int64_t>(c) * DD + lane);
+        if (accumulate) {
+          ar[r] += y[0];
+          if constexpr (C) ai[r] += y[1];
+        }
+        y[0] = ar[r];
+        if constexpr (C) y[1] = ai[r];
+      }
+    }
+  }
+}
+
+// coupling: bw[t][k][i] (+)= sum over blocks (t, s) of F (CD x CD) fw[s][k][:], all levels in one
+// launch.  A wave owns one target; its 64 lanes form G = 64/CDP groups that take every G-th block
+// and are summed with cross-lane shuffles at the end.
+template <bool C, typename FT, int CD, int RT>
+__global__ __launch_bounds__(256) void k_coupling_w(int ntgt, int nrhs, const int2* __restrict__ tgt,
+                                                    const int* __restrict__ rp, const int* __restrict__ src,
+                                                    const FT* __restrict__ F, int64_t fstride, int64_t fcomp,
+                                                    const int64_t* __restrict__ lvl_off,
+                                                    const double* __restrict__ fw, double* __restrict__ bw,
+                                                    int accumulate) {
+  constexpr int S = C ? 2 : 1;
+  constexpr int CDP = CD <= 16 ? 16 : (CD <= 32 ? 32 : 64);
+  constexpr int G = 64 / CDP;
+  const int t = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6));
+  if (t >= ntgt) return;
+  const int lane = threadIdx.x & 63;
+  const int g = lane / CDP, li = lane % CDP;
+  const int i = li < CD ? li : 0;
+  const int2 T = tgt[t];
+  const int64_t off = lvl_off[T.x] * nrhs;
+  const int p0 = rp[t], p1 = rp[t + 1];
+  for (int k0 = 0; k0 < nrhs; k0 += RT) {
+    double ar[RT], ai[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) ar[r] = 0.0, ai[r] = 0.0;
+    for (int p = p0 + g; p < p1; p += G) {
+      const int s2 = src[p];
+      const FT* f = F + S * (static_cast<int64_t>(p) * fstride + fcomp);
+      double fr[CD], fi[C ? CD : 1];
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        fr[j] = static_cast<double>(f[S * (j * CD + i)]);
+        if constexpr (C) fi[j] = static_cast<double>(f[S * (j * CD + i) + 1]);
+      }
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        const int k = k0 + r;
+        if (k >= nrhs) break;
+        const double* x = fw + S * (off + (static_cast<int64_t>(s2) * nrhs + k) * CD);
+#pragma unroll
+        for (int j = 0; j < CD; ++j) {
+          if constexpr (C) {
+            const double xr = x[2 * j], xi = x[2 * j + 1];
+            ar[r] += fr[j] * xr - fi[j] * xi;
+            ai[r] += fr[j] * xi + fi[j] * xr;
+          } else {
+            ar[r] += fr[j] * x[j];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+#pragma unroll
+      for (int m = CDP; m < 64; m <<= 1) {
+        ar[r] += __shfl_xor(ar[r], m, 64);
+        if constexpr (C) ai[r] += __shfl_xor(ai[r], m, 64);
+      }
+    }
+    if (g == 0 && li < CD) {
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        const int k = k0 + r;
+        if (k >= nrhs) break;
+        double* y = bw + S * (off + (static_cast<int64_t>(T.y) * nrhs + k) * CD + li);
+        if (accumulate) {
+          ar[r] += y[0];
+          if constexpr (C) ai[r] += y[1];
+        }
+        y[0] = ar[r];
+        if constexpr (C) y[1] = ai[r];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ host launchers
 static inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
 
@@ -392,6 +550,30 @@ void launch_coupling_apply(bool cplx, bool fp32, int ntgt, int nrhs, int cd, con
   const int64_t n = static_cast<int64_t>(ntgt) * nrhs * cd;
   if (n == 0) return;
   TimedScope ts("coupling_apply", st);
+  const dim3 wgrid(static_cast<unsigned>((ntgt + 3) / 4));
+  // wave-per-target kernels for the block sizes of the configurations (np 4, 5; Maxwell np 4)
+#define BMI_CPL_W(CC, FTT, CDD)                                                                              \
+  if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                                  \
+    if (nrhs == 1)                                                                                            \
+      hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, 1>), wgrid, dim3(256), 0, st, ntgt, nrhs, tgt, rp, src,   \
+                         static_cast<const FTT*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);             \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, 4>), wgrid, dim3(256), 0, st, ntgt, nrhs, tgt, rp, src,   \
+                         static_cast<const FTT*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);             \
+    BMI_CHECK_HIP(hipGetLastError());                                                                         \
+    return;                                                                                                   \
+  }
+  BMI_CPL_W(false, double, 16)
+  BMI_CPL_W(false, float, 16)
+  BMI_CPL_W(true, double, 16)
+  BMI_CPL_W(true, float, 16)
+  BMI_CPL_W(false, double, 25)
+  BMI_CPL_W(false, float, 25)
+  BMI_CPL_W(true, double, 25)
+  BMI_CPL_W(true, float, 25)
+  BMI_CPL_W(true, double, 32)
+  BMI_CPL_W(true, float, 32)
+#undef BMI_CPL_W
   if (cplx && !fp32)
     hipLaunchKernelGGL((k_coupling_apply<true, double>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt, rp,
                        src, static_cast<const double*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
@@ -423,6 +605,24 @@ void launch_leaf(bool cplx, int ncl, int nrhs, int dd, const int* rp, const int*
                  hipStream_t st) {
   const int64_t n = static_cast<int64_t>(ncl) * nrhs * dd;
   TimedScope ts("matvec_nearfield", st);
+  const dim3 wgrid(static_cast<unsigned>((ncl + 3) / 4));
+  // wave-per-cluster kernels for the leaf sizes 4 (P+1)^2 of P = 1, 2
+#define BMI_LEAF_W(CC, DDD)                                                                                    \
+  if (cplx == CC && dd == DDD) {                                                                               \
+    if (nrhs == 1)                                                                                             \
+      hipLaunchKernelGGL((k_leaf_w<CC, DDD, 1>), wgrid, dim3(256), 0, st, ncl, nrhs, rp, src, F, fstride, fcomp, \
+                         xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);                            \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_leaf_w<CC, DDD, 4>), wgrid, dim3(256), 0, st, ncl, nrhs, rp, src, F, fstride, fcomp, \
+                         xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);                            \
+    BMI_CHECK_HIP(hipGetLastError());                                                                          \
+    return;                                                                                                    \
+  }
+  BMI_LEAF_W(false, 16)
+  BMI_LEAF_W(true, 16)
+  BMI_LEAF_W(false, 36)
+  BMI_LEAF_W(true, 36)
+#undef BMI_LEAF_W
   if (cplx)
     hipLaunchKernelGGL(k_leaf<true>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, dd, rp, src, F, fstride, fcomp,
                        xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);

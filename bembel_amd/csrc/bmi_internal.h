@@ -106,6 +106,7 @@ struct bmi_space {
   mutable bmi::DevBuf<double> d_val, d_tval;
   mutable bmi::DevBuf<double> d_ffield;  // far-field table per element
   mutable int ffield_deg = -1;
+  mutable int ffield_stride = 0;
 };
 
 namespace bmi {
@@ -132,7 +133,7 @@ void moment_matrix(int P, int cl, int cr, int np, int comp, std::vector<double>&
 
 // ------------------------------------------------------------------ device side
 void ensure_device_space(const bmi_space* s, hipStream_t st);
-void ensure_ffield(const bmi_space* s, int deg, hipStream_t st);
+void ensure_ffield(const bmi_space* s, int deg, hipStream_t st, bool tangents);
 // Runs the near-field pair kernels; out is a device array of scalars (double or
 // interleaved complex).  Entry (i, j) of a block lands at off + i*rs + j*cs (in scalars);
 // comp_rs/comp_cs are the offsets between vector components (Maxwell).

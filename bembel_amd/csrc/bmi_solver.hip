@@ -366,8 +366,32 @@ static bool cg(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int 
 }
 
 // ------------------------------------------------------------------ right-hand sides
-// DiscreteLinearForm::compute with DirichletTrace (LinearForm/DiscreteLinearForm.hpp:44-83):
-// super-space load vector, one thread per (element, column)
+// Hertz dipole at pos with moment len, exp(-i kappa r) convention (examples/Data.hpp:76-90)
+__device__ __forceinline__ void dipole_field(const double* X, double kappa, const double* pos, const double* len,
+                                             double* Er, double* Ei) {
+  const double c0 = X[0] - pos[0], c1 = X[1] - pos[1], c2 = X[2] - pos[2];
+  const double r = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  const double n[3] = {c0 / r, c1 / r, c2 / r};
+  double sn, cs;
+  sincos(kappa * r, &sn, &cs);  // exp(-i kappa r) = cs - i sn
+  const double nl = n[0] * len[0] + n[1] * len[1] + n[2] * len[2];
+  const double ar = kappa * kappa * cs / r, ai = -kappa * kappa * sn / r;
+  // (1/r^3 + i kappa/r^2) (cs - i sn)
+  const double p = 1.0 / (r * r * r), q = kappa / (r * r);
+  const double br = p * cs + q * sn, bi = q * cs - p * sn;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double tr = len[k] - nl * n[k];   // (n x len) x n
+    const double hk = 3 * nl * n[k] - len[k];
+    Er[k] = ar * tr + br * hk;
+    Ei[k] = ai * tr + bi * hk;
+  }
+}
+
+// DiscreteLinearForm::compute (LinearForm/DiscreteLinearForm.hpp:44-83) with DirichletTrace
+// (fn 0: point source, fn 1: plane waves) or RotatedTangentialTrace of a dipole field
+// (fn 2, RotatedTangentialTrace.hpp:39-82): super-space load vector, one thread per
+// (element, column), (P+2)^2-point rule
 template <int PD, int P>
 __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* __restrict__ elem_patch,
                       const double* __restrict__ elem_llc, int nel, double h, int fn, const double* __restrict__ prm,
@@ -379,22 +403,37 @@ __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* _
   const double* gx = c_gl_nodes + c_gl_offset[n - 1];
   const double* gw = c_gl_weights + c_gl_offset[n - 1];
   const double* c = cp + static_cast<int64_t>(elem_patch[e]) * cp_stride;
-  double ar[I2], ai[I2];
+  double ar[I2], ai[I2], br[I2], bi[I2];
 #pragma unroll
-  for (int m = 0; m < I2; ++m) ar[m] = 0.0, ai[m] = 0.0;
+  for (int m = 0; m < I2; ++m) ar[m] = 0.0, ai[m] = 0.0, br[m] = 0.0, bi[m] = 0.0;
   for (int q = 0; q < n * n; ++q) {
     const double u = gx[q / n], v = gx[q % n], w = h * (gw[q / n] * gw[q % n]);
     double X[3], Ds[3], Dt[3];
     surface_point<PD>(c, elem_llc[2 * e] + h * u, elem_llc[2 * e + 1] + h * v, X, Ds, Dt);
     const double c0 = Ds[1] * Dt[2] - Ds[2] * Dt[1], c1 = Ds[2] * Dt[0] - Ds[0] * Dt[2], c2 = Ds[0] * Dt[1] - Ds[1] * Dt[0];
-    const double kap = sqrt(c0 * c0 + c1 * c1 + c2 * c2) * w;
-    double fr, fi = 0.0;
+    const double nn = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    double fr, fi = 0.0, gr = 0.0, gi = 0.0;
     if (fn == 0) {
       const double d0 = X[0] - prm[0], d1 = X[1] - prm[1], d2 = X[2] - prm[2];
-      fr = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    } else {
+      fr = nn * w / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    } else if (fn == 1) {
       const double* d = prm + 3 * k;
       sincos(kappa * (d[0] * X[0] + d[1] * X[1] + d[2] * X[2]), &fi, &fr);
+      fr *= nn * w, fi *= nn * w;
+    } else {
+      const double* d = prm + 6 * k;
+      double Er[3], Ei[3];
+      dipole_field(X, kappa, d, d + 3, Er, Ei);
+      const double nv[3] = {c0 / nn, c1 / nn, c2 / nn};
+      const double er = Er[0] * nv[0] + Er[1] * nv[1] + Er[2] * nv[2];
+      const double ei = Ei[0] * nv[0] + Ei[1] * nv[1] + Ei[2] * nv[2];
+      double tr[3], ti[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) tr[j] = (Er[j] - er * nv[j]) * w, ti[j] = (Ei[j] - ei * nv[j]) * w;
+      fr = Ds[0] * tr[0] + Ds[1] * tr[1] + Ds[2] * tr[2];
+      fi = Ds[0] * ti[0] + Ds[1] * ti[1] + Ds[2] * ti[2];
+      gr = Dt[0] * tr[0] + Dt[1] * tr[1] + Dt[2] * tr[2];
+      gi = Dt[0] * ti[0] + Dt[1] * ti[1] + Dt[2] * ti[2];
     }
     double Bu[I], Bv[I];
     bernstein_basis<P>(u, Bu);
@@ -403,17 +442,22 @@ __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* _
     for (int iy = 0; iy < I; ++iy)
 #pragma unroll
       for (int ix = 0; ix < I; ++ix) {
-        const double b = kap * Bu[ix] * Bv[iy];
+        const double b = Bu[ix] * Bv[iy];
         ar[iy * I + ix] += fr * b;
         ai[iy * I + ix] += fi * b;
+        br[iy * I + ix] += gr * b;
+        bi[iy * I + ix] += gi * b;
       }
   }
+  const int64_t vcs = static_cast<int64_t>(nel) * I2;  // second vector component (fn 2)
   for (int m = 0; m < I2; ++m) {
     const int64_t o = static_cast<int64_t>(k) * nsuper + static_cast<int64_t>(e) * I2 + m;
-    if (fn == 0)
+    if (fn == 0) {
       out[o] = ar[m];
-    else
+    } else {
       out[2 * o] = ar[m], out[2 * o + 1] = ai[m];
+      if (fn == 2) out[2 * (o + vcs)] = br[m], out[2 * (o + vcs) + 1] = bi[m];
+    }
   }
 }
 
@@ -427,14 +471,16 @@ bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, in
                             int mem, void* stream) {
   try {
     BMI_REQUIRE(s && params && out && nrhs >= 1, BMI_INVALID_ARG, "bad argument");
-    BMI_REQUIRE(fn == 0 || fn == 1, BMI_INVALID_ARG, "unknown data function");
-    BMI_REQUIRE(s->form == BMI_FORM_DISCONTINUOUS, BMI_UNSUPPORTED, "Dirichlet traces need a scalar space");
-    BMI_REQUIRE(fn == 1 || nrhs == 1, BMI_INVALID_ARG, "the point source has one column");
+    BMI_REQUIRE(fn >= 0 && fn <= 2, BMI_INVALID_ARG, "unknown data function");
+    BMI_REQUIRE((fn == 2) == (s->form == BMI_FORM_DIVCONFORMING), BMI_INVALID_ARG,
+                "Dirichlet traces need a scalar space, the rotated tangential trace a DivConforming one");
+    BMI_REQUIRE(fn != 0 || nrhs == 1, BMI_INVALID_ARG, "the point source has one column");
     hipStream_t st = static_cast<hipStream_t>(stream);
     ensure_device_space(s, st);
-    const int S = fn == 1 ? 2 : 1;
+    const int S = fn == 0 ? 1 : 2;
     DevBuf<double> prm, sup, dout;
-    prm.upload(std::vector<double>(params, params + (fn == 0 ? 3 : 3 * nrhs)), st);
+    const int per = fn == 0 ? 3 : (fn == 1 ? 3 : 6);
+    prm.upload(std::vector<double>(params, params + (fn == 0 ? 3 : per * nrhs)), st);
     sup.alloc(static_cast<size_t>(S) * s->nsuper * nrhs);
     const int64_t tot = static_cast<int64_t>(s->nel) * nrhs;
     const unsigned blocks = static_cast<unsigned>((tot + 127) / 128);
@@ -454,7 +500,7 @@ bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, in
       dout.alloc(static_cast<size_t>(S) * s->ndof * nrhs);
       y = dout.ptr;
     }
-    launch_spmv(fn == 1, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, sup.ptr, s->nsuper, y, s->ndof,
+    launch_spmv(fn != 0, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, sup.ptr, s->nsuper, y, s->ndof,
                 nullptr, nullptr, st);
     if (mem != BMI_MEM_DEVICE)
       BMI_CHECK_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, st));

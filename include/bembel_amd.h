@@ -154,10 +154,16 @@ bmi_status bmi_h2_export_leaves(const bmi_h2* h, double* data);
 /* Y = alpha * H * X + beta * Y for nrhs columns (the product of
    H2Matrix/EigenHelper/H2DenseProduct.hpp:33-165 and the Eigen scaleAndAddTo glue
    :268-290).  X, Y column-major with leading dimensions ldx, ldy (in scalars); alpha, beta
-   point to (re, im) pairs; mem says whether X and Y are host or device pointers. */
+   point to (re, im) pairs; mem says whether X and Y are host or device pointers.  With
+   device operands the product is enqueued on `stream` and the call returns without waiting;
+   with host operands it returns when Y holds the result. */
 bmi_status bmi_h2_matvec(const bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* Y,
                          int64_t ldy, const double* alpha, const double* beta, int mem,
                          void* stream);
+/* Replay the super-space part of the product from a captured HIP graph (one per RHS count,
+   default on) or launch it kernel by kernel.  No counterpart in the reference (its product is
+   CPU code); exposed so callers can compare both modes. */
+bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable);
 /* same operator applied in the super space (no T, no T^T); device pointers only */
 bmi_status bmi_h2_matvec_super(const bmi_h2* h, int nrhs, const double* Xs, double* Ys,
                                void* stream);
@@ -167,6 +173,9 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* h, int nrhs, const double* Xs, doub
    LinearForm/DirichletTrace.hpp:31-70) for built-in data functions:
    fn 0: g(x) = 1/|x - p|            (p = params[0..2]; real)
    fn 1: g(x) = exp(i kappa d_k . x) (d_k = params[3k..3k+2]; complex, one column per k)
+   fn 2: rotated tangential trace of a Hertz dipole field for the EFIE
+         (LinearForm/RotatedTangentialTrace.hpp:39-82 with examples/Data.hpp:76-90 Dipole):
+         params[6k..6k+5] = position, moment; DivConforming spaces; complex
    out: ndof x nrhs, host or device (mem) */
 bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, int nrhs,
                             double kappa, double* out, int mem, void* stream);

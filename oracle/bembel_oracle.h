@@ -117,6 +117,8 @@ void or_h2_matvec_super(const or_h2* h, int nrhs, const double* x, double* y);
 
 /* right-hand sides (LinearForm/DiscreteLinearForm.hpp:44-83, DirichletTrace.hpp:31-70).
    fn: 0 point source 1/|x-x0| (p = x0), 1 plane wave exp(i kappa d.x) (p = d, kre),
+   2 rotated tangential trace of a Hertz dipole (p = position[3], moment[3], kre;
+   DivConforming spaces, LinearForm/RotatedTangentialTrace.hpp, examples/Data.hpp:76-90),
    out: ndof entries (real) or 2*ndof (complex) */
 void or_rhs(const or_space* s, int fn, const double* p, double kre, double* out);
 

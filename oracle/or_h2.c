@@ -533,7 +533,76 @@ void or_h2_matvec(const or_h2* h, int nrhs, const double* x, double* y) {
 /* ------------------------------------------------------------ RHS */
 /* DiscreteLinearForm::compute with DirichletTrace (LinearForm/DiscreteLinearForm.hpp:44-83,
    DirichletTrace.hpp:38-66).  fn 0: 1/|x-p|, fn 1: exp(i kre p.x). */
+/* Data::Dipole (examples/Data.hpp:76-90): Hertz dipole at pos with moment len */
+static void dipole_field(const double* x, double kappa, const double* pos, const double* len,
+                         double complex* E) {
+  double c[3] = {x[0] - pos[0], x[1] - pos[1], x[2] - pos[2]};
+  double r = sqrt((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2]);
+  double n[3] = {c[0] / r, c[1] / r, c[2] / r};
+  double complex expc = cexp(-OR_I * kappa * r);
+  /* (n x len) x n and 3 (n . len) n - len */
+  double nxl[3] = {n[1] * len[2] - n[2] * len[1], n[2] * len[0] - n[0] * len[2], n[0] * len[1] - n[1] * len[0]};
+  double tr[3] = {nxl[1] * n[2] - nxl[2] * n[1], nxl[2] * n[0] - nxl[0] * n[2], nxl[0] * n[1] - nxl[1] * n[0]};
+  double nl = (n[0] * len[0] + n[1] * len[1]) + n[2] * len[2];
+  double complex a = kappa * kappa * expc / r;
+  double complex ec = ((1 / (r * r * r)) - (OR_I * (-kappa) / (r * r))) * expc;
+  for (int k = 0; k < 3; ++k) E[k] = a * tr[k] + ec * (3 * nl * n[k] - len[k]);
+}
+
+/* DiscreteLinearForm<RotatedTangentialTrace> (LinearForm/RotatedTangentialTrace.hpp:39-82) of
+   the dipole field: per element (I2 x 2) block, column c = t_c . (f - (f.n) n) w phi */
+static void rhs_rotated_tangential(const or_space* s, const double* p, double kappa, double* out) {
+  int P = s->P, I = P + 1, I2 = s->I2, nq1 = P + 2, nq = nq1 * nq1;
+  const double* qx = or_gl_nodes(nq1);
+  const double* qw = or_gl_weights(nq1);
+  size_t N = s->nsuper, vcs = (size_t)s->nel * I2;
+  double* sup = (double*)calloc(2 * N, sizeof(double));
+  for (int e = 0; e < s->nel; ++e) {
+    int node = s->elem[e];
+    double h = 1. / (1 << s->nodes[node].level);
+    double complex acc[2][1024];
+    for (int m = 0; m < I2; ++m) acc[0][m] = acc[1][m] = 0;
+    for (int i = 0; i < nq; ++i) {
+      double xi[2] = {qx[i / nq1], qx[i % nq1]}, q[12];
+      or_map2surface(s, node, xi, h * (qw[i / nq1] * qw[i % nq1]), q);
+      double c0 = q[7] * q[11] - q[8] * q[10], c1 = q[8] * q[9] - q[6] * q[11], c2 = q[6] * q[10] - q[7] * q[9];
+      double nn = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+      double n[3] = {c0 / nn, c1 / nn, c2 / nn};
+      double complex E[3];
+      dipole_field(q + 3, kappa, p, p + 3, E);
+      double complex fn_ = (E[0] * n[0] + E[1] * n[1]) + E[2] * n[2];
+      double complex t[3];
+      for (int k = 0; k < 3; ++k) t[k] = (E[k] - fn_ * n[k]) * q[2];
+      double complex cx = (q[6] * t[0] + q[7] * t[1]) + q[8] * t[2];
+      double complex cy = (q[9] * t[0] + q[10] * t[1]) + q[11] * t[2];
+      double X[32], Y[32];
+      or_bernstein(P, xi[0], X);
+      or_bernstein(P, xi[1], Y);
+      for (int iy = 0; iy < I; ++iy)
+        for (int ix = 0; ix < I; ++ix) {
+          acc[0][iy * I + ix] += cx * (X[ix] * Y[iy]);
+          acc[1][iy * I + ix] += cy * (X[ix] * Y[iy]);
+        }
+    }
+    int id = s->nodes[node].id;
+    for (int c = 0; c < 2; ++c)
+      for (int m = 0; m < I2; ++m) {
+        size_t r = c * vcs + (size_t)id * I2 + m;
+        sup[2 * r] = creal(acc[c][m]), sup[2 * r + 1] = cimag(acc[c][m]);
+      }
+  }
+  memset(out, 0, sizeof(double) * 2 * s->ndof);
+  for (size_t r = 0; r < N; ++r)
+    for (int k = s->rowptr[r]; k < s->rowptr[r + 1]; ++k)
+      for (int q = 0; q < 2; ++q) out[2 * s->colidx[k] + q] += s->vals[k] * sup[2 * r + q];
+  free(sup);
+}
+
 void or_rhs(const or_space* s, int fn, const double* p, double kre, double* out) {
+  if (fn == 2) {
+    rhs_rotated_tangential(s, p, kre, out);
+    return;
+  }
   int P = s->P, I = P + 1, I2 = s->I2, deg = P + 1, nq1 = deg + 1, nq = nq1 * nq1;
   int cplx = fn == 1;
   int c = cplx ? 2 : 1;

@@ -212,7 +212,7 @@ class Space:
         return A.reshape(D, D).T.copy()
 
     def rhs(self, fn, p, kappa=0.0):
-        cplx = fn == 1
+        cplx = fn in (1, 2)
         out = np.zeros(self.ndof * (2 if cplx else 1))
         pa = np.asarray(p, dtype=np.float64)
         lib().or_rhs(self.h, fn, _dp(pa), kappa, _dp(out))

@@ -121,3 +121,25 @@ def test_gpu_h2_full_size_laplace(gpu):
     got = ps.pair_blocks(B.LaplaceSingleLayerOperator(), pairs)
     for k, (i, j) in enumerate(pairs):
         assert rel(got[k], os_.pair_block(O.LAPLACE, int(i), int(j))) <= TOL
+
+
+def test_gpu_h2_graph_replays_match_eager(gpu):
+    """The captured-graph product equals the kernel-by-kernel product on every replay, with the
+    program's own device work and synchronisations between replays, and after the workspace
+    grows (which re-captures)."""
+    import torch
+    os_, ps, op, H = make(O.HELMHOLTZ, 4 + 0j, 3, 2, 5)
+    n = ps.number_of_dofs
+    g = torch.Generator(device="cuda").manual_seed(1)
+    xs = [torch.randn(n, k, dtype=torch.complex128, device="cuda", generator=g) for k in (1, 1, 3, 1, 3)]
+    H.use_graphs(False)
+    eager = [(H @ x).clone() for x in xs]
+    torch.cuda.synchronize()
+    H.use_graphs(True)
+    for rep in range(3):
+        for x, ye in zip(xs, eager):
+            y = H @ x
+            junk = torch.randn(1 << 16, device="cuda") * 2.0  # unrelated device work in between
+            torch.cuda.synchronize()
+            assert torch.equal(y, ye), rep
+            del junk

@@ -1,0 +1,106 @@
+"""Maxwell single layer (EFIE) on the GPU versus the oracle: element-pair blocks of every pair
+class, the dense Galerkin matrix, every H2 leaf and the H2 matvec (1e-12), on the ellipsoid with
+semi-axes (1.0, 0.7, 0.5) used by the Maxwell workload."""
+import numpy as np
+import pytest
+
+import bembel_amd as B
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+ELLIPSOID = (1.0, 0.7, 0.5)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def spaces(M, P):
+    og = O.Geometry(scale=ELLIPSOID)
+    os_ = O.Space(og, M, P, form=1)
+    op = B.MaxwellSingleLayerOperator()
+    ps = B.AnsatzSpace(B.Geometry(scale=ELLIPSOID), M, P, op)
+    return os_, ps, op
+
+
+@pytest.mark.parametrize("M,P", [(2, 1), (2, 2)])
+def test_gpu_maxwell_pair_blocks(gpu, M, P):
+    os_, ps, op = spaces(M, P)
+    kappa = 2.0 + 0.25j
+    op.set_wavenumber(kappa)
+    assert ps.vector_dimension == 2
+    rng = np.random.default_rng(M * 10 + P)
+    rows = rng.choice(ps.number_of_elements, 4, replace=False)
+    pairs = np.array([(int(e), f) for e in rows for f in range(ps.number_of_elements)], dtype=np.int32)
+    got = ps.pair_blocks(op, pairs)
+    seen = set()
+    worst = 0.0
+    for k, (i, j) in enumerate(pairs):
+        info, _ = os_.pair_info(int(i), int(j))
+        seen.add(info[0] if info[0] else (0 if info[3] == P + 2 else 1))
+        worst = max(worst, rel(got[k], os_.pair_block(O.MAXWELL, int(i), int(j), kappa)))
+    print("classes", sorted(seen), "worst", worst)
+    assert {0, 2, 3, 4} <= seen
+    assert worst <= TOL
+
+
+def test_gpu_maxwell_dense(gpu):
+    os_, ps, op = spaces(1, 1)
+    kappa = 2.0 + 0j
+    op.set_wavenumber(kappa)
+    A = B.DenseMatrix(ps, op, super_only=True).matrix
+    assert rel(A, os_.dense_super(O.MAXWELL, kappa)) <= TOL
+    G = B.DenseMatrix(ps, op).matrix
+    assert rel(G, os_.dense_galerkin(O.MAXWELL, kappa)) <= TOL
+
+
+@pytest.mark.parametrize("M,P,np_", [(2, 1, 9), (3, 2, 4)])
+def test_gpu_maxwell_h2_leaves_and_matvec(gpu, M, P, np_):
+    os_, ps, op = spaces(M, P)
+    kappa = 2.0 + 0j
+    op.set_wavenumber(kappa)
+    H = B.H2Matrix(ps, op, n_cheb=np_)
+    oh = O.H2(os_, O.MAXWELL, kappa, np_)
+    ol, gl = oh.leaves(), H.leaves()
+    okeys = {tuple(r): i for i, r in enumerate(ol.tolist())}
+    gkeys = {tuple(r): i for i, r in enumerate(gl.tolist())}
+    assert set(okeys) == set(gkeys)
+    worst = 0.0
+    for key, gi in gkeys.items():
+        for r in range(2):
+            for c in range(2):
+                worst = max(worst, rel(H.leaf_matrix(gi, r, c), oh.leaf_matrix(okeys[key], r, c)))
+    print("worst leaf rel", worst)
+    assert worst <= TOL
+    rng = np.random.default_rng(9)
+    n = ps.number_of_dofs
+    x = rng.uniform(-1, 1, (n, 2)) + 1j * rng.uniform(-1, 1, (n, 2))
+    y, yo = H @ x, oh.matvec(x)
+    print("matvec rel", rel(y, yo))
+    assert rel(y, yo) <= TOL
+
+
+def test_gpu_maxwell_rhs_and_gmres(gpu):
+    """EFIE right-hand side (rotated tangential trace of a dipole) equals the oracle; GMRES on the
+    H2 EFIE matrix reaches its tolerance and agrees with the oracle's dense solve."""
+    os_, ps, op = spaces(2, 1)
+    kappa = 2.0
+    op.set_wavenumber(kappa)
+    pos, mom = np.array([0.1, -0.05, 0.08]), np.array([0.02, 0.05, -0.07])
+    f = B.DiscreteLinearForm(B.RotatedTangentialTrace(pos, mom, kappa), ps).compute().get_discrete_linear_form()
+    fo = os_.rhs(2, np.concatenate([pos, mom]), kappa)
+    assert rel(f, fo) <= TOL
+    H = B.H2Matrix(ps, op, n_cheb=9)
+    gm = B.GMRES()
+    gm.set_tolerance(1e-10)
+    gm.set_restart(200)
+    gm.set_max_iterations(2000)
+    gm.compute(H)
+    x = gm.solve(f)
+    assert gm.info() == B.OK
+    assert rel(H @ x, f) <= 1e-9
+    D = os_.dense_galerkin(O.MAXWELL, kappa + 0j)
+    xd = np.linalg.solve(D, fo)
+    print("gmres iterations", gm.iterations(), "rel vs dense", rel(x, xd))
+    assert rel(x, xd) <= 1e-4

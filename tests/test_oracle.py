@@ -357,3 +357,162 @@ def test_rhs_and_solve_closed_form():
     assert it < 2000
     assert np.linalg.norm(A @ x - b) < 1e-10 * np.linalg.norm(b)
     assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-8, atol=1e-10)
+
+
+# ------------------------------------------------------------------ Maxwell (EFIE)
+ELLIPSOID = (1.0, 0.7, 0.5)
+
+
+def _bernstein_np(P, x):
+    """Bernstein values and derivatives of degree P from the closed form, vectorised over x."""
+    x = np.asarray(x, dtype=np.float64)
+    B = np.stack([math.comb(P, k) * x ** k * (1 - x) ** (P - k) for k in range(P + 1)])
+    low = [math.comb(P - 1, k) * x ** k * (1 - x) ** (P - 1 - k) for k in range(P)]
+    D = np.stack([P * ((low[k - 1] if k >= 1 else 0.0) - (low[k] if k <= P - 1 else 0.0)) for k in range(P + 1)])
+    return B, D
+
+
+def _maxwell_block_numpy(S, geo, e1, e2, kappa, n, div=1.0):
+    """EFIE element-pair block from its definition with an n-point tensor Gauss rule per element:
+    K = exp(-i k r) / (4 pi r), vector part K w w (t_a . t_b) phi_i phi_j, divergence part
+    -K w w / (k^2 h^2) dphi_i dphi_j, rows/cols ordered (component, iy*(P+1)+ix)."""
+    P, I = S.P, S.P + 1
+    I2 = I * I
+    h = 2.0 ** -S.M
+    t, w = np.polynomial.legendre.leggauss(n)
+    t, w = (t + 1) / 2, w / 2
+    el = S.elements()
+
+    def element_points(e):
+        xi0 = np.repeat(t, n)
+        xi1 = np.tile(t, n)
+        wt = h * np.repeat(w, n) * np.tile(w, n)
+        X, T0, T1 = [], [], []
+        for a, b in zip(xi0, xi1):
+            sp = geo.surface_point(int(el["patch"][e]), el["llc"][e] + h * np.array([a, b]))
+            X.append(sp[3:6]), T0.append(sp[6:9]), T1.append(sp[9:12])
+        Bx, Dx = _bernstein_np(P, xi0)
+        By, Dy = _bernstein_np(P, xi1)
+        phi = np.stack([Bx[ix] * By[iy] for iy in range(I) for ix in range(I)])
+        dphi = [np.stack([Dx[ix] * By[iy] for iy in range(I) for ix in range(I)]),
+                np.stack([Bx[ix] * Dy[iy] for iy in range(I) for ix in range(I)])]
+        return np.array(X), [np.array(T0), np.array(T1)], wt, phi, dphi
+
+    X1, T1, w1, phi1, d1 = element_points(e1)
+    X2, T2, w2, phi2, d2 = element_points(e2)
+    r = np.linalg.norm(X1[:, None, :] - X2[None, :, :], axis=2)
+    K = np.exp(-1j * kappa * r) / (4 * math.pi * r) * w1[:, None] * w2[None, :]
+    blk = np.zeros((2 * I2, 2 * I2), dtype=complex)
+    for rc in range(2):
+        for cc in range(2):
+            tt = T1[rc] @ T2[cc].T
+            blk[rc * I2:(rc + 1) * I2, cc * I2:(cc + 1) * I2] = (
+                phi1 @ (K * tt) @ phi2.T - div * d1[rc] @ K @ d2[cc].T / (kappa ** 2 * h ** 2))
+    return blk
+
+
+@pytest.mark.parametrize("P", [1, 2])
+def test_maxwell_integrand_vs_definition(P):
+    """The oracle's EFIE integrand (vector and divergence parts, component layout, kappa and h
+    scaling) equals the definition evaluated independently, on a regular pair with a 16-point
+    rule where both sides are converged."""
+    geo = O.Geometry(scale=ELLIPSOID)
+    S = O.Space(geo, 1, P, form=1)
+    kappa = 2.0 + 0.3j
+    pairs = []
+    for e1 in (0, 5):
+        far = [e2 for e2 in range(S.nel) if S.pair_info(e1, e2)[0][0] == 0 and S.pair_info(e1, e2)[1] > 0]
+        pairs.append((e1, far[len(far) // 2]))
+    for e1, e2 in pairs:
+        got = S.integrate(O.MAXWELL, 1, e1, 0, e2, 0, 15, kappa)
+        ref = _maxwell_block_numpy(S, geo, e1, e2, kappa, 16)
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+        # both parts matter: a dropped or sign-flipped divergence part misses by far more than
+        # the tolerance, and so do swapped vector components
+        for div in (0.0, -1.0):
+            bad = _maxwell_block_numpy(S, geo, e1, e2, kappa, 16, div)
+            assert np.linalg.norm(got - bad) > 1e-3 * np.linalg.norm(ref)
+        I2 = S.I2
+        sw = ref.copy()
+        sw[:I2, I2:], sw[I2:, :I2] = ref[I2:, :I2], ref[:I2, I2:]
+        assert np.linalg.norm(got - sw) > 1e-3 * np.linalg.norm(ref)
+
+
+def test_maxwell_h2_vs_dense():
+    """The Maxwell interpolation FMM (derivative moments, 4x4 interpolant per vector component
+    pair) reproduces the dense EFIE matrix (tests/test_LazyEigenSum.cpp analogue)."""
+    geo = O.Geometry(scale=ELLIPSOID)
+    S = O.Space(geo, 2, 1, form=1)
+    kappa = 2.0 + 0j
+    D = S.dense_galerkin(O.MAXWELL, kappa)
+    assert np.abs(D - D.T).max() <= 1e-12 * np.abs(D).max()
+    H = O.H2(S, O.MAXWELL, kappa, 9)
+    a = np.random.default_rng(5).uniform(-1, 1, S.ndof) + 0j
+    Da, Ha = D @ a, H.matvec(a)
+    assert np.linalg.norm(Da - Ha) <= 1e-7 * np.linalg.norm(Da)
+    # a coarse interpolation is visibly worse, so the comparison has power
+    H3 = O.H2(S, O.MAXWELL, kappa, 3)
+    assert np.linalg.norm(Da - H3.matvec(a)) > 10 * np.linalg.norm(Da - Ha)
+
+
+def _dipole_np(x, kappa, pos, length):
+    """Hertz dipole field (Jackson, Classical Electrodynamics, 3rd ed., p. 411) with the
+    exp(-i kappa r) convention of the reference (examples/Data.hpp:76-90)."""
+    c = np.asarray(x, dtype=float) - pos
+    r = np.linalg.norm(c)
+    n = c / r
+    e = np.exp(-1j * kappa * r)
+    far = kappa ** 2 * e / r * (length - (n @ length) * n)
+    near = (1 / r ** 3 + 1j * kappa / r ** 2) * e * (3 * (n @ length) * n - length)
+    return far + near
+
+
+def test_dipole_field_solves_maxwell():
+    """curl curl E = kappa^2 E and div E = 0 away from the dipole (fourth-order differences)."""
+    kappa, pos, length = 2.0, np.array([0.1, -0.05, 0.08]), np.array([0.02, 0.05, -0.07])
+    x0, hstep = np.array([0.6, 0.3, -0.4]), 1e-3
+
+    def jac(f, x):
+        J = np.zeros((3, 3), dtype=complex)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = hstep
+            J[:, k] = (-f(x + 2 * e) + 8 * f(x + e) - 8 * f(x - e) + f(x - 2 * e)) / (12 * hstep)
+        return J
+
+    E = lambda x: _dipole_np(x, kappa, pos, length)
+    curl = lambda f: (lambda x: (lambda J: np.array([J[2, 1] - J[1, 2], J[0, 2] - J[2, 0], J[1, 0] - J[0, 1]]))(jac(f, x)))
+    cc = curl(curl(E))(x0)
+    assert np.linalg.norm(cc - kappa ** 2 * E(x0)) < 1e-5 * np.linalg.norm(kappa ** 2 * E(x0))
+    assert abs(np.trace(jac(E, x0))) < 1e-6 * np.linalg.norm(E(x0))
+
+
+def test_rotated_tangential_trace_vs_definition():
+    """RHS of the EFIE: Galerkin projection of the rotated tangential trace of the dipole field
+    onto the DivConforming space, evaluated independently with the same (P+2)-point rule."""
+    geo = O.Geometry(scale=ELLIPSOID)
+    for P in (1, 2):
+        S = O.Space(geo, 1, P, form=1)
+        kappa, pos, length = 2.0, np.array([0.1, -0.05, 0.08]), np.array([0.02, 0.05, -0.07])
+        got = S.rhs(2, np.concatenate([pos, length]), kappa)
+        I, I2, h, n = P + 1, S.I2, 2.0 ** -S.M, P + 2
+        t, w = np.polynomial.legendre.leggauss(n)
+        t, w = (t + 1) / 2, w / 2
+        el = S.elements()
+        sup = np.zeros(S.nsuper, dtype=complex)
+        for e in range(S.nel):
+            for a in range(n):
+                for b in range(n):
+                    sp = geo.surface_point(int(el["patch"][e]), el["llc"][e] + h * np.array([t[a], t[b]]))
+                    ds, dt = sp[6:9], sp[9:12]
+                    nrm = np.cross(ds, dt)
+                    nrm /= np.linalg.norm(nrm)
+                    f = _dipole_np(sp[3:6], kappa, pos, length)
+                    tang = (f - (f @ nrm) * nrm) * (h * w[a] * w[b])
+                    Bx, _ = _bernstein_np(P, np.array([t[a]]))
+                    By, _ = _bernstein_np(P, np.array([t[b]]))
+                    phi = np.array([Bx[ix, 0] * By[iy, 0] for iy in range(I) for ix in range(I)])
+                    sup[e * I2:(e + 1) * I2] += (ds @ tang) * phi
+                    sup[S.nel * I2 + e * I2:S.nel * I2 + (e + 1) * I2] += (dt @ tang) * phi
+        ref = S.T_dense().T @ sup
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
