@@ -429,11 +429,12 @@ class H2Matrix:
         """alpha * H x for a vector or a (n x nrhs) matrix (numpy on host or torch on device)."""
         n = self.rows()
         X, nrhs, ld, mem, squeeze, backend = _columns(x, n, self.scalar)
+        # beta = 0: Y is written, never read
         if backend == "torch":
             import torch
-            Y = torch.zeros_like(X)
+            Y = torch.empty_like(X)
         else:
-            Y = np.zeros_like(X)
+            Y = np.empty_like(X)
         al = np.array([complex(alpha).real, complex(alpha).imag])
         be = np.zeros(2)
         _check(library().bmi_h2_matvec(self._h, nrhs, _dptr(X, backend), ld, _dptr(Y, backend), ld, _dp(al), _dp(be),
@@ -443,6 +444,20 @@ class H2Matrix:
 
     def __matmul__(self, x):
         return self.matvec(x)
+
+    def matvec_super(self, xs, stream=None):
+        """Super-space product A_super xs (no T, T^T) for a torch device tensor of shape
+        (number_of_super_dofs, nrhs) or (number_of_super_dofs,)."""
+        import torch
+        if not _is_torch_cuda(xs):
+            raise BembelError(INVALID_ARG, "matvec_super takes a device tensor")
+        N = self.space.number_of_super_dofs
+        X, nrhs, ld, mem, squeeze, backend = _columns(xs, N, self.scalar)
+        Y = torch.empty_like(X)
+        _check(library().bmi_h2_matvec_super(self._h, nrhs, _dptr(X, backend), _dptr(Y, backend),
+                                             _stream_ptr(stream)))
+        Y = Y.t()
+        return Y.reshape(-1) if squeeze else Y
 
     def use_graphs(self, enable=True):
         """Replay the product from captured HIP graphs (default) or launch kernel by kernel."""

@@ -72,6 +72,16 @@ __global__ void k_fill(double* p, int64_t n, double v) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_add(double* y, const double* x, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+// y += x (n doubles)
+static void device_add(double* y, const double* x, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_add, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, y, x, n);
+  BMI_CHECK_HIP(hipGetLastError());
+}
 void device_fill(double* p, int64_t n, double v, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, p, n, v);
@@ -114,6 +124,15 @@ struct bmi_h2 {
   bool use_graphs = true;
   hipStream_t cap_stream = nullptr;
   std::map<int, hipGraphExec_t> graphs;
+  // the far-field chain runs on a side stream, concurrently with the near-field sweep, and
+  // writes its leaf rows into d_yf; the final T^T product adds both
+  hipStream_t side_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  DevBuf<double> d_yf;
+  // sweep kernel variants picked by timing on first use (-1: not yet); the fused single-RHS
+  // sweep has its own pair
+  int leaf_variant = -1, cpl_variant = -1;
+  int fused_leaf_variant = -1, fused_cpl_variant = -1;
   void drop_graphs() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     graphs.clear();
@@ -121,11 +140,14 @@ struct bmi_h2 {
   ~bmi_h2() {
     drop_graphs();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    if (side_stream) (void)hipStreamDestroy(side_stream);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
   }
   size_t bytes() const {
     return d_dense.bytes() + d_coup.bytes() + d_coup32.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() +
            d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_xs.bytes() +
-           d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes();
+           d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes() + d_yf.bytes();
   }
 };
 
@@ -143,11 +165,14 @@ static void ensure_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
   h->d_ys.alloc(S * N * nrhs);
   for (int c = 0; c < h->vd; ++c) h->d_fw[c].alloc(S * h->lvl_total * nrhs);
   h->d_bw.alloc(S * h->lvl_total * nrhs);
+  h->d_yf.alloc(S * N * nrhs);
   h->ws_rhs = nrhs;
 }
 
-// super-space apply: ys = A_super xs (device, column stride nsuper)
-static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, hipStream_t st) {
+static bool has_far(const bmi_h2* h) { return h->L >= 0 && h->lvl_total > 0; }
+
+// far-field chain: forward transformation, coupling, backward transformation, leaf rows -> yf
+static void apply_far(bmi_h2* h, int nrhs, const double* xs, double* yf, hipStream_t st) {
   const bmi_space* s = h->s;
   const bool C = h->cplx;
   const int S = C ? 2 : 1;
@@ -157,45 +182,143 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, hipSt
   const bool fp32 = h->op.precision == BMI_PRECISION_MIXED;
   const void* coupF = fp32 ? static_cast<const void*>(h->d_coup32.ptr) : static_cast<const void*>(h->d_coup.ptr);
   const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
-  const int64_t dblk = static_cast<int64_t>(h->dd) * h->dd;
-  const bool has_far = h->L >= 0 && h->lvl_total > 0;
-  // forward transformation per column component
-  if (has_far)
-    for (int cc = 0; cc < h->vd; ++cc) {
-      double* fw = h->d_fw[cc].ptr;
-      TimedScope ts("matvec_forward", st);
-      launch_moment(C, ncl_at(s, h->L), nrhs, rows, h->dd, h->d_mom[cc].ptr, xs, N, cc * vcs,
-                    fw + S * h->lvl_off[h->L] * nrhs, st);
-      for (int l = h->L - 1; l >= 0; --l)
-        launch_m2m(C, ncl_at(s, l), nrhs, rows, h->np2, h->d_T.ptr, fw + S * h->lvl_off[l + 1] * nrhs,
-                   fw + S * h->lvl_off[l] * nrhs, st);
-    }
-  for (int rc = 0; rc < h->vd; ++rc) {
-    if (has_far) {
-      device_fill(h->d_bw.ptr, S * h->lvl_total * nrhs, 0.0, st);
-      for (int cc = 0; cc < h->vd; ++cc)
-        launch_coupling_apply(C, fp32, h->ntgt, nrhs, rows, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, coupF,
-                              h->vd * h->vd * cblk, (rc * h->vd + cc) * cblk, h->d_lvl_off.ptr, h->d_fw[cc].ptr,
-                              h->d_bw.ptr, 1, st);
-      TimedScope ts("matvec_backward", st);
-      for (int l = 0; l < h->L; ++l)
-        launch_l2l(C, ncl_at(s, l + 1), nrhs, rows, h->np2, h->d_T.ptr, h->d_bw.ptr + S * h->lvl_off[l] * nrhs,
-                   h->d_bw.ptr + S * h->lvl_off[l + 1] * nrhs, st);
-    }
-    for (int cc = 0; cc < h->vd; ++cc)
-      launch_leaf(C, h->ncl_leaf, nrhs, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, h->vd * h->vd * dblk,
-                  (rc * h->vd + cc) * dblk, xs, N, cc * vcs, h->d_mom[rc].ptr, rows,
-                  (cc == 0 && has_far) ? h->d_bw.ptr + S * h->lvl_off[h->L] * nrhs : nullptr, ys, N, rc * vcs,
-                  cc > 0 ? 1 : 0, st);
+  for (int cc = 0; cc < h->vd; ++cc) {
+    TimedScope ts("matvec_forward", st);
+    launch_forward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[cc].ptr, h->d_T.ptr, xs, N, cc * vcs,
+                   h->d_fw[cc].ptr, h->d_lvl_off.ptr, st);
   }
+  for (int rc = 0; rc < h->vd; ++rc) {
+    device_fill(h->d_bw.ptr, S * h->lvl_total * nrhs, 0.0, st);
+    for (int cc = 0; cc < h->vd; ++cc)
+      launch_coupling_apply(C, fp32, std::max(h->cpl_variant, 0), h->ntgt, nrhs, rows, h->d_tgt.ptr, h->d_crp.ptr,
+                            h->d_csrc.ptr, coupF, h->vd * h->vd * cblk, (rc * h->vd + cc) * cblk, h->d_lvl_off.ptr,
+                            h->d_fw[cc].ptr, h->d_bw.ptr, 1, st);
+    TimedScope ts("matvec_backward", st);
+    launch_backward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[rc].ptr, h->d_T.ptr, h->d_bw.ptr,
+                    h->d_lvl_off.ptr, yf, N, rc * vcs, st);
+  }
+}
+
+// near-field sweep: ys = A_near xs
+static void apply_near(bmi_h2* h, int nrhs, const double* xs, double* ys, hipStream_t st) {
+  const bmi_space* s = h->s;
+  const int64_t N = s->nsuper;
+  const int64_t vcs = static_cast<int64_t>(s->nel) * s->I2;
+  const int64_t dblk = static_cast<int64_t>(h->dd) * h->dd;
+  for (int rc = 0; rc < h->vd; ++rc)
+    for (int cc = 0; cc < h->vd; ++cc)
+      launch_leaf(h->cplx, std::max(h->leaf_variant, 0), h->ncl_leaf, nrhs, h->dd, h->d_drp.ptr, h->d_dsrc.ptr,
+                  h->d_dense.ptr, h->vd * h->vd * dblk, (rc * h->vd + cc) * dblk, xs, N, cc * vcs, ys, N, rc * vcs,
+                  cc > 0 ? 1 : 0, st);
+}
+
+// time both variants of the two sweeps once (1 RHS) and keep the faster; the variants give
+// identical bits, so the choice affects speed only
+static void autotune(bmi_h2* h, const double* xs, hipStream_t st) {
+  if (h->leaf_variant >= 0 && h->cpl_variant >= 0) return;
+  hipEvent_t a, b;
+  BMI_CHECK_HIP(hipEventCreate(&a));
+  BMI_CHECK_HIP(hipEventCreate(&b));
+  auto time_it = [&](auto&& fn) {
+    fn();  // warm-up
+    BMI_CHECK_HIP(hipEventRecord(a, st));
+    for (int r = 0; r < 3; ++r) fn();
+    BMI_CHECK_HIP(hipEventRecord(b, st));
+    BMI_CHECK_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    BMI_CHECK_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  };
+  float best = 0.f;
+  int pick = 0;
+  for (int v = 0; v < 2; ++v) {
+    h->leaf_variant = v;
+    const float t = time_it([&] { apply_near(h, 1, xs, h->d_ys.ptr, st); });
+    if (v == 0 || t < best) best = t, pick = v;
+  }
+  h->leaf_variant = pick;
+  if (has_far(h)) {
+    const int S = h->cplx ? 2 : 1;
+    const bool fp32 = h->op.precision == BMI_PRECISION_MIXED;
+    const void* coupF = fp32 ? static_cast<const void*>(h->d_coup32.ptr) : static_cast<const void*>(h->d_coup.ptr);
+    const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
+    device_fill(h->d_fw[0].ptr, S * h->lvl_total, 0.0, st);
+    pick = 0;
+    for (int v = 0; v < 2; ++v) {
+      const float t = time_it([&] {
+        launch_coupling_apply(h->cplx, fp32, v, h->ntgt, 1, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, coupF,
+                              h->vd * h->vd * cblk, 0, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, 0, st);
+      });
+      if (v == 0 || t < best) best = t, pick = v;
+    }
+    h->cpl_variant = pick;
+    // the fused sweep: all four combinations (register use, and so occupancy, differs)
+    if (h->vd == 1) {
+      int pl = -1, pc = -1;
+      for (int vl = 0; vl < 2; ++vl)
+        for (int vc = 0; vc < 2; ++vc) {
+          bool ok = true;
+          const float t = time_it([&] {
+            ok = launch_sweep_fused(h->cplx, fp32, vl, vc, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr,
+                                    h->d_dense.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
+                                    h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st);
+          });
+          if (ok && (pl < 0 || t < best)) best = t, pl = vl, pc = vc;
+        }
+      h->fused_leaf_variant = pl;
+      h->fused_cpl_variant = pc;
+    }
+  } else {
+    h->cpl_variant = 0;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+}
+
+// super-space apply: ys = A_near xs on `st`, yf = A_far xs on the side stream (joined back into
+// `st` before returning); the caller adds the two
+static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, double* yf, hipStream_t st) {
+  const bool far = has_far(h);
+  if (far && nrhs == 1 && h->vd == 1 && h->fused_leaf_variant >= 0 && h->fused_cpl_variant >= 0) {
+    // one RHS: forward transformation (also clears bw), both sweeps in one launch, backward
+    const bmi_space* s = h->s;
+    const bool fp32 = h->op.precision == BMI_PRECISION_MIXED;
+    const void* coupF = fp32 ? static_cast<const void*>(h->d_coup32.ptr) : static_cast<const void*>(h->d_coup.ptr);
+    const int64_t N = s->nsuper;
+    {
+      TimedScope ts("matvec_forward", st);
+      launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
+                     h->d_fw[0].ptr, h->d_lvl_off.ptr, st, h->d_bw.ptr);
+    }
+    if (launch_sweep_fused(h->cplx, fp32, h->fused_leaf_variant, h->fused_cpl_variant, h->ncl_leaf, h->dd, h->d_drp.ptr,
+                           h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
+                           h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st)) {
+      TimedScope ts("matvec_backward", st);
+      launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
+                      h->d_lvl_off.ptr, yf, N, 0, st);
+      return;
+    }
+  }
+  if (far) {
+    if (!h->side_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking));
+    if (!h->ev_fork) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    if (!h->ev_join) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    BMI_CHECK_HIP(hipEventRecord(h->ev_fork, st));
+    BMI_CHECK_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
+    apply_far(h, nrhs, xs, yf, h->side_stream);
+    BMI_CHECK_HIP(hipEventRecord(h->ev_join, h->side_stream));
+  }
+  apply_near(h, nrhs, xs, ys, st);
+  if (far) BMI_CHECK_HIP(hipStreamWaitEvent(st, h->ev_join, 0));
 }
 
 // ys = A_super xs on the workspace, through a captured graph when possible.  The graph holds
 // only kernel nodes (buffers are cleared by a fill kernel, not a memset node); in-library
 // timing needs host synchronisation per kernel and therefore runs eagerly.
 static void apply_super_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
+  autotune(h, h->d_xs.ptr, st);
   if (!h->use_graphs || timing_enabled()) {
-    apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, st);
+    apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, st);
     return;
   }
   auto it = h->graphs.find(nrhs);
@@ -204,7 +327,7 @@ static void apply_super_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
     hipGraph_t g = nullptr;
     BMI_CHECK_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
     try {
-      apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->cap_stream);
+      apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, h->cap_stream);
     } catch (...) {
       (void)hipStreamEndCapture(h->cap_stream, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -229,7 +352,7 @@ void h2_apply_device(bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* 
               nullptr, st);
   apply_super_workspace(h, nrhs, st);
   launch_spmv(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, h->d_ys.ptr, N, Y, ldy, alpha,
-              beta, st);
+              beta, st, has_far(h) ? h->d_yf.ptr : nullptr, true);
 }
 
 int64_t h2_rows(const bmi_h2* h) { return h->s->ndof; }
@@ -832,8 +955,11 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* hc, int nrhs, const double* Xs, dou
   BMI_REQUIRE(hc && Xs && Ys && nrhs >= 0, BMI_INVALID_ARG, "bad argument");
   bmi_h2* h = const_cast<bmi_h2*>(hc);
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (nrhs == 0) return BMI_OK;
   ensure_workspace(h, nrhs, st);
-  apply_super(h, nrhs, Xs, Ys, st);
+  apply_super(h, nrhs, Xs, Ys, h->d_yf.ptr, st);
+  const int S = h->cplx ? 2 : 1;
+  if (has_far(h)) device_add(Ys, h->d_yf.ptr, static_cast<int64_t>(S) * h->s->nsuper * nrhs, st);
   BMI_CHECK_HIP(hipStreamSynchronize(st));
   BMI_API_END
 }

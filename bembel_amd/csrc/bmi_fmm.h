@@ -15,20 +15,38 @@ void launch_cluster_points(const bmi_space* s, int level, int np, const double* 
 void launch_coupling_assemble(int kind, const int4* d_leaves, int nleaves, int np2,
                               const double* const* d_level_pts, double kre, double kim, double* out,
                               hipStream_t st);
+// y = alpha A (x + x2) + beta y with A in CSR; x2 may be null; long_rows: 16 lanes per row
 void launch_spmv(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* x,
-                 int64_t ldx, double* y, int64_t ldy, const double* alpha, const double* beta, hipStream_t st);
+                 int64_t ldx, double* y, int64_t ldy, const double* alpha, const double* beta, hipStream_t st,
+                 const double* x2 = nullptr, bool long_rows = false);
 void launch_moment(bool cplx, int ncl, int nrhs, int rows, int mc, const double* mom, const double* xs,
                    int64_t ldxs, int64_t seg, double* fw, hipStream_t st);
 void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T, const double* child,
                 double* parent, hipStream_t st);
-void launch_coupling_apply(bool cplx, bool fp32, int ntgt, int nrhs, int cd, const int2* tgt, const int* rp,
-                           const int* src, const void* F, int64_t fstride, int64_t fcomp, const int64_t* lvl_off,
-                           const double* fw, double* bw, int accumulate, hipStream_t st);
+// variant: 0 = one block per lane group per iteration, 1 = two (chosen by the autotuner)
+void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs, int cd, const int2* tgt,
+                           const int* rp, const int* src, const void* F, int64_t fstride, int64_t fcomp,
+                           const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st);
+// forward transformation of all levels (moments at level L, transfers up to level 0)
+void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
+                    const double* T, const double* xs, int64_t ldxs, int64_t xseg, double* fw, const int64_t* lvl_off,
+                    hipStream_t st, double* bw_zero = nullptr);
+// backward transformation of all levels, ending in the leaf rows yf = Mom^T bw_L (written)
+void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
+                     const double* T, double* bw, const int64_t* lvl_off, double* yf, int64_t ldy, int64_t yseg,
+                     hipStream_t st);
 void launch_l2l(bool cplx, int nchild, int nrhs, int rows, int np2, const double* T, const double* parent,
                 double* child, hipStream_t st);
-void launch_leaf(bool cplx, int ncl, int nrhs, int dd, const int* rp, const int* src, const double* F,
-                 int64_t fstride, int64_t fcomp, const double* xs, int64_t ldxs, int64_t xseg, const double* mom,
-                 int rows, const double* bw, double* ys, int64_t ldys, int64_t yseg, int accumulate,
-                 hipStream_t st);
+void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* rp, const int* src, const double* F,
+                 int64_t fstride, int64_t fcomp, const double* xs, int64_t ldxs, int64_t xseg, double* ys, int64_t ldys,
+                 int64_t yseg, int accumulate, hipStream_t st);
+// near-field sweep and coupling sweep of a single-RHS, single-component product in one launch
+// (ys written, bw written); false when no fused kernel exists for these sizes
+bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
+                        const int* dsrc, const double* dF, const double* xs, double* ys, int ntgt, int cd,
+                        const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
+                        const double* fw, double* bw, hipStream_t st);
+void launch_leaf_backward(bool cplx, int ncl, int nrhs, int dd, const double* mom, int rows, const double* bw,
+                          double* yf, int64_t ldy, int64_t yseg, hipStream_t st);
 
 }  // namespace bmi

@@ -13,7 +13,9 @@
 //   multipole levels fw/bw: [cluster][rhs][row], row = comp*np2 + node.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "bmi_device.h"
 #include "bmi_fmm.h"
@@ -198,17 +200,71 @@ struct Val<true> {
 };
 
 // ------------------------------------------------------------------ sparse T, T^T
-// y[row, k] = alpha * sum_j A[row, j] x[j, k] + beta * y[row, k]
+// y[row, k] = alpha * sum_j A[row, j] (x[j, k] + x2[j, k]) + beta * y[row, k]   (x2 optional)
 template <bool C>
 __global__ void k_spmv(int nrows, int nrhs, const int* __restrict__ rp, const int* __restrict__ ci,
-                       const double* __restrict__ v, const double* __restrict__ x, int64_t ldx, double* y,
-                       int64_t ldy, double ar, double ai, double br, double bi, int use_beta) {
+                       const double* __restrict__ v, const double* __restrict__ x, const double* __restrict__ x2,
+                       int64_t ldx, double* y, int64_t ldy, double ar, double ai, double br, double bi, int use_beta) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= static_cast<int64_t>(nrows) * nrhs) return;
   const int row = static_cast<int>(gid % nrows), k = static_cast<int>(gid / nrows);
   Val<C> acc;
   const double* xk = x + (C ? 2 : 1) * static_cast<int64_t>(k) * ldx;
-  for (int p = rp[row]; p < rp[row + 1]; ++p) acc.fma_real(v[p], Val<C>::load(xk, ci[p]));
+  if (x2) {
+    const double* x2k = x2 + (C ? 2 : 1) * static_cast<int64_t>(k) * ldx;
+    for (int p = rp[row]; p < rp[row + 1]; ++p) {
+      Val<C> a = Val<C>::load(xk, ci[p]);
+      a.fma_real(1.0, Val<C>::load(x2k, ci[p]));
+      acc.fma_real(v[p], a);
+    }
+  } else {
+    for (int p = rp[row]; p < rp[row + 1]; ++p) acc.fma_real(v[p], Val<C>::load(xk, ci[p]));
+  }
+  double* yk = y + (C ? 2 : 1) * static_cast<int64_t>(k) * ldy;
+  if constexpr (C) {
+    double r = ar * acc.r - ai * acc.i, im = ar * acc.i + ai * acc.r;
+    if (use_beta) {
+      const double yr = yk[2 * row], yi = yk[2 * row + 1];
+      r += br * yr - bi * yi;
+      im += br * yi + bi * yr;
+    }
+    yk[2 * row] = r;
+    yk[2 * row + 1] = im;
+  } else {
+    double r = ar * acc.r;
+    if (use_beta) r += br * yk[row];
+    yk[row] = r;
+  }
+}
+
+// long rows (T^T: about (P+1)^4 entries per row): LPR lanes share a row and are summed by
+// cross-lane shuffles
+template <bool C, int LPR>
+__global__ void k_spmv_sub(int nrows, int nrhs, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ v, const double* __restrict__ x, const double* __restrict__ x2,
+                           int64_t ldx, double* y, int64_t ldy, double ar, double ai, double br, double bi,
+                           int use_beta) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t rk = gid / LPR;
+  const int l = static_cast<int>(gid % LPR);
+  const bool valid = rk < static_cast<int64_t>(nrows) * nrhs;
+  const int row = valid ? static_cast<int>(rk % nrows) : 0, k = valid ? static_cast<int>(rk / nrows) : 0;
+  Val<C> acc;
+  if (valid) {
+    const double* xk = x + (C ? 2 : 1) * static_cast<int64_t>(k) * ldx;
+    const double* x2k = x2 ? x2 + (C ? 2 : 1) * static_cast<int64_t>(k) * ldx : nullptr;
+    for (int p = rp[row] + l; p < rp[row + 1]; p += LPR) {
+      Val<C> a = Val<C>::load(xk, ci[p]);
+      if (x2k) a.fma_real(1.0, Val<C>::load(x2k, ci[p]));
+      acc.fma_real(v[p], a);
+    }
+  }
+#pragma unroll
+  for (int m = LPR / 2; m > 0; m >>= 1) {
+    acc.r += __shfl_xor(acc.r, m, 64);
+    if constexpr (C) acc.i += __shfl_xor(acc.i, m, 64);
+  }
+  if (!valid || l != 0) return;
   double* yk = y + (C ? 2 : 1) * static_cast<int64_t>(k) * ldy;
   if constexpr (C) {
     double r = ar * acc.r - ai * acc.i, im = ar * acc.i + ai * acc.r;
@@ -259,6 +315,157 @@ __global__ void k_m2m(int ncl, int nrhs, int rows, int np2, const double* __rest
       acc.fma_real(T[static_cast<int64_t>(q * np2 + j) * np2 + i], Val<C>::load(child, cbase + j));
   }
   acc.store(parent, gid);
+}
+
+// ------------------------------------------------------------------ fused tree transformations
+// One workgroup owns one cluster at level `top` and walks its subtree down to level `bot`
+// (bot - top <= 3) in LDS; blockIdx.y is the right-hand side.  This replaces one launch per level
+// (each a few microseconds of latency for very little work) by one launch per three levels.
+// Level buffers in LDS: level top + d holds 4^d clusters x rows values.
+__device__ __forceinline__ int tree_lds_off(int d, int rows, int S) {
+  return S * rows * (((1 << (2 * d)) - 1) / 3);  // sum_{e<d} 4^e clusters before level top + d
+}
+
+// forward transformation (H2Multipole.hpp:380-422): level bot from the moments of the super
+// vector (from_moment) or from fw; then parents by the transfer matrices up to level top
+template <bool C>
+__global__ __launch_bounds__(256) void k_forward_tree(int top, int bot, int from_moment, int rows, int np2, int dd,
+                                                      int nrhs, const double* __restrict__ mom,
+                                                      const double* __restrict__ T, const double* __restrict__ xs,
+                                                      int64_t ldxs, int64_t xseg, double* __restrict__ fw,
+                                                      const int64_t* __restrict__ lvl_off, double* __restrict__ bwz) {
+  extern __shared__ double lds[];
+  constexpr int S = C ? 2 : 1;
+  const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int D = bot - top;
+  // bottom level
+  {
+    const int n = (1 << (2 * D)) * rows;
+    double* L = lds + tree_lds_off(D, rows, S);
+    const int64_t base = lvl_off[bot] * nrhs;
+    for (int idx = tid; idx < n; idx += blockDim.x) {
+      const int cl = idx / rows, row = idx - (idx / rows) * rows;
+      const int64_t c = (static_cast<int64_t>(ctop) << (2 * D)) + cl;
+      const int64_t g = S * (base + (c * nrhs + k) * rows + row);
+      if (from_moment) {
+        const double* x = xs + S * (static_cast<int64_t>(k) * ldxs + xseg + c * dd);
+        double ar = 0.0, ai = 0.0;
+        for (int j = 0; j < dd; ++j) {
+          const double m = mom[static_cast<int64_t>(j) * rows + row];
+          ar += m * x[S * j];
+          if constexpr (C) ai += m * x[S * j + 1];
+        }
+        L[S * idx] = ar;
+        fw[g] = ar;
+        if constexpr (C) L[S * idx + 1] = ai, fw[g + 1] = ai;
+        if (bwz) {
+          bwz[g] = 0.0;
+          if constexpr (C) bwz[g + 1] = 0.0;
+        }
+      } else {
+        L[S * idx] = fw[g];
+        if constexpr (C) L[S * idx + 1] = fw[g + 1];
+      }
+    }
+  }
+  __syncthreads();
+  for (int d = D - 1; d >= 0; --d) {
+    const int n = (1 << (2 * d)) * rows;
+    const double* Ch = lds + tree_lds_off(d + 1, rows, S);
+    double* Pa = lds + tree_lds_off(d, rows, S);
+    const int64_t base = lvl_off[top + d] * nrhs;
+    for (int idx = tid; idx < n; idx += blockDim.x) {
+      const int cl = idx / rows, row = idx - (idx / rows) * rows;
+      const int comp = row / np2, i = row - comp * np2;
+      double ar = 0.0, ai = 0.0;
+      for (int q = 0; q < 4; ++q) {
+        const double* ch = Ch + S * ((4 * cl + q) * rows + comp * np2);
+        for (int j = 0; j < np2; ++j) {
+          const double t = T[static_cast<int64_t>(q * np2 + j) * np2 + i];
+          ar += t * ch[S * j];
+          if constexpr (C) ai += t * ch[S * j + 1];
+        }
+      }
+      Pa[S * idx] = ar;
+      const int64_t c = (static_cast<int64_t>(ctop) << (2 * d)) + cl;
+      const int64_t g = S * (base + (c * nrhs + k) * rows + row);
+      fw[g] = ar;
+      if constexpr (C) Pa[S * idx + 1] = ai, fw[g + 1] = ai;
+      if (bwz) {
+        bwz[g] = 0.0;
+        if constexpr (C) bwz[g + 1] = 0.0;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// backward transformation (H2Multipole.hpp:428-469): bw of level top read from global; children
+// get the coupling sums already in bw plus T^T of their parent, down to level bot; level bot is
+// written back, or, with to_leaf, turned into the leaf rows yf = Mom^T bw_bot
+template <bool C>
+__global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_leaf, int rows, int np2, int dd,
+                                                       int nrhs, const double* __restrict__ mom,
+                                                       const double* __restrict__ T, double* __restrict__ bw,
+                                                       const int64_t* __restrict__ lvl_off, double* __restrict__ yf,
+                                                       int64_t ldy, int64_t yseg) {
+  extern __shared__ double lds[];
+  constexpr int S = C ? 2 : 1;
+  const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int D = bot - top;
+  {
+    const int64_t g = S * (lvl_off[top] * nrhs + (static_cast<int64_t>(ctop) * nrhs + k) * rows);
+    for (int idx = tid; idx < rows; idx += blockDim.x) {
+      lds[S * idx] = bw[g + S * idx];
+      if constexpr (C) lds[S * idx + 1] = bw[g + S * idx + 1];
+    }
+  }
+  __syncthreads();
+  for (int d = 1; d <= D; ++d) {
+    const int n = (1 << (2 * d)) * rows;
+    const double* Pa = lds + tree_lds_off(d - 1, rows, S);
+    double* Ch = lds + tree_lds_off(d, rows, S);
+    const int64_t base = lvl_off[top + d] * nrhs;
+    for (int idx = tid; idx < n; idx += blockDim.x) {
+      const int ch = idx / rows, row = idx - (idx / rows) * rows;
+      const int comp = row / np2, j = row - comp * np2;
+      const int pc = ch >> 2, q = ch & 3;
+      const int64_t c = (static_cast<int64_t>(ctop) << (2 * d)) + ch;
+      const int64_t g = S * (base + (c * nrhs + k) * rows + row);
+      double ar = bw[g], ai = C ? bw[g + 1] : 0.0;
+      const double* pa = Pa + S * (pc * rows + comp * np2);
+      for (int i = 0; i < np2; ++i) {
+        const double t = T[static_cast<int64_t>(q * np2 + j) * np2 + i];
+        ar += t * pa[S * i];
+        if constexpr (C) ai += t * pa[S * i + 1];
+      }
+      Ch[S * idx] = ar;
+      if constexpr (C) Ch[S * idx + 1] = ai;
+      if (d == D && !to_leaf) {
+        bw[g] = ar;
+        if constexpr (C) bw[g + 1] = ai;
+      }
+    }
+    __syncthreads();
+  }
+  if (to_leaf) {
+    const double* Lv = lds + tree_lds_off(D, rows, S);
+    const int n = (1 << (2 * D)) * dd;
+    for (int idx = tid; idx < n; idx += blockDim.x) {
+      const int cl = idx / dd, i = idx - (idx / dd) * dd;
+      const double* m = mom + static_cast<int64_t>(i) * rows;
+      const double* b = Lv + S * cl * rows;
+      double ar = 0.0, ai = 0.0;
+      for (int q = 0; q < rows; ++q) {
+        ar += m[q] * b[S * q];
+        if constexpr (C) ai += m[q] * b[S * q + 1];
+      }
+      const int64_t c = (static_cast<int64_t>(ctop) << (2 * D)) + cl;
+      double* y = yf + S * (static_cast<int64_t>(k) * ldy + yseg + c * dd + i);
+      y[0] = ar;
+      if constexpr (C) y[1] = ai;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ coupling apply
@@ -356,69 +563,80 @@ __global__ void k_leaf(int ncl, int nrhs, int dd, const int* __restrict__ rp, co
 // whole block in flight, and reads the source segment of the input, which is wave-uniform, via
 // scalar loads.  RT right-hand sides share each block load.
 
-// near field: ys[seg + c*DD + i, k] = sum_blocks F (DD x DD) xs[c2*DD + .., k] + Mom^T bw_L[c][k]
-template <bool C, int DD, int RT>
-__global__ __launch_bounds__(256) void k_leaf_w(int ncl, int nrhs, const int* __restrict__ rp,
-                                                const int* __restrict__ src, const double* __restrict__ F,
-                                                int64_t fstride, int64_t fcomp, const double* __restrict__ xs,
-                                                int64_t ldxs, int64_t xseg, const double* __restrict__ mom, int rows,
-                                                const double* __restrict__ bw, double* __restrict__ ys, int64_t ldys,
-                                                int64_t yseg, int accumulate) {
+// near field: ys[seg + c*DD + i, k] (+)= sum_blocks F (DD x DD) xs[c2*DD + .., k].  UB blocks
+// are loaded per iteration (their products are still added in block order, so every UB gives the
+// same bits).
+struct LeafArgs {
+  int ncl, nrhs;
+  const int* rp;
+  const int* src;
+  const double* F;
+  int64_t fstride, fcomp;
+  const double* xs;
+  int64_t ldxs, xseg;
+  double* ys;
+  int64_t ldys, yseg;
+  int accumulate;
+};
+
+template <bool C, int DD, int RT, int UB>
+__device__ __forceinline__ void leaf_wave(int c, const LeafArgs& a) {
   constexpr int S = C ? 2 : 1;
-  const int c = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6));
-  if (c >= ncl) return;
   const int lane = threadIdx.x & 63;
   const int i = lane < DD ? lane : 0;  // spare lanes shadow row 0 and store nothing
-  const int p0 = rp[c], p1 = rp[c + 1];
-  for (int k0 = 0; k0 < nrhs; k0 += RT) {
+  const int p0 = a.rp[c], p1 = a.rp[c + 1];
+  const double* __restrict__ F = a.F;
+  const double* __restrict__ xs = a.xs;
+  for (int k0 = 0; k0 < a.nrhs; k0 += RT) {
     double ar[RT], ai[RT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) ar[r] = 0.0, ai[r] = 0.0;
-    for (int p = p0; p < p1; ++p) {
-      const int s2 = src[p];
-      const double* f = F + S * (static_cast<int64_t>(p) * fstride + fcomp);
-      double fr[DD], fi[C ? DD : 1];
+    for (int p = p0; p < p1; p += UB) {
+      const int nb = p1 - p < UB ? p1 - p : UB;
+      double fr[UB][DD], fi[UB][C ? DD : 1];
 #pragma unroll
-      for (int j = 0; j < DD; ++j) {
-        if constexpr (C) {
-          const double2 v = *reinterpret_cast<const double2*>(f + 2 * (j * DD + i));
-          fr[j] = v.x, fi[j] = v.y;
-        } else {
-          fr[j] = f[j * DD + i];
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        const int k = k0 + r;
-        if (k >= nrhs) break;
-        const double* x = xs + S * (static_cast<int64_t>(k) * ldxs + xseg + static_cast<int64_t>(s2) * DD);
+      for (int u = 0; u < UB; ++u) {
+        if (u >= nb) break;
+        const double* f = F + S * (static_cast<int64_t>(p + u) * a.fstride + a.fcomp);
 #pragma unroll
         for (int j = 0; j < DD; ++j) {
           if constexpr (C) {
-            const double xr = x[2 * j], xi = x[2 * j + 1];
-            ar[r] += fr[j] * xr - fi[j] * xi;
-            ai[r] += fr[j] * xi + fi[j] * xr;
+            const double2 v = *reinterpret_cast<const double2*>(f + 2 * (j * DD + i));
+            fr[u][j] = v.x, fi[u][j] = v.y;
           } else {
-            ar[r] += fr[j] * x[j];
+            fr[u][j] = f[j * DD + i];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        if (u >= nb) break;
+        const int s2 = a.src[p + u];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+          const int k = k0 + r;
+          if (k >= a.nrhs) break;
+          const double* x = xs + S * (static_cast<int64_t>(k) * a.ldxs + a.xseg + static_cast<int64_t>(s2) * DD);
+#pragma unroll
+          for (int j = 0; j < DD; ++j) {
+            if constexpr (C) {
+              const double xr = x[2 * j], xi = x[2 * j + 1];
+              ar[r] += fr[u][j] * xr - fi[u][j] * xi;
+              ai[r] += fr[u][j] * xi + fi[u][j] * xr;
+            } else {
+              ar[r] += fr[u][j] * x[j];
+            }
           }
         }
       }
     }
+    if (lane < DD) {
 #pragma unroll
-    for (int r = 0; r < RT; ++r) {
-      const int k = k0 + r;
-      if (k >= nrhs) break;
-      if (bw) {
-        const double* b = bw + S * ((static_cast<int64_t>(c) * nrhs + k) * rows);
-        const double* m = mom + static_cast<int64_t>(i) * rows;
-        for (int q = 0; q < rows; ++q) {
-          ar[r] += m[q] * b[S * q];
-          if constexpr (C) ai[r] += m[q] * b[S * q + 1];
-        }
-      }
-      if (lane < DD) {
-        double* y = ys + S * (static_cast<int64_t>(k) * ldys + yseg + static_cast<int64_t>(c) * DD + lane);
-        if (accumulate) {
+      for (int r = 0; r < RT; ++r) {
+        const int k = k0 + r;
+        if (k >= a.nrhs) break;
+        double* y = a.ys + S * (static_cast<int64_t>(k) * a.ldys + a.yseg + static_cast<int64_t>(c) * DD + lane);
+        if (a.accumulate) {
           ar[r] += y[0];
           if constexpr (C) ai[r] += y[1];
         }
@@ -429,53 +647,98 @@ __global__ __launch_bounds__(256) void k_leaf_w(int ncl, int nrhs, const int* __
   }
 }
 
+template <bool C, int DD, int RT, int UB>
+__global__ __launch_bounds__(256) void k_leaf_w(LeafArgs a) {
+  const int c = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6));
+  if (c >= a.ncl) return;
+  leaf_wave<C, DD, RT, UB>(c, a);
+}
+
+// leaf rows of the backward transformation: yf[seg + c*dd + i, k] = sum_q Mom(q, i) bw_L[c][k][q]
+template <bool C>
+__global__ void k_leaf_backward(int ncl, int nrhs, int dd, const double* __restrict__ mom, int rows,
+                                const double* __restrict__ bw, double* __restrict__ yf, int64_t ldy, int64_t yseg) {
+  constexpr int S = C ? 2 : 1;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(ncl) * nrhs * dd) return;
+  const int i = static_cast<int>(gid % dd);
+  const int64_t ck = gid / dd;
+  const int c = static_cast<int>(ck % ncl), k = static_cast<int>(ck / ncl);
+  const double* b = bw + S * ((static_cast<int64_t>(c) * nrhs + k) * rows);
+  const double* m = mom + static_cast<int64_t>(i) * rows;
+  double ar = 0.0, ai = 0.0;
+  for (int q = 0; q < rows; ++q) {
+    ar += m[q] * b[S * q];
+    if constexpr (C) ai += m[q] * b[S * q + 1];
+  }
+  double* y = yf + S * (static_cast<int64_t>(k) * ldy + yseg + static_cast<int64_t>(c) * dd + i);
+  y[0] = ar;
+  if constexpr (C) y[1] = ai;
+}
+
 // coupling: bw[t][k][i] (+)= sum over blocks (t, s) of F (CD x CD) fw[s][k][:], all levels in one
 // launch.  A wave owns one target; its 64 lanes form G = 64/CDP groups that take every G-th block
 // and are summed with cross-lane shuffles at the end.
-template <bool C, typename FT, int CD, int RT>
-__global__ __launch_bounds__(256) void k_coupling_w(int ntgt, int nrhs, const int2* __restrict__ tgt,
-                                                    const int* __restrict__ rp, const int* __restrict__ src,
-                                                    const FT* __restrict__ F, int64_t fstride, int64_t fcomp,
-                                                    const int64_t* __restrict__ lvl_off,
-                                                    const double* __restrict__ fw, double* __restrict__ bw,
-                                                    int accumulate) {
+struct CplArgs {
+  int ntgt, nrhs;
+  const int2* tgt;
+  const int* rp;
+  const int* src;
+  const void* F;
+  int64_t fstride, fcomp;
+  const int64_t* lvl_off;
+  const double* fw;
+  double* bw;
+  int accumulate;
+};
+
+template <bool C, typename FT, int CD, int RT, int UB>
+__device__ __forceinline__ void coupling_wave(int t, const CplArgs& a) {
   constexpr int S = C ? 2 : 1;
   constexpr int CDP = CD <= 16 ? 16 : (CD <= 32 ? 32 : 64);
   constexpr int G = 64 / CDP;
-  const int t = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6));
-  if (t >= ntgt) return;
   const int lane = threadIdx.x & 63;
   const int g = lane / CDP, li = lane % CDP;
   const int i = li < CD ? li : 0;
-  const int2 T = tgt[t];
-  const int64_t off = lvl_off[T.x] * nrhs;
-  const int p0 = rp[t], p1 = rp[t + 1];
-  for (int k0 = 0; k0 < nrhs; k0 += RT) {
+  const int2 T = a.tgt[t];
+  const int64_t off = a.lvl_off[T.x] * a.nrhs;
+  const int p0 = a.rp[t], p1 = a.rp[t + 1];
+  const FT* __restrict__ F = static_cast<const FT*>(a.F);
+  const double* __restrict__ fw = a.fw;
+  for (int k0 = 0; k0 < a.nrhs; k0 += RT) {
     double ar[RT], ai[RT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) ar[r] = 0.0, ai[r] = 0.0;
-    for (int p = p0 + g; p < p1; p += G) {
-      const int s2 = src[p];
-      const FT* f = F + S * (static_cast<int64_t>(p) * fstride + fcomp);
-      double fr[CD], fi[C ? CD : 1];
+    for (int p = p0 + g; p < p1; p += G * UB) {
+      double fr[UB][CD], fi[UB][C ? CD : 1];
 #pragma unroll
-      for (int j = 0; j < CD; ++j) {
-        fr[j] = static_cast<double>(f[S * (j * CD + i)]);
-        if constexpr (C) fi[j] = static_cast<double>(f[S * (j * CD + i) + 1]);
-      }
-#pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        const int k = k0 + r;
-        if (k >= nrhs) break;
-        const double* x = fw + S * (off + (static_cast<int64_t>(s2) * nrhs + k) * CD);
+      for (int u = 0; u < UB; ++u) {
+        if (p + G * u >= p1) break;
+        const FT* f = F + S * (static_cast<int64_t>(p + G * u) * a.fstride + a.fcomp);
 #pragma unroll
         for (int j = 0; j < CD; ++j) {
-          if constexpr (C) {
-            const double xr = x[2 * j], xi = x[2 * j + 1];
-            ar[r] += fr[j] * xr - fi[j] * xi;
-            ai[r] += fr[j] * xi + fi[j] * xr;
-          } else {
-            ar[r] += fr[j] * x[j];
+          fr[u][j] = static_cast<double>(f[S * (j * CD + i)]);
+          if constexpr (C) fi[u][j] = static_cast<double>(f[S * (j * CD + i) + 1]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        if (p + G * u >= p1) break;
+        const int s2 = a.src[p + G * u];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+          const int k = k0 + r;
+          if (k >= a.nrhs) break;
+          const double* x = fw + S * (off + (static_cast<int64_t>(s2) * a.nrhs + k) * CD);
+#pragma unroll
+          for (int j = 0; j < CD; ++j) {
+            if constexpr (C) {
+              const double xr = x[2 * j], xi = x[2 * j + 1];
+              ar[r] += fr[u][j] * xr - fi[u][j] * xi;
+              ai[r] += fr[u][j] * xi + fi[u][j] * xr;
+            } else {
+              ar[r] += fr[u][j] * x[j];
+            }
           }
         }
       }
@@ -492,9 +755,9 @@ __global__ __launch_bounds__(256) void k_coupling_w(int ntgt, int nrhs, const in
 #pragma unroll
       for (int r = 0; r < RT; ++r) {
         const int k = k0 + r;
-        if (k >= nrhs) break;
-        double* y = bw + S * (off + (static_cast<int64_t>(T.y) * nrhs + k) * CD + li);
-        if (accumulate) {
+        if (k >= a.nrhs) break;
+        double* y = a.bw + S * (off + (static_cast<int64_t>(T.y) * a.nrhs + k) * CD + li);
+        if (a.accumulate) {
           ar[r] += y[0];
           if constexpr (C) ai[r] += y[1];
         }
@@ -505,21 +768,55 @@ __global__ __launch_bounds__(256) void k_coupling_w(int ntgt, int nrhs, const in
   }
 }
 
+template <bool C, typename FT, int CD, int RT, int UB>
+__global__ __launch_bounds__(256) void k_coupling_w(CplArgs a) {
+  const int t = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6));
+  if (t >= a.ntgt) return;
+  coupling_wave<C, FT, CD, RT, UB>(t, a);
+}
+
+// the two sweeps of a single-RHS product in one launch: workgroups [0, nb_near) run the near
+// field, the rest the coupling blocks, so both streams of leaves share the memory system
+template <bool C, int DD, typename FT, int CD, int UBN, int UBC>
+__global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, int nb_near) {
+  const int b = static_cast<int>(blockIdx.x);
+  const int w = static_cast<int>(threadIdx.x >> 6);
+  if (b < nb_near) {
+    const int c = __builtin_amdgcn_readfirstlane(b * 4 + w);
+    if (c < la.ncl) leaf_wave<C, DD, 1, UBN>(c, la);
+  } else {
+    const int t = __builtin_amdgcn_readfirstlane((b - nb_near) * 4 + w);
+    if (t < ca.ntgt) coupling_wave<C, FT, CD, 1, UBC>(t, ca);
+  }
+}
+
 // ------------------------------------------------------------------ host launchers
 static inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
 
 void launch_spmv(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* x,
-                 int64_t ldx, double* y, int64_t ldy, const double* alpha, const double* beta, hipStream_t st) {
+                 int64_t ldx, double* y, int64_t ldy, const double* alpha, const double* beta, hipStream_t st,
+                 const double* x2, bool long_rows) {
   const int64_t n = static_cast<int64_t>(nrows) * nrhs;
   if (n == 0) return;
   const double ar = alpha ? alpha[0] : 1.0, ai = alpha ? alpha[1] : 0.0;
   const double br = beta ? beta[0] : 0.0, bi = beta ? beta[1] : 0.0;
   const int ub = (beta && (br != 0.0 || bi != 0.0)) ? 1 : 0;
+  if (long_rows) {
+    const int64_t nt = n * 16;
+    if (cplx)
+      hipLaunchKernelGGL((k_spmv_sub<true, 16>), dim3(nblk(nt)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, x, x2, ldx,
+                         y, ldy, ar, ai, br, bi, ub);
+    else
+      hipLaunchKernelGGL((k_spmv_sub<false, 16>), dim3(nblk(nt)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, x, x2, ldx,
+                         y, ldy, ar, ai, br, bi, ub);
+    BMI_CHECK_HIP(hipGetLastError());
+    return;
+  }
   if (cplx)
-    hipLaunchKernelGGL(k_spmv<true>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, x, ldx, y, ldy,
+    hipLaunchKernelGGL(k_spmv<true>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, x, x2, ldx, y, ldy,
                        ar, ai, br, bi, ub);
   else
-    hipLaunchKernelGGL(k_spmv<false>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, x, ldx, y, ldy,
+    hipLaunchKernelGGL(k_spmv<false>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, x, x2, ldx, y, ldy,
                        ar, ai, br, bi, ub);
   BMI_CHECK_HIP(hipGetLastError());
 }
@@ -544,9 +841,9 @@ void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T
   BMI_CHECK_HIP(hipGetLastError());
 }
 
-void launch_coupling_apply(bool cplx, bool fp32, int ntgt, int nrhs, int cd, const int2* tgt, const int* rp,
-                           const int* src, const void* F, int64_t fstride, int64_t fcomp, const int64_t* lvl_off,
-                           const double* fw, double* bw, int accumulate, hipStream_t st) {
+void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs, int cd, const int2* tgt,
+                           const int* rp, const int* src, const void* F, int64_t fstride, int64_t fcomp,
+                           const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st) {
   const int64_t n = static_cast<int64_t>(ntgt) * nrhs * cd;
   if (n == 0) return;
   TimedScope ts("coupling_apply", st);
@@ -554,12 +851,13 @@ void launch_coupling_apply(bool cplx, bool fp32, int ntgt, int nrhs, int cd, con
   // wave-per-target kernels for the block sizes of the configurations (np 4, 5; Maxwell np 4)
 #define BMI_CPL_W(CC, FTT, CDD)                                                                              \
   if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                                  \
-    if (nrhs == 1)                                                                                            \
-      hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, 1>), wgrid, dim3(256), 0, st, ntgt, nrhs, tgt, rp, src,   \
-                         static_cast<const FTT*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);             \
+    const CplArgs ca{ntgt, nrhs, tgt, rp, src, F, fstride, fcomp, lvl_off, fw, bw, accumulate};               \
+    if (nrhs == 1 && variant == 1)                                                                            \
+      hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, 1, 2>), wgrid, dim3(256), 0, st, ca);                    \
+    else if (nrhs == 1)                                                                                       \
+      hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, 1, 1>), wgrid, dim3(256), 0, st, ca);                    \
     else                                                                                                      \
-      hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, 4>), wgrid, dim3(256), 0, st, ntgt, nrhs, tgt, rp, src,   \
-                         static_cast<const FTT*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);             \
+      hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, (CC ? 2 : 4), 1>), wgrid, dim3(256), 0, st, ca);         \
     BMI_CHECK_HIP(hipGetLastError());                                                                         \
     return;                                                                                                   \
   }
@@ -589,6 +887,80 @@ void launch_coupling_apply(bool cplx, bool fp32, int ntgt, int nrhs, int cd, con
   BMI_CHECK_HIP(hipGetLastError());
 }
 
+// deepest subtree (at most three levels below its root) whose level buffers fit in 64 KB of LDS
+static int tree_max_depth(int doubles_per_cluster) {
+  int d = 3;
+  while (d > 0 && sizeof(double) * doubles_per_cluster * (((1 << (2 * (d + 1))) - 1) / 3) > 64 * 1024) --d;
+  return d;
+}
+
+// whole forward transformation: leaf moments at level L, then every level up to 0, in subtree
+// launches of at most three levels
+void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
+                    const double* T, const double* xs, int64_t ldxs, int64_t xseg, double* fw, const int64_t* lvl_off,
+                    hipStream_t st, double* bw_zero) {
+  const int S = cplx ? 2 : 1;
+  // the bottom segment spans up to two levels (many workgroups for the moments), the others up
+  // to three, as far as the level buffers fit in 64 KB of LDS
+  const int maxd = tree_max_depth(rows * S);
+  BMI_REQUIRE(maxd >= 1, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
+  int bot = L;
+  bool first = true;
+  while (true) {
+    const int depth = first ? std::min(2, maxd) : maxd;
+    const int top = bot - depth > 0 ? bot - depth : 0;
+    const int D = bot - top;
+    const size_t lds = sizeof(double) * S * rows * (((1 << (2 * (D + 1))) - 1) / 3);
+    BMI_REQUIRE(lds <= 64 * 1024, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
+    const dim3 grid(static_cast<unsigned>(npatch << (2 * top)), static_cast<unsigned>(nrhs));
+    if (cplx)
+      hipLaunchKernelGGL(k_forward_tree<true>, grid, dim3(256), lds, st, top, bot, first ? 1 : 0, rows, np2, dd, nrhs,
+                         mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero);
+    else
+      hipLaunchKernelGGL(k_forward_tree<false>, grid, dim3(256), lds, st, top, bot, first ? 1 : 0, rows, np2, dd, nrhs,
+                         mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero);
+    BMI_CHECK_HIP(hipGetLastError());
+    first = false;
+    if (top == 0) break;
+    bot = top;
+  }
+}
+
+// whole backward transformation from level 0 to the leaf rows yf (segments of at most three
+// levels, the last one ending in the leaf rows)
+void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
+                     const double* T, double* bw, const int64_t* lvl_off, double* yf, int64_t ldy, int64_t yseg,
+                     hipStream_t st) {
+  const int S = cplx ? 2 : 1;
+  // segment boundaries from the bottom: L, L-3, ... down to 0, walked top-down
+  const int maxd = tree_max_depth(rows * S);
+  BMI_REQUIRE(maxd >= 1, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
+  std::vector<int> cuts;
+  for (int b = L; b > 0;) {
+    cuts.push_back(b);
+    const int depth = b == L ? std::min(2, maxd) : maxd;
+    b = b - depth > 0 ? b - depth : 0;
+  }
+  cuts.push_back(0);
+  std::reverse(cuts.begin(), cuts.end());
+  if (cuts.size() == 1) cuts.push_back(0);  // L = 0: one segment of depth 0 ending in the leaf rows
+  for (size_t k = 0; k + 1 < cuts.size(); ++k) {
+    const int top = cuts[k], bot = cuts[k + 1];
+    const int D = bot - top;
+    const int to_leaf = k + 2 == cuts.size() ? 1 : 0;
+    const size_t lds = sizeof(double) * S * rows * (((1 << (2 * (D + 1))) - 1) / 3);
+    BMI_REQUIRE(lds <= 64 * 1024, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
+    const dim3 grid(static_cast<unsigned>(npatch << (2 * top)), static_cast<unsigned>(nrhs));
+    if (cplx)
+      hipLaunchKernelGGL(k_backward_tree<true>, grid, dim3(256), lds, st, top, bot, to_leaf, rows, np2, dd, nrhs, mom,
+                         T, bw, lvl_off, yf, ldy, yseg);
+    else
+      hipLaunchKernelGGL(k_backward_tree<false>, grid, dim3(256), lds, st, top, bot, to_leaf, rows, np2, dd, nrhs, mom,
+                         T, bw, lvl_off, yf, ldy, yseg);
+    BMI_CHECK_HIP(hipGetLastError());
+  }
+}
+
 void launch_l2l(bool cplx, int nchild, int nrhs, int rows, int np2, const double* T, const double* parent,
                 double* child, hipStream_t st) {
   const int64_t n = static_cast<int64_t>(nchild) * nrhs * rows;
@@ -599,24 +971,24 @@ void launch_l2l(bool cplx, int nchild, int nrhs, int rows, int np2, const double
   BMI_CHECK_HIP(hipGetLastError());
 }
 
-void launch_leaf(bool cplx, int ncl, int nrhs, int dd, const int* rp, const int* src, const double* F,
-                 int64_t fstride, int64_t fcomp, const double* xs, int64_t ldxs, int64_t xseg, const double* mom,
-                 int rows, const double* bw, double* ys, int64_t ldys, int64_t yseg, int accumulate,
-                 hipStream_t st) {
+void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* rp, const int* src, const double* F,
+                 int64_t fstride, int64_t fcomp, const double* xs, int64_t ldxs, int64_t xseg, double* ys, int64_t ldys,
+                 int64_t yseg, int accumulate, hipStream_t st) {
   const int64_t n = static_cast<int64_t>(ncl) * nrhs * dd;
   TimedScope ts("matvec_nearfield", st);
   const dim3 wgrid(static_cast<unsigned>((ncl + 3) / 4));
   // wave-per-cluster kernels for the leaf sizes 4 (P+1)^2 of P = 1, 2
-#define BMI_LEAF_W(CC, DDD)                                                                                    \
-  if (cplx == CC && dd == DDD) {                                                                               \
-    if (nrhs == 1)                                                                                             \
-      hipLaunchKernelGGL((k_leaf_w<CC, DDD, 1>), wgrid, dim3(256), 0, st, ncl, nrhs, rp, src, F, fstride, fcomp, \
-                         xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);                            \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_leaf_w<CC, DDD, 4>), wgrid, dim3(256), 0, st, ncl, nrhs, rp, src, F, fstride, fcomp, \
-                         xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);                            \
-    BMI_CHECK_HIP(hipGetLastError());                                                                          \
-    return;                                                                                                    \
+#define BMI_LEAF_W(CC, DDD)                                                                                     \
+  if (cplx == CC && dd == DDD) {                                                                                \
+    const LeafArgs la{ncl, nrhs, rp, src, F, fstride, fcomp, xs, ldxs, xseg, ys, ldys, yseg, accumulate};       \
+    if (nrhs == 1 && variant == 1)                                                                              \
+      hipLaunchKernelGGL((k_leaf_w<CC, DDD, 1, 2>), wgrid, dim3(256), 0, st, la);                               \
+    else if (nrhs == 1)                                                                                         \
+      hipLaunchKernelGGL((k_leaf_w<CC, DDD, 1, 1>), wgrid, dim3(256), 0, st, la);                               \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_leaf_w<CC, DDD, 4, 1>), wgrid, dim3(256), 0, st, la);                               \
+    BMI_CHECK_HIP(hipGetLastError());                                                                           \
+    return;                                                                                                     \
   }
   BMI_LEAF_W(false, 16)
   BMI_LEAF_W(true, 16)
@@ -625,10 +997,55 @@ void launch_leaf(bool cplx, int ncl, int nrhs, int dd, const int* rp, const int*
 #undef BMI_LEAF_W
   if (cplx)
     hipLaunchKernelGGL(k_leaf<true>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, dd, rp, src, F, fstride, fcomp,
-                       xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);
+                       xs, ldxs, xseg, nullptr, 0, nullptr, ys, ldys, yseg, accumulate);
   else
     hipLaunchKernelGGL(k_leaf<false>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, dd, rp, src, F, fstride, fcomp,
-                       xs, ldxs, xseg, mom, rows, bw, ys, ldys, yseg, accumulate);
+                       xs, ldxs, xseg, nullptr, 0, nullptr, ys, ldys, yseg, accumulate);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
+                        const int* dsrc, const double* dF, const double* xs, double* ys, int ntgt, int cd,
+                        const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
+                        const double* fw, double* bw, hipStream_t st) {
+  const LeafArgs la{ncl, 1, drp, dsrc, dF, static_cast<int64_t>(dd) * dd, 0, xs, 0, 0, ys, 0, 0, 0};
+  const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(cd) * cd, 0, lvl_off, fw, bw, 0};
+  const int nbn = (ncl + 3) / 4, nbc = (ntgt + 3) / 4;
+  const dim3 grid(static_cast<unsigned>(nbn + nbc));
+  TimedScope ts("matvec_sweep", st);
+#define BMI_FUSED(CC, DDD, FTT, CDD, UN, UC)                                                                    \
+  if (cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&                        \
+      leaf_variant == UN - 1 && cpl_variant == UC - 1) {                                                        \
+    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC>), grid, dim3(256), 0, st, la, ca, nbn);        \
+    BMI_CHECK_HIP(hipGetLastError());                                                                           \
+    return true;                                                                                                \
+  }
+#define BMI_FUSED_V(CC, DDD, FTT, CDD) \
+  BMI_FUSED(CC, DDD, FTT, CDD, 1, 1) BMI_FUSED(CC, DDD, FTT, CDD, 1, 2) BMI_FUSED(CC, DDD, FTT, CDD, 2, 1) \
+  BMI_FUSED(CC, DDD, FTT, CDD, 2, 2)
+  BMI_FUSED_V(false, 36, double, 16)
+  BMI_FUSED_V(false, 36, float, 16)
+  BMI_FUSED_V(true, 36, double, 25)
+  BMI_FUSED_V(true, 36, float, 25)
+  BMI_FUSED_V(true, 36, double, 16)
+  BMI_FUSED_V(true, 36, float, 16)
+  BMI_FUSED_V(false, 16, double, 16)
+  BMI_FUSED_V(true, 16, double, 16)
+#undef BMI_FUSED_V
+#undef BMI_FUSED
+  return false;
+}
+
+void launch_leaf_backward(bool cplx, int ncl, int nrhs, int dd, const double* mom, int rows, const double* bw,
+                          double* yf, int64_t ldy, int64_t yseg, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(ncl) * nrhs * dd;
+  if (n == 0) return;
+  if (cplx)
+    hipLaunchKernelGGL(k_leaf_backward<true>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, dd, mom, rows, bw, yf, ldy,
+                       yseg);
+  else
+    hipLaunchKernelGGL(k_leaf_backward<false>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, dd, mom, rows, bw, yf, ldy,
+                       yseg);
   BMI_CHECK_HIP(hipGetLastError());
 }
 

@@ -168,17 +168,21 @@ def main():
     for _ in range(max(3, a.steps // 4)):
         H.matvec(x)
     B.timing.enable(False)
-    nf_ms, nf_n = B.timing.get("matvec_nearfield")
-    cp_ms, cp_n = B.timing.get("coupling_apply")
-    nf_avg = nf_ms / max(nf_n, 1)
+    sw_ms, sw_n = B.timing.get("matvec_sweep")
+    fw_ms, fw_n = B.timing.get("matvec_forward")
+    bw_ms, bw_n = B.timing.get("matvec_backward")
+    sw_avg = sw_ms / max(sw_n, 1)
     dd = info["dense_dim"]
     nsuper = space.number_of_super_dofs
     rows = info["coupling_dim"]
-    ncl_leaf = space.number_of_elements // 4
-    nf_bytes = info["dense"] * dd * dd * 8 + 2 * nsuper * 8 + ncl_leaf * rows * 8 + rows * dd * 8
-    achieved = nf_bytes / (nf_avg * 1e-3) / 1e9
-    log("near-field sweep %.3f ms/launch, %.1f MB -> %.0f GB/s; coupling %.3f ms/launch" % (
-        nf_avg, nf_bytes / 1e6, achieved, cp_ms / max(cp_n, 1)))
+    L = M - 1
+    lvl_total = 6 * (4 ** (L + 1) - 1) // 3 * rows
+    # algorithmic bytes of the fused sweep: every stored leaf once, x read and y written once,
+    # the multipole vectors read and the local expansions written once
+    sw_bytes = (info["dense"] * dd * dd + info["lowrank"] * rows * rows) * 8 + 2 * nsuper * 8 + 2 * lvl_total * 8
+    achieved = sw_bytes / (sw_avg * 1e-3) / 1e9 if sw_avg > 0 else 0.0
+    log("sweep (near field + coupling) %.3f ms/launch, %.1f MB -> %.0f GB/s; forward %.3f ms, backward %.3f ms" % (
+        sw_avg, sw_bytes / 1e6, achieved, fw_ms / max(fw_n, 1), bw_ms / max(bw_n, 1)))
 
     # near-field quadrature roofline (FP64)
     pairs_by_class = info["pair_classes"]
@@ -264,8 +268,8 @@ def main():
                   "mixed_rel_err_vs_fp64": rel_err_mixed, "solves": a.solves},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_leaf (near-field leaf sweep of the matvec)", "bytes_per_launch": nf_bytes,
-                     "avg_launch_ms": nf_avg},
+                     "kernel": "k_sweep_fused (near-field leaves + coupling blocks of the matvec)",
+                     "bytes_per_launch": sw_bytes, "avg_launch_ms": sw_avg},
         "roofline_assembly": {"bound": "fp64", "achieved": duffy_tflops, "peak": FP64_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": duffy_tflops / FP64_PEAK_TFLOPS,
                               "kernel": "k_nearfield identical/edge/vertex (Duffy)", "flops": fl},

@@ -309,6 +309,20 @@ class H2:
         b = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1)
         return lib().or_time_assembly_sample(self.h, len(p) // 2, _ip(p), len(b), _ip(b), threads)
 
+    def matvec_super(self, xs):
+        """super-space product (no T, T^T), xs of shape (nsuper, nrhs)"""
+        X = np.asarray(xs).reshape(xs.shape[0], -1)
+        nrhs, N = X.shape[1], self.space.nsuper
+        if self.cplx:
+            xin = np.ascontiguousarray(X.T.astype(np.complex128)).view(np.float64)
+            out = np.zeros(2 * N * nrhs)
+            lib().or_h2_matvec_super(self.h, nrhs, _dp(xin), _dp(out))
+            return out.view(np.complex128).reshape(nrhs, N).T
+        xin = np.ascontiguousarray(X.T.astype(np.float64))
+        out = np.zeros(N * nrhs)
+        lib().or_h2_matvec_super(self.h, nrhs, _dp(xin), _dp(out))
+        return out.reshape(nrhs, N).T
+
     def matvec(self, x):
         x = np.asarray(x)
         X = x.reshape(x.shape[0], -1)

@@ -143,3 +143,19 @@ def test_gpu_h2_graph_replays_match_eager(gpu):
             torch.cuda.synchronize()
             assert torch.equal(y, ye), rep
             del junk
+
+
+@pytest.mark.parametrize("kind,kappa,M,P,np_", [CASES[0], CASES[1]])
+def test_gpu_h2_super_space_product(gpu, kind, kappa, M, P, np_):
+    """A_super xs without T (bmi_h2_matvec_super) equals the oracle's super-space product."""
+    import torch
+    os_, ps, op, H = make(kind, kappa, M, P, np_)
+    oh = O.H2(os_, kind, kappa, np_)
+    N = ps.number_of_super_dofs
+    rng = np.random.default_rng(4)
+    xs = rng.uniform(-1, 1, (N, 2))
+    if kind != O.LAPLACE:
+        xs = xs + 1j * rng.uniform(-1, 1, (N, 2))
+    dt = torch.complex128 if kind != O.LAPLACE else torch.float64
+    ys = H.matvec_super(torch.tensor(xs, dtype=dt, device="cuda")).cpu().numpy()
+    assert rel(ys, oh.matvec_super(xs)) <= TOL
