@@ -21,7 +21,7 @@ __all__ = [
     "BembelError", "Geometry", "AnsatzSpace", "LaplaceSingleLayerOperator", "HelmholtzSingleLayerOperator",
     "MaxwellSingleLayerOperator", "DummyOperator", "DiscreteOperator", "H2Matrix", "DenseMatrix",
     "DiscreteLinearForm", "DirichletTrace", "PlaneWaves", "RotatedTangentialTrace", "GMRES", "ConjugateGradient", "library", "timing",
-    "SPHERE_DAT", "FP64", "MIXED",
+    "SPHERE_DAT", "FP64", "MIXED", "MIXED_BF16",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -30,7 +30,7 @@ SPHERE_DAT = os.path.join(HERE, "geo", "sphere.dat")
 
 OK, INVALID_ARG, OUT_OF_MEMORY, DEVICE_ERROR, NOT_CONVERGED, ADMISSION_REJECTED, IO_ERROR, UNSUPPORTED = range(8)
 LAPLACE, HELMHOLTZ, MAXWELL, DUMMY = 0, 1, 2, 3
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_BF16 = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
 
 

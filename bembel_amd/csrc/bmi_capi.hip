@@ -104,6 +104,8 @@ struct bmi_h2 {
   // leaf storage
   DevBuf<double> d_dense, d_coup;
   DevBuf<float> d_coup32;
+  DevBuf<unsigned short> d_coup16;  // BF16 copies of the coarsest coupling blocks (MIXED_BF16)
+  int ncoarse = 0;
   // dense CSR over leaf-level clusters
   int ncl_leaf = 0;
   DevBuf<int> d_drp, d_dsrc;
@@ -123,7 +125,7 @@ struct bmi_h2 {
   // until the workspace grows, which drops the graphs)
   bool use_graphs = true;
   hipStream_t cap_stream = nullptr;
-  std::map<int, hipGraphExec_t> graphs;
+  std::map<int, hipGraphExec_t> graphs;  // key: 2 * nrhs + reduced
   // the far-field chain runs on a side stream, concurrently with the near-field sweep, and
   // writes its leaf rows into d_yf; the final T^T product adds both
   hipStream_t side_stream = nullptr;
@@ -155,6 +157,19 @@ namespace bmi {
 
 static int ncl_at(const bmi_space* s, int l) { return s->npatch << (2 * l); }
 
+// coupling blocks of the FP64 operator, or of the reduced-precision one of a mixed handle
+struct CouplingOperand {
+  bool fp32;
+  const void* F;
+  const unsigned short* Fc;
+  int ncoarse;
+};
+static CouplingOperand coupling_operand(const bmi_h2* h, bool reduced) {
+  if (reduced && h->op.precision != BMI_PRECISION_FP64)
+    return {true, h->d_coup32.ptr, h->ncoarse > 0 ? h->d_coup16.ptr : nullptr, h->ncoarse};
+  return {false, h->d_coup.ptr, nullptr, 0};
+}
+
 static void ensure_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
   (void)st;
   if (nrhs <= h->ws_rhs) return;
@@ -172,15 +187,14 @@ static void ensure_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
 static bool has_far(const bmi_h2* h) { return h->L >= 0 && h->lvl_total > 0; }
 
 // far-field chain: forward transformation, coupling, backward transformation, leaf rows -> yf
-static void apply_far(bmi_h2* h, int nrhs, const double* xs, double* yf, hipStream_t st) {
+static void apply_far(bmi_h2* h, int nrhs, const double* xs, double* yf, bool reduced, hipStream_t st) {
   const bmi_space* s = h->s;
   const bool C = h->cplx;
   const int S = C ? 2 : 1;
   const int64_t N = s->nsuper;
   const int rows = h->cd;
   const int64_t vcs = static_cast<int64_t>(s->nel) * s->I2;  // super DOFs per component
-  const bool fp32 = h->op.precision == BMI_PRECISION_MIXED;
-  const void* coupF = fp32 ? static_cast<const void*>(h->d_coup32.ptr) : static_cast<const void*>(h->d_coup.ptr);
+  const CouplingOperand co = coupling_operand(h, reduced);
   const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
   for (int cc = 0; cc < h->vd; ++cc) {
     TimedScope ts("matvec_forward", st);
@@ -190,9 +204,9 @@ static void apply_far(bmi_h2* h, int nrhs, const double* xs, double* yf, hipStre
   for (int rc = 0; rc < h->vd; ++rc) {
     device_fill(h->d_bw.ptr, S * h->lvl_total * nrhs, 0.0, st);
     for (int cc = 0; cc < h->vd; ++cc)
-      launch_coupling_apply(C, fp32, std::max(h->cpl_variant, 0), h->ntgt, nrhs, rows, h->d_tgt.ptr, h->d_crp.ptr,
-                            h->d_csrc.ptr, coupF, h->vd * h->vd * cblk, (rc * h->vd + cc) * cblk, h->d_lvl_off.ptr,
-                            h->d_fw[cc].ptr, h->d_bw.ptr, 1, st);
+      launch_coupling_apply(C, co.fp32, std::max(h->cpl_variant, 0), h->ntgt, nrhs, rows, h->d_tgt.ptr,
+                            h->d_crp.ptr, h->d_csrc.ptr, co.F, h->vd * h->vd * cblk, (rc * h->vd + cc) * cblk,
+                            h->d_lvl_off.ptr, h->d_fw[cc].ptr, h->d_bw.ptr, 1, st, co.Fc, co.ncoarse);
     TimedScope ts("matvec_backward", st);
     launch_backward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[rc].ptr, h->d_T.ptr, h->d_bw.ptr,
                     h->d_lvl_off.ptr, yf, N, rc * vcs, st);
@@ -214,7 +228,7 @@ static void apply_near(bmi_h2* h, int nrhs, const double* xs, double* ys, hipStr
 
 // time both variants of the two sweeps once (1 RHS) and keep the faster; the variants give
 // identical bits, so the choice affects speed only
-static void autotune(bmi_h2* h, const double* xs, hipStream_t st) {
+static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) {
   if (h->leaf_variant >= 0 && h->cpl_variant >= 0) return;
   hipEvent_t a, b;
   BMI_CHECK_HIP(hipEventCreate(&a));
@@ -239,15 +253,17 @@ static void autotune(bmi_h2* h, const double* xs, hipStream_t st) {
   h->leaf_variant = pick;
   if (has_far(h)) {
     const int S = h->cplx ? 2 : 1;
-    const bool fp32 = h->op.precision == BMI_PRECISION_MIXED;
-    const void* coupF = fp32 ? static_cast<const void*>(h->d_coup32.ptr) : static_cast<const void*>(h->d_coup.ptr);
+    const CouplingOperand co = coupling_operand(h, reduced);
+    const bool fp32 = co.fp32;
+    const void* coupF = co.F;
     const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
     device_fill(h->d_fw[0].ptr, S * h->lvl_total, 0.0, st);
     pick = 0;
     for (int v = 0; v < 2; ++v) {
       const float t = time_it([&] {
         launch_coupling_apply(h->cplx, fp32, v, h->ntgt, 1, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, coupF,
-                              h->vd * h->vd * cblk, 0, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, 0, st);
+                              h->vd * h->vd * cblk, 0, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, 0, st, co.Fc,
+                              co.ncoarse);
       });
       if (v == 0 || t < best) best = t, pick = v;
     }
@@ -261,7 +277,8 @@ static void autotune(bmi_h2* h, const double* xs, hipStream_t st) {
           const float t = time_it([&] {
             ok = launch_sweep_fused(h->cplx, fp32, vl, vc, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr,
                                     h->d_dense.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
-                                    h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st);
+                                    h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
+                                    co.ncoarse);
           });
           if (ok && (pl < 0 || t < best)) best = t, pl = vl, pc = vc;
         }
@@ -277,22 +294,23 @@ static void autotune(bmi_h2* h, const double* xs, hipStream_t st) {
 
 // super-space apply: ys = A_near xs on `st`, yf = A_far xs on the side stream (joined back into
 // `st` before returning); the caller adds the two
-static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, double* yf, hipStream_t st) {
+static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, double* yf, bool reduced,
+                        hipStream_t st) {
   const bool far = has_far(h);
   if (far && nrhs == 1 && h->vd == 1 && h->fused_leaf_variant >= 0 && h->fused_cpl_variant >= 0) {
     // one RHS: forward transformation (also clears bw), both sweeps in one launch, backward
     const bmi_space* s = h->s;
-    const bool fp32 = h->op.precision == BMI_PRECISION_MIXED;
-    const void* coupF = fp32 ? static_cast<const void*>(h->d_coup32.ptr) : static_cast<const void*>(h->d_coup.ptr);
+    const CouplingOperand co = coupling_operand(h, reduced);
     const int64_t N = s->nsuper;
     {
       TimedScope ts("matvec_forward", st);
       launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
                      h->d_fw[0].ptr, h->d_lvl_off.ptr, st, h->d_bw.ptr);
     }
-    if (launch_sweep_fused(h->cplx, fp32, h->fused_leaf_variant, h->fused_cpl_variant, h->ncl_leaf, h->dd, h->d_drp.ptr,
-                           h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
-                           h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st)) {
+    if (launch_sweep_fused(h->cplx, co.fp32, h->fused_leaf_variant, h->fused_cpl_variant, h->ncl_leaf, h->dd,
+                           h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr,
+                           h->d_crp.ptr, h->d_csrc.ptr, co.F, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
+                           co.ncoarse)) {
       TimedScope ts("matvec_backward", st);
       launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
                       h->d_lvl_off.ptr, yf, N, 0, st);
@@ -305,7 +323,7 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
     if (!h->ev_join) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
     BMI_CHECK_HIP(hipEventRecord(h->ev_fork, st));
     BMI_CHECK_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
-    apply_far(h, nrhs, xs, yf, h->side_stream);
+    apply_far(h, nrhs, xs, yf, reduced, h->side_stream);
     BMI_CHECK_HIP(hipEventRecord(h->ev_join, h->side_stream));
   }
   apply_near(h, nrhs, xs, ys, st);
@@ -315,19 +333,20 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
 // ys = A_super xs on the workspace, through a captured graph when possible.  The graph holds
 // only kernel nodes (buffers are cleared by a fill kernel, not a memset node); in-library
 // timing needs host synchronisation per kernel and therefore runs eagerly.
-static void apply_super_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
-  autotune(h, h->d_xs.ptr, st);
+static void apply_super_workspace(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
+  autotune(h, h->d_xs.ptr, reduced, st);
   if (!h->use_graphs || timing_enabled()) {
-    apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, st);
+    apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, reduced, st);
     return;
   }
-  auto it = h->graphs.find(nrhs);
+  const int key = 2 * nrhs + (reduced ? 1 : 0);
+  auto it = h->graphs.find(key);
   if (it == h->graphs.end()) {
     if (!h->cap_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
     hipGraph_t g = nullptr;
     BMI_CHECK_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
     try {
-      apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, h->cap_stream);
+      apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, reduced, h->cap_stream);
     } catch (...) {
       (void)hipStreamEndCapture(h->cap_stream, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -338,19 +357,19 @@ static void apply_super_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
     const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     BMI_CHECK_HIP(e);
-    it = h->graphs.emplace(nrhs, ex).first;
+    it = h->graphs.emplace(key, ex).first;
   }
   BMI_CHECK_HIP(hipGraphLaunch(it->second, st));
 }
 
 void h2_apply_device(bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy, const double* alpha,
-                     const double* beta, hipStream_t st) {
+                     const double* beta, hipStream_t st, bool reduced) {
   const bmi_space* s = h->s;
   ensure_workspace(h, nrhs, st);
   const int64_t N = s->nsuper;
   launch_spmv(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, h->d_xs.ptr, N, nullptr,
               nullptr, st);
-  apply_super_workspace(h, nrhs, st);
+  apply_super_workspace(h, nrhs, reduced, st);
   launch_spmv(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, h->d_ys.ptr, N, Y, ldy, alpha,
               beta, st, has_far(h) ? h->d_yf.ptr : nullptr, true);
 }
@@ -358,6 +377,7 @@ void h2_apply_device(bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* 
 int64_t h2_rows(const bmi_h2* h) { return h->s->ndof; }
 bool h2_complex(const bmi_h2* h) { return h->cplx; }
 bool h2_is_symmetric_real(const bmi_h2* h) { return h->op.kind == BMI_LAPLACE_SL; }
+bool h2_is_mixed(const bmi_h2* h) { return h->op.precision != BMI_PRECISION_FP64; }
 
 // ------------------------------------------------------------------ assembly
 static void assemble_h2(bmi_h2* h, hipStream_t st) {
@@ -473,9 +493,17 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   h->d_tgt.upload(tgt, st);
   h->d_crp.upload(crp, st);
   h->d_csrc.upload(csrc, st);
-  if (op.precision == BMI_PRECISION_MIXED) h->d_coup32.alloc(h->d_coup.n);
+  if (op.precision != BMI_PRECISION_FP64) h->d_coup32.alloc(h->d_coup.n);
   BMI_CHECK_HIP(hipStreamSynchronize(st));
-  if (op.precision == BMI_PRECISION_MIXED && h->d_coup.n) convert_to_fp32(h->d_coup.ptr, h->d_coup32.ptr, h->d_coup.n, st);
+  if (op.precision != BMI_PRECISION_FP64 && h->d_coup.n) convert_to_fp32(h->d_coup.ptr, h->d_coup32.ptr, h->d_coup.n, st);
+  if (op.precision == BMI_PRECISION_MIXED_BF16) {
+    // blocks are sorted by level: the coarse ones (levels 0 and 1) come first
+    h->ncoarse = 0;
+    while (h->ncoarse < nl && h->lowrank[h->ncoarse].level <= 1) ++h->ncoarse;
+    const int64_t n16 = static_cast<int64_t>(S) * vd * vd * cblk * h->ncoarse;
+    h->d_coup16.alloc(static_cast<size_t>(n16));
+    convert_to_bf16(h->d_coup.ptr, h->d_coup16.ptr, n16, st);
+  }
   BMI_CHECK_HIP(hipStreamSynchronize(st));
 }
 
@@ -846,7 +874,7 @@ bmi_status bmi_h2_assemble(const bmi_space* s, const bmi_op_desc* op, void* stre
   BMI_REQUIRE(op->n_cheb >= 1 && op->n_cheb <= 12, BMI_INVALID_ARG, "n_cheb out of range");
   BMI_REQUIRE(op->eta > 0, BMI_INVALID_ARG, "eta must be positive");
   BMI_REQUIRE(op->min_cluster_level >= 0 && op->min_cluster_level <= 3, BMI_INVALID_ARG, "min_cluster_level out of range");
-  BMI_REQUIRE(op->precision == BMI_PRECISION_FP64 || op->precision == BMI_PRECISION_MIXED, BMI_INVALID_ARG,
+  BMI_REQUIRE(op->precision >= BMI_PRECISION_FP64 && op->precision <= BMI_PRECISION_MIXED_BF16, BMI_INVALID_ARG,
               "unknown precision");
   *out = nullptr;
   auto h = std::make_unique<bmi_h2>();
@@ -931,7 +959,7 @@ bmi_status bmi_h2_matvec(const bmi_h2* hc, int nrhs, const double* X, int64_t ld
   const int S = h->cplx ? 2 : 1;
   if (mem == BMI_MEM_DEVICE) {
     // device operands: enqueued on the stream, returns without waiting
-    h2_apply_device(h, nrhs, X, ldx, Y, ldy, alpha, beta, st);
+    h2_apply_device(h, nrhs, X, ldx, Y, ldy, alpha, beta, st, h2_is_mixed(h));
     BMI_CHECK_HIP(hipGetLastError());
     return BMI_OK;
   } else {
@@ -942,7 +970,7 @@ bmi_status bmi_h2_matvec(const bmi_h2* hc, int nrhs, const double* X, int64_t ld
       BMI_CHECK_HIP(hipMemcpyAsync(dx.ptr + S * D * k, X + S * ldx * k, S * D * sizeof(double), hipMemcpyHostToDevice, st));
       BMI_CHECK_HIP(hipMemcpyAsync(dy.ptr + S * D * k, Y + S * ldy * k, S * D * sizeof(double), hipMemcpyHostToDevice, st));
     }
-    h2_apply_device(h, nrhs, dx.ptr, D, dy.ptr, D, alpha, beta, st);
+    h2_apply_device(h, nrhs, dx.ptr, D, dy.ptr, D, alpha, beta, st, h2_is_mixed(h));
     for (int k = 0; k < nrhs; ++k)
       BMI_CHECK_HIP(hipMemcpyAsync(Y + S * ldy * k, dy.ptr + S * D * k, S * D * sizeof(double), hipMemcpyDeviceToHost, st));
   }
@@ -957,7 +985,7 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* hc, int nrhs, const double* Xs, dou
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (nrhs == 0) return BMI_OK;
   ensure_workspace(h, nrhs, st);
-  apply_super(h, nrhs, Xs, Ys, h->d_yf.ptr, st);
+  apply_super(h, nrhs, Xs, Ys, h->d_yf.ptr, h2_is_mixed(h), st);
   const int S = h->cplx ? 2 : 1;
   if (has_far(h)) device_add(Ys, h->d_yf.ptr, static_cast<int64_t>(S) * h->s->nsuper * nrhs, st);
   BMI_CHECK_HIP(hipStreamSynchronize(st));

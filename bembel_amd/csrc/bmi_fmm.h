@@ -24,9 +24,12 @@ void launch_moment(bool cplx, int ncl, int nrhs, int rows, int mc, const double*
 void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T, const double* child,
                 double* parent, hipStream_t st);
 // variant: 0 = one block per lane group per iteration, 1 = two (chosen by the autotuner)
+// Fc/ncoarse: BF16 copies of the first ncoarse blocks (fp32 only)
 void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs, int cd, const int2* tgt,
                            const int* rp, const int* src, const void* F, int64_t fstride, int64_t fcomp,
-                           const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st);
+                           const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st,
+                           const unsigned short* Fc = nullptr, int ncoarse = 0);
+void convert_to_bf16(const double* src, unsigned short* dst, int64_t n, hipStream_t st);
 // forward transformation of all levels (moments at level L, transfers up to level 0)
 void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
                     const double* T, const double* xs, int64_t ldxs, int64_t xseg, double* fw, const int64_t* lvl_off,
@@ -45,7 +48,8 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
                         const int* dsrc, const double* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
-                        const double* fw, double* bw, hipStream_t st);
+                        const double* fw, double* bw, hipStream_t st, const unsigned short* Fc = nullptr,
+                        int ncoarse = 0);
 void launch_leaf_backward(bool cplx, int ncl, int nrhs, int dd, const double* mom, int rows, const double* bw,
                           double* yf, int64_t ldy, int64_t yseg, hipStream_t st);
 

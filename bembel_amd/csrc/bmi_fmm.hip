@@ -690,7 +690,13 @@ struct CplArgs {
   const double* fw;
   double* bw;
   int accumulate;
+  const unsigned short* Fc;  // BF16 copies of the first ncoarse blocks (coarsest levels), or null
+  int ncoarse;
 };
+
+__device__ __forceinline__ double bf16_to_double(unsigned short b) {
+  return static_cast<double>(__uint_as_float(static_cast<unsigned>(b) << 16));
+}
 
 template <bool C, typename FT, int CD, int RT, int UB>
 __device__ __forceinline__ void coupling_wave(int t, const CplArgs& a) {
@@ -714,11 +720,21 @@ __device__ __forceinline__ void coupling_wave(int t, const CplArgs& a) {
 #pragma unroll
       for (int u = 0; u < UB; ++u) {
         if (p + G * u >= p1) break;
-        const FT* f = F + S * (static_cast<int64_t>(p + G * u) * a.fstride + a.fcomp);
+        const int64_t fo = S * (static_cast<int64_t>(p + G * u) * a.fstride + a.fcomp);
+        if (std::is_same<FT, float>::value && p + G * u < a.ncoarse) {
+          const unsigned short* f = a.Fc + fo;
 #pragma unroll
-        for (int j = 0; j < CD; ++j) {
-          fr[u][j] = static_cast<double>(f[S * (j * CD + i)]);
-          if constexpr (C) fi[u][j] = static_cast<double>(f[S * (j * CD + i) + 1]);
+          for (int j = 0; j < CD; ++j) {
+            fr[u][j] = bf16_to_double(f[S * (j * CD + i)]);
+            if constexpr (C) fi[u][j] = bf16_to_double(f[S * (j * CD + i) + 1]);
+          }
+        } else {
+          const FT* f = F + fo;
+#pragma unroll
+          for (int j = 0; j < CD; ++j) {
+            fr[u][j] = static_cast<double>(f[S * (j * CD + i)]);
+            if constexpr (C) fi[u][j] = static_cast<double>(f[S * (j * CD + i) + 1]);
+          }
         }
       }
 #pragma unroll
@@ -790,6 +806,15 @@ __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, in
   }
 }
 
+// BF16 copy (round to nearest even of the FP32 value) of n doubles
+__global__ void k_to_bf16(const double* __restrict__ src, unsigned short* __restrict__ dst, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned u = __float_as_uint(static_cast<float>(src[i]));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  dst[i] = static_cast<unsigned short>(u >> 16);
+}
+
 // ------------------------------------------------------------------ host launchers
 static inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
 
@@ -843,7 +868,8 @@ void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T
 
 void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs, int cd, const int2* tgt,
                            const int* rp, const int* src, const void* F, int64_t fstride, int64_t fcomp,
-                           const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st) {
+                           const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st,
+                           const unsigned short* Fc, int ncoarse) {
   const int64_t n = static_cast<int64_t>(ntgt) * nrhs * cd;
   if (n == 0) return;
   TimedScope ts("coupling_apply", st);
@@ -851,7 +877,7 @@ void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs
   // wave-per-target kernels for the block sizes of the configurations (np 4, 5; Maxwell np 4)
 #define BMI_CPL_W(CC, FTT, CDD)                                                                              \
   if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                                  \
-    const CplArgs ca{ntgt, nrhs, tgt, rp, src, F, fstride, fcomp, lvl_off, fw, bw, accumulate};               \
+    const CplArgs ca{ntgt, nrhs, tgt, rp, src, F, fstride, fcomp, lvl_off, fw, bw, accumulate, Fc, ncoarse};   \
     if (nrhs == 1 && variant == 1)                                                                            \
       hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, 1, 2>), wgrid, dim3(256), 0, st, ca);                    \
     else if (nrhs == 1)                                                                                       \
@@ -1007,9 +1033,9 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
                         const int* dsrc, const double* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
-                        const double* fw, double* bw, hipStream_t st) {
+                        const double* fw, double* bw, hipStream_t st, const unsigned short* Fc, int ncoarse) {
   const LeafArgs la{ncl, 1, drp, dsrc, dF, static_cast<int64_t>(dd) * dd, 0, xs, 0, 0, ys, 0, 0, 0};
-  const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(cd) * cd, 0, lvl_off, fw, bw, 0};
+  const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(cd) * cd, 0, lvl_off, fw, bw, 0, Fc, ncoarse};
   const int nbn = (ncl + 3) / 4, nbc = (ntgt + 3) / 4;
   const dim3 grid(static_cast<unsigned>(nbn + nbc));
   TimedScope ts("matvec_sweep", st);
@@ -1034,6 +1060,12 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
 #undef BMI_FUSED_V
 #undef BMI_FUSED
   return false;
+}
+
+void convert_to_bf16(const double* src, unsigned short* dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(n)), dim3(256), 0, st, src, dst, n);
+  BMI_CHECK_HIP(hipGetLastError());
 }
 
 void launch_leaf_backward(bool cplx, int ncl, int nrhs, int dd, const double* mom, int rows, const double* bw,

@@ -148,6 +148,13 @@ struct Vecs {
     BMI_CHECK_HIP(hipMemcpyAsync(host.data(), d_out, host.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     BMI_CHECK_HIP(hipStreamSynchronize(st));
   }
+  void dots_async(const double* V, int nv, const double* W, double* d_out) const {
+    if (C)
+      hipLaunchKernelGGL(k_dots<true>, dim3(nv * nrhs), dim3(256), 0, st, V, block(), W, n, nrhs, d_out);
+    else
+      hipLaunchKernelGGL(k_dots<false>, dim3(nv * nrhs), dim3(256), 0, st, V, block(), W, n, nrhs, d_out);
+    BMI_CHECK_HIP(hipGetLastError());
+  }
   void subtract(const double* V, int nv, const double* d_h, double* W) const {
     if (C)
       hipLaunchKernelGGL(k_subtract<true>, dim3(nb(n * nrhs)), dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs);
@@ -175,7 +182,7 @@ using cplx = std::complex<double>;
 
 // batched restarted GMRES; returns true if every column converged
 static bool gmres(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int restart, int maxit,
-                  std::vector<int>& iters, std::vector<double>& resid, hipStream_t st) {
+                  std::vector<int>& iters, std::vector<double>& resid, hipStream_t st, bool reduced) {
   const bool C = h2_complex(h);
   const int64_t n = h2_rows(h);
   Vecs vs{C, n, nrhs, st};
@@ -209,7 +216,7 @@ static bool gmres(bmi_h2* h, int nrhs, const double* B, double* X, double tol, i
   while (true) {
     // R = B - A X
     BMI_CHECK_HIP(hipMemcpyAsync(R.ptr, B, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
-    h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st);
+    h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st, reduced);
     vs.dots(R.ptr, 1, R.ptr, d_small.ptr, hbuf);
     std::vector<double> beta(nrhs);
     bool all = true;
@@ -234,7 +241,7 @@ static bool gmres(bmi_h2* h, int nrhs, const double* B, double* X, double tol, i
     int k = 0;
     for (; k < m && total < maxit; ++k, ++total) {
       double* Vk = V.ptr + vs.block() * k;
-      h2_apply_device(h, nrhs, Vk, n, W.ptr, n, one, nullptr, st);
+      h2_apply_device(h, nrhs, Vk, n, W.ptr, n, one, nullptr, st, reduced);
       // classical Gram-Schmidt, twice
       vs.dots(V.ptr, k + 1, W.ptr, d_small.ptr, hbuf);
       vs.subtract(V.ptr, k + 1, d_small.ptr, W.ptr);
@@ -302,66 +309,175 @@ static bool gmres(bmi_h2* h, int nrhs, const double* B, double* X, double tol, i
   }
 }
 
-// batched conjugate gradients (real symmetric operators)
+// CG scalars on the device: no host round trip per iteration
+// alpha_c = rr_c / pAp_c (active columns); X += alpha P; R -= alpha AP
+__global__ void k_cg_xr(const double* __restrict__ rr, const double* __restrict__ pap, const char* __restrict__ active,
+                        const double* __restrict__ P, const double* __restrict__ AP, double* X, double* R, int64_t n,
+                        int nrhs) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n * nrhs) return;
+  const int c = static_cast<int>(gid / n);
+  if (!active[c] || pap[c] == 0.0) return;
+  const double a = rr[c] / pap[c];
+  X[gid] += a * P[gid];
+  R[gid] -= a * AP[gid];
+}
+// per column: count the step, beta = rr_new / rr, deactivate converged columns
+__global__ void k_cg_scalars(const double* __restrict__ rr, double* __restrict__ rrn, const double* __restrict__ bn,
+                             char* active, int* iters, double* beta, double tol, int nrhs) {
+  const int c = threadIdx.x + blockIdx.x * blockDim.x;
+  if (c >= nrhs) return;
+  if (!active[c]) {
+    beta[c] = 0.0;
+    rrn[c] = rr[c];
+    return;
+  }
+  iters[c] += 1;
+  beta[c] = rr[c] != 0.0 ? rrn[c] / rr[c] : 0.0;
+  if (sqrt(fmax(rrn[c], 0.0)) <= tol * bn[c]) active[c] = 0;
+}
+// P = R + beta P (active columns)
+__global__ void k_cg_p(const double* __restrict__ beta, const char* __restrict__ active, const double* __restrict__ R,
+                       double* P, int64_t n, int nrhs) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n * nrhs) return;
+  const int c = static_cast<int>(gid / n);
+  if (!active[c]) return;
+  P[gid] = R[gid] + beta[c] * P[gid];
+}
+__global__ void k_cg_init(const double* __restrict__ rr, const double* __restrict__ bn, char* active, int* iters,
+                          double tol, int nrhs) {
+  const int c = threadIdx.x + blockIdx.x * blockDim.x;
+  if (c >= nrhs) return;
+  active[c] = sqrt(fmax(rr[c], 0.0)) > tol * bn[c] ? 1 : 0;
+  iters[c] = 0;
+}
+
+// batched conjugate gradients (real symmetric operators); convergence is read back every
+// `check` iterations
 static bool cg(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int maxit, std::vector<int>& iters,
-               std::vector<double>& resid, hipStream_t st) {
+               std::vector<double>& resid, hipStream_t st, bool reduced) {
   const int64_t n = h2_rows(h);
   Vecs vs{false, n, nrhs, st};
-  DevBuf<double> R, Pv, AP, d_small, d_coef;
+  DevBuf<double> R, Pv, AP, d_rr[2], d_pap, d_bn, d_beta;
+  DevBuf<char> d_active;
+  DevBuf<int> d_iters;
   R.alloc(vs.block());
   Pv.alloc(vs.block());
   AP.alloc(vs.block());
-  d_small.alloc(2 * nrhs);
-  d_coef.alloc(2 * nrhs);
+  d_rr[0].alloc(nrhs);
+  d_rr[1].alloc(nrhs);
+  d_pap.alloc(nrhs);
+  d_bn.alloc(nrhs);
+  d_beta.alloc(nrhs);
+  d_active.alloc(nrhs);
+  d_iters.alloc(nrhs);
   const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
   std::vector<double> buf;
-  vs.dots(B, 1, B, d_small.ptr, buf);
+  vs.dots(B, 1, B, d_pap.ptr, buf);
   std::vector<double> bn(nrhs);
   for (int c = 0; c < nrhs; ++c) bn[c] = std::sqrt(buf[c]) > 0 ? std::sqrt(buf[c]) : 1.0;
+  d_bn.upload(bn, st);
   BMI_CHECK_HIP(hipMemcpyAsync(R.ptr, B, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
-  h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st);
+  h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st, reduced);
   BMI_CHECK_HIP(hipMemcpyAsync(Pv.ptr, R.ptr, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
-  vs.dots(R.ptr, 1, R.ptr, d_small.ptr, buf);
-  std::vector<double> rr(buf.begin(), buf.end());
-  iters.assign(nrhs, 0);
-  resid.assign(nrhs, 0.0);
-  std::vector<char> active(nrhs, 1);
-  auto upload = [&](const std::vector<double>& a, const std::vector<double>& b) {
-    std::vector<double> t(a);
-    t.insert(t.end(), b.begin(), b.end());
-    BMI_CHECK_HIP(hipMemcpyAsync(d_coef.ptr, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, st));
-  };
+  vs.dots_async(R.ptr, 1, R.ptr, d_rr[0].ptr);
+  const unsigned sb = static_cast<unsigned>((nrhs + 63) / 64);
+  hipLaunchKernelGGL(k_cg_init, dim3(sb), dim3(64), 0, st, d_rr[0].ptr, d_bn.ptr, d_active.ptr, d_iters.ptr, tol, nrhs);
+  BMI_CHECK_HIP(hipGetLastError());
+  const int check = 8;
+  int cur = 0;
+  std::vector<char> act(nrhs);
+  const unsigned eb = nb(n * nrhs);
   for (int it = 0; it < maxit; ++it) {
-    bool any = false;
-    for (int c = 0; c < nrhs; ++c) {
-      resid[c] = std::sqrt(std::max(rr[c], 0.0)) / bn[c];
-      if (resid[c] <= tol) active[c] = 0;
-      any = any || active[c];
+    if (it % check == 0) {
+      BMI_CHECK_HIP(hipMemcpyAsync(act.data(), d_active.ptr, nrhs, hipMemcpyDeviceToHost, st));
+      BMI_CHECK_HIP(hipStreamSynchronize(st));
+      if (std::none_of(act.begin(), act.end(), [](char a) { return a != 0; })) break;
     }
-    if (!any) return true;
-    h2_apply_device(h, nrhs, Pv.ptr, n, AP.ptr, n, one, nullptr, st);
-    vs.dots(Pv.ptr, 1, AP.ptr, d_small.ptr, buf);
-    std::vector<double> alpha(nrhs, 0.0), ones(nrhs, 1.0), malpha(nrhs, 0.0);
-    for (int c = 0; c < nrhs; ++c)
-      if (active[c] && buf[c] != 0.0) alpha[c] = rr[c] / buf[c], malpha[c] = -alpha[c], iters[c]++;
-    upload(alpha, ones);
-    vs.axpby(d_coef.ptr, Pv.ptr, d_coef.ptr + nrhs, X);
-    upload(malpha, ones);
-    vs.axpby(d_coef.ptr, AP.ptr, d_coef.ptr + nrhs, R.ptr);
-    vs.dots(R.ptr, 1, R.ptr, d_small.ptr, buf);
-    std::vector<double> beta(nrhs, 0.0);
-    for (int c = 0; c < nrhs; ++c) {
-      if (active[c] && rr[c] != 0.0) beta[c] = buf[c] / rr[c];
-      rr[c] = active[c] ? buf[c] : rr[c];
-    }
-    upload(ones, beta);
-    vs.axpby(d_coef.ptr, R.ptr, d_coef.ptr + nrhs, Pv.ptr);
+    h2_apply_device(h, nrhs, Pv.ptr, n, AP.ptr, n, one, nullptr, st, reduced);
+    vs.dots_async(Pv.ptr, 1, AP.ptr, d_pap.ptr);
+    hipLaunchKernelGGL(k_cg_xr, dim3(eb), dim3(256), 0, st, d_rr[cur].ptr, d_pap.ptr, d_active.ptr, Pv.ptr, AP.ptr, X,
+                       R.ptr, n, nrhs);
+    vs.dots_async(R.ptr, 1, R.ptr, d_rr[1 - cur].ptr);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(sb), dim3(64), 0, st, d_rr[cur].ptr, d_rr[1 - cur].ptr, d_bn.ptr,
+                       d_active.ptr, d_iters.ptr, d_beta.ptr, tol, nrhs);
+    hipLaunchKernelGGL(k_cg_p, dim3(eb), dim3(256), 0, st, d_beta.ptr, d_active.ptr, R.ptr, Pv.ptr, n, nrhs);
+    BMI_CHECK_HIP(hipGetLastError());
+    cur = 1 - cur;
   }
+  std::vector<double> rr(nrhs);
+  iters.assign(nrhs, 0);
+  BMI_CHECK_HIP(hipMemcpyAsync(rr.data(), d_rr[cur].ptr, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+  BMI_CHECK_HIP(hipMemcpyAsync(iters.data(), d_iters.ptr, nrhs * sizeof(int), hipMemcpyDeviceToHost, st));
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+  resid.assign(nrhs, 0.0);
   bool all = true;
   for (int c = 0; c < nrhs; ++c) {
     resid[c] = std::sqrt(std::max(rr[c], 0.0)) / bn[c];
     all = all && resid[c] <= tol;
   }
+  return all;
+}
+
+// mixed-precision iterative refinement: residuals with the FP64 operator, corrections from the
+// inner solver with the reduced-precision operator
+static bool refine(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int restart, int maxit,
+                   std::vector<int>& iters, std::vector<double>& resid, hipStream_t st, bool use_cg, int* outer_steps) {
+  const bool C = h2_complex(h);
+  const int64_t n = h2_rows(h);
+  Vecs vs{C, n, nrhs, st};
+  const int S = vs.S();
+  DevBuf<double> R, D, d_small, d_coef;
+  R.alloc(vs.block());
+  D.alloc(vs.block());
+  d_small.alloc(static_cast<size_t>(S) * nrhs);
+  d_coef.alloc(static_cast<size_t>(2) * S * nrhs);
+  const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
+  std::vector<double> buf;
+  vs.dots(B, 1, B, d_small.ptr, buf);
+  std::vector<double> bn(nrhs);
+  for (int c = 0; c < nrhs; ++c) bn[c] = std::sqrt(std::max(buf[S * c], 0.0)) > 0 ? std::sqrt(buf[S * c]) : 1.0;
+  std::vector<double> ones(static_cast<size_t>(2) * S * nrhs, 0.0);
+  for (int c = 0; c < nrhs; ++c) ones[S * c] = 1.0, ones[S * (nrhs + c)] = 1.0;
+  d_coef.upload(ones, st);
+  iters.assign(nrhs, 0);
+  resid.assign(nrhs, 0.0);
+  int used = 0, outer = 0;
+  bool all = false;
+  while (true) {
+    // R = B - A64 X
+    BMI_CHECK_HIP(hipMemcpyAsync(R.ptr, B, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
+    h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st, false);
+    vs.dots(R.ptr, 1, R.ptr, d_small.ptr, buf);
+    all = true;
+    double need = 1.0;
+    for (int c = 0; c < nrhs; ++c) {
+      const double rn = std::sqrt(std::max(buf[S * c], 0.0));
+      resid[c] = rn / bn[c];
+      if (resid[c] > tol) {
+        all = false;
+        need = std::min(need, tol * bn[c] / rn);
+      }
+    }
+    if (all || used >= maxit || outer >= 30) break;
+    // inner solve A_reduced D = R to the accuracy this step needs (not below what the
+    // reduced operator can resolve)
+    const double tin = std::max(need, 1e-6);
+    device_fill(D.ptr, vs.block(), 0.0, st);
+    std::vector<int> it;
+    std::vector<double> rs;
+    if (use_cg)
+      cg(h, nrhs, R.ptr, D.ptr, tin, maxit - used, it, rs, st, true);
+    else
+      gmres(h, nrhs, R.ptr, D.ptr, tin, restart, maxit - used, it, rs, st, true);
+    int mx = 0;
+    for (int c = 0; c < nrhs; ++c) iters[c] += it[c], mx = std::max(mx, it[c]);
+    used += std::max(mx, 1);
+    vs.axpby(d_coef.ptr, D.ptr, d_coef.ptr + S * nrhs, X);  // X = D + X
+    ++outer;
+  }
+  if (outer_steps) *outer_steps = outer;
   return all;
 }
 
@@ -536,8 +652,12 @@ static bmi_status solve_common(const bmi_h2* hc, int nrhs, const double* B, doub
     if (zero_guess) device_fill(x, S * n * nrhs, 0.0, st);
     std::vector<int> it;
     std::vector<double> rs;
-    const bool ok = use_cg ? cg(h, nrhs, b, x, tol, maxit, it, rs, st)
-                           : gmres(h, nrhs, b, x, tol, restart, maxit, it, rs, st);
+    bool ok;
+    if (h2_is_mixed(h))
+      ok = refine(h, nrhs, b, x, tol, restart, maxit, it, rs, st, use_cg, nullptr);
+    else
+      ok = use_cg ? cg(h, nrhs, b, x, tol, maxit, it, rs, st, false)
+                  : gmres(h, nrhs, b, x, tol, restart, maxit, it, rs, st, false);
     if (mem != BMI_MEM_DEVICE) BMI_CHECK_HIP(hipMemcpyAsync(X, dX.ptr, dX.bytes(), hipMemcpyDeviceToHost, st));
     BMI_CHECK_HIP(hipStreamSynchronize(st));
     if (iters) std::copy(it.begin(), it.end(), iters);

@@ -54,7 +54,8 @@ def nearfield_flops(classes_deg, P, pd):
 
 
 def run_solves(B, torch, H, space, op, npts, b, nsolves):
-    """CG to 1e-10 in FP64 and with the mixed-precision far field; latencies and error"""
+    """CG to 1e-10 in FP64, and with the mixed-precision operator inside FP64 iterative refinement;
+    latencies and the mixed solution's error against the FP64 solution"""
     cg = B.ConjugateGradient()
     cg.set_tolerance(1e-10)
     cg.set_max_iterations(2000)
@@ -162,6 +163,20 @@ def main():
     value = n / t_mv
     log("matvec %.3f ms -> %.3e DOF/s" % (t_mv * 1e3, value))
 
+    # mixed precision product (FP32 coupling blocks, FP64 near field): speed and error vs FP64
+    Hm = B.H2Matrix(space, op, n_cheb=npts, precision=B.MIXED)
+    for _ in range(a.warmup):
+        ym = Hm.matvec(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ym = Hm.matvec(x)
+    torch.cuda.synchronize()
+    t_mv_mixed = (time.perf_counter() - t0) / a.steps
+    mixed_err = float(torch.linalg.norm(ym - y) / torch.linalg.norm(y))
+    log("mixed matvec %.3f ms -> %.3e DOF/s, rel err vs FP64 %.2e" % (t_mv_mixed * 1e3, n / t_mv_mixed, mixed_err))
+    del Hm
+
     # kernel timing pass for the roofline (HIP events on the launch stream, inside the library)
     B.timing.reset()
     B.timing.enable(True)
@@ -259,6 +274,8 @@ def main():
         "config": {"workload": "laplace_single_layer_h2_sphere_M%d_P%d_np%d" % (M, P, npts),
                    "elements": space.number_of_elements, "dofs": n, "polynomial_degree": P, "n_cheb": npts,
                    "eta": 1.6, "min_cluster_level": 1, "nrhs": 1},
+        "mixed_matvec": {"ms_per_step": t_mv_mixed * 1e3, "dof_per_s": n / t_mv_mixed, "rel_err_vs_fp64": mixed_err,
+                         "storage": "coupling blocks FP32, near field FP64, sums in FP64"},
         "h2_assembly_s": t_asm,
         "space_setup_s": t_space,
         "assembly_kernels_ms": {"nf_duffy": duffy_ms, "nf_regular": reg_ms, "coupling_assemble": cpl_ms},

@@ -43,8 +43,10 @@ typedef enum {
 
 /* storage / arithmetic of the far field (coupling and transfer) */
 typedef enum {
-  BMI_PRECISION_FP64 = 0,  /* everything FP64 */
-  BMI_PRECISION_MIXED = 1  /* coupling + transfer FP32 storage/arith; near field FP64 */
+  BMI_PRECISION_FP64 = 0,       /* everything FP64 */
+  BMI_PRECISION_MIXED = 1,      /* coupling blocks stored FP32 (products summed in FP64), near field FP64;
+                                   solvers refine with the FP64 operator */
+  BMI_PRECISION_MIXED_BF16 = 2  /* as MIXED, and the coupling blocks of the two coarsest levels in BF16 */
 } bmi_precision;
 
 /* spaces: DifferentialForm of the operator (LinearOperator/DifferentialFormEnum.hpp) */

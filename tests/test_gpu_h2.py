@@ -159,3 +159,52 @@ def test_gpu_h2_super_space_product(gpu, kind, kappa, M, P, np_):
     dt = torch.complex128 if kind != O.LAPLACE else torch.float64
     ys = H.matvec_super(torch.tensor(xs, dtype=dt, device="cuda")).cpu().numpy()
     assert rel(ys, oh.matvec_super(xs)) <= TOL
+
+
+def test_gpu_mixed_precision_matvec_and_refinement(gpu):
+    """Mixed handles: FP32 coupling blocks keep the product within 1e-6 of FP64; BF16 coarse
+    blocks are coarser but bounded; iterative refinement with the FP64 residual brings CG and GMRES
+    solutions to the FP64 solve."""
+    M, P, np_ = 4, 2, 4
+    ps = B.AnsatzSpace(B.Geometry(), M, P)
+    op = B.LaplaceSingleLayerOperator()
+    H64 = B.H2Matrix(ps, op, n_cheb=np_)
+    H32 = B.H2Matrix(ps, op, n_cheb=np_, precision=B.MIXED)
+    H16 = B.H2Matrix(ps, op, n_cheb=np_, precision=B.MIXED_BF16)
+    x = np.random.default_rng(2).uniform(-1, 1, ps.number_of_dofs)
+    y64 = H64 @ x
+    e32, e16 = rel(H32 @ x, y64), rel(H16 @ x, y64)
+    print("mixed matvec rel err fp32 %.2e bf16 %.2e" % (e32, e16))
+    assert e32 <= 1e-6 and e16 <= 1e-2 and e16 >= e32
+    b = B.DiscreteLinearForm(B.DirichletTrace([0.1, -0.2, 0.05]), ps).compute().get_discrete_linear_form()
+    sols = {}
+    for name, H in (("fp64", H64), ("mixed", H32), ("bf16", H16)):
+        cg = B.ConjugateGradient()
+        cg.set_tolerance(1e-10)
+        cg.set_max_iterations(3000)
+        cg.compute(H)
+        sols[name] = cg.solve(b)
+        assert cg.info() == B.OK, name
+        assert rel(H64 @ sols[name], b) <= 2e-10, name
+    # both residuals are at 1e-10 of the FP64 operator; solutions of a first-kind equation with
+    # condition number ~1e3 may still differ by ~1e-7: the stated bound is 1e-6
+    print("refined vs fp64 solve:", rel(sols["mixed"], sols["fp64"]), rel(sols["bf16"], sols["fp64"]))
+    assert rel(sols["mixed"], sols["fp64"]) <= 1e-6 and rel(sols["bf16"], sols["fp64"]) <= 1e-6
+    # Helmholtz GMRES with refinement
+    oph = B.HelmholtzSingleLayerOperator()
+    oph.set_wavenumber(2.0)
+    ps3 = B.AnsatzSpace(B.Geometry(), 3, 2)
+    Hh64 = B.H2Matrix(ps3, oph, n_cheb=5)
+    Hh32 = B.H2Matrix(ps3, oph, n_cheb=5, precision=B.MIXED)
+    bh = B.DiscreteLinearForm(B.PlaneWaves([[0.0, 0.0, 1.0]], 2.0), ps3).compute().get_discrete_linear_form()
+    out = []
+    for H in (Hh64, Hh32):
+        gm = B.GMRES()
+        gm.set_tolerance(1e-10)
+        gm.set_restart(60)
+        gm.set_max_iterations(3000)
+        gm.compute(H)
+        out.append(gm.solve(bh))
+        assert gm.info() == B.OK
+        assert rel(Hh64 @ out[-1], bh) <= 2e-10
+    assert rel(out[1], out[0]) <= 1e-6
