@@ -54,9 +54,10 @@ def library():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    path = os.environ.get("BEMBEL_AMD_LIBRARY", LIB_PATH)  # another build of the same library
+    if not os.path.exists(path):
         raise RuntimeError("libbembel_amd.so is missing: build it with `python -m bembel_amd.build`")
-    L = C.CDLL(LIB_PATH)
+    L = C.CDLL(path)
     vp, ip, dp, i64p = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int64)
     pp = C.POINTER(C.c_void_p)
     sig = {

@@ -93,119 +93,170 @@ __device__ __forceinline__ double dummy_value(int fn, const PT& a, const PT& b) 
   return 1.0;
 }
 
-// per-lane block accumulator
+// Block accumulation split over groups of G lanes.  Each lane evaluates its own integrand point
+// (surface points, kernel, Bernstein values); the G lanes of a group exchange those values and
+// each adds all G evaluations to its part of the block: for complex operators the lane role
+// (lane mod G) picks real or imaginary parts (role & 1) and, with G = 4, a half of the rows
+// (role >> 1).  Per lane this divides the accumulator registers by G at the same number of
+// multiply-adds; real blocks (G = 1) stay whole in every lane.
 template <int P, int KIND>
 struct BlockAcc {
   static constexpr int I = P + 1;
   static constexpr int I2 = I * I;
   static constexpr bool CPLX = KIND == BMI_HELMHOLTZ_SL || KIND == BMI_MAXWELL_SL;
-  double re[I2 * I2];
-  double im[CPLX ? I2 * I2 : 1];
+  static constexpr bool MAXW = KIND == BMI_MAXWELL_SL;
+  // lanes per group: real blocks fit one lane (P <= 2), complex ones are split two or four ways
+  static constexpr int G = CPLX ? (P >= 2 ? 4 : 2) : 1;
+  static constexpr int RH = CPLX ? G / 2 : G;         // row halves
+  static constexpr int NR = (I2 + RH - 1) / RH;       // rows held per lane
+  static constexpr int ID = MAXW ? I : 1;
+  static constexpr int TOT = CPLX ? 2 * I2 * I2 : I2 * I2;
+  double acc[NR * I2];
+
+  // what the block update needs from one evaluation
+  struct Eval {
+    double vr, vi;  // kernel x weights (x tangent product for Maxwell)
+    double qr, qi;  // Maxwell divergence factor
+    double X[I], Y[I], U[I], V[I];
+    double DX[ID], DY[ID], DU[ID], DV[ID];
+  };
 
   __device__ __forceinline__ void zero() {
 #pragma unroll
-    for (int k = 0; k < I2 * I2; ++k) re[k] = 0.0;
-    if (CPLX) {
-#pragma unroll
-      for (int k = 0; k < I2 * I2; ++k) im[k] = 0.0;
-    }
+    for (int k = 0; k < NR * I2; ++k) acc[k] = 0.0;
   }
 
-  // Maxwell: component block (rc, cc) of the EFIE integrand, vector part
-  // K w w (t_rc(a) . t_cc(b)) phi_i(a) phi_j(b) plus divergence part
-  // -K w w / (kappa^2 h^2) dphi_i^rc(a) dphi_j^cc(b) (Maxwell/SingleLayerOperator.hpp:44-88,
-  // SuperSpace.hpp:236-283, Basis.hpp:170-200)
-  __device__ __forceinline__ void add_maxwell(const Pt<KIND>& a, const Pt<KIND>& b, const NFParams& prm, int rc,
-                                              int cc) {
+  // one evaluation of the integrand; a lane without work (valid = false) contributes zero
+  __device__ __forceinline__ Eval eval(const Pt<KIND>& a, const Pt<KIND>& b, const NFParams& prm, int rc, int cc,
+                                       bool valid) const {
+    Eval e;
     const double d0 = a.x0 - b.x0, d1 = a.x1 - b.x1, d2 = a.x2 - b.x2;
     const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    double sn, cs;
-    sincos(prm.kre * r, &sn, &cs);
-    const double mag = exp(prm.kim * r) * (a.W * b.W) * (0.25 * M_1_PI) / r;
-    const double kr = mag * cs, ki = -mag * sn;
-    const double* ta = a.t + 3 * rc;
-    const double* tb = b.t + 3 * cc;
-    const double dt = ta[0] * tb[0] + ta[1] * tb[1] + ta[2] * tb[2];
-    const double vr = kr * dt, vi = ki * dt;
-    const double qr = kr * prm.qre - ki * prm.qim, qi = kr * prm.qim + ki * prm.qre;
-    double X[I], Y[I], U[I], V[I], DX[I], DY[I], DU[I], DV[I];
-    bernstein_basis_der<P>(a.s0, X, DX);
-    bernstein_basis_der<P>(a.s1, Y, DY);
-    bernstein_basis_der<P>(b.s0, U, DU);
-    bernstein_basis_der<P>(b.s1, V, DV);
-    double bv[I2], bd[I2];
+    e.vi = 0.0, e.qr = 0.0, e.qi = 0.0;
+    if constexpr (KIND == BMI_LAPLACE_SL) {
+      e.vr = (a.W * b.W) * (0.25 * M_1_PI) / r;
+    } else if constexpr (CPLX) {
+      // exp(-i kappa r) / (4 pi r), kappa = kre + i kim (Helmholtz/SingleLayerOperator.hpp:115-120)
+      double sn, cs;
+      sincos(prm.kre * r, &sn, &cs);
+      const double mag = exp(prm.kim * r) * (a.W * b.W) * (0.25 * M_1_PI) / r;
+      const double kr = mag * cs, ki = -mag * sn;
+      if constexpr (MAXW) {
+        // vector part K w w (t_rc(a) . t_cc(b)), divergence part -K w w / (kappa^2 h^2)
+        // (Maxwell/SingleLayerOperator.hpp:44-88)
+        const double* ta = a.t + 3 * rc;
+        const double* tb = b.t + 3 * cc;
+        const double dt = ta[0] * tb[0] + ta[1] * tb[1] + ta[2] * tb[2];
+        e.vr = kr * dt, e.vi = ki * dt;
+        e.qr = kr * prm.qre - ki * prm.qim, e.qi = kr * prm.qim + ki * prm.qre;
+      } else {
+        e.vr = kr, e.vi = ki;
+      }
+    } else {
+      e.vr = dummy_value(prm.dummy_fn, a, b) * a.W * b.W;
+    }
+    if (!valid) e.vr = 0.0, e.vi = 0.0, e.qr = 0.0, e.qi = 0.0;
+    if constexpr (MAXW) {
+      bernstein_basis_der<P>(a.s0, e.X, e.DX);
+      bernstein_basis_der<P>(a.s1, e.Y, e.DY);
+      bernstein_basis_der<P>(b.s0, e.U, e.DU);
+      bernstein_basis_der<P>(b.s1, e.V, e.DV);
+    } else {
+      bernstein_basis<P>(a.s0, e.X);
+      bernstein_basis<P>(a.s1, e.Y);
+      bernstein_basis<P>(b.s0, e.U);
+      bernstein_basis<P>(b.s1, e.V);
+    }
+    return e;
+  }
+
+  // the evaluation held by lane `src` of the wave
+  __device__ __forceinline__ static Eval fetch(const Eval& e, int src) {
+    Eval o;
+    o.vr = __shfl(e.vr, src, 64);
+    if constexpr (CPLX) o.vi = __shfl(e.vi, src, 64);
+    if constexpr (MAXW) o.qr = __shfl(e.qr, src, 64), o.qi = __shfl(e.qi, src, 64);
+#pragma unroll
+    for (int k = 0; k < I; ++k) {
+      o.X[k] = __shfl(e.X[k], src, 64), o.Y[k] = __shfl(e.Y[k], src, 64);
+      o.U[k] = __shfl(e.U[k], src, 64), o.V[k] = __shfl(e.V[k], src, 64);
+      if constexpr (MAXW) {
+        o.DX[k] = __shfl(e.DX[k], src, 64), o.DY[k] = __shfl(e.DY[k], src, 64);
+        o.DU[k] = __shfl(e.DU[k], src, 64), o.DV[k] = __shfl(e.DV[k], src, 64);
+      }
+    }
+    return o;
+  }
+
+  // add one evaluation to this lane's part of the block
+  __device__ __forceinline__ void accumulate(const Eval& e, int role, int rc, int cc) {
+    const int part = CPLX ? (role & 1) : 0;
+    const int half = CPLX ? (role >> 1) : role;
+    double bv[I2], bd[MAXW ? I2 : 1], av[I2], ad[MAXW ? I2 : 1];
 #pragma unroll
     for (int jy = 0; jy < I; ++jy)
 #pragma unroll
       for (int jx = 0; jx < I; ++jx) {
-        bv[jy * I + jx] = U[jx] * V[jy];
-        bd[jy * I + jx] = cc == 0 ? DU[jx] * V[jy] : U[jx] * DV[jy];
+        bv[jy * I + jx] = e.U[jx] * e.V[jy];
+        if constexpr (MAXW) bd[jy * I + jx] = cc == 0 ? e.DU[jx] * e.V[jy] : e.U[jx] * e.DV[jy];
       }
 #pragma unroll
     for (int iy = 0; iy < I; ++iy)
 #pragma unroll
       for (int ix = 0; ix < I; ++ix) {
-        const double av = X[ix] * Y[iy];
-        const double ad = rc == 0 ? DX[ix] * Y[iy] : X[ix] * DY[iy];
-        const double ar = vr * av, ai = vi * av, dr = qr * ad, di = qi * ad;
-        const int row = (iy * I + ix) * I2;
+        av[iy * I + ix] = e.X[ix] * e.Y[iy];
+        if constexpr (MAXW) ad[iy * I + ix] = rc == 0 ? e.DX[ix] * e.Y[iy] : e.X[ix] * e.DY[iy];
+      }
+    const double s = part ? e.vi : e.vr;
+    const double q = part ? e.qi : e.qr;
 #pragma unroll
-        for (int j = 0; j < I2; ++j) {
-          re[row + j] += ar * bv[j] + dr * bd[j];
-          im[row + j] += ai * bv[j] + di * bd[j];
+    for (int ii = 0; ii < NR; ++ii) {
+      // row ii of this lane's half (rows past the end hold zero)
+      constexpr int dummy = 0;
+      (void)dummy;
+      double ar = av[ii], dr = MAXW ? ad[ii < (MAXW ? I2 : 1) ? ii : 0] : 0.0;
+      if constexpr (RH == 2) {
+        const bool in = NR + ii < I2;
+        const double ah = in ? av[in ? NR + ii : 0] : 0.0;
+        ar = half ? ah : ar;
+        if constexpr (MAXW) {
+          const double dh = in ? ad[in ? NR + ii : 0] : 0.0;
+          dr = half ? dh : dr;
         }
       }
+      const double as = s * ar;
+      const double aq = MAXW ? q * dr : 0.0;
+#pragma unroll
+      for (int j = 0; j < I2; ++j) {
+        if constexpr (MAXW)
+          acc[ii * I2 + j] += as * bv[j] + aq * bd[j];
+        else
+          acc[ii * I2 + j] += as * bv[j];
+      }
+    }
   }
 
-  // integrand of the pair (a, b) times the basis product, added to the block
-  __device__ __forceinline__ void add(const Pt<KIND>& a, const Pt<KIND>& b, const NFParams& prm, int rc = 0,
-                                      int cc = 0) {
-    if constexpr (KIND == BMI_MAXWELL_SL) {
-      add_maxwell(a, b, prm, rc, cc);
-      return;
+  // the lane's own evaluation and those of the other lanes of its group, in role order (all
+  // lanes of the wave must call this together)
+  __device__ __forceinline__ void add(const Pt<KIND>& a, const Pt<KIND>& b, const NFParams& prm, int rc, int cc,
+                                      bool valid, int lane) {
+    const Eval e = eval(a, b, prm, rc, cc, valid);
+    if constexpr (G == 1) {
+      accumulate(e, 0, rc, cc);
     } else {
-    double vr, vi = 0.0;
-    const double d0 = a.x0 - b.x0, d1 = a.x1 - b.x1, d2 = a.x2 - b.x2;
-    if (KIND == BMI_LAPLACE_SL) {
-      const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-      vr = (a.W * b.W) * (0.25 * M_1_PI) / r;
-    } else if (KIND == BMI_HELMHOLTZ_SL) {
-      // exp(-i kappa r) / (4 pi r), kappa = kre + i kim (Helmholtz/SingleLayerOperator.hpp:115-120)
-      const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-      double sn, cs;
-      sincos(prm.kre * r, &sn, &cs);
-      const double mag = exp(prm.kim * r) * (a.W * b.W) * (0.25 * M_1_PI) / r;
-      vr = mag * cs;
-      vi = -mag * sn;
-    } else {
-      vr = dummy_value(prm.dummy_fn, a, b) * a.W * b.W;
+      const int role = lane & (G - 1), base = lane & ~(G - 1);
+#pragma unroll 1
+      for (int r = 0; r < G; ++r) accumulate(fetch(e, base + r), role, rc, cc);
     }
-    double X[I], Y[I], U[I], V[I];
-    bernstein_basis<P>(a.s0, X);
-    bernstein_basis<P>(a.s1, Y);
-    bernstein_basis<P>(b.s0, U);
-    bernstein_basis<P>(b.s1, V);
-    double bv[I2];
-#pragma unroll
-    for (int jy = 0; jy < I; ++jy)
-#pragma unroll
-      for (int jx = 0; jx < I; ++jx) bv[jy * I + jx] = U[jx] * V[jy];
-#pragma unroll
-    for (int iy = 0; iy < I; ++iy)
-#pragma unroll
-      for (int ix = 0; ix < I; ++ix) {
-        const double av = X[ix] * Y[iy];
-        const double ar = vr * av;
-        const int row = (iy * I + ix) * I2;
-#pragma unroll
-        for (int j = 0; j < I2; ++j) re[row + j] += ar * bv[j];
-        if (CPLX) {
-          const double ai = vi * av;
-#pragma unroll
-          for (int j = 0; j < I2; ++j) im[row + j] += ai * bv[j];
-        }
-      }
-    }
+  }
+
+  // output slot of accumulator k of a lane with this role (complex blocks store all real parts,
+  // then all imaginary parts); -1 if unused
+  __device__ __forceinline__ static int slot(int role, int k) {
+    const int part = CPLX ? (role & 1) : 0;
+    const int half = CPLX ? (role >> 1) : role;
+    const int row = half * NR + k / I2;
+    return row < I2 ? part * I2 * I2 + row * I2 + k % I2 : -1;
   }
 };
 
@@ -219,29 +270,34 @@ __device__ __forceinline__ void tau(double x, double y, int c, double& u, double
   }
 }
 
-template <int NT, int N>
-__device__ __forceinline__ void block_reduce_store(double (&acc)[N], double* lds, int tid,
-                                                   double* out_sum) {
-  // lane reduction inside each wave, then across waves through LDS
+// sum the per-lane partial blocks over the workgroup: lanes of equal role within each wave by
+// shuffles, then the waves through LDS; the result is the full block in `sum` (Acc::TOT values)
+template <int NT, class Acc>
+__device__ __forceinline__ void reduce_block(Acc& acc, double* lds, int tid, double* sum) {
+  constexpr int N = Acc::NR * Acc::I2;
+  constexpr int TOT = Acc::TOT;
+  constexpr int NW = NT / 64;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    double v = acc[k];
+    double v = acc.acc[k];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    acc[k] = v;
+    for (int off = 32; off >= Acc::G; off >>= 1) v += __shfl_xor(v, off, 64);
+    acc.acc[k] = v;
   }
-  constexpr int NW = NT / 64;
   const int wave = tid >> 6, lane = tid & 63;
-  if (lane == 0) {
+  if (lane < Acc::G) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) lds[wave * N + k] = acc[k];
+    for (int k = 0; k < N; ++k) {
+      const int sl = Acc::slot(lane, k);
+      if (sl >= 0) lds[wave * TOT + sl] = acc.acc[k];
+    }
   }
   __syncthreads();
-  for (int k = tid; k < N; k += NT) {
-    double s = 0;
+  for (int k = tid; k < TOT; k += NT) {
+    double v = 0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) s += lds[w * N + k];
-    out_sum[k] = s;
+    for (int w = 0; w < NW; ++w) v += lds[w * TOT + k];
+    sum[k] = v;
   }
   __syncthreads();
 }
@@ -253,6 +309,7 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
   constexpr int I2 = Acc::I2;
   const NFPair pr = pairs[blockIdx.x];
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const double* cp1 = prm.cp + static_cast<int64_t>(prm.elem_patch[pr.e1]) * prm.cp_stride;
   const double* cp2 = prm.cp + static_cast<int64_t>(prm.elem_patch[pr.e2]) * prm.cp_stride;
   const double l1x = prm.elem_llc[2 * pr.e1], l1y = prm.elem_llc[2 * pr.e1 + 1];
@@ -268,13 +325,18 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
   const int cc = KIND == BMI_MAXWELL_SL ? static_cast<int>(blockIdx.y) & 1 : 0;
   Acc acc;
   acc.zero();
+  // every lane runs the same number of iterations (lane groups swap values); lanes past the end
+  // evaluate a valid point with zero weight
 
   if constexpr (CLS == 0) {
     // integrate0: far-field tables of both elements
     const int nf = prm.nff, fs = prm.ffs;
     const double* f1 = prm.ffield + static_cast<int64_t>(pr.e1) * nf * fs;
     const double* f2 = prm.ffield + static_cast<int64_t>(pr.e2) * nf * fs;
-    for (int idx = tid; idx < nf * nf; idx += NT) {
+    const int total = nf * nf;
+    for (int base = 0; base < total; base += NT) {
+      const bool valid = base + tid < total;
+      const int idx = valid ? base + tid : total - 1;
       const int i = idx / nf, j = idx - (idx / nf) * nf;
       const double* a = f1 + i * fs;
       const double* b = f2 + j * fs;
@@ -286,7 +348,7 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
 #pragma unroll
         for (int k = 0; k < 6; ++k) pa.t[k] = a[8 + k], pb.t[k] = b[8 + k];
       }
-      acc.add(pa, pb, prm, rc, cc);
+      acc.add(pa, pb, prm, rc, cc, valid, lane);
     }
   } else if constexpr (CLS == 1) {
     // integrate1: tensor Gauss rule; surface points of both elements staged in LDS
@@ -298,14 +360,20 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       q2[k] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, w);
     }
     __syncthreads();
-    for (int idx = tid; idx < nq * nq; idx += NT) {
+    const int total = nq * nq;
+    for (int base = 0; base < total; base += NT) {
+      const bool valid = base + tid < total;
+      const int idx = valid ? base + tid : total - 1;
       const int i = idx / nq, j = idx - (idx / nq) * nq;
-      acc.add(q1[i], q2[j], prm, rc, cc);
+      acc.add(q1[i], q2[j], prm, rc, cc, valid, lane);
     }
     __syncthreads();
   } else if constexpr (CLS == 2) {
     // integrate2: identical elements, 8 Duffy sub-simplices, symmetrised
-    for (int idx = tid; idx < nq * nq; idx += NT) {
+    const int total = nq * nq;
+    for (int base = 0; base < total; base += NT) {
+      const bool valid = base + tid < total;
+      const int idx = valid ? base + tid : total - 1;
       const int i = idx / nq, j = idx - (idx / nq) * nq;
       const double xi0 = gx[i / n], xi1 = gx[i % n], wi = gw[i / n] * gw[i % n];
       const double xj0 = gx[j / n], xj1 = gx[j % n], wj = gw[j / n] * gw[j % n];
@@ -314,20 +382,27 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       const double t2 = xj1 * (1 - xi0 * xi1);
       const double t3 = t1 + xi0;
       const double t4 = t2 + xi0 * xi1;
-      const double pu[8] = {t1, t3, t1, t3, t2, t4, t2, t4};
-      const double pv[8] = {t2, t4, t4, t2, t1, t3, t3, t1};
-#pragma unroll
+      // point pairs (t1,t2)-(t3,t4), (t1,t4)-(t3,t2), (t2,t1)-(t4,t3), (t2,t3)-(t4,t1), written as
+      // selects; the loop is unrolled only when the block accumulator leaves registers to spare
+      constexpr int UNR2 = Acc::G == 1 ? 4 : 1;
+#pragma unroll UNR2
       for (int k = 0; k < 4; ++k) {
-        const PtK a = make_point<PD, KIND>(cp1, l1x, l1y, h, pu[2 * k], pv[2 * k], w);
-        const PtK b = make_point<PD, KIND>(cp1, l1x, l1y, h, pu[2 * k + 1], pv[2 * k + 1], wj);
-        acc.add(a, b, prm, rc, cc);
-        acc.add(b, a, prm, rc, cc);
+        const bool sw = k >= 2, odd = k & 1;
+        const double pu = sw ? t2 : t1, qu = sw ? t4 : t3;
+        const double r = sw ? t1 : t2, sv = sw ? t3 : t4;
+        const PtK a = make_point<PD, KIND>(cp1, l1x, l1y, h, pu, odd ? sv : r, w);
+        const PtK b = make_point<PD, KIND>(cp1, l1x, l1y, h, qu, odd ? r : sv, wj);
+        acc.add(a, b, prm, rc, cc, valid, lane);
+        acc.add(b, a, prm, rc, cc, valid, lane);
       }
     }
   } else if constexpr (CLS == 3) {
     // integrate3: common edge, 6 Duffy sub-regions
     const int r1 = pr.rot1, r2 = pr.rot2;
-    for (int idx = tid; idx < nq * nq; idx += NT) {
+    const int total = nq * nq;
+    for (int base = 0; base < total; base += NT) {
+      const bool valid = base + tid < total;
+      const int idx = valid ? base + tid : total - 1;
       const int i = idx / nq, j = idx - (idx / nq) * nq;
       const double xi0 = gx[i / n], xi1 = gx[i % n], wi = gw[i / n] * gw[i % n];
       const double xj0 = gx[j / n], xj1 = gx[j % n], wj = gw[j / n] * gw[j % n];
@@ -338,19 +413,25 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       const double t3 = xi0 * (1 - e0);
       const double t4 = (1 - xi0) * (1 - e0);
       const double wa = wj * (1 - xi1), wb = wj * (1 - e0);
-      const double A0[6] = {t1, 1 - t1, t3, 1 - t3, t4, 1 - t4};
-      const double A1[6] = {e0, e0, xi1, xi1, e1, e1};
-      const double B0[6] = {t2, 1 - t2, t4, 1 - t4, t3, 1 - t3};
-      const double B1[6] = {e1, e1, e1, e1, xi1, xi1};
-      const double BW[6] = {wa, wa, wb, wb, wb, wb};
-#pragma unroll
+      // sub-region k: first coordinates {t1, 1-t1, t3, 1-t3, t4, 1-t4} and {t2, 1-t2, t4, 1-t4, t3,
+      // 1-t3}, second {e0, e0, xi1, xi1, e1, e1} and {e1, e1, e1, e1, xi1, xi1}, weights
+      // {wa, wa, wb, wb, wb, wb}, written as selects (see above for the unrolling)
+      constexpr int UNR3 = Acc::G == 1 ? 6 : 1;
+#pragma unroll UNR3
       for (int k = 0; k < 6; ++k) {
+        const int g = k >> 1;
+        const bool odd = k & 1;
+        const double a0 = g == 0 ? t1 : (g == 1 ? t3 : t4);
+        const double b0 = g == 0 ? t2 : (g == 1 ? t4 : t3);
+        const double a1 = g == 0 ? e0 : (g == 1 ? xi1 : e1);
+        const double b1 = g == 2 ? xi1 : e1;
+        const double bw = g == 0 ? wa : wb;
         double u, v;
-        tau(A0[k], A1[k], r1, u, v);
+        tau(odd ? 1 - a0 : a0, a1, r1, u, v);
         const PtK a = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, w);
-        tau(B0[k], B1[k], r2, u, v);
-        const PtK b = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, BW[k]);
-        acc.add(a, b, prm, rc, cc);
+        tau(odd ? 1 - b0 : b0, b1, r2, u, v);
+        const PtK b = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, bw);
+        acc.add(a, b, prm, rc, cc, valid, lane);
       }
     }
   } else {
@@ -372,7 +453,10 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       q[4 * i + 3] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, w);
     }
     __syncthreads();
-    for (int idx = tid; idx < nq * nq; idx += NT) {
+    const int total = nq * nq;
+    for (int base = 0; base < total; base += NT) {
+      const bool valid = base + tid < total;
+      const int idx = valid ? base + tid : total - 1;
       const int i = idx / nq, j = idx - (idx / nq) * nq;
       const double xi0 = gx[i / n];
       const double e0 = xi0 * gx[j / n], e1 = xi0 * gx[j % n], wj = gw[j / n] * gw[j % n];
@@ -381,36 +465,29 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       const PtK p5 = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, wj);
       tau(e0, e1, r1, u, v);
       const PtK p6 = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, wj);
-      acc.add(q[4 * i + 0], p5, prm, rc, cc);
-      acc.add(q[4 * i + 1], p5, prm, rc, cc);
-      acc.add(p6, q[4 * i + 2], prm, rc, cc);
-      acc.add(p6, q[4 * i + 3], prm, rc, cc);
+      acc.add(q[4 * i + 0], p5, prm, rc, cc, valid, lane);
+      acc.add(q[4 * i + 1], p5, prm, rc, cc, valid, lane);
+      acc.add(p6, q[4 * i + 2], prm, rc, cc, valid, lane);
+      acc.add(p6, q[4 * i + 3], prm, rc, cc, valid, lane);
     }
     __syncthreads();
   }
 
-  // reduce the per-lane blocks and write the result
+  // reduce the per-lane partial blocks and write the result
   double* red = lds;
-  double* sum = lds + (NT / 64) * I2 * I2;
-  block_reduce_store<NT>(acc.re, red, tid, sum);
+  double* sum = lds + (NT / 64) * Acc::TOT;
+  reduce_block<NT>(acc, red, tid, sum);
   double* o = prm.out;
   constexpr bool CPLX = Acc::CPLX;
   const int64_t base = pr.off + rc * prm.comp_rs + cc * prm.comp_cs;
   for (int k = tid; k < I2 * I2; k += NT) {
     const int i = k / I2, j = k - (k / I2) * I2;
     const int64_t idx = base + i * prm.rs + j * prm.cs;
-    if (CPLX)
+    if (CPLX) {
       o[2 * idx] = sum[k];
-    else
+      o[2 * idx + 1] = sum[I2 * I2 + k];
+    } else {
       o[idx] = sum[k];
-  }
-  if constexpr (CPLX) {
-    __syncthreads();
-    block_reduce_store<NT>(acc.im, red, tid, sum);
-    for (int k = tid; k < I2 * I2; k += NT) {
-      const int i = k / I2, j = k - (k / I2) * I2;
-      const int64_t idx = base + i * prm.rs + j * prm.cs;
-      o[2 * idx + 1] = sum[k];
     }
   }
 }
@@ -472,9 +549,8 @@ template <int P, int KIND, int PD, int CLS>
 static void launch_class(const NFPair* d_pairs, int count, int deg, NFParams prm, hipStream_t st) {
   if (count == 0) return;
   constexpr int NT = (CLS == 0) ? 64 : 256;
-  constexpr int I2 = (P + 1) * (P + 1);
   const int nq = (deg + 1) * (deg + 1);
-  size_t lds = sizeof(double) * ((NT / 64) * I2 * I2 + I2 * I2);
+  size_t lds = sizeof(double) * ((NT / 64) + 1) * BlockAcc<P, KIND>::TOT;
   if (CLS == 1) lds = std::max(lds, 2 * nq * sizeof(Pt<KIND>));
   if (CLS == 4) lds = std::max(lds, 4 * nq * sizeof(Pt<KIND>));
   BMI_REQUIRE(lds <= 160 * 1024, BMI_UNSUPPORTED, "quadrature degree too large for LDS staging");

@@ -87,6 +87,133 @@ def run_solves(B, torch, H, space, op, npts, b, nsolves):
     return lat64, latm, it64, rel_err_mixed
 
 
+def unit_directions(rng, k):
+    d = rng.normal(size=(k, 3))
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def timed(torch, fn, reps=1):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps, out
+
+
+def gmres_solve(B, H, b, tol=1e-10, restart=30, maxit=5000):
+    gm = B.GMRES()
+    gm.set_tolerance(tol)
+    gm.set_restart(restart)
+    gm.set_max_iterations(maxit)
+    gm.compute(H)
+    x = gm.solve(b)
+    return x, gm
+
+
+def run_batched(B, torch, rng, steps):
+    """Helmholtz single layer, kappa 4, 6,144 elements, P 2, 5x5 Chebyshev nodes, 64 plane-wave
+    right-hand sides: batched product throughput and batched GMRES (FP64 and mixed)"""
+    M, P, npts, kappa, nrhs = 5, 2, 5, 4.0, 64
+    op = B.HelmholtzSingleLayerOperator()
+    op.set_wavenumber(kappa)
+    space = B.AnsatzSpace(B.Geometry(), M, P, op)
+    n = space.number_of_dofs
+    t_asm, H = timed(torch, lambda: B.H2Matrix(space, op, n_cheb=npts))
+    Hm = B.H2Matrix(space, op, n_cheb=npts, precision=B.MIXED)
+    rhs = B.DiscreteLinearForm(B.PlaneWaves(unit_directions(rng, nrhs), kappa), space).compute(
+        device=True).get_discrete_linear_form()
+    out = {"workload": "helmholtz_single_layer_h2_sphere_M5_P2_np5_kappa4_64rhs", "dofs": n, "nrhs": nrhs,
+           "h2_assembly_s": t_asm}
+    for name, HH in (("fp64", H), ("mixed", Hm)):
+        for _ in range(2):
+            HH.matvec(rhs)
+        t, y = timed(torch, lambda: HH.matvec(rhs), max(3, steps // 4))
+        out[name + "_matvec_ms"] = t * 1e3
+        out[name + "_matvec_dof_rhs_per_s"] = n * nrhs / t
+        t1, _ = timed(torch, lambda: HH.matvec(rhs[:, :1]), max(3, steps // 4))
+        out[name + "_matvec_1rhs_ms"] = t1 * 1e3
+    t, (x64, gm) = timed(torch, lambda: gmres_solve(B, H, rhs))
+    out["fp64_gmres_s"] = t
+    out["fp64_gmres_iterations_max"] = int(np.max(gm.iterations()))
+    t, (xm, gmm) = timed(torch, lambda: gmres_solve(B, Hm, rhs))
+    out["mixed_gmres_s"] = t
+    out["mixed_gmres_iterations_max"] = int(np.max(gmm.iterations()))
+    out["mixed_rel_err_vs_fp64_max"] = float(torch.max(torch.linalg.norm(xm - x64, dim=0) /
+                                                   torch.linalg.norm(x64, dim=0)))
+    log("batched 64 RHS: matvec fp64 %.2f ms, mixed %.2f ms (1 RHS %.2f ms); GMRES fp64 %.2f s (%d it), mixed %.2f s, "
+        "err %.1e" % (out["fp64_matvec_ms"], out["mixed_matvec_ms"], out["fp64_matvec_1rhs_ms"], out["fp64_gmres_s"],
+                      out["fp64_gmres_iterations_max"], out["mixed_gmres_s"], out["mixed_rel_err_vs_fp64_max"]))
+    return out
+
+
+def run_headline(B, torch, rng, cpu_threads, with_cpu):
+    """Helmholtz single layer, kappa 4, 24,576 elements, P 2, 5x5 Chebyshev nodes, one plane wave:
+    H2 assembly + GMRES(30) to 1e-10, plain FP64 and mixed precision, and the CPU estimate"""
+    M, P, npts, kappa = 6, 2, 5, 4.0
+    op = B.HelmholtzSingleLayerOperator()
+    op.set_wavenumber(kappa)
+    space = B.AnsatzSpace(B.Geometry(), M, P, op)
+    n = space.number_of_dofs
+    d = unit_directions(rng, 1)
+    out = {"workload": "helmholtz_single_layer_h2_sphere_M6_P2_np5_kappa4", "dofs": n}
+    sols = {}
+    for name, prec in (("fp64", B.FP64), ("mixed", B.MIXED)):
+        t0 = time.perf_counter()
+        H = B.H2Matrix(space, op, n_cheb=npts, precision=prec)
+        b = B.DiscreteLinearForm(B.PlaneWaves(d, kappa), space).compute(device=True).get_discrete_linear_form()
+        torch.cuda.synchronize()
+        t_asm = time.perf_counter() - t0
+        t_sol, (x, gm) = timed(torch, lambda: gmres_solve(B, H, b))
+        sols[name] = x
+        out[name] = {"assembly_s": t_asm, "gmres_s": t_sol, "total_s": t_asm + t_sol,
+                     "iterations": int(gm.iterations()[0]), "residual": float(gm.error()[0])}
+        if name == "fp64":
+            keep = H
+        else:
+            del H
+    out["mixed_rel_err_vs_fp64"] = float(torch.linalg.norm(sols["mixed"] - sols["fp64"]) /
+                                         torch.linalg.norm(sols["fp64"]))
+    out["mixed_speedup"] = out["fp64"]["total_s"] / out["mixed"]["total_s"]
+    log("Helmholtz M6: FP64 assembly %.2f s + GMRES %.2f s (%d it); mixed %.2f + %.2f s; err %.1e" % (
+        out["fp64"]["assembly_s"], out["fp64"]["gmres_s"], out["fp64"]["iterations"], out["mixed"]["assembly_s"],
+        out["mixed"]["gmres_s"], out["mixed_rel_err_vs_fp64"]))
+    if with_cpu:
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib as O
+            osp = O.Space(O.Geometry(), M, P)
+            oh = O.H2(osp, O.HELMHOLTZ, complex(kappa), npts, assemble=False)
+            keys, data = keep.export_leaves()
+            assert oh.import_leaves(keys, data) == len(keys)
+            del data
+            t_mv = oh.time_matvec(1, 2, cpu_threads)
+            dense = keys[keys[:, 0] == 2]
+            lr_idx = np.nonzero(keys[:, 0] == 1)[0]
+            ns = 128
+            pick = rng.choice(len(dense), ns, replace=False)
+            pairs = np.array([(4 * c1 + i, 4 * c2 + j) for (_, _, c1, c2) in dense[pick] for i in range(4)
+                              for j in range(4)], dtype=np.int32)
+            okey = {tuple(r): i for i, r in enumerate(oh.leaves().tolist())}
+            blocks = np.array([okey[tuple(keys[k])] for k in rng.choice(lr_idx, 2000, replace=False)], dtype=np.int32)
+            t_pairs = oh.time_assembly_sample(pairs, [], cpu_threads)
+            t_blocks = oh.time_assembly_sample(np.zeros((0, 2)), blocks, cpu_threads)
+            cpu_asm = t_pairs * len(dense) / ns + t_blocks * len(lr_idx) / len(blocks)
+            it = out["fp64"]["iterations"]
+            cpu_total = cpu_asm + t_mv * (it + it // 30 + 1)
+            out["cpu_estimate"] = {
+                "cores": cpu_threads, "kind": "port", "assembly_s": cpu_asm, "matvec_s": t_mv,
+                "total_s": cpu_total, "gpu_fp64_speedup": cpu_total / out["fp64"]["total_s"],
+                "sample": "%d near-field pairs + 2000 coupling blocks scaled to the operator; oracle OpenMP matvec "
+                          "of the same leaves x (GMRES iterations + restarts)" % len(pairs)}
+            log("Helmholtz M6 CPU estimate: assembly %.0f s, matvec %.3f s, total %.0f s (%d threads)" % (
+                cpu_asm, t_mv, cpu_total, cpu_threads))
+        except Exception as e:
+            log("Helmholtz CPU estimate failed:", repr(e))
+    del keep
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +226,8 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MB first (counter calibration)")
     ap.add_argument("--no-breakdown", action="store_true", help="skip the per-kernel assembly breakdown")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-RHS Helmholtz workload")
+    ap.add_argument("--no-headline", action="store_true", help="skip the Helmholtz 24,576-element assembly + GMRES")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the H2 path on a single MI355X "
@@ -217,13 +346,16 @@ def main():
     if a.solves > 0:
         lat64, latm, it64, rel_err_mixed = run_solves(B, torch, H, space, op, npts, b, a.solves)
 
+    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    batched = None if a.no_batched else run_batched(B, torch, rng, a.steps)
+    headline = None if a.no_headline else run_headline(B, torch, rng, threads, not a.no_cpu_baseline)
+
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not a.no_cpu_baseline:
         try:
             sys.path.insert(0, os.path.join(ROOT, "tests"))
             import oracle_lib as O
-            threads = a.cpu_threads or min(16, os.cpu_count() or 1)
             og = O.Geometry()
             osp = O.Space(og, M, P)
             oh = O.H2(osp, O.LAPLACE, 0j, npts, assemble=False)
@@ -291,6 +423,8 @@ def main():
                               "unit": "TFLOP/s", "frac": duffy_tflops / FP64_PEAK_TFLOPS,
                               "kernel": "k_nearfield identical/edge/vertex (Duffy)", "flops": fl},
         "cpu_baseline": cpu,
+        "batched_64rhs": batched,
+        "helmholtz_assembly_gmres": headline,
     }
     print(json.dumps(out), flush=True)
 
