@@ -21,14 +21,14 @@ __all__ = [
     "BembelError", "Geometry", "AnsatzSpace", "LaplaceSingleLayerOperator", "HelmholtzSingleLayerOperator",
     "MaxwellSingleLayerOperator", "DummyOperator", "DiscreteOperator", "H2Matrix", "DenseMatrix",
     "DiscreteLinearForm", "DirichletTrace", "PlaneWaves", "RotatedTangentialTrace", "GMRES", "ConjugateGradient", "library", "timing",
-    "SPHERE_DAT", "FP64", "MIXED", "MIXED_BF16",
+    "SPHERE_DAT", "FP64", "MIXED", "MIXED_BF16", "Server", "synchronize",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbembel_amd.so")
 SPHERE_DAT = os.path.join(HERE, "geo", "sphere.dat")
 
-OK, INVALID_ARG, OUT_OF_MEMORY, DEVICE_ERROR, NOT_CONVERGED, ADMISSION_REJECTED, IO_ERROR, UNSUPPORTED = range(8)
+OK, INVALID_ARG, OUT_OF_MEMORY, DEVICE_ERROR, NOT_CONVERGED, ADMISSION_REJECTED, IO_ERROR, UNSUPPORTED, PENDING = range(9)
 LAPLACE, HELMHOLTZ, MAXWELL, DUMMY = 0, 1, 2, 3
 FP64, MIXED, MIXED_BF16 = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
@@ -46,6 +46,23 @@ class OpDesc(C.Structure):
                 ("dummy_fn", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_active", C.c_int), ("max_batch_rhs", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("geometry_path", C.c_char_p), ("scale", C.c_double * 3), ("refinement_level", C.c_int),
+                ("polynomial_degree", C.c_int), ("op", OpDesc), ("data_fn", C.c_int), ("nrhs", C.c_int),
+                ("data_params", C.POINTER(C.c_double)), ("data_kappa", C.c_double), ("tol", C.c_double),
+                ("restart", C.c_int), ("maxit", C.c_int)]
+
+
+class ResultInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("ndof", C.c_int64), ("nrhs", C.c_int), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("end_s", C.c_double), ("iterations", C.c_int), ("residual", C.c_double),
+                ("batch_requests", C.c_int), ("batch_rhs", C.c_int), ("cache_hit", C.c_int)]
+
+
 _lib = None
 
 
@@ -54,6 +71,12 @@ def library():
     global _lib
     if _lib is not None:
         return _lib
+    # torch bundles its own HIP runtime on the same HSA runtime as /opt/rocm; when torch is
+    # loaded first, this library binds to that HSA runtime instead of starting a second one
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     path = os.environ.get("BEMBEL_AMD_LIBRARY", LIB_PATH)  # another build of the same library
     if not os.path.exists(path):
         raise RuntimeError("libbembel_amd.so is missing: build it with `python -m bembel_amd.build`")
@@ -87,6 +110,7 @@ def library():
         "bmi_h2_matvec": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, C.c_int, vp]),
         "bmi_h2_matvec_super": (C.c_int, [vp, C.c_int, dp, dp, vp]),
         "bmi_h2_use_graphs": (C.c_int, [vp, C.c_int]),
+        "bmi_synchronize": (C.c_int, [vp]),
         "bmi_rhs_assemble": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp, C.c_int, vp]),
         "bmi_solve_gmres": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, ip, dp,
                                       C.c_int, vp]),
@@ -94,6 +118,14 @@ def library():
         "bmi_timing_enable": (C.c_int, [C.c_int]),
         "bmi_timing_reset": (C.c_int, []),
         "bmi_timing_get": (C.c_int, [C.c_char_p, dp, i64p]),
+        "bmi_h2_estimate_bytes": (C.c_int, [vp, vp, C.c_int, i64p]),
+        "bmi_server_config_default": (C.c_int, [vp]),
+        "bmi_server_create": (C.c_int, [vp, pp]),
+        "bmi_server_submit": (C.c_int, [vp, vp, i64p]),
+        "bmi_server_wait": (C.c_int, [vp, C.c_int64, C.c_double, vp]),
+        "bmi_server_fetch": (C.c_int, [vp, C.c_int64, dp]),
+        "bmi_server_stats": (C.c_int, [vp, i64p]),
+        "bmi_server_destroy": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -118,17 +150,29 @@ def _ip(a):
 
 
 def _stream_ptr(stream):
+    """A stream of this library's HIP runtime (an integer handle) or None for its default
+    stream.  torch streams belong to torch's own HIP runtime and are not passed through: work on
+    them is synchronised at the boundary instead (_torch_sync)."""
     if stream is None:
-        try:
-            import torch
-            if torch.cuda.is_available() and torch.cuda.is_initialized():
-                return C.c_void_p(torch.cuda.current_stream().cuda_stream)
-        except Exception:
-            pass
         return None
     if isinstance(stream, int):
         return C.c_void_p(stream)
-    return C.c_void_p(stream.cuda_stream)
+    raise BembelError(INVALID_ARG, "pass None or an integer stream handle of this library's HIP runtime")
+
+
+def _torch_sync():
+    """Finish pending torch work before the library reads or writes torch tensors."""
+    try:
+        import torch
+        if torch.cuda.is_available() and torch.cuda.is_initialized():
+            torch.cuda.current_stream().synchronize()
+    except ImportError:
+        pass
+
+
+def synchronize(stream=None):
+    """Wait for the library's work on `stream` (default stream if None)."""
+    _check(library().bmi_synchronize(_stream_ptr(stream)))
 
 
 def _is_torch_cuda(x):
@@ -426,10 +470,16 @@ class H2Matrix:
         _check(library().bmi_h2_export_leaves(self._h, _dp(data)))
         return self.leaves(), data
 
-    def matvec(self, x, alpha=1.0, stream=None):
-        """alpha * H x for a vector or a (n x nrhs) matrix (numpy on host or torch on device)."""
+    def matvec(self, x, alpha=1.0, stream=None, sync=True):
+        """alpha * H x for a vector or a (n x nrhs) matrix (numpy on host or torch on device).
+        With device tensors the call waits for pending torch work before and for the product
+        after (torch runs on its own HIP runtime).  sync=False leaves that ordering to the caller
+        (torch.cuda.synchronize() before, bembel_amd.synchronize() after): the product is only
+        enqueued, so back-to-back products overlap their host overhead with the device work."""
         n = self.rows()
         X, nrhs, ld, mem, squeeze, backend = _columns(x, n, self.scalar)
+        if backend == "torch" and sync:
+            _torch_sync()
         # beta = 0: Y is written, never read
         if backend == "torch":
             import torch
@@ -440,6 +490,8 @@ class H2Matrix:
         be = np.zeros(2)
         _check(library().bmi_h2_matvec(self._h, nrhs, _dptr(X, backend), ld, _dptr(Y, backend), ld, _dp(al), _dp(be),
                                        mem, _stream_ptr(stream)))
+        if backend == "torch" and sync:
+            synchronize(stream)
         Y = Y.t() if backend == "torch" else Y.T
         return Y.reshape(-1) if squeeze else Y
 
@@ -455,6 +507,7 @@ class H2Matrix:
         N = self.space.number_of_super_dofs
         X, nrhs, ld, mem, squeeze, backend = _columns(xs, N, self.scalar)
         Y = torch.empty_like(X)
+        _torch_sync()
         _check(library().bmi_h2_matvec_super(self._h, nrhs, _dptr(X, backend), _dptr(Y, backend),
                                              _stream_ptr(stream)))
         Y = Y.t()
@@ -515,12 +568,13 @@ class DiscreteOperator:
 
 # ---------------------------------------------------------------------------- linear forms
 class DirichletTrace:
-    """g(x) = 1/|x - x0| (a point source outside the domain's exterior)."""
+    """g_k(x) = 1/|x - x0_k| (point sources; one column per source)."""
 
     def __init__(self, x0):
+        p = np.asarray(x0, dtype=np.float64).reshape(-1, 3)
         self.fn = 0
-        self.params = np.asarray(x0, dtype=np.float64).reshape(3)
-        self.nrhs = 1
+        self.params = np.ascontiguousarray(p)
+        self.nrhs = p.shape[0]
         self.kappa = 0.0
 
 
@@ -568,6 +622,7 @@ class DiscreteLinearForm:
         if device:
             import torch
             out = torch.zeros(S * n * nrhs, dtype=torch.float64, device="cuda")
+            _torch_sync()
             _check(library().bmi_rhs_assemble(self.space._h, self.data.fn, _dp(self.data.params), nrhs,
                                               self.data.kappa, _dptr(out, "torch"), MEM_DEVICE, _stream_ptr(stream)))
             v = torch.view_as_complex(out.view(-1, 2)) if cplx else out
@@ -632,6 +687,8 @@ class _Solver:
             zero = 0
         it = np.zeros(nrhs, dtype=np.int32)
         rs = np.zeros(nrhs)
+        if backend == "torch":
+            _torch_sync()
         st = fn(A._h, nrhs, _dptr(B, backend), _dptr(X, backend), zero, _ip(it), _dp(rs), mem, _stream_ptr(stream))
         self._info = _check(st, allow=(NOT_CONVERGED,))
         self._iters = it
@@ -669,3 +726,85 @@ class ConjugateGradient(_Solver):
             return L.bmi_solve_cg(h, nrhs, B, X, self.tol, self.maxit, zero, it, rs, mem, s)
 
         return self._solve(b, x0, fn, stream)
+
+
+# ---------------------------------------------------------------------------- serving
+class Server:
+    """Request loop on the GPU with continuous batching (bmi_server_*): submit() returns at once,
+    result() waits and returns (solution, info)."""
+
+    def __init__(self, hbm_budget_bytes=None, max_active=4, max_batch_rhs=64):
+        cfg = ServerConfig()
+        _check(library().bmi_server_config_default(C.byref(cfg)))
+        if hbm_budget_bytes is not None:
+            cfg.hbm_budget_bytes = float(hbm_budget_bytes)
+        cfg.max_active = int(max_active)
+        cfg.max_batch_rhs = int(max_batch_rhs)
+        h = C.c_void_p()
+        _check(library().bmi_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._lib = library()
+        self._meta = {}
+
+    def submit(self, operator, refinement_level, polynomial_degree, data, scale=(1.0, 1.0, 1.0),
+               geometry=SPHERE_DAT, n_cheb=9, eta=1.6, min_cluster_level=1, precision=FP64, tol=1e-10,
+               restart=30, maxit=5000):
+        """data: DirichletTrace (Laplace) or PlaneWaves (Helmholtz)"""
+        r = Request()
+        r.geometry_path = geometry.encode()
+        for k in range(3):
+            r.scale[k] = float(scale[k])
+        r.refinement_level = int(refinement_level)
+        r.polynomial_degree = int(polynomial_degree)
+        r.op = operator.desc(n_cheb=n_cheb, eta=eta, min_cluster_level=min_cluster_level, precision=precision)
+        params = np.ascontiguousarray(data.params, dtype=np.float64).reshape(-1)
+        r.data_fn = data.fn
+        r.nrhs = data.nrhs
+        r.data_params = _dp(params)
+        r.data_kappa = data.kappa
+        r.tol = float(tol)
+        r.restart = int(restart)
+        r.maxit = int(maxit)
+        rid = C.c_int64()
+        _check(library().bmi_server_submit(self._h, C.byref(r), C.byref(rid)))
+        self._meta[rid.value] = (data.nrhs, operator.scalar)
+        return rid.value
+
+    def result(self, rid, timeout=-1.0):
+        """(solution ndof x nrhs or None on failure, info dict); raises TimeoutError if pending"""
+        info = ResultInfo()
+        st = library().bmi_server_wait(self._h, rid, float(timeout), C.byref(info))
+        if st == PENDING:
+            raise TimeoutError("request %d still running" % rid)
+        _check(st)
+        nrhs, scalar = self._meta.pop(rid)
+        out = dict(status=info.status, ndof=info.ndof, submit_s=info.submit_s, start_s=info.start_s,
+                   end_s=info.end_s, latency_s=info.end_s - info.submit_s, iterations=info.iterations,
+                   residual=info.residual, batch_requests=info.batch_requests, batch_rhs=info.batch_rhs,
+                   cache_hit=bool(info.cache_hit))
+        if info.status != OK and info.status != NOT_CONVERGED:
+            out["error"] = library().bmi_last_error().decode()
+            _check(library().bmi_server_fetch(self._h, rid, None))
+            return None, out
+        S = 2 if scalar == np.complex128 else 1
+        x = np.zeros(S * info.ndof * nrhs)
+        _check(library().bmi_server_fetch(self._h, rid, _dp(x)))
+        X = (x.view(np.complex128) if S == 2 else x).reshape(nrhs, info.ndof).T
+        return (X[:, 0] if nrhs == 1 else X), out
+
+    def stats(self):
+        v = np.zeros(6, dtype=np.int64)
+        _check(library().bmi_server_stats(self._h, v.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(submitted=int(v[0]), finished=int(v[1]), queued=int(v[2]), running=int(v[3]),
+                    resident_operators=int(v[4]), reserved_bytes=int(v[5]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bmi_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

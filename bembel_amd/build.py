@@ -27,6 +27,7 @@ SOURCES = {
     "bmi_fmm.hip": [],
     "bmi_solver.hip": [],
     "bmi_capi.hip": [],
+    "bmi_server.hip": [],
 }
 
 

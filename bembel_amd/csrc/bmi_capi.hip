@@ -885,6 +885,28 @@ bmi_status bmi_h2_assemble(const bmi_space* s, const bmi_op_desc* op, void* stre
   BMI_API_END
 }
 
+bmi_status bmi_h2_estimate_bytes(const bmi_space* s, const bmi_op_desc* op, int nrhs, int64_t* bytes) {
+  BMI_API_BEGIN
+  check_op(s, op);
+  BMI_REQUIRE(bytes && nrhs >= 0, BMI_INVALID_ARG, "bad argument");
+  std::vector<Leaf> dense, lowrank;
+  const int mcl = op->min_cluster_level;
+  BMI_REQUIRE(s->M - mcl >= 0, BMI_UNSUPPORTED, "refinement level below min_cluster_level");
+  build_block_tree(s, op->eta, mcl, dense, lowrank);
+  const int64_t S = (op->kind == BMI_HELMHOLTZ_SL || op->kind == BMI_MAXWELL_SL) ? 2 : 1;
+  const int64_t vd = s->vd, nc = op->kind == BMI_MAXWELL_SL ? 2 : 1;
+  const int64_t dd = (int64_t(1) << (2 * mcl)) * s->I2, cd = nc * op->n_cheb * op->n_cheb;
+  const int L = s->M - mcl;
+  const int64_t lvl_total = int64_t(s->npatch) * (((int64_t(1) << (2 * (L + 1))) - 1) / 3) * cd;
+  int64_t b = S * vd * vd * (int64_t(dense.size()) * dd * dd + int64_t(lowrank.size()) * cd * cd) * 8;
+  if (op->precision != BMI_PRECISION_FP64) b += S * vd * vd * int64_t(lowrank.size()) * cd * cd * 4;
+  b += (int64_t(dense.size()) + int64_t(lowrank.size())) * 8;                           // CSR indices
+  b += std::max(nrhs, 1) * S * (3 * int64_t(s->nsuper) + (vd + 1) * lvl_total) * 8;   // workspace
+  b += int64_t(s->nel) * 25 * 16 * 8 + int64_t(s->val.size()) * 24;                    // tables, T
+  *bytes = b;
+  BMI_API_END
+}
+
 bmi_status bmi_h2_destroy(bmi_h2* h) {
   delete h;
   return BMI_OK;
@@ -989,6 +1011,12 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* hc, int nrhs, const double* Xs, dou
   const int S = h->cplx ? 2 : 1;
   if (has_far(h)) device_add(Ys, h->d_yf.ptr, static_cast<int64_t>(S) * h->s->nsuper * nrhs, st);
   BMI_CHECK_HIP(hipStreamSynchronize(st));
+  BMI_API_END
+}
+
+bmi_status bmi_synchronize(void* stream) {
+  BMI_API_BEGIN
+  BMI_CHECK_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   BMI_API_END
 }
 

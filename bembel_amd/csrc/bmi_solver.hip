@@ -530,7 +530,8 @@ __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* _
     const double nn = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
     double fr, fi = 0.0, gr = 0.0, gi = 0.0;
     if (fn == 0) {
-      const double d0 = X[0] - prm[0], d1 = X[1] - prm[1], d2 = X[2] - prm[2];
+      const double* x0 = prm + 3 * k;
+      const double d0 = X[0] - x0[0], d1 = X[1] - x0[1], d2 = X[2] - x0[2];
       fr = nn * w / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
     } else if (fn == 1) {
       const double* d = prm + 3 * k;
@@ -590,13 +591,12 @@ bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, in
     BMI_REQUIRE(fn >= 0 && fn <= 2, BMI_INVALID_ARG, "unknown data function");
     BMI_REQUIRE((fn == 2) == (s->form == BMI_FORM_DIVCONFORMING), BMI_INVALID_ARG,
                 "Dirichlet traces need a scalar space, the rotated tangential trace a DivConforming one");
-    BMI_REQUIRE(fn != 0 || nrhs == 1, BMI_INVALID_ARG, "the point source has one column");
     hipStream_t st = static_cast<hipStream_t>(stream);
     ensure_device_space(s, st);
     const int S = fn == 0 ? 1 : 2;
     DevBuf<double> prm, sup, dout;
-    const int per = fn == 0 ? 3 : (fn == 1 ? 3 : 6);
-    prm.upload(std::vector<double>(params, params + (fn == 0 ? 3 : per * nrhs)), st);
+    const int per = fn == 2 ? 6 : 3;
+    prm.upload(std::vector<double>(params, params + per * nrhs), st);
     sup.alloc(static_cast<size_t>(S) * s->nsuper * nrhs);
     const int64_t tot = static_cast<int64_t>(s->nel) * nrhs;
     const unsigned blocks = static_cast<unsigned>((tot + 127) / 128);

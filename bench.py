@@ -93,10 +93,12 @@ def unit_directions(rng, k):
 
 
 def timed(torch, fn, reps=1):
+    import bembel_amd as B
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
         out = fn()
+    B.synchronize()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / reps, out
 
@@ -128,10 +130,11 @@ def run_batched(B, torch, rng, steps):
     for name, HH in (("fp64", H), ("mixed", Hm)):
         for _ in range(2):
             HH.matvec(rhs)
-        t, y = timed(torch, lambda: HH.matvec(rhs), max(3, steps // 4))
+        t, y = timed(torch, lambda: HH.matvec(rhs, sync=False), max(3, steps // 4))
         out[name + "_matvec_ms"] = t * 1e3
         out[name + "_matvec_dof_rhs_per_s"] = n * nrhs / t
-        t1, _ = timed(torch, lambda: HH.matvec(rhs[:, :1]), max(3, steps // 4))
+        r1 = rhs[:, :1].contiguous()
+        t1, _ = timed(torch, lambda: HH.matvec(r1, sync=False), max(3, steps // 4))
         out[name + "_matvec_1rhs_ms"] = t1 * 1e3
     t, (x64, gm) = timed(torch, lambda: gmres_solve(B, H, rhs))
     out["fp64_gmres_s"] = t
@@ -281,13 +284,16 @@ def main():
     g = torch.Generator(device="cuda")
     g.manual_seed(1234)
     x = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) - 0.5
+    # steps are enqueued back to back on the library's stream; the clock stops when the last
+    # product has finished on the device
+    torch.cuda.synchronize()  # x is written by torch (its own HIP runtime)
     for _ in range(a.warmup):
-        y = H.matvec(x)
-    torch.cuda.synchronize()
+        y = H.matvec(x, sync=False)
+    B.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        y = H.matvec(x)
-    torch.cuda.synchronize()
+        y = H.matvec(x, sync=False)
+    B.synchronize()
     t_mv = (time.perf_counter() - t0) / a.steps
     value = n / t_mv
     log("matvec %.3f ms -> %.3e DOF/s" % (t_mv * 1e3, value))
@@ -295,12 +301,12 @@ def main():
     # mixed precision product (FP32 coupling blocks, FP64 near field): speed and error vs FP64
     Hm = B.H2Matrix(space, op, n_cheb=npts, precision=B.MIXED)
     for _ in range(a.warmup):
-        ym = Hm.matvec(x)
-    torch.cuda.synchronize()
+        ym = Hm.matvec(x, sync=False)
+    B.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        ym = Hm.matvec(x)
-    torch.cuda.synchronize()
+        ym = Hm.matvec(x, sync=False)
+    B.synchronize()
     t_mv_mixed = (time.perf_counter() - t0) / a.steps
     mixed_err = float(torch.linalg.norm(ym - y) / torch.linalg.norm(y))
     log("mixed matvec %.3f ms -> %.3e DOF/s, rel err vs FP64 %.2e" % (t_mv_mixed * 1e3, n / t_mv_mixed, mixed_err))

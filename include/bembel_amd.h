@@ -27,7 +27,8 @@ typedef enum {
   BMI_NOT_CONVERGED = 4,
   BMI_ADMISSION_REJECTED = 5,
   BMI_IO_ERROR = 6,
-  BMI_UNSUPPORTED = 7
+  BMI_UNSUPPORTED = 7,
+  BMI_PENDING = 8 /* a served request did not finish within the wait time */
 } bmi_status;
 
 /* operator kinds: the single-layer operators of the reference
@@ -139,6 +140,10 @@ bmi_status bmi_dense_assemble(const bmi_space* s, const bmi_op_desc* op, int sup
    All leaves are assembled on the device and stay resident. The space must outlive it. */
 bmi_status bmi_h2_assemble(const bmi_space* s, const bmi_op_desc* op, void* stream,
                            bmi_h2** out);
+/* device bytes an H2 assembly of `op` on `s` will hold, including the workspace of an
+   nrhs-column product (block tree built on the host, nothing allocated).  No counterpart in the
+   reference; used for admission by device-memory budget. */
+bmi_status bmi_h2_estimate_bytes(const bmi_space* s, const bmi_op_desc* op, int nrhs, int64_t* bytes);
 bmi_status bmi_h2_destroy(bmi_h2* h);
 /* info: [0] leaves [1] dense leaves [2] low-rank leaves [3] near-field pairs
    [4] dense leaf dimension [5] coupling dimension [6] FMM levels [7] rows=cols (DOFs)
@@ -166,6 +171,8 @@ bmi_status bmi_h2_matvec(const bmi_h2* h, int nrhs, const double* X, int64_t ldx
    default on) or launch it kernel by kernel.  No counterpart in the reference (its product is
    CPU code); exposed so callers can compare both modes. */
 bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable);
+/* wait for the library's work on `stream` (NULL: the default stream) */
+bmi_status bmi_synchronize(void* stream);
 /* same operator applied in the super space (no T, no T^T); device pointers only */
 bmi_status bmi_h2_matvec_super(const bmi_h2* h, int nrhs, const double* Xs, double* Ys,
                                void* stream);
@@ -173,7 +180,7 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* h, int nrhs, const double* Xs, doub
 /* ------------------------------------------------------------------ right-hand sides */
 /* DiscreteLinearForm<DirichletTrace>::compute (LinearForm/DiscreteLinearForm.hpp:44-83,
    LinearForm/DirichletTrace.hpp:31-70) for built-in data functions:
-   fn 0: g(x) = 1/|x - p|            (p = params[0..2]; real)
+   fn 0: g(x) = 1/|x - p_k|          (p_k = params[3k..3k+2]; real, one column per k)
    fn 1: g(x) = exp(i kappa d_k . x) (d_k = params[3k..3k+2]; complex, one column per k)
    fn 2: rotated tangential trace of a Hertz dipole field for the EFIE
          (LinearForm/RotatedTangentialTrace.hpp:39-82 with examples/Data.hpp:76-90 Dipole):
@@ -194,6 +201,56 @@ bmi_status bmi_solve_gmres(const bmi_h2* h, int nrhs, const double* B, double* X
 bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, double tol,
                         int maxit, int zero_guess, int* iters, double* resid, int mem,
                         void* stream);
+
+/* ------------------------------------------------------------------ serving */
+/* A request loop on one GPU (no counterpart in the reference, which solves one problem per
+   program).  Submitted solves are queued; a scheduler admits them while fewer than max_active
+   batches run and the device-memory budget holds the batch's workspace plus, for an operator
+   that is not resident yet, its estimated H2 state (idle resident operators are evicted least
+   recently used first).  Queued requests with equal operator and solver settings are merged into
+   one batched solve of up to max_batch_rhs right-hand sides.  Each batch runs on its own stream,
+   so batches overlap on the device; assembled operators stay resident between requests. */
+typedef struct bmi_server bmi_server;
+typedef struct {
+  double hbm_budget_bytes; /* device memory the server may hold */
+  int max_active;          /* batches on the GPU at once */
+  int max_batch_rhs;       /* right-hand sides merged into one solve */
+} bmi_server_config;
+typedef struct {
+  const char* geometry_path; /* .dat file (GeometryIO.hpp:25-122) */
+  double scale[3];           /* per-axis scaling of the control points (0 = 1) */
+  int refinement_level, polynomial_degree;
+  bmi_op_desc op;            /* Laplace or Helmholtz single layer */
+  int data_fn;               /* 0: point sources (Laplace), 1: plane waves (Helmholtz) */
+  int nrhs;
+  const double* data_params; /* 3 per right-hand side: source point or direction */
+  double data_kappa;         /* plane-wave wavenumber */
+  double tol;                /* relative residual */
+  int restart, maxit;        /* GMRES restart (Helmholtz), iteration limit */
+} bmi_request;
+typedef struct {
+  bmi_status status;         /* of the solve itself */
+  int64_t ndof;
+  int nrhs;
+  double submit_s, start_s, end_s; /* seconds since the server started */
+  int iterations;            /* largest over the request's columns */
+  double residual;           /* largest relative residual over the request's columns */
+  int batch_requests, batch_rhs; /* size of the batch the request was solved in */
+  int cache_hit;             /* the operator was resident when the batch started */
+} bmi_result_info;
+bmi_status bmi_server_config_default(bmi_server_config* cfg);
+bmi_status bmi_server_create(const bmi_server_config* cfg, bmi_server** out);
+/* queue a request (returns at once); *id names it in wait/fetch */
+bmi_status bmi_server_submit(bmi_server* srv, const bmi_request* req, int64_t* id);
+/* wait up to timeout_s seconds (< 0: no limit) for the request; BMI_PENDING if still running */
+bmi_status bmi_server_wait(bmi_server* srv, int64_t id, double timeout_s, bmi_result_info* info);
+/* copy the solution (ndof x nrhs, column-major, complex interleaved for Helmholtz) and release
+   the request */
+bmi_status bmi_server_fetch(bmi_server* srv, int64_t id, double* x);
+/* submitted, finished, queued, running batches, resident operators, reserved device bytes */
+bmi_status bmi_server_stats(bmi_server* srv, int64_t out[6]);
+/* waits for running batches, then frees everything */
+bmi_status bmi_server_destroy(bmi_server* srv);
 
 /* ------------------------------------------------------------------ measurement */
 /* per-kernel timing with HIP events recorded on the stream each kernel is launched on */

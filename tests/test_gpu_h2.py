@@ -208,3 +208,20 @@ def test_gpu_mixed_precision_matvec_and_refinement(gpu):
         assert gm.info() == B.OK
         assert rel(Hh64 @ out[-1], bh) <= 2e-10
     assert rel(out[1], out[0]) <= 1e-6
+
+
+def test_gpu_h2_torch_work_is_ordered_before_the_product(gpu):
+    """torch runs on its own HIP runtime: pending torch work that produces x finishes before the
+    library reads x, and the product is complete when torch reads the result."""
+    import torch
+    os_, ps, op, H = make(O.LAPLACE, 0j, 3, 2, 4)
+    n = ps.number_of_dofs
+    big = torch.randn(1 << 24, device="cuda", dtype=torch.float64)
+    x = torch.zeros(n, device="cuda", dtype=torch.float64)
+    for _ in range(20):  # a queue of torch kernels in front of the one that writes x
+        big = big * 1.0000001
+    x += big[:n] * 0 + 1.0
+    y = H @ x
+    z = y * 2.0  # torch reads the product right away
+    ref = H @ np.ones(n)
+    assert rel(z.cpu().numpy() / 2.0, ref) <= 1e-14

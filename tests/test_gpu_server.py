@@ -1,0 +1,80 @@
+"""Serving runtime on the GPU: results in a concurrent mix (batched, overlapping on the device)
+equal the results of the same requests solved alone, resident operators are reused, and the
+solutions satisfy the FP64 operator."""
+import numpy as np
+import pytest
+
+import bembel_amd as B
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def requests():
+    rng = np.random.default_rng(17)
+    out = []
+    for k in range(8):
+        scale = tuple(rng.uniform(0.5, 1.5, 3))
+        M = int(rng.choice([2, 3]))
+        P = int(rng.choice([1, 2]))
+        nrhs = int(rng.integers(1, 4))
+        if k % 2 == 0:
+            op = B.LaplaceSingleLayerOperator()
+            pts = rng.normal(size=(nrhs, 3))
+            pts = 3.0 * pts / np.linalg.norm(pts, axis=1, keepdims=True)
+            data = B.DirichletTrace(pts)
+        else:
+            kappa = float(rng.uniform(1, 4))
+            op = B.HelmholtzSingleLayerOperator()
+            op.set_wavenumber(kappa)
+            d = rng.normal(size=(nrhs, 3))
+            data = B.PlaneWaves(d / np.linalg.norm(d, axis=1, keepdims=True), kappa)
+        out.append(dict(op=op, M=M, P=P, data=data, scale=scale))
+    # a repeat of the first request (same operator): reuses the resident operator or joins its batch
+    out.append(dict(out[0]))
+    return out
+
+
+def solve_all(srv, reqs):
+    ids = [srv.submit(r["op"], r["M"], r["P"], r["data"], scale=r["scale"], n_cheb=5, tol=1e-10) for r in reqs]
+    return [srv.result(i, timeout=240) for i in ids]
+
+
+def test_gpu_server_mix_equals_alone(gpu):
+    reqs = requests()
+    alone = []
+    srv1 = B.Server(max_active=1, max_batch_rhs=1)
+    for r in reqs:
+        alone.append(solve_all(srv1, [r])[0])
+    srv1.close()
+    srv = B.Server(max_active=4, max_batch_rhs=64)
+    mixed = solve_all(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st["finished"] == len(reqs)
+    batched = 0
+    for (xa, ia), (xm, im), r in zip(alone, mixed, reqs):
+        assert ia["status"] == B.OK and im["status"] == B.OK, (ia, im)
+        assert ia["residual"] <= 1e-10 and im["residual"] <= 1e-10
+        # same tolerance-level answer alone and in the mix (batched columns, other streams busy)
+        assert rel(xm, xa) <= 1e-8, (r["M"], r["P"], rel(xm, xa))
+        batched += im["batch_requests"] > 1
+    # the repeated request found its operator resident or shared a batch with its twin
+    assert mixed[-1][1]["cache_hit"] or mixed[-1][1]["batch_requests"] > 1
+    print("batches with more than one request:", batched)
+
+
+def test_gpu_server_solution_satisfies_operator(gpu):
+    op = B.HelmholtzSingleLayerOperator()
+    op.set_wavenumber(3.0)
+    data = B.PlaneWaves([[0.0, 0.6, 0.8]], 3.0)
+    srv = B.Server()
+    x, info = srv.result(srv.submit(op, 3, 2, data, scale=(1.0, 0.7, 0.5), n_cheb=5, tol=1e-10))
+    srv.close()
+    sp = B.AnsatzSpace(B.Geometry(scale=(1.0, 0.7, 0.5)), 3, 2, op)
+    H = B.H2Matrix(sp, op, n_cheb=5)
+    b = B.DiscreteLinearForm(data, sp).compute().get_discrete_linear_form()
+    assert info["status"] == B.OK and rel(H @ x, b) <= 2e-10
