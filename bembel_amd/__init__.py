@@ -733,7 +733,7 @@ class Server:
     """Request loop on the GPU with continuous batching (bmi_server_*): submit() returns at once,
     result() waits and returns (solution, info)."""
 
-    def __init__(self, hbm_budget_bytes=None, max_active=4, max_batch_rhs=64):
+    def __init__(self, hbm_budget_bytes=None, max_active=1, max_batch_rhs=64):
         cfg = ServerConfig()
         _check(library().bmi_server_config_default(C.byref(cfg)))
         if hbm_budget_bytes is not None:

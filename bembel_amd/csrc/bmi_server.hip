@@ -331,7 +331,9 @@ extern "C" {
 bmi_status bmi_server_config_default(bmi_server_config* cfg) {
   if (!cfg) return BMI_INVALID_ARG;
   cfg->hbm_budget_bytes = 200.0 * (1ull << 30);
-  cfg->max_active = 4;
+  // one batch on the device at a time by default: several concurrent batches (one host thread
+  // and stream each) are supported by the scheduler but not yet validated on the device
+  cfg->max_active = 1;
   cfg->max_batch_rhs = 64;
   return BMI_OK;
 }

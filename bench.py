@@ -217,6 +217,67 @@ def run_headline(B, torch, rng, cpu_threads, with_cpu):
     return out
 
 
+def run_serving(B, torch, rng, nreq):
+    """Serving stream: Poisson arrivals (4 requests/s), each a Laplace or Helmholtz single-layer
+    solve on a 6-patch ellipsoid with semi-axes uniform in [0.5, 1.5], M in {3,4,5,6}, P in {1,2},
+    kappa uniform in [1, 8], 1-16 right-hand sides (point sources / plane waves), 5x5 Chebyshev
+    nodes, mixed precision, relative residual 1e-6 (GMRES restart 30, at most 3000 iterations)"""
+    import threading
+    reqs = []
+    t = 0.0
+    for k in range(nreq):
+        t += rng.exponential(1 / 4.0)
+        helm = rng.uniform() < 0.5
+        scale = tuple(rng.uniform(0.5, 1.5, 3))
+        M = int(rng.choice([3, 4, 5, 6]))
+        P = int(rng.choice([1, 2]))
+        nrhs = int(rng.integers(1, 17))
+        if helm:
+            kappa = float(rng.uniform(1, 8))
+            op = B.HelmholtzSingleLayerOperator()
+            op.set_wavenumber(kappa)
+            data = B.PlaneWaves(unit_directions(rng, nrhs), kappa)
+        else:
+            op = B.LaplaceSingleLayerOperator()
+            data = B.DirichletTrace(3.0 * unit_directions(rng, nrhs))
+        reqs.append(dict(at=t, op=op, M=M, P=P, data=data, scale=scale, helm=helm, nrhs=nrhs))
+    srv = B.Server(max_active=1, max_batch_rhs=64)
+    ids = [None] * nreq
+    t0 = time.perf_counter()
+
+    def feed():
+        for k, r in enumerate(reqs):
+            dt = r["at"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids[k] = srv.submit(r["op"], r["M"], r["P"], r["data"], scale=r["scale"], n_cheb=5,
+                                precision=B.MIXED, tol=1e-6, restart=30, maxit=3000)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    infos = []
+    for k in range(nreq):
+        _, info = srv.result(ids[k])
+        infos.append(info)
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = [i["latency_s"] for i in infos]
+    ok = sum(1 for i in infos if i["status"] == B.OK)
+    first = min(i["submit_s"] for i in infos)
+    last = max(i["end_s"] for i in infos)
+    out = {"requests": nreq, "ok": ok, "arrival_rate_per_s": 4.0, "wall_s": wall,
+           "p50_latency_s": pct(lat, 50), "p99_latency_s": pct(lat, 99), "max_latency_s": max(lat),
+           "solves_per_s": nreq / (last - first), "rhs_solved": int(sum(r["nrhs"] for r in reqs)),
+           "batched_requests": int(sum(1 for i in infos if i["batch_requests"] > 1)),
+           "precision": "mixed (FP32 coupling, FP64 near field, FP64 refinement)", "tol": 1e-6,
+           "mix": [{"M": r["M"], "P": r["P"], "helmholtz": r["helm"], "nrhs": r["nrhs"], "latency_s": i["latency_s"],
+                    "iterations": i["iterations"], "status": i["status"]} for r, i in zip(reqs, infos)]}
+    log("serving: %d requests (%d ok) in %.1f s, p50 %.2f s, p99 %.2f s, %.2f solves/s" % (
+        nreq, ok, wall, out["p50_latency_s"], out["p99_latency_s"], out["solves_per_s"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +292,7 @@ def main():
     ap.add_argument("--no-breakdown", action="store_true", help="skip the per-kernel assembly breakdown")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-RHS Helmholtz workload")
     ap.add_argument("--no-headline", action="store_true", help="skip the Helmholtz 24,576-element assembly + GMRES")
+    ap.add_argument("--serving", type=int, default=16, help="requests in the serving stream (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the H2 path on a single MI355X "
@@ -355,6 +417,7 @@ def main():
     threads = a.cpu_threads or min(16, os.cpu_count() or 1)
     batched = None if a.no_batched else run_batched(B, torch, rng, a.steps)
     headline = None if a.no_headline else run_headline(B, torch, rng, threads, not a.no_cpu_baseline)
+    serving = run_serving(B, torch, np.random.default_rng(55), a.serving) if a.serving > 0 else None
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
@@ -431,6 +494,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_64rhs": batched,
         "helmholtz_assembly_gmres": headline,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
