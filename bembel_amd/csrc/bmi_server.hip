@@ -1,7 +1,8 @@
 // Serving runtime: a request loop that batches BEM solves as they arrive on one GPU.
 //
 // Requests (geometry, refinement level, degree, operator, right-hand sides, solver settings) are
-// queued by bmi_server_submit and admitted by a scheduler thread while
+// queued by bmi_server_submit and admitted by a scheduler thread (smallest problems first, any
+// request older than a few seconds before all others) while
 //   - fewer than `max_active` batches are on the GPU, and
 //   - the device-memory budget holds the batch's workspace plus, when the operator is not yet
 //     resident, its assembled H2 state (estimated from the block tree before anything is
@@ -22,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -171,7 +173,23 @@ void bmi_server::run_scheduler() {
         qi = static_cast<size_t>(-1);
       }
     }
-    // admit in arrival order whatever fits
+    // admission order: requests that waited longer than aging_s first (in arrival order), then
+    // the smallest problems first (degrees of freedom x right-hand sides), so that short solves
+    // do not queue behind long ones
+    {
+      const double tnow = now(), aging_s = 5.0;
+      auto size_of = [&](const std::shared_ptr<Request>& q) {
+        auto oi = ops.find(q->key);
+        const int64_t nd = oi != ops.end() ? oi->second->ndof : (q->pending ? q->pending->ndof : -1);
+        return nd < 0 ? std::numeric_limits<double>::max() : static_cast<double>(nd) * q->nrhs;
+      };
+      std::stable_sort(queue.begin(), queue.end(), [&](const std::shared_ptr<Request>& a, const std::shared_ptr<Request>& b) {
+        const bool oa = tnow - a->t_submit > aging_s, ob = tnow - b->t_submit > aging_s;
+        if (oa != ob) return oa;
+        if (oa) return a->t_submit < b->t_submit;
+        return size_of(a) < size_of(b);
+      });
+    }
     bool admitted = false;
     for (size_t qi = 0; qi < queue.size() && active < cfg.max_active; ++qi) {
       std::shared_ptr<Request> r = queue[qi];
