@@ -47,7 +47,8 @@ class OpDesc(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_active", C.c_int), ("max_batch_rhs", C.c_int)]
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_active", C.c_int), ("max_batch_rhs", C.c_int),
+                ("slice_iterations", C.c_int)]
 
 
 class Request(C.Structure):
@@ -60,7 +61,7 @@ class Request(C.Structure):
 class ResultInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("ndof", C.c_int64), ("nrhs", C.c_int), ("submit_s", C.c_double),
                 ("start_s", C.c_double), ("end_s", C.c_double), ("iterations", C.c_int), ("residual", C.c_double),
-                ("batch_requests", C.c_int), ("batch_rhs", C.c_int), ("cache_hit", C.c_int)]
+                ("batch_requests", C.c_int), ("batch_rhs", C.c_int), ("cache_hit", C.c_int), ("slices", C.c_int)]
 
 
 _lib = None
@@ -731,15 +732,19 @@ class ConjugateGradient(_Solver):
 # ---------------------------------------------------------------------------- serving
 class Server:
     """Request loop on the GPU with continuous batching (bmi_server_*): submit() returns at once,
-    result() waits and returns (solution, info)."""
+    result() waits and returns (solution, info).  Up to max_active batches are admitted and take
+    turns on the device in slices of slice_iterations GMRES steps."""
 
-    def __init__(self, hbm_budget_bytes=None, max_active=1, max_batch_rhs=64):
+    def __init__(self, hbm_budget_bytes=None, max_active=None, max_batch_rhs=64, slice_iterations=None):
         cfg = ServerConfig()
         _check(library().bmi_server_config_default(C.byref(cfg)))
         if hbm_budget_bytes is not None:
             cfg.hbm_budget_bytes = float(hbm_budget_bytes)
-        cfg.max_active = int(max_active)
+        if max_active is not None:
+            cfg.max_active = int(max_active)
         cfg.max_batch_rhs = int(max_batch_rhs)
+        if slice_iterations is not None:
+            cfg.slice_iterations = int(slice_iterations)
         h = C.c_void_p()
         _check(library().bmi_server_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -781,7 +786,7 @@ class Server:
         out = dict(status=info.status, ndof=info.ndof, submit_s=info.submit_s, start_s=info.start_s,
                    end_s=info.end_s, latency_s=info.end_s - info.submit_s, iterations=info.iterations,
                    residual=info.residual, batch_requests=info.batch_requests, batch_rhs=info.batch_rhs,
-                   cache_hit=bool(info.cache_hit))
+                   cache_hit=bool(info.cache_hit), slices=info.slices)
         if info.status != OK and info.status != NOT_CONVERGED:
             out["error"] = library().bmi_last_error().decode()
             _check(library().bmi_server_fetch(self._h, rid, None))

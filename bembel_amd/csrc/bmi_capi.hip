@@ -106,6 +106,12 @@ struct bmi_h2 {
   DevBuf<float> d_coup32;
   DevBuf<unsigned short> d_coup16;  // BF16 copies of the coarsest coupling blocks (MIXED_BF16)
   int ncoarse = 0;
+  // MIXED single-component operators: FP32 copy of the near-field blocks and the FP32 node
+  // tables from which the single-RHS product evaluates the coupling blocks on the fly
+  bool fly = false;
+  DevBuf<float> d_dense32;
+  DevBuf<float4> d_fly_pts;
+  DevBuf<double4> d_fly_org;
   // dense CSR over leaf-level clusters
   int ncl_leaf = 0;
   DevBuf<int> d_drp, d_dsrc;
@@ -135,6 +141,7 @@ struct bmi_h2 {
   // sweep has its own pair
   int leaf_variant = -1, cpl_variant = -1;
   int fused_leaf_variant = -1, fused_cpl_variant = -1;
+  int fly_variant = -1;
   void drop_graphs() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     graphs.clear();
@@ -147,7 +154,8 @@ struct bmi_h2 {
     if (ev_join) (void)hipEventDestroy(ev_join);
   }
   size_t bytes() const {
-    return d_dense.bytes() + d_coup.bytes() + d_coup32.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() +
+    return d_dense.bytes() + d_coup.bytes() + d_coup32.bytes() + d_coup16.bytes() + d_dense32.bytes() +
+           d_fly_pts.bytes() + d_fly_org.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() +
            d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_xs.bytes() +
            d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes() + d_yf.bytes();
   }
@@ -285,6 +293,20 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
       h->fused_leaf_variant = pl;
       h->fused_cpl_variant = pc;
     }
+    if (h->fly) {
+      int pv = -1;
+      for (int v = 0; v < 2; ++v) {
+        bool ok = true;
+        const float t = time_it([&] {
+          ok = launch_sweep_fly(h->cplx, v, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs,
+                                h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr,
+                                h->d_lvl_off.ptr, h->d_fly_pts.ptr, h->d_fly_org.ptr, h->op.kappa_re, h->op.kappa_im,
+                                h->d_fw[0].ptr, h->d_bw.ptr, st);
+        });
+        if (ok && (pv < 0 || t < best)) best = t, pv = v;
+      }
+      h->fly_variant = pv;
+    }
   } else {
     h->cpl_variant = 0;
   }
@@ -297,6 +319,24 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
 static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, double* yf, bool reduced,
                         hipStream_t st) {
   const bool far = has_far(h);
+  if (far && reduced && h->fly && nrhs == 1) {
+    // reduced operator, one RHS: FP32 near-field blocks and the coupling evaluated on the fly
+    const bmi_space* s = h->s;
+    const int64_t N = s->nsuper;
+    {
+      TimedScope ts("matvec_forward", st);
+      launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
+                     h->d_fw[0].ptr, h->d_lvl_off.ptr, st, h->d_bw.ptr);
+    }
+    if (launch_sweep_fly(h->cplx, std::max(h->fly_variant, 0), h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys, h->ntgt,
+                         h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, h->d_lvl_off.ptr, h->d_fly_pts.ptr,
+                         h->d_fly_org.ptr, h->op.kappa_re, h->op.kappa_im, h->d_fw[0].ptr, h->d_bw.ptr, st)) {
+      TimedScope ts("matvec_backward", st);
+      launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
+                      h->d_lvl_off.ptr, yf, N, 0, st);
+      return;
+    }
+  }
   if (far && nrhs == 1 && h->vd == 1 && h->fused_leaf_variant >= 0 && h->fused_cpl_variant >= 0) {
     // one RHS: forward transformation (also clears bw), both sweeps in one launch, backward
     const bmi_space* s = h->s;
@@ -475,6 +515,15 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
     DevBuf<int4> d_lv;
     d_lv.upload(lv, st);
     launch_coupling_assemble(op.kind, d_lv.ptr, nl, h->np2, d_ptrs.ptr, op.kappa_re, op.kappa_im, h->d_coup.ptr, st);
+    if (op.precision == BMI_PRECISION_MIXED && vd == 1) {
+      // node tables of every level at the cluster offsets of the multipole arrays
+      h->d_fly_pts.alloc(static_cast<size_t>(h->lvl_total));
+      h->d_fly_org.alloc(static_cast<size_t>(h->lvl_total / h->cd));
+      for (int l = 0; l <= h->L; ++l)
+        launch_fly_points(pts[l].ptr, ncl_at(s, l), h->np2, h->d_fly_pts.ptr + h->lvl_off[l],
+                          h->d_fly_org.ptr + h->lvl_off[l] / h->cd, st);
+      h->fly = true;
+    }
     BMI_CHECK_HIP(hipStreamSynchronize(st));
   }
   // targets (level, cluster) and CSR over the low-rank leaves (sorted by level, c1, c2)
@@ -496,6 +545,10 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   if (op.precision != BMI_PRECISION_FP64) h->d_coup32.alloc(h->d_coup.n);
   BMI_CHECK_HIP(hipStreamSynchronize(st));
   if (op.precision != BMI_PRECISION_FP64 && h->d_coup.n) convert_to_fp32(h->d_coup.ptr, h->d_coup32.ptr, h->d_coup.n, st);
+  if (h->fly) {
+    h->d_dense32.alloc(h->d_dense.n);
+    if (h->d_dense.n) convert_to_fp32(h->d_dense.ptr, h->d_dense32.ptr, h->d_dense.n, st);
+  }
   if (op.precision == BMI_PRECISION_MIXED_BF16) {
     // blocks are sorted by level: the coarse ones (levels 0 and 1) come first
     h->ncoarse = 0;
@@ -900,6 +953,8 @@ bmi_status bmi_h2_estimate_bytes(const bmi_space* s, const bmi_op_desc* op, int 
   const int64_t lvl_total = int64_t(s->npatch) * (((int64_t(1) << (2 * (L + 1))) - 1) / 3) * cd;
   int64_t b = S * vd * vd * (int64_t(dense.size()) * dd * dd + int64_t(lowrank.size()) * cd * cd) * 8;
   if (op->precision != BMI_PRECISION_FP64) b += S * vd * vd * int64_t(lowrank.size()) * cd * cd * 4;
+  if (op->precision == BMI_PRECISION_MIXED && vd == 1)
+    b += S * int64_t(dense.size()) * dd * dd * 4 + lvl_total * 16 + lvl_total / cd * 32;  // FP32 near field, nodes
   b += (int64_t(dense.size()) + int64_t(lowrank.size())) * 8;                           // CSR indices
   b += std::max(nrhs, 1) * S * (3 * int64_t(s->nsuper) + (vd + 1) * lvl_total) * 8;   // workspace
   b += int64_t(s->nel) * 25 * 16 * 8 + int64_t(s->val.size()) * 24;                    // tables, T

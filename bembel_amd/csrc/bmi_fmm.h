@@ -50,6 +50,15 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
                         const double* fw, double* bw, hipStream_t st, const unsigned short* Fc = nullptr,
                         int ncoarse = 0);
+// FP32 node tables (offsets from each cluster's node 0, weights; node 0 in FP64) of one level
+void launch_fly_points(const double* pts, int ncl, int np2, float4* out, double4* org, hipStream_t st);
+// single-RHS sweep of the reduced operator: FP32 near-field blocks, coupling evaluated on the fly
+// in FP32 (ys and bw written); false when no kernel exists for these sizes
+// variant: near-field blocks loaded per iteration, 0 = one, 1 = two
+bool launch_sweep_fly(bool cplx, int variant, int ncl, int dd, const int* drp, const int* dsrc, const float* dF32,
+                      const double* xs, double* ys, int ntgt, int cd, const int2* tgt, const int* crp,
+                      const int* csrc, const int64_t* lvl_off, const float4* pts, const double4* org,
+                      double kre, double kim, const double* fw, double* bw, hipStream_t st);
 void launch_leaf_backward(bool cplx, int ncl, int nrhs, int dd, const double* mom, int rows, const double* bw,
                           double* yf, int64_t ldy, int64_t yseg, hipStream_t st);
 

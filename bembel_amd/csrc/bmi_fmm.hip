@@ -570,7 +570,7 @@ struct LeafArgs {
   int ncl, nrhs;
   const int* rp;
   const int* src;
-  const double* F;
+  const void* F;  // double, or float for the reduced-precision copy
   int64_t fstride, fcomp;
   const double* xs;
   int64_t ldxs, xseg;
@@ -579,13 +579,13 @@ struct LeafArgs {
   int accumulate;
 };
 
-template <bool C, int DD, int RT, int UB>
+template <bool C, int DD, int RT, int UB, typename FT = double>
 __device__ __forceinline__ void leaf_wave(int c, const LeafArgs& a) {
   constexpr int S = C ? 2 : 1;
   const int lane = threadIdx.x & 63;
   const int i = lane < DD ? lane : 0;  // spare lanes shadow row 0 and store nothing
   const int p0 = a.rp[c], p1 = a.rp[c + 1];
-  const double* __restrict__ F = a.F;
+  const FT* __restrict__ F = static_cast<const FT*>(a.F);
   const double* __restrict__ xs = a.xs;
   for (int k0 = 0; k0 < a.nrhs; k0 += RT) {
     double ar[RT], ai[RT];
@@ -593,15 +593,18 @@ __device__ __forceinline__ void leaf_wave(int c, const LeafArgs& a) {
     for (int r = 0; r < RT; ++r) ar[r] = 0.0, ai[r] = 0.0;
     for (int p = p0; p < p1; p += UB) {
       const int nb = p1 - p < UB ? p1 - p : UB;
-      double fr[UB][DD], fi[UB][C ? DD : 1];
+      FT fr[UB][DD], fi[UB][C ? DD : 1];
 #pragma unroll
       for (int u = 0; u < UB; ++u) {
         if (u >= nb) break;
-        const double* f = F + S * (static_cast<int64_t>(p + u) * a.fstride + a.fcomp);
+        const FT* f = F + S * (static_cast<int64_t>(p + u) * a.fstride + a.fcomp);
 #pragma unroll
         for (int j = 0; j < DD; ++j) {
-          if constexpr (C) {
+          if constexpr (C && std::is_same<FT, double>::value) {
             const double2 v = *reinterpret_cast<const double2*>(f + 2 * (j * DD + i));
+            fr[u][j] = v.x, fi[u][j] = v.y;
+          } else if constexpr (C) {
+            const float2 v = *reinterpret_cast<const float2*>(f + 2 * (j * DD + i));
             fr[u][j] = v.x, fi[u][j] = v.y;
           } else {
             fr[u][j] = f[j * DD + i];
@@ -619,12 +622,14 @@ __device__ __forceinline__ void leaf_wave(int c, const LeafArgs& a) {
           const double* x = xs + S * (static_cast<int64_t>(k) * a.ldxs + a.xseg + static_cast<int64_t>(s2) * DD);
 #pragma unroll
           for (int j = 0; j < DD; ++j) {
+            const double f_r = static_cast<double>(fr[u][j]);
             if constexpr (C) {
+              const double f_i = static_cast<double>(fi[u][j]);
               const double xr = x[2 * j], xi = x[2 * j + 1];
-              ar[r] += fr[u][j] * xr - fi[u][j] * xi;
-              ai[r] += fr[u][j] * xi + fi[u][j] * xr;
+              ar[r] += f_r * xr - f_i * xi;
+              ai[r] += f_r * xi + f_i * xr;
             } else {
-              ar[r] += fr[u][j] * x[j];
+              ar[r] += f_r * x[j];
             }
           }
         }
@@ -806,6 +811,231 @@ __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, in
   }
 }
 
+// ------------------------------------------------------------------ many right-hand sides
+// Block rows for many right-hand sides: one wave per row owner (leaf cluster or coupling target)
+// and 64 columns, lane k owns column k and keeps all ROWS rows of its column in registers.  The
+// blocks are wave-uniform and read through the scalar cache, each once per 64 columns, so the
+// products run at the arithmetic rate instead of re-reading every block per few columns.
+// Addresses: X(s, j, k) = x[S * (xoff + s * xcs + k * xks + j)], Y(r, i, k) likewise with the
+// owner's cluster; coupling owners shift both by their level's start.
+struct MultiArgs {
+  int nown, nrhs;
+  const int2* tgt;  // coupling: (level, cluster) of each owner; near field: null (owner = cluster)
+  const int64_t* lvl_off;
+  const int* rp;
+  const int* src;
+  const void* F;
+  int64_t fstride, fcomp;
+  const double* x;
+  int64_t xoff, xcs, xks;
+  double* y;
+  int64_t yoff, ycs, yks;
+  int accumulate;
+};
+
+template <bool C, typename FT, int ROWS>
+__global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
+  constexpr int S = C ? 2 : 1;
+  constexpr bool F32 = std::is_same<FT, float>::value;
+  const int r = blockIdx.x;
+  const int k = static_cast<int>(blockIdx.y) * 64 + static_cast<int>(threadIdx.x);
+  const bool valid = k < a.nrhs;
+  const int kk = valid ? k : 0;
+  int cl = r;
+  int64_t xo = a.xoff, yo = a.yoff;
+  if (a.tgt) {
+    const int2 T = a.tgt[r];
+    cl = T.y;
+    const int64_t lo = a.lvl_off[T.x] * a.nrhs;  // level start of [level][cluster][rhs][row]
+    xo += lo;
+    yo += lo;
+  }
+  double ar[ROWS], ai[C ? ROWS : 1];
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    ar[i] = 0.0;
+    if constexpr (C) ai[i] = 0.0;
+  }
+  const FT* __restrict__ F = static_cast<const FT*>(a.F);
+  const int p0 = a.rp[r], p1 = a.rp[r + 1];
+  for (int p = p0; p < p1; ++p) {
+    const FT* f = F + S * (static_cast<int64_t>(p) * a.fstride + a.fcomp);
+    const double* x = a.x + S * (xo + static_cast<int64_t>(a.src[p]) * a.xcs + static_cast<int64_t>(kk) * a.xks);
+    if constexpr (F32) {
+      // FP32 blocks: FP32 products summed per block, the block sums added in FP64
+      float br[ROWS], bi[C ? ROWS : 1];
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i) {
+        br[i] = 0.f;
+        if constexpr (C) bi[i] = 0.f;
+      }
+      for (int j = 0; j < ROWS; ++j) {
+        const float xr = static_cast<float>(x[S * j]);
+        const float xi = C ? static_cast<float>(x[S * j + 1]) : 0.f;
+        const float* fj = reinterpret_cast<const float*>(f) + S * j * ROWS;
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) {
+          const float fr = fj[S * i];
+          if constexpr (C) {
+            const float fi = fj[S * i + 1];
+            br[i] += fr * xr - fi * xi;
+            bi[i] += fr * xi + fi * xr;
+          } else {
+            br[i] += fr * xr;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i) {
+        ar[i] += static_cast<double>(br[i]);
+        if constexpr (C) ai[i] += static_cast<double>(bi[i]);
+      }
+    } else {
+      for (int j = 0; j < ROWS; ++j) {
+        const double xr = x[S * j];
+        const double xi = C ? x[S * j + 1] : 0.0;
+        const double* fj = reinterpret_cast<const double*>(f) + S * j * ROWS;
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i) {
+          const double fr = fj[S * i];
+          if constexpr (C) {
+            const double fi = fj[S * i + 1];
+            ar[i] += fr * xr - fi * xi;
+            ai[i] += fr * xi + fi * xr;
+          } else {
+            ar[i] += fr * xr;
+          }
+        }
+      }
+    }
+  }
+  if (!valid) return;
+  double* y = a.y + S * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k) * a.yks);
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    double vr = ar[i], vi = C ? ai[i] : 0.0;
+    if (a.accumulate) {
+      vr += y[S * i];
+      if constexpr (C) vi += y[S * i + 1];
+    }
+    y[S * i] = vr;
+    if constexpr (C) y[S * i + 1] = vi;
+  }
+}
+
+// ------------------------------------------------------------------ coupling evaluated on the fly
+// Reduced-precision far field without stored coupling blocks: the interpolated kernel
+// F(i, j) = w_i w_j k(|x_i - y_j|) of an admissible block (the entry k_coupling_assemble stores)
+// is recomputed in FP32 from the Chebyshev surface points of the two clusters, which costs far
+// fewer bytes than reading the block.  Points are kept as FP32 offsets from their cluster's first
+// point, whose FP64 position gives the block's centre shift, so |x_i - y_j| keeps FP32 relative
+// precision for close clusters; each block's FP32 partial sums are added in FP64.
+struct FlyArgs {
+  int ntgt;
+  const int2* tgt;
+  const int* rp;
+  const int* src;
+  const int64_t* lvl_off;  // per level, in clusters x CD rows
+  const float4* pts;       // [level cluster][node]: offset from the cluster's node 0 (x, y, z), weight
+  const double4* org;      // [level cluster]: node 0 (x, y, z)
+  const double* fw;
+  double* bw;
+  float krev, kim;         // kappa / (2 pi) (phase in revolutions), Im kappa
+};
+
+template <bool C, int CD>
+__device__ __forceinline__ void coupling_fly_wave(int t, const FlyArgs& a) {
+  constexpr int S = C ? 2 : 1;
+  constexpr int CDP = CD <= 16 ? 16 : (CD <= 32 ? 32 : 64);
+  constexpr int G = 64 / CDP;
+  const int lane = threadIdx.x & 63;
+  const int g = lane / CDP, li = lane % CDP;
+  const int i = li < CD ? li : 0;
+  const int2 T = a.tgt[t];
+  const int64_t off = a.lvl_off[T.x];  // rows of the level start (one RHS)
+  const int64_t coff = off / CD;       // clusters of the level start
+  const int p0 = a.rp[t], p1 = a.rp[t + 1];
+  const double4 ot = a.org[coff + T.y];
+  const float4 xi = a.pts[(coff + T.y) * CD + i];
+  double ar = 0.0, ai = 0.0;
+  for (int p = p0 + g; p < p1; p += G) {
+    const int s2 = a.src[p];
+    const double4 os = a.org[coff + s2];
+    // x_i - y_j = (o_t - o_s) + (xi - yj)
+    const float ex = static_cast<float>(ot.x - os.x) + xi.x;
+    const float ey = static_cast<float>(ot.y - os.y) + xi.y;
+    const float ez = static_cast<float>(ot.z - os.z) + xi.z;
+    const float4* yp = a.pts + (coff + s2) * CD;
+    const double* x = a.fw + S * (off + static_cast<int64_t>(s2) * CD);
+    float br = 0.f, bi = 0.f;
+#pragma unroll
+    for (int j = 0; j < CD; ++j) {
+      const float4 y = yp[j];
+      const float dx = ex - y.x, dy = ey - y.y, dz = ez - y.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      if constexpr (C) {
+        const float vr = y.w * static_cast<float>(x[2 * j]), vi = y.w * static_cast<float>(x[2 * j + 1]);
+        const float r = r2 * rinv;
+        float mag = rinv;
+        if (a.kim != 0.f) mag *= __expf(a.kim * r);
+        float ph = r * a.krev;
+        ph -= rintf(ph);
+        const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+        const float kr = mag * cs, ki = -mag * sn;
+        br += kr * vr - ki * vi;
+        bi += kr * vi + ki * vr;
+      } else {
+        br += rinv * (y.w * static_cast<float>(x[j]));
+      }
+    }
+    ar += static_cast<double>(br);
+    if constexpr (C) ai += static_cast<double>(bi);
+  }
+#pragma unroll
+  for (int m = CDP; m < 64; m <<= 1) {
+    ar += __shfl_xor(ar, m, 64);
+    if constexpr (C) ai += __shfl_xor(ai, m, 64);
+  }
+  if (g == 0 && li < CD) {
+    const double wq = static_cast<double>(xi.w) * (0.25 * M_1_PI);
+    double* y = a.bw + S * (off + static_cast<int64_t>(T.y) * CD + li);
+    y[0] = ar * wq;
+    if constexpr (C) y[1] = ai * wq;
+  }
+}
+
+// single-RHS product sweep of the reduced operator: FP32 near-field blocks and the coupling
+// evaluated on the fly in one launch.  Near-field workgroups stream HBM, coupling workgroups
+// compute; the two kinds are interleaved along the grid so that both run at once on every CU.
+template <bool C, int DD, int CD, int UBN>
+__global__ __launch_bounds__(256) void k_sweep_fly(LeafArgs la, FlyArgs fa, int nb_near, int nb_cpl) {
+  const int64_t b = blockIdx.x, tot = static_cast<int64_t>(nb_near) + nb_cpl;
+  const int64_t cb = b * nb_cpl / tot, cb1 = (b + 1) * nb_cpl / tot;
+  const int w = static_cast<int>(threadIdx.x >> 6);
+  if (cb1 > cb) {
+    const int t = __builtin_amdgcn_readfirstlane(static_cast<int>(cb) * 4 + w);
+    if (t < fa.ntgt) coupling_fly_wave<C, CD>(t, fa);
+  } else {
+    const int c = __builtin_amdgcn_readfirstlane(static_cast<int>(b - cb) * 4 + w);
+    if (c < la.ncl) leaf_wave<C, DD, 1, UBN, float>(c, la);
+  }
+}
+
+// FP32 node tables of one level for the on-the-fly coupling: offsets from the cluster's node 0
+// and the weights, from the FP64 table of k_cluster_points
+__global__ void k_fly_points(const double* __restrict__ pts, int ncl, int np2, float4* __restrict__ out,
+                             double4* __restrict__ org) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(ncl) * np2) return;
+  const int64_t c = gid / np2;
+  const double* p = pts + gid * CP_STRIDE;
+  const double* o = pts + c * np2 * CP_STRIDE;
+  out[gid] = make_float4(static_cast<float>(p[0] - o[0]), static_cast<float>(p[1] - o[1]),
+                         static_cast<float>(p[2] - o[2]), static_cast<float>(p[3]));
+  if (gid % np2 == 0) org[c] = make_double4(o[0], o[1], o[2], 0.0);
+}
+
 // BF16 copy (round to nearest even of the FP32 value) of n doubles
 __global__ void k_to_bf16(const double* __restrict__ src, unsigned short* __restrict__ dst, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -817,6 +1047,8 @@ __global__ void k_to_bf16(const double* __restrict__ src, unsigned short* __rest
 
 // ------------------------------------------------------------------ host launchers
 static inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
+// from this many right-hand sides on, block rows run one wave per owner and 64 columns
+constexpr int MULTI_MIN_RHS = 8;
 
 void launch_spmv(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* x,
                  int64_t ldx, double* y, int64_t ldy, const double* alpha, const double* beta, hipStream_t st,
@@ -873,6 +1105,28 @@ void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs
   const int64_t n = static_cast<int64_t>(ntgt) * nrhs * cd;
   if (n == 0) return;
   TimedScope ts("coupling_apply", st);
+  if (nrhs >= MULTI_MIN_RHS && (!Fc || ncoarse == 0)) {
+    const MultiArgs ma{ntgt, nrhs, tgt, lvl_off, rp, src, F, fstride, fcomp, fw, 0,
+                       static_cast<int64_t>(nrhs) * cd, cd, bw, 0, static_cast<int64_t>(nrhs) * cd, cd, accumulate};
+    const dim3 grid(static_cast<unsigned>(ntgt), static_cast<unsigned>((nrhs + 63) / 64));
+#define BMI_CPL_M(CC, FTT, CDD)                                                                   \
+  if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                       \
+    hipLaunchKernelGGL((k_multi_rows<CC, FTT, CDD>), grid, dim3(64), 0, st, ma);                  \
+    BMI_CHECK_HIP(hipGetLastError());                                                             \
+    return;                                                                                       \
+  }
+    BMI_CPL_M(false, double, 16)
+    BMI_CPL_M(false, float, 16)
+    BMI_CPL_M(true, double, 16)
+    BMI_CPL_M(true, float, 16)
+    BMI_CPL_M(false, double, 25)
+    BMI_CPL_M(false, float, 25)
+    BMI_CPL_M(true, double, 25)
+    BMI_CPL_M(true, float, 25)
+    BMI_CPL_M(true, double, 32)
+    BMI_CPL_M(true, float, 32)
+#undef BMI_CPL_M
+  }
   const dim3 wgrid(static_cast<unsigned>((ntgt + 3) / 4));
   // wave-per-target kernels for the block sizes of the configurations (np 4, 5; Maxwell np 4)
 #define BMI_CPL_W(CC, FTT, CDD)                                                                              \
@@ -1002,6 +1256,22 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
                  int64_t yseg, int accumulate, hipStream_t st) {
   const int64_t n = static_cast<int64_t>(ncl) * nrhs * dd;
   TimedScope ts("matvec_nearfield", st);
+  if (nrhs >= MULTI_MIN_RHS) {
+    const MultiArgs ma{ncl, nrhs, nullptr, nullptr, rp, src, F, fstride, fcomp, xs, xseg, dd, ldxs, ys, yseg, dd, ldys,
+                       accumulate};
+    const dim3 grid(static_cast<unsigned>(ncl), static_cast<unsigned>((nrhs + 63) / 64));
+#define BMI_LEAF_M(CC, DDD)                                                                       \
+  if (cplx == CC && dd == DDD) {                                                                  \
+    hipLaunchKernelGGL((k_multi_rows<CC, double, DDD>), grid, dim3(64), 0, st, ma);               \
+    BMI_CHECK_HIP(hipGetLastError());                                                             \
+    return;                                                                                       \
+  }
+    BMI_LEAF_M(false, 16)
+    BMI_LEAF_M(true, 16)
+    BMI_LEAF_M(false, 36)
+    BMI_LEAF_M(true, 36)
+#undef BMI_LEAF_M
+  }
   const dim3 wgrid(static_cast<unsigned>((ncl + 3) / 4));
   // wave-per-cluster kernels for the leaf sizes 4 (P+1)^2 of P = 1, 2
 #define BMI_LEAF_W(CC, DDD)                                                                                     \
@@ -1059,6 +1329,42 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
   BMI_FUSED_V(true, 16, double, 16)
 #undef BMI_FUSED_V
 #undef BMI_FUSED
+  return false;
+}
+
+void launch_fly_points(const double* pts, int ncl, int np2, float4* out, double4* org, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(ncl) * np2;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fly_points, dim3(nblk(n)), dim3(256), 0, st, pts, ncl, np2, out, org);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+bool launch_sweep_fly(bool cplx, int variant, int ncl, int dd, const int* drp, const int* dsrc, const float* dF32,
+                      const double* xs, double* ys, int ntgt, int cd, const int2* tgt, const int* crp,
+                      const int* csrc, const int64_t* lvl_off, const float4* pts, const double4* org,
+                      double kre, double kim, const double* fw, double* bw, hipStream_t st) {
+  const LeafArgs la{ncl, 1, drp, dsrc, dF32, static_cast<int64_t>(dd) * dd, 0, xs, 0, 0, ys, 0, 0, 0};
+  const FlyArgs fa{ntgt, tgt, crp, csrc, lvl_off, pts, org, fw, bw, static_cast<float>(kre * 0.5 * M_1_PI),
+                   static_cast<float>(kim)};
+  const int nbn = (ncl + 3) / 4, nbc = (ntgt + 3) / 4;
+  const dim3 grid(static_cast<unsigned>(nbn + nbc));
+  TimedScope ts("matvec_sweep", st);
+#define BMI_FLY(CC, DDD, CDD)                                                                              \
+  if (cplx == CC && dd == DDD && cd == CDD) {                                                              \
+    if (variant == 1)                                                                                      \
+      hipLaunchKernelGGL((k_sweep_fly<CC, DDD, CDD, 2>), grid, dim3(256), 0, st, la, fa, nbn, nbc);        \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_sweep_fly<CC, DDD, CDD, 1>), grid, dim3(256), 0, st, la, fa, nbn, nbc);        \
+    BMI_CHECK_HIP(hipGetLastError());                                                                      \
+    return true;                                                                                           \
+  }
+  BMI_FLY(false, 36, 16)
+  BMI_FLY(false, 36, 25)
+  BMI_FLY(true, 36, 25)
+  BMI_FLY(true, 36, 16)
+  BMI_FLY(false, 16, 16)
+  BMI_FLY(true, 16, 16)
+#undef BMI_FLY
   return false;
 }
 

@@ -8,10 +8,16 @@
 //     resident, its assembled H2 state (estimated from the block tree before anything is
 //     allocated); idle resident operators are evicted least recently used first.
 // Queued requests with the same operator and solver settings are merged into one batched solve
-// (up to `max_batch_rhs` right-hand sides).  Each admitted batch runs on a worker thread with its
-// own HIP stream, so batches overlap on the device; requests that arrive meanwhile are admitted as
-// slots and memory free up.  Assembled operators stay resident between requests (keyed by
-// geometry file, scale, level, degree and operator), so a repeated geometry skips assembly.
+// (up to `max_batch_rhs` right-hand sides).  Admitted batches hold their memory and are advanced in
+// time slices by one device thread: a GMRES batch runs `slice_iterations` steps (whole restart
+// cycles; a restarted GMRES resumed from its iterate continues exactly as it would have), then the
+// device turns to the next admitted batch, so a short solve that arrives while a long one runs
+// starts after at most one slice of each batch ahead of it.  CG batches run to the end in one
+// slice (a restarted CG converges more slowly).  Every HIP call of the server is made by the device
+// thread; the scheduler thread does host work only (geometry, spaces, memory estimates), and
+// operators it evicts are released by the device thread.  Assembled operators stay resident
+// between requests (keyed by geometry file, scale, level, degree and operator), so a repeated
+// geometry skips assembly.
 //
 // There is no counterpart in the reference (Bembel solves one problem per program); the
 // operator, right-hand side and solver calls are the library's own C ABI.
@@ -81,7 +87,24 @@ struct Request {
   std::vector<int> iters;
   std::vector<double> resid;
   int64_t ndof = 0;
-  int batch_requests = 0, batch_rhs = 0, cache_hit = 0;
+  int batch_requests = 0, batch_rhs = 0, cache_hit = 0, slices = 0;
+};
+
+// an admitted batch: its device buffers and progress between time slices
+struct Job {
+  std::vector<std::shared_ptr<Request>> batch;
+  std::shared_ptr<Operator> op;
+  int64_t ws = 0;
+  hipStream_t stream = nullptr;
+  double* dB = nullptr;
+  double* dX = nullptr;
+  int rhs = 0, S = 1, used = 0, slices = 0, cache_hit = 0;
+  bool started = false;
+  bmi_status st = BMI_OK;
+  std::string err;
+  std::vector<int> iters;
+  std::vector<double> resid;
+  std::vector<double> hx;
 };
 
 std::string operator_key(const Request& r) {
@@ -103,17 +126,21 @@ struct bmi_server {
   std::deque<std::shared_ptr<Request>> queue;
   std::map<int64_t, std::shared_ptr<Request>> reqs;
   std::map<std::string, std::shared_ptr<Operator>> ops;
+  std::deque<std::shared_ptr<Job>> ready;              // admitted batches, in turn order
+  std::vector<std::shared_ptr<Operator>> graveyard;    // evicted, released by the device thread
   int64_t used_bytes = 0;
   int active = 0;
   int64_t next_id = 1;
   bool stop = false;
-  std::thread scheduler;
+  std::thread scheduler, device;
   int64_t submitted = 0, finished = 0;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
   void run_scheduler();
-  void run_batch(std::vector<std::shared_ptr<Request>> batch, std::shared_ptr<Operator> op, int64_t ws);
-  bool estimate(Request& r);  // called without the lock
+  void run_device();
+  bool run_slice(Job& j);                 // device thread, without the lock; true when finished
+  void finish(const std::shared_ptr<Job>& j);  // with the lock
+  bool estimate(Request& r);              // called without the lock
 };
 
 // geometry, space and memory estimate of a queued request whose operator is not resident
@@ -219,6 +246,8 @@ void bmi_server::run_scheduler() {
         if (!victim) break;
         used_bytes -= victim->bytes;
         ops.erase(victim->key);
+        graveyard.push_back(std::move(victim));  // its device memory is released by the device thread
+        cv.notify_all();
       }
       if (used_bytes + need > cfg.hbm_budget_bytes) continue;
       std::shared_ptr<Operator> op;
@@ -241,107 +270,170 @@ void bmi_server::run_scheduler() {
       }
       qi = static_cast<size_t>(-1);
       admitted = true;
-      std::thread(&bmi_server::run_batch, this, std::move(batch), op, ws).detach();
+      auto job = std::make_shared<Job>();
+      job->batch = std::move(batch);
+      job->op = op;
+      job->ws = ws;
+      ready.push_back(std::move(job));
+      cv.notify_all();
     }
     if (!admitted) cv.wait_for(lk, std::chrono::milliseconds(5));
   }
 }
 
-void bmi_server::run_batch(std::vector<std::shared_ptr<Request>> batch, std::shared_ptr<Operator> op, int64_t ws) {
-  bmi_status st = BMI_OK;
-  std::string err;
-  hipStream_t stream = nullptr;
-  double* dB = nullptr;
-  double* dX = nullptr;
-  int cache_hit = op->h != nullptr ? 1 : 0;
-  const Request& r0 = *batch[0];
-  const int S = r0.op.kind == BMI_LAPLACE_SL ? 1 : 2;
-  int rhs = 0;
-  for (auto& q : batch) rhs += q->nrhs;
-  std::vector<int> iters(rhs, 0);
-  std::vector<double> resid(rhs, 0.0);
-  std::vector<double> hx;
-  auto fail = [&](bmi_status s) {
-    if (st == BMI_OK) st = s, err = bmi_last_error();
-  };
-  if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
-    st = BMI_DEVICE_ERROR;
-    err = "stream creation failed";
-  }
-  if (st == BMI_OK && !op->h) {
-    const bmi_status s = bmi_h2_assemble(op->s, &op->op, stream, &op->h);
-    if (s != BMI_OK) fail(s);
-    // products launch kernel by kernel here: stream capture would have to exclude the
-    // device-wide synchronisation of other workers' allocations
-    if (s == BMI_OK) bmi_h2_use_graphs(op->h, 0);
-  }
-  const int64_t n = op->ndof;
-  if (st == BMI_OK) {
-    if (hipMalloc(&dB, sizeof(double) * S * n * rhs) != hipSuccess || hipMalloc(&dX, sizeof(double) * S * n * rhs) != hipSuccess) {
-      st = BMI_OUT_OF_MEMORY;
-      err = "workspace allocation failed";
+void bmi_server::run_device() {
+  std::unique_lock<std::mutex> lk(mu);
+  while (true) {
+    cv.wait(lk, [&] { return stop || !ready.empty() || !graveyard.empty(); });
+    if (!graveyard.empty()) {
+      std::vector<std::shared_ptr<Operator>> g;
+      g.swap(graveyard);
+      lk.unlock();
+      g.clear();  // bmi_h2_destroy and friends, outside the lock
+      lk.lock();
+      continue;
     }
-  }
-  int col = 0;
-  for (auto& q : batch) {
-    if (st != BMI_OK) break;
-    const bmi_status s = bmi_rhs_assemble(op->s, q->data_fn, q->params.data(), q->nrhs, q->data_kappa,
-                                          dB + static_cast<int64_t>(S) * n * col, BMI_MEM_DEVICE, stream);
-    if (s != BMI_OK) fail(s);
-    col += q->nrhs;
-  }
-  if (st == BMI_OK) {
-    bmi_status s;
-    if (r0.op.kind == BMI_LAPLACE_SL)
-      s = bmi_solve_cg(op->h, rhs, dB, dX, r0.tol, r0.maxit, 1, iters.data(), resid.data(), BMI_MEM_DEVICE, stream);
+    if (ready.empty()) {
+      if (stop) return;
+      continue;
+    }
+    std::shared_ptr<Job> j = ready.front();
+    ready.pop_front();
+    lk.unlock();
+    const bool done = run_slice(*j);
+    lk.lock();
+    if (done)
+      finish(j);
     else
-      s = bmi_solve_gmres(op->h, rhs, dB, dX, r0.tol, r0.restart, r0.maxit, 1, iters.data(), resid.data(),
-                          BMI_MEM_DEVICE, stream);
-    if (s != BMI_OK && s != BMI_NOT_CONVERGED) fail(s);
-    if (s == BMI_NOT_CONVERGED) st = BMI_NOT_CONVERGED, err = "solver did not reach the tolerance";
+      ready.push_back(j);  // the next admitted batch takes the device
+    cv.notify_all();
   }
-  if (dX && (st == BMI_OK || st == BMI_NOT_CONVERGED)) {
-    hx.resize(static_cast<size_t>(S) * n * rhs);
-    if (hipMemcpyAsync(hx.data(), dX, hx.size() * sizeof(double), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess) {
-      st = BMI_DEVICE_ERROR;
-      err = "copy of the solution failed";
-      hx.clear();
+}
+
+// one time slice of a batch on the device: on the first, assembly (when the operator is not
+// resident), buffers and right-hand sides; then solver steps
+bool bmi_server::run_slice(Job& j) {
+  const Request& r0 = *j.batch[0];
+  const bool gm = r0.op.kind != BMI_LAPLACE_SL;
+  auto fail = [&](bmi_status s, const char* what) {
+    if (j.st == BMI_OK) j.st = s, j.err = what ? std::string(what) : std::string(bmi_last_error());
+  };
+  if (!j.started) {
+    j.started = true;
+    j.S = gm ? 2 : 1;
+    for (auto& q : j.batch) j.rhs += q->nrhs;
+    j.iters.assign(j.rhs, 0);
+    j.resid.assign(j.rhs, 0.0);
+    j.cache_hit = j.op->h != nullptr ? 1 : 0;
+    if (hipStreamCreateWithFlags(&j.stream, hipStreamNonBlocking) != hipSuccess) {
+      j.stream = nullptr;
+      fail(BMI_DEVICE_ERROR, "stream creation failed");
+    }
+    if (j.st == BMI_OK && !j.op->h) {
+      const bmi_status s = bmi_h2_assemble(j.op->s, &j.op->op, j.stream, &j.op->h);
+      if (s != BMI_OK) fail(s, nullptr);
+      if (s == BMI_OK) bmi_h2_use_graphs(j.op->h, 0);
+    }
+    const int64_t n = j.op->ndof;
+    if (j.st == BMI_OK && (hipMalloc(&j.dB, sizeof(double) * j.S * n * j.rhs) != hipSuccess ||
+                           hipMalloc(&j.dX, sizeof(double) * j.S * n * j.rhs) != hipSuccess))
+      fail(BMI_OUT_OF_MEMORY, "workspace allocation failed");
+    int col = 0;
+    for (auto& q : j.batch) {
+      if (j.st != BMI_OK) break;
+      const bmi_status s = bmi_rhs_assemble(j.op->s, q->data_fn, q->params.data(), q->nrhs, q->data_kappa,
+                                            j.dB + static_cast<int64_t>(j.S) * n * col, BMI_MEM_DEVICE, j.stream);
+      if (s != BMI_OK) fail(s, nullptr);
+      col += q->nrhs;
     }
   }
-  if (stream) (void)hipStreamSynchronize(stream);
-  if (dB) (void)hipFree(dB);
-  if (dX) (void)hipFree(dX);
-  if (stream) (void)hipStreamDestroy(stream);
+  bool done = j.st != BMI_OK;
+  if (!done) {
+    // GMRES slices are whole restart cycles; CG runs to the end
+    int steps = r0.maxit - j.used;
+    if (gm && cfg.slice_iterations > 0) {
+      const int cyc = std::max(1, (cfg.slice_iterations + r0.restart - 1) / r0.restart) * r0.restart;
+      steps = std::min(steps, cyc);
+    }
+    std::vector<int> it(j.rhs, 0);
+    std::vector<double> rs(j.rhs, 0.0);
+    const int zero = j.slices == 0 ? 1 : 0;
+    bmi_status s;
+    if (!gm)
+      s = bmi_solve_cg(j.op->h, j.rhs, j.dB, j.dX, r0.tol, steps, zero, it.data(), rs.data(), BMI_MEM_DEVICE,
+                       j.stream);
+    else
+      s = bmi_solve_gmres(j.op->h, j.rhs, j.dB, j.dX, r0.tol, r0.restart, steps, zero, it.data(), rs.data(),
+                          BMI_MEM_DEVICE, j.stream);
+    ++j.slices;
+    j.used += steps;
+    for (int c = 0; c < j.rhs; ++c) j.iters[c] += it[c];
+    j.resid = rs;
+    if (s == BMI_OK) {
+      done = true;
+    } else if (s == BMI_NOT_CONVERGED) {
+      if (j.used >= r0.maxit) done = true, j.st = BMI_NOT_CONVERGED, j.err = "solver did not reach the tolerance";
+    } else {
+      fail(s, nullptr);
+      done = true;
+    }
+  }
+  if (!done) return false;
+  if (j.dX && (j.st == BMI_OK || j.st == BMI_NOT_CONVERGED)) {
+    j.hx.resize(static_cast<size_t>(j.S) * j.op->ndof * j.rhs);
+    if (hipMemcpyAsync(j.hx.data(), j.dX, j.hx.size() * sizeof(double), hipMemcpyDeviceToHost, j.stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(j.stream) != hipSuccess) {
+      j.st = BMI_DEVICE_ERROR;
+      j.err = "copy of the solution failed";
+      j.hx.clear();
+    }
+  }
+  if (j.stream) (void)hipStreamSynchronize(j.stream);
+  if (j.dB) (void)hipFree(j.dB);
+  if (j.dX) (void)hipFree(j.dX);
+  if (j.stream) (void)hipStreamDestroy(j.stream);
+  j.dB = j.dX = nullptr;
+  j.stream = nullptr;
+  return true;
+}
 
-  std::lock_guard<std::mutex> lk(mu);
+void bmi_server::finish(const std::shared_ptr<Job>& jp) {
+  Job& j = *jp;
   const double t = now();
-  col = 0;
-  for (auto& q : batch) {
-    q->status = st;
-    q->error = err;
+  const int64_t n = j.op->ndof;
+  int col = 0;
+  for (auto& q : j.batch) {
+    q->status = j.st;
+    q->error = j.err;
     q->ndof = n;
-    q->batch_requests = static_cast<int>(batch.size());
-    q->batch_rhs = rhs;
-    q->cache_hit = cache_hit;
-    if (!hx.empty())
-      q->x.assign(hx.begin() + static_cast<int64_t>(S) * n * col, hx.begin() + static_cast<int64_t>(S) * n * (col + q->nrhs));
-    q->iters.assign(iters.begin() + col, iters.begin() + col + q->nrhs);
-    q->resid.assign(resid.begin() + col, resid.begin() + col + q->nrhs);
+    q->batch_requests = static_cast<int>(j.batch.size());
+    q->batch_rhs = j.rhs;
+    q->cache_hit = j.cache_hit;
+    q->slices = j.slices;
+    if (!j.hx.empty())
+      q->x.assign(j.hx.begin() + static_cast<int64_t>(j.S) * n * col,
+                  j.hx.begin() + static_cast<int64_t>(j.S) * n * (col + q->nrhs));
+    if (static_cast<int>(j.iters.size()) >= col + q->nrhs) {
+      q->iters.assign(j.iters.begin() + col, j.iters.begin() + col + q->nrhs);
+      q->resid.assign(j.resid.begin() + col, j.resid.begin() + col + q->nrhs);
+    }
     q->t_end = t;
     q->state = 2;
     col += q->nrhs;
     ++finished;
   }
-  used_bytes -= ws;
-  op->busy = false;
-  op->last_used = t;
-  if (!op->h) {  // assembly failed: forget the operator
-    used_bytes -= op->bytes;
-    ops.erase(op->key);
+  j.hx.clear();
+  used_bytes -= j.ws;
+  j.op->busy = false;
+  j.op->last_used = t;
+  if (!j.op->h) {  // assembly failed: forget the operator
+    used_bytes -= j.op->bytes;
+    ops.erase(j.op->key);
+    graveyard.push_back(j.op);
   }
+  j.op.reset();
   --active;
-  cv.notify_all();
 }
 
 extern "C" {
@@ -349,21 +441,22 @@ extern "C" {
 bmi_status bmi_server_config_default(bmi_server_config* cfg) {
   if (!cfg) return BMI_INVALID_ARG;
   cfg->hbm_budget_bytes = 200.0 * (1ull << 30);
-  // one batch on the device at a time by default: several concurrent batches (one host thread
-  // and stream each) are supported by the scheduler but not yet validated on the device
-  cfg->max_active = 1;
+  cfg->max_active = 4;
   cfg->max_batch_rhs = 64;
+  cfg->slice_iterations = 120;
   return BMI_OK;
 }
 
 bmi_status bmi_server_create(const bmi_server_config* cfg, bmi_server** out) {
   try {
     BMI_REQUIRE(cfg && out, BMI_INVALID_ARG, "null argument");
-    BMI_REQUIRE(cfg->max_active >= 1 && cfg->max_batch_rhs >= 1 && cfg->hbm_budget_bytes > 0, BMI_INVALID_ARG,
-                "bad server configuration");
+    BMI_REQUIRE(cfg->max_active >= 1 && cfg->max_batch_rhs >= 1 && cfg->hbm_budget_bytes > 0 &&
+                    cfg->slice_iterations >= 0,
+                BMI_INVALID_ARG, "bad server configuration");
     auto s = std::make_unique<bmi_server>();
     s->cfg = *cfg;
     s->scheduler = std::thread(&bmi_server::run_scheduler, s.get());
+    s->device = std::thread(&bmi_server::run_device, s.get());
     *out = s.release();
     return BMI_OK;
   } catch (const bmi::Error& e) {
@@ -447,6 +540,7 @@ bmi_status bmi_server_wait(bmi_server* srv, int64_t id, double timeout_s, bmi_re
     info->batch_requests = r->batch_requests;
     info->batch_rhs = r->batch_rhs;
     info->cache_hit = r->cache_hit;
+    info->slices = r->slices;
   }
   if (r->status != BMI_OK) bmi::set_last_error(r->error);
   return BMI_OK;
@@ -486,6 +580,7 @@ bmi_status bmi_server_destroy(bmi_server* srv) {
     srv->cv.notify_all();
   }
   if (srv->scheduler.joinable()) srv->scheduler.join();
+  if (srv->device.joinable()) srv->device.join();
   delete srv;
   return BMI_OK;
 }

@@ -178,9 +178,141 @@ struct Vecs {
   }
 };
 
-using cplx = std::complex<double>;
+// ------------------------------------------------------------------ GMRES state on the device
+// Per column c (complex entries as S doubles): Hessenberg H[c] ((m+1) x m, column-major), the
+// rotated right-hand side g[c] (m+1), rotations cs[c], sn[c] (m), the active flag, the number of
+// Arnoldi columns kc[c] of this cycle, the step count and the residual estimate.  The host reads
+// the flags back only every few steps, so an Arnoldi step costs no host round trip.
+struct GmresState {
+  double *H, *g, *cs, *sn, *bn, *resid, *coef;
+  int *kc, *iters;
+  char* active;
+};
 
-// batched restarted GMRES; returns true if every column converged
+__device__ __forceinline__ void cmul(double ar, double ai, double br, double bi, double& r, double& i) {
+  r = ar * br - ai * bi;
+  i = ar * bi + ai * br;
+}
+
+// start of a cycle: beta = ||R||, g = (beta, 0, ...), V0 = R / beta (coefficients for axpby)
+template <bool C>
+__global__ void k_gm_cycle(const double* __restrict__ nrm, GmresState s, int m, int nrhs, double tol, int first) {
+  constexpr int S = C ? 2 : 1;
+  const int c = threadIdx.x + blockIdx.x * blockDim.x;
+  if (c >= nrhs) return;
+  const double beta = sqrt(fmax(nrm[S * c], 0.0));
+  if (first) {
+    s.bn[c] = beta > 0 ? beta : 1.0;
+    s.iters[c] = 0;
+  }
+  s.resid[c] = beta / s.bn[c];
+  s.active[c] = (s.resid[c] > tol && beta > 0) ? 1 : 0;
+  s.kc[c] = 0;
+  double* g = s.g + static_cast<int64_t>(c) * (m + 1) * S;
+  for (int j = 0; j <= m; ++j) {
+    g[S * j] = j == 0 ? beta : 0.0;
+    if (C) g[S * j + 1] = 0.0;
+  }
+  s.coef[S * c] = beta > 0 ? 1.0 / beta : 0.0;
+  s.coef[S * (nrhs + c)] = 0.0;
+  if (C) s.coef[S * c + 1] = 0.0, s.coef[S * (nrhs + c) + 1] = 0.0;
+}
+
+// Arnoldi step k: H(:, k) = h1 + h2 (two Gram-Schmidt passes), H(k+1, k) = ||w||, previous
+// rotations, a new rotation, g and the residual estimate |g(k+1)| / ||b||; coefficients 1/||w||
+// for the next basis vector
+template <bool C>
+__global__ void k_gm_givens(const double* __restrict__ h1, const double* __restrict__ h2, const double* __restrict__ nrm,
+                            GmresState s, int k, int m, int nrhs, double tol) {
+  constexpr int S = C ? 2 : 1;
+  const int c = threadIdx.x + blockIdx.x * blockDim.x;
+  if (c >= nrhs) return;
+  const double hn = sqrt(fmax(nrm[S * c], 0.0));
+  s.coef[S * c] = hn > 0 ? 1.0 / hn : 0.0;
+  s.coef[S * (nrhs + c)] = 0.0;
+  if (C) s.coef[S * c + 1] = 0.0, s.coef[S * (nrhs + c) + 1] = 0.0;
+  if (!s.active[c]) return;
+  double* col = s.H + (static_cast<int64_t>(c) * (m + 1) * m + static_cast<int64_t>(k) * (m + 1)) * S;
+  double* cs = s.cs + static_cast<int64_t>(c) * m * S;
+  double* sn = s.sn + static_cast<int64_t>(c) * m * S;
+  double* g = s.g + static_cast<int64_t>(c) * (m + 1) * S;
+  for (int j = 0; j <= k; ++j) {
+    const int64_t q = S * (static_cast<int64_t>(j) * nrhs + c);
+    col[S * j] = h1[q] + h2[q];
+    if (C) col[S * j + 1] = h1[q + 1] + h2[q + 1];
+  }
+  col[S * (k + 1)] = hn;
+  if (C) col[S * (k + 1) + 1] = 0.0;
+  for (int j = 0; j < k; ++j) {
+    const double t1r = col[S * j], t1i = C ? col[S * j + 1] : 0.0;
+    const double t2r = col[S * (j + 1)], t2i = C ? col[S * (j + 1) + 1] : 0.0;
+    const double cr = cs[S * j], ci = C ? cs[S * j + 1] : 0.0, sr = sn[S * j], si = C ? sn[S * j + 1] : 0.0;
+    double ar, ai, br, bi;
+    cmul(cr, -ci, t1r, t1i, ar, ai);  // conj(cs) t1 + conj(sn) t2
+    cmul(sr, -si, t2r, t2i, br, bi);
+    col[S * j] = ar + br;
+    if (C) col[S * j + 1] = ai + bi;
+    cmul(-sr, -si, t1r, t1i, ar, ai);  // -sn t1 + cs t2
+    cmul(cr, ci, t2r, t2i, br, bi);
+    col[S * (j + 1)] = ar + br;
+    if (C) col[S * (j + 1) + 1] = ai + bi;
+  }
+  const double x1r = col[S * k], x1i = C ? col[S * k + 1] : 0.0;
+  const double x2r = col[S * (k + 1)], x2i = C ? col[S * (k + 1) + 1] : 0.0;
+  const double den = sqrt((x1r * x1r + x1i * x1i) + (x2r * x2r + x2i * x2i));
+  const double cr = den > 0 ? x1r / den : 1.0, ci = den > 0 ? x1i / den : 0.0;
+  const double sr = den > 0 ? x2r / den : 0.0, si = den > 0 ? x2i / den : 0.0;
+  cs[S * k] = cr, sn[S * k] = sr;
+  if (C) cs[S * k + 1] = ci, sn[S * k + 1] = si;
+  col[S * k] = den;
+  col[S * (k + 1)] = 0.0;
+  if (C) col[S * k + 1] = 0.0, col[S * (k + 1) + 1] = 0.0;
+  const double gr = g[S * k], gi = C ? g[S * k + 1] : 0.0;
+  double nr, ni, kr, ki;
+  cmul(-sr, -si, gr, gi, nr, ni);  // g(k+1) = -sn g(k)
+  cmul(cr, -ci, gr, gi, kr, ki);   // g(k) = conj(cs) g(k)
+  g[S * (k + 1)] = nr;
+  g[S * k] = kr;
+  if (C) g[S * (k + 1) + 1] = ni, g[S * k + 1] = ki;
+  s.kc[c] = k + 1;
+  s.iters[c] += 1;
+  s.resid[c] = hypot(nr, ni) / s.bn[c];
+  if (s.resid[c] <= tol || hn == 0.0) s.active[c] = 0;
+}
+
+// end of a cycle: y from the triangular system R y = g (kc[c] unknowns), zero beyond
+template <bool C>
+__global__ void k_gm_backsolve(GmresState s, int m, int nv, int nrhs, double* y) {
+  constexpr int S = C ? 2 : 1;
+  const int c = threadIdx.x + blockIdx.x * blockDim.x;
+  if (c >= nrhs) return;
+  const int kk = s.kc[c];
+  const double* H = s.H + static_cast<int64_t>(c) * (m + 1) * m * S;
+  const double* g = s.g + static_cast<int64_t>(c) * (m + 1) * S;
+  for (int j = kk; j < nv; ++j) {
+    y[S * (static_cast<int64_t>(j) * nrhs + c)] = 0.0;
+    if (C) y[S * (static_cast<int64_t>(j) * nrhs + c) + 1] = 0.0;
+  }
+  for (int i = kk - 1; i >= 0; --i) {
+    double ar = g[S * i], ai = C ? g[S * i + 1] : 0.0;
+    for (int j = i + 1; j < kk; ++j) {
+      const double* hij = H + S * (static_cast<int64_t>(j) * (m + 1) + i);
+      const double* yj = y + S * (static_cast<int64_t>(j) * nrhs + c);
+      double pr, pi;
+      cmul(hij[0], C ? hij[1] : 0.0, yj[0], C ? yj[1] : 0.0, pr, pi);
+      ar -= pr;
+      ai -= pi;
+    }
+    const double d = H[S * (static_cast<int64_t>(i) * (m + 1) + i)];  // real after the rotation
+    y[S * (static_cast<int64_t>(i) * nrhs + c)] = ar / d;
+    if (C) y[S * (static_cast<int64_t>(i) * nrhs + c) + 1] = ai / d;
+  }
+}
+
+// batched restarted GMRES (classical Gram-Schmidt applied twice, FP64 Arnoldi); returns true
+// if every column converged.  The host waits for the device once per cycle and every
+// `check` Arnoldi steps; a column that converges inside such a window takes no further
+// steps (its flag is cleared on the device), the others continue.
 static bool gmres(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int restart, int maxit,
                   std::vector<int>& iters, std::vector<double>& resid, hipStream_t st, bool reduced) {
   const bool C = h2_complex(h);
@@ -188,125 +320,96 @@ static bool gmres(bmi_h2* h, int nrhs, const double* B, double* X, double tol, i
   Vecs vs{C, n, nrhs, st};
   const int S = vs.S();
   const int m = std::max(1, std::min(restart, maxit));
-  DevBuf<double> V, W, R, d_small, d_coef;
+  const int check = 4;
+  DevBuf<double> V, W, R, d_h1, d_h2, d_nrm, d_H, d_g, d_cs, d_sn, d_bn, d_resid, d_coef, d_y;
+  DevBuf<int> d_kc, d_iters;
+  DevBuf<char> d_active;
   V.alloc(static_cast<size_t>(vs.block()) * (m + 1));
   W.alloc(vs.block());
   R.alloc(vs.block());
-  d_small.alloc(static_cast<size_t>(S) * (m + 2) * nrhs * 2);
-  d_coef.alloc(static_cast<size_t>(S) * nrhs * 2);
-  std::vector<double> hbuf, h2buf;
-  // norms of B
-  vs.dots(B, 1, B, d_small.ptr, hbuf);
-  std::vector<double> bn(nrhs);
-  for (int c = 0; c < nrhs; ++c) bn[c] = std::sqrt(std::max(hbuf[S * c], 0.0)), bn[c] = bn[c] > 0 ? bn[c] : 1.0;
-  iters.assign(nrhs, 0);
-  resid.assign(nrhs, 0.0);
-  std::vector<char> done(nrhs, 0);
+  d_h1.alloc(static_cast<size_t>(S) * (m + 1) * nrhs);
+  d_h2.alloc(static_cast<size_t>(S) * (m + 1) * nrhs);
+  d_nrm.alloc(static_cast<size_t>(S) * nrhs);
+  d_H.alloc(static_cast<size_t>(S) * (m + 1) * m * nrhs);
+  d_g.alloc(static_cast<size_t>(S) * (m + 1) * nrhs);
+  d_cs.alloc(static_cast<size_t>(S) * m * nrhs);
+  d_sn.alloc(static_cast<size_t>(S) * m * nrhs);
+  d_bn.alloc(nrhs);
+  d_resid.alloc(nrhs);
+  d_coef.alloc(static_cast<size_t>(2) * S * nrhs);
+  d_y.alloc(static_cast<size_t>(S) * m * nrhs);
+  d_kc.alloc(nrhs);
+  d_iters.alloc(nrhs);
+  d_active.alloc(nrhs);
+  const GmresState gs{d_H.ptr, d_g.ptr, d_cs.ptr, d_sn.ptr, d_bn.ptr, d_resid.ptr, d_coef.ptr,
+                      d_kc.ptr, d_iters.ptr, d_active.ptr};
+  const unsigned sb = static_cast<unsigned>((nrhs + 63) / 64);
   const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
-  int total = 0;
-  auto upload_coef = [&](const std::vector<cplx>& a, const std::vector<cplx>& b) {
-    std::vector<double> buf(static_cast<size_t>(2) * S * nrhs);
-    for (int c = 0; c < nrhs; ++c) {
-      buf[S * c] = a[c].real();
-      buf[S * (nrhs + c)] = b[c].real();
-      if (C) buf[S * c + 1] = a[c].imag(), buf[S * (nrhs + c) + 1] = b[c].imag();
-    }
-    BMI_CHECK_HIP(hipMemcpyAsync(d_coef.ptr, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  std::vector<char> act(nrhs);
+  auto any_active = [&]() {
+    BMI_CHECK_HIP(hipMemcpyAsync(act.data(), d_active.ptr, nrhs, hipMemcpyDeviceToHost, st));
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+    return std::any_of(act.begin(), act.end(), [](char a) { return a != 0; });
   };
+  // ||B|| per column
+  vs.dots_async(B, 1, B, d_nrm.ptr);
+  if (C)
+    hipLaunchKernelGGL(k_gm_cycle<true>, dim3(sb), dim3(64), 0, st, d_nrm.ptr, gs, m, nrhs, tol, 1);
+  else
+    hipLaunchKernelGGL(k_gm_cycle<false>, dim3(sb), dim3(64), 0, st, d_nrm.ptr, gs, m, nrhs, tol, 1);
+  BMI_CHECK_HIP(hipGetLastError());
+  int total = 0;
+  bool converged = false;
   while (true) {
-    // R = B - A X
+    // R = B - A X, beta, V0 = R / beta
     BMI_CHECK_HIP(hipMemcpyAsync(R.ptr, B, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
     h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st, reduced);
-    vs.dots(R.ptr, 1, R.ptr, d_small.ptr, hbuf);
-    std::vector<double> beta(nrhs);
-    bool all = true;
-    for (int c = 0; c < nrhs; ++c) {
-      beta[c] = std::sqrt(std::max(hbuf[S * c], 0.0));
-      resid[c] = beta[c] / bn[c];
-      if (resid[c] <= tol) done[c] = 1;
-      all = all && done[c];
+    vs.dots_async(R.ptr, 1, R.ptr, d_nrm.ptr);
+    if (C)
+      hipLaunchKernelGGL(k_gm_cycle<true>, dim3(sb), dim3(64), 0, st, d_nrm.ptr, gs, m, nrhs, tol, 0);
+    else
+      hipLaunchKernelGGL(k_gm_cycle<false>, dim3(sb), dim3(64), 0, st, d_nrm.ptr, gs, m, nrhs, tol, 0);
+    BMI_CHECK_HIP(hipGetLastError());
+    if (!any_active()) {
+      converged = true;
+      break;
     }
-    if (all || total >= maxit) return all;
-    // V0 = R / beta
-    std::vector<cplx> a(nrhs), b(nrhs, 0.0);
-    for (int c = 0; c < nrhs; ++c) a[c] = beta[c] > 0 ? 1.0 / beta[c] : 0.0;
-    upload_coef(a, b);
+    if (total >= maxit) break;
     vs.axpby(d_coef.ptr, R.ptr, d_coef.ptr + S * nrhs, V.ptr);
-    // per-column Hessenberg (column-major (m+1) x m), rotations, rhs g
-    std::vector<std::vector<cplx>> H(nrhs, std::vector<cplx>((m + 1) * m, 0.0)), g(nrhs, std::vector<cplx>(m + 1, 0.0));
-    std::vector<std::vector<cplx>> cs(nrhs, std::vector<cplx>(m)), sn(nrhs, std::vector<cplx>(m));
-    std::vector<int> kc(nrhs, 0);
-    std::vector<char> active(nrhs, 0);
-    for (int c = 0; c < nrhs; ++c) g[c][0] = beta[c], active[c] = !done[c] && beta[c] > 0;
     int k = 0;
     for (; k < m && total < maxit; ++k, ++total) {
+      if (k > 0 && k % check == 0 && !any_active()) break;
       double* Vk = V.ptr + vs.block() * k;
       h2_apply_device(h, nrhs, Vk, n, W.ptr, n, one, nullptr, st, reduced);
-      // classical Gram-Schmidt, twice
-      vs.dots(V.ptr, k + 1, W.ptr, d_small.ptr, hbuf);
-      vs.subtract(V.ptr, k + 1, d_small.ptr, W.ptr);
-      vs.dots(V.ptr, k + 1, W.ptr, d_small.ptr, h2buf);
-      vs.subtract(V.ptr, k + 1, d_small.ptr, W.ptr);
-      for (size_t q = 0; q < hbuf.size(); ++q) hbuf[q] += h2buf[q];
-      vs.dots(W.ptr, 1, W.ptr, d_small.ptr, h2buf);
-      std::vector<cplx> inv(nrhs), zero(nrhs, 0.0);
-      bool any = false;
-      for (int c = 0; c < nrhs; ++c) {
-        const double hn = std::sqrt(std::max(h2buf[S * c], 0.0));
-        inv[c] = hn > 0 ? 1.0 / hn : 0.0;
-        if (!active[c]) continue;
-        std::vector<cplx>& Hc = H[c];
-        for (int j = 0; j <= k; ++j)
-          Hc[k * (m + 1) + j] = C ? cplx(hbuf[S * (j * nrhs + c)], hbuf[S * (j * nrhs + c) + 1]) : cplx(hbuf[j * nrhs + c], 0.0);
-        Hc[k * (m + 1) + k + 1] = hn;
-        for (int j = 0; j < k; ++j) {
-          const cplx t1 = Hc[k * (m + 1) + j], t2 = Hc[k * (m + 1) + j + 1];
-          Hc[k * (m + 1) + j] = std::conj(cs[c][j]) * t1 + std::conj(sn[c][j]) * t2;
-          Hc[k * (m + 1) + j + 1] = -sn[c][j] * t1 + cs[c][j] * t2;
-        }
-        const cplx x1 = Hc[k * (m + 1) + k], x2 = Hc[k * (m + 1) + k + 1];
-        const double den = std::sqrt(std::norm(x1) + std::norm(x2));
-        cs[c][k] = den > 0 ? x1 / den : 1.0;
-        sn[c][k] = den > 0 ? x2 / den : 0.0;
-        Hc[k * (m + 1) + k] = den;
-        Hc[k * (m + 1) + k + 1] = 0.0;
-        g[c][k + 1] = -sn[c][k] * g[c][k];
-        g[c][k] = std::conj(cs[c][k]) * g[c][k];
-        kc[c] = k + 1;
-        iters[c] += 1;
-        resid[c] = std::abs(g[c][k + 1]) / bn[c];
-        if (resid[c] <= tol || hn == 0.0) active[c] = 0;
-        any = any || active[c];
-      }
-      upload_coef(inv, zero);
+      vs.dots_async(V.ptr, k + 1, W.ptr, d_h1.ptr);
+      vs.subtract(V.ptr, k + 1, d_h1.ptr, W.ptr);
+      vs.dots_async(V.ptr, k + 1, W.ptr, d_h2.ptr);
+      vs.subtract(V.ptr, k + 1, d_h2.ptr, W.ptr);
+      vs.dots_async(W.ptr, 1, W.ptr, d_nrm.ptr);
+      if (C)
+        hipLaunchKernelGGL(k_gm_givens<true>, dim3(sb), dim3(64), 0, st, d_h1.ptr, d_h2.ptr, d_nrm.ptr, gs, k, m,
+                           nrhs, tol);
+      else
+        hipLaunchKernelGGL(k_gm_givens<false>, dim3(sb), dim3(64), 0, st, d_h1.ptr, d_h2.ptr, d_nrm.ptr, gs, k, m,
+                           nrhs, tol);
+      BMI_CHECK_HIP(hipGetLastError());
       vs.axpby(d_coef.ptr, W.ptr, d_coef.ptr + S * nrhs, V.ptr + vs.block() * (k + 1));
-      if (!any) {
-        ++k, ++total;
-        break;
-      }
     }
-    // X += V y, y from the triangular systems
-    std::vector<double> ybuf(static_cast<size_t>(S) * m * nrhs, 0.0);
-    for (int c = 0; c < nrhs; ++c) {
-      const int kk = kc[c];
-      std::vector<cplx> y(kk);
-      for (int i = kk - 1; i >= 0; --i) {
-        cplx acc = g[c][i];
-        for (int j = i + 1; j < kk; ++j) acc -= H[c][j * (m + 1) + i] * y[j];
-        y[i] = acc / H[c][i * (m + 1) + i];
-      }
-      for (int j = 0; j < kk; ++j) {
-        ybuf[S * (j * nrhs + c)] = y[j].real();
-        if (C) ybuf[S * (j * nrhs + c) + 1] = y[j].imag();
-      }
-    }
-    const int nv = *std::max_element(kc.begin(), kc.end());
-    if (nv == 0) return false;
-    DevBuf<double> d_y;
-    d_y.upload(ybuf, st);
-    vs.combine(V.ptr, nv, d_y.ptr, X);
-    BMI_CHECK_HIP(hipStreamSynchronize(st));
+    // X += V y
+    if (k == 0) break;
+    if (C)
+      hipLaunchKernelGGL(k_gm_backsolve<true>, dim3(sb), dim3(64), 0, st, gs, m, k, nrhs, d_y.ptr);
+    else
+      hipLaunchKernelGGL(k_gm_backsolve<false>, dim3(sb), dim3(64), 0, st, gs, m, k, nrhs, d_y.ptr);
+    BMI_CHECK_HIP(hipGetLastError());
+    vs.combine(V.ptr, k, d_y.ptr, X);
   }
+  iters.assign(nrhs, 0);
+  resid.assign(nrhs, 0.0);
+  BMI_CHECK_HIP(hipMemcpyAsync(iters.data(), d_iters.ptr, nrhs * sizeof(int), hipMemcpyDeviceToHost, st));
+  BMI_CHECK_HIP(hipMemcpyAsync(resid.data(), d_resid.ptr, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+  return converged;
 }
 
 // CG scalars on the device: no host round trip per iteration

@@ -45,8 +45,11 @@ typedef enum {
 /* storage / arithmetic of the far field (coupling and transfer) */
 typedef enum {
   BMI_PRECISION_FP64 = 0,       /* everything FP64 */
-  BMI_PRECISION_MIXED = 1,      /* coupling blocks stored FP32 (products summed in FP64), near field FP64;
-                                   solvers refine with the FP64 operator */
+  BMI_PRECISION_MIXED = 1,      /* reduced operator for products and inner solves: coupling blocks in FP32
+                                   (one RHS: recomputed on the fly in FP32 from the Chebyshev surface
+                                   points and combined with an FP32 copy of the near-field blocks;
+                                   several RHS: stored FP32 blocks with the FP64 near field), sums in
+                                   FP64; the FP64 operator is kept and solvers refine with it */
   BMI_PRECISION_MIXED_BF16 = 2  /* as MIXED, and the coupling blocks of the two coarsest levels in BF16 */
 } bmi_precision;
 
@@ -204,17 +207,21 @@ bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, d
 
 /* ------------------------------------------------------------------ serving */
 /* A request loop on one GPU (no counterpart in the reference, which solves one problem per
-   program).  Submitted solves are queued; a scheduler admits them while fewer than max_active
-   batches run and the device-memory budget holds the batch's workspace plus, for an operator
-   that is not resident yet, its estimated H2 state (idle resident operators are evicted least
-   recently used first).  Queued requests with equal operator and solver settings are merged into
-   one batched solve of up to max_batch_rhs right-hand sides.  Each batch runs on its own stream,
-   so batches overlap on the device; assembled operators stay resident between requests. */
+   program).  Submitted solves are queued; a scheduler admits them (smallest first, anything
+   older than 5 s before all else) while fewer than max_active batches are admitted and the
+   device-memory budget holds the batch's workspace plus, for an operator that is not resident
+   yet, its estimated H2 state (idle resident operators are evicted least recently used first).
+   Queued requests with equal operator and solver settings are merged into one batched solve of
+   up to max_batch_rhs right-hand sides.  Admitted batches take turns on the device in time
+   slices of slice_iterations GMRES steps (whole restart cycles; CG batches run to the end), all
+   driven by one device thread, so a short solve does not wait for a long one to finish.
+   Assembled operators stay resident between requests. */
 typedef struct bmi_server bmi_server;
 typedef struct {
   double hbm_budget_bytes; /* device memory the server may hold */
-  int max_active;          /* batches on the GPU at once */
+  int max_active;          /* batches admitted at once (they take turns on the device) */
   int max_batch_rhs;       /* right-hand sides merged into one solve */
+  int slice_iterations;    /* GMRES steps of a batch before the next batch's turn (0: to the end) */
 } bmi_server_config;
 typedef struct {
   const char* geometry_path; /* .dat file (GeometryIO.hpp:25-122) */
@@ -237,6 +244,7 @@ typedef struct {
   double residual;           /* largest relative residual over the request's columns */
   int batch_requests, batch_rhs; /* size of the batch the request was solved in */
   int cache_hit;             /* the operator was resident when the batch started */
+  int slices;                /* device turns the batch took */
 } bmi_result_info;
 bmi_status bmi_server_config_default(bmi_server_config* cfg);
 bmi_status bmi_server_create(const bmi_server_config* cfg, bmi_server** out);

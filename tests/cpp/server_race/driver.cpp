@@ -4,7 +4,11 @@
 #include <random>
 #include <thread>
 #include <vector>
+#include <atomic>
 #include "bembel_amd.h"
+extern std::atomic<bool> g_teardown;
+extern std::atomic<int> g_wrong_thread;
+void reset_device_owner();
 int main(int argc, char** argv) {
   const int rounds = argc > 1 ? atoi(argv[1]) : 20;
   for (int round = 0; round < rounds; ++round) {
@@ -13,6 +17,7 @@ int main(int argc, char** argv) {
     cfg.max_active = 1 + round % 4;
     cfg.max_batch_rhs = round % 3 == 0 ? 1 : 64;
     cfg.hbm_budget_bytes = round % 5 == 0 ? 3e7 : 1e12;  // small budgets force eviction
+    cfg.slice_iterations = round % 2 ? 30 : 0;              // GMRES batches in two turns, or one
     bmi_server* srv;
     if (bmi_server_create(&cfg, &srv) != BMI_OK) return 1;
     std::vector<std::thread> clients;
@@ -51,8 +56,12 @@ int main(int argc, char** argv) {
         }
       });
     for (auto& t : clients) t.join();
+    g_teardown = true;
     bmi_server_destroy(srv);
+    g_teardown = false;
+    reset_device_owner();
     if (bad) { printf("round %d: %d bad results\n", round, bad); return 2; }
+    if (g_wrong_thread.load()) { printf("round %d: device calls from more than one thread\n", round); return 3; }
   }
   printf("ok\n");
   return 0;

@@ -6,19 +6,40 @@
 #include <cstdint>
 #include <random>
 #include <string>
+#include <mutex>
 #include <thread>
 #include "bembel_amd.h"
 static void nap(int max_us) {
   thread_local std::mt19937 g(std::hash<std::thread::id>()(std::this_thread::get_id()));
   std::this_thread::sleep_for(std::chrono::microseconds(g() % (max_us + 1)));
 }
+// every device call of a running server must come from one thread (its device thread); calls
+// during teardown (after the driver set g_teardown) are exempt
+std::atomic<bool> g_teardown{false};
+std::atomic<int> g_wrong_thread{0};
+static std::mutex g_owner_mu;
+static std::thread::id g_owner;
+void reset_device_owner() {
+  std::lock_guard<std::mutex> lk(g_owner_mu);
+  g_owner = std::thread::id();
+}
+static void device_call() {
+  if (g_teardown.load()) return;
+  std::lock_guard<std::mutex> lk(g_owner_mu);
+  if (g_owner == std::thread::id()) g_owner = std::this_thread::get_id();
+  else if (g_owner != std::this_thread::get_id()) g_wrong_thread.fetch_add(1);
+}
 struct ihipStream_t { int x; };
-hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { nap(50); *s = new ihipStream_t{1}; return 0; }
-hipError_t hipStreamDestroy(hipStream_t s) { delete s; return 0; }
-hipError_t hipStreamSynchronize(hipStream_t) { nap(200); return 0; }
-hipError_t hipMalloc(void** p, size_t n) { *p = std::malloc(n ? n : 1); return 0; }
-hipError_t hipFree(void* p) { std::free(p); return 0; }
-hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { std::memcpy(d, s, n); return 0; }
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { device_call(); nap(50); *s = new ihipStream_t{1}; return 0; }
+hipError_t hipStreamDestroy(hipStream_t s) { device_call(); delete s; return 0; }
+hipError_t hipStreamSynchronize(hipStream_t) { device_call(); nap(200); return 0; }
+hipError_t hipMalloc(void** p, size_t n) { device_call(); *p = std::malloc(n ? n : 1); return 0; }
+hipError_t hipFree(void* p) { device_call(); std::free(p); return 0; }
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  device_call();
+  std::memcpy(d, s, n);
+  return 0;
+}
 const char* hipGetErrorString(hipError_t) { return "stub"; }
 struct bmi_geometry { int k; };
 struct bmi_space { int64_t ndof, nsuper; };
@@ -37,26 +58,38 @@ bmi_status bmi_space_create(const bmi_geometry*, int M, int P, int, bmi_space** 
 bmi_status bmi_space_destroy(bmi_space* s) { delete s; return BMI_OK; }
 bmi_status bmi_space_info(const bmi_space* s, int64_t info[8]) { for (int k = 0; k < 8; ++k) info[k] = 0; info[4] = s->nsuper; info[5] = s->ndof; return BMI_OK; }
 bmi_status bmi_h2_estimate_bytes(const bmi_space* s, const bmi_op_desc*, int, int64_t* b) { *b = s->ndof * 1000; return BMI_OK; }
-bmi_status bmi_h2_assemble(const bmi_space* s, const bmi_op_desc* op, void*, bmi_h2** h) { nap(2000); *h = new bmi_h2{s->ndof, op->kind != 0}; return BMI_OK; }
-bmi_status bmi_h2_destroy(bmi_h2* h) { delete h; return BMI_OK; }
+bmi_status bmi_h2_assemble(const bmi_space* s, const bmi_op_desc* op, void*, bmi_h2** h) {
+  device_call();
+  nap(2000);
+  *h = new bmi_h2{s->ndof, op->kind != 0};
+  return BMI_OK;
+}
+bmi_status bmi_h2_destroy(bmi_h2* h) { device_call(); delete h; return BMI_OK; }
 bmi_status bmi_h2_use_graphs(bmi_h2*, int) { return BMI_OK; }
 bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* p, int nrhs, double, double* out, int, void*) {
+  device_call();
   nap(200);
   const int S = fn == 0 ? 1 : 2;
   for (int64_t i = 0; i < S * s->ndof * nrhs; ++i) out[i] = p[(i % nrhs) * 3];
   return BMI_OK;
 }
-static bmi_status solve(const bmi_h2* h, int nrhs, const double* B, double* X, int* it, double* rs) {
+// a solve that needs 45 steps: X = B / 4 after a first call shorter than that (time slice),
+// X = B / 2 when it converges; a resumed call (zero_guess 0) starts from X
+static bmi_status solve(const bmi_h2* h, int nrhs, const double* B, double* X, int maxit, int zero, int* it,
+                        double* rs) {
+  device_call();
   nap(3000);
   const int S = h->cplx ? 2 : 1;
-  for (int64_t i = 0; i < S * h->ndof * nrhs; ++i) X[i] = 0.5 * B[i];
-  for (int c = 0; c < nrhs; ++c) it[c] = 10 + c, rs[c] = 1e-11;
-  return BMI_OK;
+  const bool first = zero != 0;
+  const bool conv = !first || maxit >= 45;
+  for (int64_t i = 0; i < S * h->ndof * nrhs; ++i) X[i] = (conv ? 0.5 : 0.25) * B[i];
+  for (int c = 0; c < nrhs; ++c) it[c] = conv ? 10 + c : maxit, rs[c] = conv ? 1e-11 : 1e-3;
+  return conv ? BMI_OK : BMI_NOT_CONVERGED;
 }
-bmi_status bmi_solve_gmres(const bmi_h2* h, int nrhs, const double* B, double* X, double, int, int, int, int* it,
-                           double* rs, int, void*) { return solve(h, nrhs, B, X, it, rs); }
-bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, double, int, int, int* it, double* rs,
-                        int, void*) { return solve(h, nrhs, B, X, it, rs); }
+bmi_status bmi_solve_gmres(const bmi_h2* h, int nrhs, const double* B, double* X, double, int, int maxit, int zero,
+                           int* it, double* rs, int, void*) { return solve(h, nrhs, B, X, maxit, zero, it, rs); }
+bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, double, int maxit, int zero, int* it,
+                        double* rs, int, void*) { return solve(h, nrhs, B, X, maxit, zero, it, rs); }
 }
 extern "C" bmi_status bmi_op_desc_default(int kind, bmi_op_desc* d) {
   d->kind = kind; d->kappa_re = 0; d->kappa_im = 0; d->n_cheb = 9; d->eta = 1.6; d->min_cluster_level = 1;

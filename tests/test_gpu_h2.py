@@ -57,9 +57,10 @@ def test_gpu_h2_matvec_matches_oracle(gpu, kind, kappa, M, P, np_):
     oh = O.H2(os_, kind, kappa, np_)
     rng = np.random.default_rng(7)
     n = ps.number_of_dofs
-    x = rng.uniform(-1, 1, (n, 3))
+    # 67 columns: one full and one partial tile of 64 right-hand sides
+    x = rng.uniform(-1, 1, (n, 67))
     if kind != O.LAPLACE:
-        x = x + 1j * rng.uniform(-1, 1, (n, 3))
+        x = x + 1j * rng.uniform(-1, 1, (n, 67))
     y = H @ x
     yo = oh.matvec(x)
     r = rel(y, yo)
@@ -196,6 +197,15 @@ def test_gpu_mixed_precision_matvec_and_refinement(gpu):
     ps3 = B.AnsatzSpace(B.Geometry(), 3, 2)
     Hh64 = B.H2Matrix(ps3, oph, n_cheb=5)
     Hh32 = B.H2Matrix(ps3, oph, n_cheb=5, precision=B.MIXED)
+    # one column: FP32 near field with the coupling evaluated on the fly; several columns: FP32
+    # stored coupling blocks with the FP64 near field (a few columns per pass, or 64 per wave)
+    rng = np.random.default_rng(5)
+    nh = ps3.number_of_dofs
+    xh = rng.uniform(-1, 1, (nh, 12)) + 1j * rng.uniform(-1, 1, (nh, 12))
+    e1, e3, e12 = (rel(Hh32 @ xh[:, 0], Hh64 @ xh[:, 0]), rel(Hh32 @ xh[:, :3], Hh64 @ xh[:, :3]),
+                   rel(Hh32 @ xh, Hh64 @ xh))
+    print("helmholtz mixed matvec rel err: 1 column %.2e, 3 columns %.2e, 12 columns %.2e" % (e1, e3, e12))
+    assert e1 <= 1e-6 and e3 <= 1e-6 and e12 <= 1e-6
     bh = B.DiscreteLinearForm(B.PlaneWaves([[0.0, 0.0, 1.0]], 2.0), ps3).compute().get_discrete_linear_form()
     out = []
     for H in (Hh64, Hh32):

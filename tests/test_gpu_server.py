@@ -1,6 +1,6 @@
-"""Serving runtime on the GPU: results in a concurrent mix (batched, overlapping on the device)
-equal the results of the same requests solved alone, resident operators are reused, and the
-solutions satisfy the FP64 operator."""
+"""Serving runtime on the GPU: results in a mix (batched, taking turns on the device in time
+slices) equal the results of the same requests solved alone, resident operators are reused, and
+the solutions satisfy the FP64 operator."""
 import numpy as np
 import pytest
 
@@ -43,10 +43,11 @@ def solve_all(srv, reqs):
     return [srv.result(i, timeout=240) for i in ids]
 
 
-@pytest.mark.skip(reason="several batches on the device at once (max_active=4) ended once in a host segmentation "
-                         "fault inside a worker thread; the scheduler is clean under ThreadSanitizer and "
-                         "AddressSanitizer with stubbed device calls, the cause below it is not found yet, so this "
-                         "case stays off the GPU")
+@pytest.mark.skip(reason="several batches on the device at once, one host worker thread per batch, ended once in "
+                         "a host segmentation fault inside a worker thread; the scheduler is clean under "
+                         "ThreadSanitizer and AddressSanitizer with stubbed device calls and the cause below it is "
+                         "not found, so concurrent batches stay off the GPU (the server now drives all batches "
+                         "from one device thread in time slices: test_gpu_server_time_sliced_mix_equals_alone)")
 def test_gpu_server_mix_equals_alone(gpu):
     reqs = requests()
     alone = []
@@ -69,6 +70,29 @@ def test_gpu_server_mix_equals_alone(gpu):
     # the repeated request found its operator resident or shared a batch with its twin
     assert mixed[-1][1]["cache_hit"] or mixed[-1][1]["batch_requests"] > 1
     print("batches with more than one request:", batched)
+
+
+def test_gpu_server_time_sliced_mix_equals_alone(gpu):
+    """Several batches admitted at once take turns on the device in slices of one GMRES restart
+    cycle; a GMRES resumed from its iterate continues as it would have, so every result equals the
+    same request solved alone in one go."""
+    reqs = requests()
+    srv1 = B.Server(max_active=1, max_batch_rhs=1, slice_iterations=0)
+    alone = [solve_all(srv1, [r])[0] for r in reqs]
+    srv1.close()
+    srv = B.Server(max_active=4, max_batch_rhs=64, slice_iterations=30)
+    mixed = solve_all(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st["finished"] == len(reqs)
+    for (xa, ia), (xm, im), r in zip(alone, mixed, reqs):
+        assert ia["status"] == B.OK and im["status"] == B.OK, (ia, im)
+        assert rel(xm, xa) <= 1e-8, (r["M"], r["P"], rel(xm, xa))
+    assert mixed[-1][1]["cache_hit"] or mixed[-1][1]["batch_requests"] > 1
+    # some batch needed more than one turn, and batches were admitted while others were running
+    assert max(i["slices"] for _, i in mixed) > 1
+    spans = [(i["start_s"], i["end_s"]) for _, i in mixed]
+    assert any(a[0] < b[1] and b[0] < a[1] and a != b for a in spans for b in spans)
 
 
 def test_gpu_server_solution_satisfies_operator(gpu):

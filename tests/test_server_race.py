@@ -1,8 +1,9 @@
-"""The serving runtime's scheduler and worker threads under ThreadSanitizer and AddressSanitizer
+"""The serving runtime's scheduler and device threads under ThreadSanitizer and AddressSanitizer
 on the CPU: bmi_server.hip is compiled for the host against stand-ins of the HIP calls and of the
-operator/solver C ABI (random delays, real buffers); four clients submit and wait concurrently over
-rounds with different slot counts, batch limits and memory budgets (eviction), and every result is
-checked."""
+operator/solver C ABI (random delays, real buffers, solves that need two time slices); four
+clients submit and wait concurrently over rounds with different slot counts, batch limits, slice
+lengths and memory budgets (eviction).  Every result is checked, and every device call of a
+running server must come from its one device thread."""
 import os
 import shutil
 import subprocess
