@@ -294,14 +294,23 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
       h->fused_cpl_variant = pc;
     }
     if (h->fly) {
+      // reduced single-RHS sweep: coupling recomputed on the fly (variants 0, 1) or read from the
+      // FP32 copy (variants 2 + 2 * near blocks per iteration + coupling blocks per iteration),
+      // both with the FP32 near-field blocks; whichever is faster (arithmetic vs bytes)
       int pv = -1;
-      for (int v = 0; v < 2; ++v) {
+      for (int v = 0; v < 6; ++v) {
         bool ok = true;
         const float t = time_it([&] {
-          ok = launch_sweep_fly(h->cplx, v, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs,
-                                h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr,
-                                h->d_lvl_off.ptr, h->d_fly_pts.ptr, h->d_fly_org.ptr, h->op.kappa_re, h->op.kappa_im,
-                                h->d_fw[0].ptr, h->d_bw.ptr, st);
+          if (v < 2)
+            ok = launch_sweep_fly(h->cplx, v, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs,
+                                  h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr,
+                                  h->d_lvl_off.ptr, h->d_fly_pts.ptr, h->d_fly_org.ptr, h->op.kappa_re,
+                                  h->op.kappa_im, h->d_fw[0].ptr, h->d_bw.ptr, st);
+          else
+            ok = launch_sweep_fused(h->cplx, true, (v - 2) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd, h->d_drp.ptr,
+                                    h->d_dsrc.ptr, h->d_dense32.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr,
+                                    h->d_crp.ptr, h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, h->d_fw[0].ptr,
+                                    h->d_bw.ptr, st, nullptr, 0, true);
         });
         if (ok && (pv < 0 || t < best)) best = t, pv = v;
       }
@@ -328,9 +337,17 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
       launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
                      h->d_fw[0].ptr, h->d_lvl_off.ptr, st, h->d_bw.ptr);
     }
-    if (launch_sweep_fly(h->cplx, std::max(h->fly_variant, 0), h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys, h->ntgt,
-                         h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, h->d_lvl_off.ptr, h->d_fly_pts.ptr,
-                         h->d_fly_org.ptr, h->op.kappa_re, h->op.kappa_im, h->d_fw[0].ptr, h->d_bw.ptr, st)) {
+    const int v = std::max(h->fly_variant, 0);
+    const bool ok =
+        v < 2 ? launch_sweep_fly(h->cplx, v, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys,
+                                 h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, h->d_lvl_off.ptr,
+                                 h->d_fly_pts.ptr, h->d_fly_org.ptr, h->op.kappa_re, h->op.kappa_im, h->d_fw[0].ptr,
+                                 h->d_bw.ptr, st)
+              : launch_sweep_fused(h->cplx, true, (v - 2) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd, h->d_drp.ptr,
+                                   h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
+                                   h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st,
+                                   nullptr, 0, true);
+    if (ok) {
       TimedScope ts("matvec_backward", st);
       launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
                       h->d_lvl_off.ptr, yf, N, 0, st);

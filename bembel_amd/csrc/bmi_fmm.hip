@@ -333,11 +333,25 @@ __global__ __launch_bounds__(256) void k_forward_tree(int top, int bot, int from
                                                       int nrhs, const double* __restrict__ mom,
                                                       const double* __restrict__ T, const double* __restrict__ xs,
                                                       int64_t ldxs, int64_t xseg, double* __restrict__ fw,
-                                                      const int64_t* __restrict__ lvl_off, double* __restrict__ bwz) {
+                                                      const int64_t* __restrict__ lvl_off, double* __restrict__ bwz,
+                                                      int stage) {
   extern __shared__ double lds[];
   constexpr int S = C ? 2 : 1;
   const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
   const int D = bot - top;
+  // the transfer matrix (and the moment matrix for the bottom level) are read many times per
+  // workgroup: stage them in LDS after the level buffers when they fit (stage != 0)
+  if (stage) {
+    double* Ts = lds + tree_lds_off(D + 1, rows, S);
+    for (int q = tid; q < 4 * np2 * np2; q += blockDim.x) Ts[q] = T[q];
+    T = Ts;
+    if (from_moment) {
+      double* Ms = Ts + 4 * np2 * np2;
+      for (int q = tid; q < dd * rows; q += blockDim.x) Ms[q] = mom[q];
+      mom = Ms;
+    }
+    __syncthreads();
+  }
   // bottom level
   {
     const int n = (1 << (2 * D)) * rows;
@@ -408,11 +422,22 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
                                                        int nrhs, const double* __restrict__ mom,
                                                        const double* __restrict__ T, double* __restrict__ bw,
                                                        const int64_t* __restrict__ lvl_off, double* __restrict__ yf,
-                                                       int64_t ldy, int64_t yseg) {
+                                                       int64_t ldy, int64_t yseg, int stage) {
   extern __shared__ double lds[];
   constexpr int S = C ? 2 : 1;
   const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
   const int D = bot - top;
+  if (stage) {  // as in k_forward_tree
+    double* Ts = lds + tree_lds_off(D + 1, rows, S);
+    for (int q = tid; q < 4 * np2 * np2; q += blockDim.x) Ts[q] = T[q];
+    T = Ts;
+    if (to_leaf) {
+      double* Ms = Ts + 4 * np2 * np2;
+      for (int q = tid; q < dd * rows; q += blockDim.x) Ms[q] = mom[q];
+      mom = Ms;
+    }
+    __syncthreads();
+  }
   {
     const int64_t g = S * (lvl_off[top] * nrhs + (static_cast<int64_t>(ctop) * nrhs + k) * rows);
     for (int idx = tid; idx < rows; idx += blockDim.x) {
@@ -798,13 +823,13 @@ __global__ __launch_bounds__(256) void k_coupling_w(CplArgs a) {
 
 // the two sweeps of a single-RHS product in one launch: workgroups [0, nb_near) run the near
 // field, the rest the coupling blocks, so both streams of leaves share the memory system
-template <bool C, int DD, typename FT, int CD, int UBN, int UBC>
+template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT = double>
 __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, int nb_near) {
   const int b = static_cast<int>(blockIdx.x);
   const int w = static_cast<int>(threadIdx.x >> 6);
   if (b < nb_near) {
     const int c = __builtin_amdgcn_readfirstlane(b * 4 + w);
-    if (c < la.ncl) leaf_wave<C, DD, 1, UBN>(c, la);
+    if (c < la.ncl) leaf_wave<C, DD, 1, UBN, NFT>(c, la);
   } else {
     const int t = __builtin_amdgcn_readfirstlane((b - nb_near) * 4 + w);
     if (t < ca.ntgt) coupling_wave<C, FT, CD, 1, UBC>(t, ca);
@@ -833,12 +858,15 @@ struct MultiArgs {
   int accumulate;
 };
 
-template <bool C, typename FT, int ROWS>
+// One wave covers RPW of the ROWS rows (blockIdx.z picks which): fewer accumulators per lane and
+// several waves per owner keep enough waves resident to hide the latency of the scalar block loads.
+template <bool C, typename FT, int ROWS, int RPW>
 __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
   constexpr int S = C ? 2 : 1;
   constexpr bool F32 = std::is_same<FT, float>::value;
   const int r = blockIdx.x;
   const int k = static_cast<int>(blockIdx.y) * 64 + static_cast<int>(threadIdx.x);
+  const int i0 = RPW == ROWS ? 0 : static_cast<int>(blockIdx.z) * RPW;
   const bool valid = k < a.nrhs;
   const int kk = valid ? k : 0;
   int cl = r;
@@ -850,22 +878,22 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
     xo += lo;
     yo += lo;
   }
-  double ar[ROWS], ai[C ? ROWS : 1];
+  double ar[RPW], ai[C ? RPW : 1];
 #pragma unroll
-  for (int i = 0; i < ROWS; ++i) {
+  for (int i = 0; i < RPW; ++i) {
     ar[i] = 0.0;
     if constexpr (C) ai[i] = 0.0;
   }
   const FT* __restrict__ F = static_cast<const FT*>(a.F);
   const int p0 = a.rp[r], p1 = a.rp[r + 1];
   for (int p = p0; p < p1; ++p) {
-    const FT* f = F + S * (static_cast<int64_t>(p) * a.fstride + a.fcomp);
+    const FT* f = F + S * (static_cast<int64_t>(p) * a.fstride + a.fcomp + i0);
     const double* x = a.x + S * (xo + static_cast<int64_t>(a.src[p]) * a.xcs + static_cast<int64_t>(kk) * a.xks);
     if constexpr (F32) {
       // FP32 blocks: FP32 products summed per block, the block sums added in FP64
-      float br[ROWS], bi[C ? ROWS : 1];
+      float br[RPW], bi[C ? RPW : 1];
 #pragma unroll
-      for (int i = 0; i < ROWS; ++i) {
+      for (int i = 0; i < RPW; ++i) {
         br[i] = 0.f;
         if constexpr (C) bi[i] = 0.f;
       }
@@ -874,10 +902,11 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
         const float xi = C ? static_cast<float>(x[S * j + 1]) : 0.f;
         const float* fj = reinterpret_cast<const float*>(f) + S * j * ROWS;
 #pragma unroll
-        for (int i = 0; i < ROWS; ++i) {
-          const float fr = fj[S * i];
+        for (int i = 0; i < RPW; ++i) {
+          const int ii = RPW == ROWS ? i : (i0 + i < ROWS ? i : ROWS - 1 - i0);  // rows past the end repeat the last
+          const float fr = fj[S * ii];
           if constexpr (C) {
-            const float fi = fj[S * i + 1];
+            const float fi = fj[S * ii + 1];
             br[i] += fr * xr - fi * xi;
             bi[i] += fr * xi + fi * xr;
           } else {
@@ -886,7 +915,7 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
         }
       }
 #pragma unroll
-      for (int i = 0; i < ROWS; ++i) {
+      for (int i = 0; i < RPW; ++i) {
         ar[i] += static_cast<double>(br[i]);
         if constexpr (C) ai[i] += static_cast<double>(bi[i]);
       }
@@ -896,10 +925,11 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
         const double xi = C ? x[S * j + 1] : 0.0;
         const double* fj = reinterpret_cast<const double*>(f) + S * j * ROWS;
 #pragma unroll
-        for (int i = 0; i < ROWS; ++i) {
-          const double fr = fj[S * i];
+        for (int i = 0; i < RPW; ++i) {
+          const int ii = RPW == ROWS ? i : (i0 + i < ROWS ? i : ROWS - 1 - i0);
+          const double fr = fj[S * ii];
           if constexpr (C) {
-            const double fi = fj[S * i + 1];
+            const double fi = fj[S * ii + 1];
             ar[i] += fr * xr - fi * xi;
             ai[i] += fr * xi + fi * xr;
           } else {
@@ -910,9 +940,10 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
     }
   }
   if (!valid) return;
-  double* y = a.y + S * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k) * a.yks);
+  double* y = a.y + S * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k) * a.yks + i0);
 #pragma unroll
-  for (int i = 0; i < ROWS; ++i) {
+  for (int i = 0; i < RPW; ++i) {
+    if (RPW != ROWS && i0 + i >= ROWS) continue;
     double vr = ar[i], vi = C ? ai[i] : 0.0;
     if (a.accumulate) {
       vr += y[S * i];
@@ -1108,12 +1139,14 @@ void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs
   if (nrhs >= MULTI_MIN_RHS && (!Fc || ncoarse == 0)) {
     const MultiArgs ma{ntgt, nrhs, tgt, lvl_off, rp, src, F, fstride, fcomp, fw, 0,
                        static_cast<int64_t>(nrhs) * cd, cd, bw, 0, static_cast<int64_t>(nrhs) * cd, cd, accumulate};
-    const dim3 grid(static_cast<unsigned>(ntgt), static_cast<unsigned>((nrhs + 63) / 64));
-#define BMI_CPL_M(CC, FTT, CDD)                                                                   \
-  if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                       \
-    hipLaunchKernelGGL((k_multi_rows<CC, FTT, CDD>), grid, dim3(64), 0, st, ma);                  \
-    BMI_CHECK_HIP(hipGetLastError());                                                             \
-    return;                                                                                       \
+#define BMI_CPL_M(CC, FTT, CDD)                                                                       \
+  if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                           \
+    constexpr int RPW = CDD;                                                                          \
+    const dim3 grid(static_cast<unsigned>(ntgt), static_cast<unsigned>((nrhs + 63) / 64),             \
+                    static_cast<unsigned>((CDD + RPW - 1) / RPW));                                    \
+    hipLaunchKernelGGL((k_multi_rows<CC, FTT, CDD, RPW>), grid, dim3(64), 0, st, ma);                 \
+    BMI_CHECK_HIP(hipGetLastError());                                                                 \
+    return;                                                                                           \
   }
     BMI_CPL_M(false, double, 16)
     BMI_CPL_M(false, float, 16)
@@ -1190,15 +1223,20 @@ void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int
     const int depth = first ? std::min(2, maxd) : maxd;
     const int top = bot - depth > 0 ? bot - depth : 0;
     const int D = bot - top;
-    const size_t lds = sizeof(double) * S * rows * (((1 << (2 * (D + 1))) - 1) / 3);
+    size_t lds = sizeof(double) * S * rows * (((1 << (2 * (D + 1))) - 1) / 3);
     BMI_REQUIRE(lds <= 64 * 1024, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
+    // staging pays off when a workgroup's tree is the whole work (one RHS); with many RHS the
+    // matrices stay in the caches across the workgroups of the same subtree
+    const size_t staged = lds + sizeof(double) * (4 * np2 * np2 + (first ? dd * rows : 0));
+    const int stage = (nrhs == 1 && staged <= 64 * 1024) ? 1 : 0;
+    if (stage) lds = staged;
     const dim3 grid(static_cast<unsigned>(npatch << (2 * top)), static_cast<unsigned>(nrhs));
     if (cplx)
       hipLaunchKernelGGL(k_forward_tree<true>, grid, dim3(256), lds, st, top, bot, first ? 1 : 0, rows, np2, dd, nrhs,
-                         mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero);
+                         mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero, stage);
     else
       hipLaunchKernelGGL(k_forward_tree<false>, grid, dim3(256), lds, st, top, bot, first ? 1 : 0, rows, np2, dd, nrhs,
-                         mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero);
+                         mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero, stage);
     BMI_CHECK_HIP(hipGetLastError());
     first = false;
     if (top == 0) break;
@@ -1228,15 +1266,18 @@ void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, in
     const int top = cuts[k], bot = cuts[k + 1];
     const int D = bot - top;
     const int to_leaf = k + 2 == cuts.size() ? 1 : 0;
-    const size_t lds = sizeof(double) * S * rows * (((1 << (2 * (D + 1))) - 1) / 3);
+    size_t lds = sizeof(double) * S * rows * (((1 << (2 * (D + 1))) - 1) / 3);
     BMI_REQUIRE(lds <= 64 * 1024, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
+    const size_t staged = lds + sizeof(double) * (4 * np2 * np2 + (to_leaf ? dd * rows : 0));
+    const int stage = (nrhs == 1 && staged <= 64 * 1024) ? 1 : 0;
+    if (stage) lds = staged;
     const dim3 grid(static_cast<unsigned>(npatch << (2 * top)), static_cast<unsigned>(nrhs));
     if (cplx)
       hipLaunchKernelGGL(k_backward_tree<true>, grid, dim3(256), lds, st, top, bot, to_leaf, rows, np2, dd, nrhs, mom,
-                         T, bw, lvl_off, yf, ldy, yseg);
+                         T, bw, lvl_off, yf, ldy, yseg, stage);
     else
       hipLaunchKernelGGL(k_backward_tree<false>, grid, dim3(256), lds, st, top, bot, to_leaf, rows, np2, dd, nrhs, mom,
-                         T, bw, lvl_off, yf, ldy, yseg);
+                         T, bw, lvl_off, yf, ldy, yseg, stage);
     BMI_CHECK_HIP(hipGetLastError());
   }
 }
@@ -1259,12 +1300,14 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
   if (nrhs >= MULTI_MIN_RHS) {
     const MultiArgs ma{ncl, nrhs, nullptr, nullptr, rp, src, F, fstride, fcomp, xs, xseg, dd, ldxs, ys, yseg, dd, ldys,
                        accumulate};
-    const dim3 grid(static_cast<unsigned>(ncl), static_cast<unsigned>((nrhs + 63) / 64));
-#define BMI_LEAF_M(CC, DDD)                                                                       \
-  if (cplx == CC && dd == DDD) {                                                                  \
-    hipLaunchKernelGGL((k_multi_rows<CC, double, DDD>), grid, dim3(64), 0, st, ma);               \
-    BMI_CHECK_HIP(hipGetLastError());                                                             \
-    return;                                                                                       \
+#define BMI_LEAF_M(CC, DDD)                                                                           \
+  if (cplx == CC && dd == DDD) {                                                                      \
+    constexpr int RPW = DDD;                                                                          \
+    const dim3 grid(static_cast<unsigned>(ncl), static_cast<unsigned>((nrhs + 63) / 64),              \
+                    static_cast<unsigned>((DDD + RPW - 1) / RPW));                                    \
+    hipLaunchKernelGGL((k_multi_rows<CC, double, DDD, RPW>), grid, dim3(64), 0, st, ma);              \
+    BMI_CHECK_HIP(hipGetLastError());                                                                 \
+    return;                                                                                           \
   }
     BMI_LEAF_M(false, 16)
     BMI_LEAF_M(true, 16)
@@ -1301,32 +1344,40 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
 }
 
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
-                        const int* dsrc, const double* dF, const double* xs, double* ys, int ntgt, int cd,
+                        const int* dsrc, const void* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
-                        const double* fw, double* bw, hipStream_t st, const unsigned short* Fc, int ncoarse) {
+                        const double* fw, double* bw, hipStream_t st, const unsigned short* Fc, int ncoarse,
+                        bool near32) {
   const LeafArgs la{ncl, 1, drp, dsrc, dF, static_cast<int64_t>(dd) * dd, 0, xs, 0, 0, ys, 0, 0, 0};
   const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(cd) * cd, 0, lvl_off, fw, bw, 0, Fc, ncoarse};
   const int nbn = (ncl + 3) / 4, nbc = (ntgt + 3) / 4;
   const dim3 grid(static_cast<unsigned>(nbn + nbc));
   TimedScope ts("matvec_sweep", st);
-#define BMI_FUSED(CC, DDD, FTT, CDD, UN, UC)                                                                    \
+#define BMI_FUSED(CC, DDD, FTT, CDD, UN, UC, NFTT)                                                              \
   if (cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&                        \
-      leaf_variant == UN - 1 && cpl_variant == UC - 1) {                                                        \
-    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC>), grid, dim3(256), 0, st, la, ca, nbn);        \
+      near32 == std::is_same<NFTT, float>::value && leaf_variant == UN - 1 && cpl_variant == UC - 1) {          \
+    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC, NFTT>), grid, dim3(256), 0, st, la, ca, nbn);  \
     BMI_CHECK_HIP(hipGetLastError());                                                                           \
     return true;                                                                                                \
   }
-#define BMI_FUSED_V(CC, DDD, FTT, CDD) \
-  BMI_FUSED(CC, DDD, FTT, CDD, 1, 1) BMI_FUSED(CC, DDD, FTT, CDD, 1, 2) BMI_FUSED(CC, DDD, FTT, CDD, 2, 1) \
-  BMI_FUSED(CC, DDD, FTT, CDD, 2, 2)
-  BMI_FUSED_V(false, 36, double, 16)
-  BMI_FUSED_V(false, 36, float, 16)
-  BMI_FUSED_V(true, 36, double, 25)
-  BMI_FUSED_V(true, 36, float, 25)
-  BMI_FUSED_V(true, 36, double, 16)
-  BMI_FUSED_V(true, 36, float, 16)
-  BMI_FUSED_V(false, 16, double, 16)
-  BMI_FUSED_V(true, 16, double, 16)
+#define BMI_FUSED_V(CC, DDD, FTT, CDD, NFTT)                                                   \
+  BMI_FUSED(CC, DDD, FTT, CDD, 1, 1, NFTT) BMI_FUSED(CC, DDD, FTT, CDD, 1, 2, NFTT)            \
+  BMI_FUSED(CC, DDD, FTT, CDD, 2, 1, NFTT) BMI_FUSED(CC, DDD, FTT, CDD, 2, 2, NFTT)
+  BMI_FUSED_V(false, 36, double, 16, double)
+  BMI_FUSED_V(false, 36, float, 16, double)
+  BMI_FUSED_V(true, 36, double, 25, double)
+  BMI_FUSED_V(true, 36, float, 25, double)
+  BMI_FUSED_V(true, 36, double, 16, double)
+  BMI_FUSED_V(true, 36, float, 16, double)
+  BMI_FUSED_V(false, 16, double, 16, double)
+  BMI_FUSED_V(true, 16, double, 16, double)
+  // reduced operators with the FP32 near-field copy
+  BMI_FUSED_V(false, 36, float, 16, float)
+  BMI_FUSED_V(false, 36, float, 25, float)
+  BMI_FUSED_V(true, 36, float, 25, float)
+  BMI_FUSED_V(true, 36, float, 16, float)
+  BMI_FUSED_V(false, 16, float, 16, float)
+  BMI_FUSED_V(true, 16, float, 16, float)
 #undef BMI_FUSED_V
 #undef BMI_FUSED
   return false;

@@ -98,15 +98,18 @@ __device__ __forceinline__ double dummy_value(int fn, const PT& a, const PT& b) 
 // each adds all G evaluations to its part of the block: for complex operators the lane role
 // (lane mod G) picks real or imaginary parts (role & 1) and, with G = 4, a half of the rows
 // (role >> 1).  Per lane this divides the accumulator registers by G at the same number of
-// multiply-adds; real blocks (G = 1) stay whole in every lane.
+// multiply-adds; real blocks (G = 1) stay whole in every lane.  Complex blocks use G = 2 (a
+// four-way split halves the accumulators again but doubles the exchanges, which measured
+// slower for P = 2).
 template <int P, int KIND>
 struct BlockAcc {
   static constexpr int I = P + 1;
   static constexpr int I2 = I * I;
   static constexpr bool CPLX = KIND == BMI_HELMHOLTZ_SL || KIND == BMI_MAXWELL_SL;
   static constexpr bool MAXW = KIND == BMI_MAXWELL_SL;
-  // lanes per group: real blocks fit one lane (P <= 2), complex ones are split two or four ways
-  static constexpr int G = CPLX ? (P >= 2 ? 4 : 2) : 1;
+  // lanes per group: real blocks fit one lane (P <= 2), complex ones are split into their real
+  // and imaginary parts (two lanes)
+  static constexpr int G = CPLX ? 2 : 1;
   static constexpr int RH = CPLX ? G / 2 : G;         // row halves
   static constexpr int NR = (I2 + RH - 1) / RH;       // rows held per lane
   static constexpr int ID = MAXW ? I : 1;
@@ -139,7 +142,8 @@ struct BlockAcc {
       // exp(-i kappa r) / (4 pi r), kappa = kre + i kim (Helmholtz/SingleLayerOperator.hpp:115-120)
       double sn, cs;
       sincos(prm.kre * r, &sn, &cs);
-      const double mag = exp(prm.kim * r) * (a.W * b.W) * (0.25 * M_1_PI) / r;
+      const double damp = prm.kim != 0.0 ? exp(prm.kim * r) : 1.0;
+      const double mag = damp * (a.W * b.W) * (0.25 * M_1_PI) / r;
       const double kr = mag * cs, ki = -mag * sn;
       if constexpr (MAXW) {
         // vector part K w w (t_rc(a) . t_cc(b)), divergence part -K w w / (kappa^2 h^2)
@@ -259,6 +263,37 @@ struct BlockAcc {
     return row < I2 ? part * I2 * I2 + row * I2 + k % I2 : -1;
   }
 };
+
+// Duffy rules reuse surface points: some points of the edge rule depend on the outer index i and
+// one coordinate of the inner index j only, and the inner points of the vertex rule on the outer
+// first coordinate and j only.  Those points are tabulated in LDS for a chunk of outer indices and
+// the (i, j) work of the chunk is spread over the workgroup.  The chunk is the smallest number of
+// outer indices that keeps at least 90 % of the lanes busy (or the best one that fits).
+__host__ __device__ constexpr int pick_chunk(int items_per_unit, int nt, int max_units) {
+  int best = 1;
+  int best_num = 0, best_den = 1;
+  for (int c = 1; c <= max_units; ++c) {
+    const int items = c * items_per_unit;
+    const int slots = ((items + nt - 1) / nt) * nt;
+    if (10 * items >= 9 * slots) return c;
+    if (items * best_den > best_num * slots) best = c, best_num = items, best_den = slots;
+  }
+  return best;
+}
+// LDS for the point tables (bytes), shared by the launch and the kernel
+constexpr int DUFFY_TABLE_BYTES = 48 * 1024;
+template <class PtK>
+__host__ __device__ constexpr int edge_chunk(int n, int nt) {
+  const int cap = DUFFY_TABLE_BYTES / (8 * n * static_cast<int>(sizeof(PtK)));
+  const int mx = cap < n * n ? cap : n * n;
+  return pick_chunk(n * n, nt, mx > 0 ? mx : 1);
+}
+template <class PtK>
+__host__ __device__ constexpr int vertex_chunk(int n, int nt) {
+  const int cap = DUFFY_TABLE_BYTES / (2 * n * n * static_cast<int>(sizeof(PtK)));
+  const int mx = cap < n ? cap : n;
+  return pick_chunk(n * n * n, nt, mx > 0 ? mx : 1);
+}
 
 // DuffyTrick::tau (DuffyTrick/tau.hpp:26-42)
 __device__ __forceinline__ void tau(double x, double y, int c, double& u, double& v) {
@@ -397,47 +432,93 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       }
     }
   } else if constexpr (CLS == 3) {
-    // integrate3: common edge, 6 Duffy sub-regions
+    // integrate3: common edge, 6 Duffy sub-regions.  With t1 = xi0 (1 - xi1), t2 = (1 - xi0)(1 - xi1),
+    // e0 = xi1 xj0, e1 = xi1 xj1, t3 = xi0 (1 - e0), t4 = (1 - xi0)(1 - e0), sub-region k = 2g + odd
+    // pairs a = tau(a0 or 1 - a0, a1; r1) on element 1 (weight w) with b = tau(b0 or 1 - b0, b1; r2)
+    // on element 2 (weight wa for g = 0, else wb), where (a0, a1) = (t1, e0), (t3, xi1), (t4, e1)
+    // and (b0, b1) = (t2, e1), (t4, e1), (t3, xi1).  The points of g = 0 (a: i, xj0; b: i, xj1), the
+    // a points of g = 1 and the b points of g = 2 (i, xj0) come from per-chunk tables (W holds the
+    // surface measure; the weight is applied where the point is used, as make_point does).
     const int r1 = pr.rot1, r2 = pr.rot2;
-    const int total = nq * nq;
-    for (int base = 0; base < total; base += NT) {
-      const bool valid = base + tid < total;
-      const int idx = valid ? base + tid : total - 1;
-      const int i = idx / nq, j = idx - (idx / nq) * nq;
-      const double xi0 = gx[i / n], xi1 = gx[i % n], wi = gw[i / n] * gw[i % n];
-      const double xj0 = gx[j / n], xj1 = gx[j % n], wj = gw[j / n] * gw[j % n];
-      const double w = h * h * wi * xi1 * xi1;
-      const double t1 = xi0 * (1 - xi1);
-      const double t2 = (1 - xi0) * (1 - xi1);
-      const double e0 = xi1 * xj0, e1 = xi1 * xj1;
-      const double t3 = xi0 * (1 - e0);
-      const double t4 = (1 - xi0) * (1 - e0);
-      const double wa = wj * (1 - xi1), wb = wj * (1 - e0);
-      // sub-region k: first coordinates {t1, 1-t1, t3, 1-t3, t4, 1-t4} and {t2, 1-t2, t4, 1-t4, t3,
-      // 1-t3}, second {e0, e0, xi1, xi1, e1, e1} and {e1, e1, e1, e1, xi1, xi1}, weights
-      // {wa, wa, wb, wb, wb, wb}, written as selects (see above for the unrolling)
-      constexpr int UNR3 = Acc::G == 1 ? 6 : 1;
-#pragma unroll UNR3
-      for (int k = 0; k < 6; ++k) {
-        const int g = k >> 1;
-        const bool odd = k & 1;
-        const double a0 = g == 0 ? t1 : (g == 1 ? t3 : t4);
-        const double b0 = g == 0 ? t2 : (g == 1 ? t4 : t3);
-        const double a1 = g == 0 ? e0 : (g == 1 ? xi1 : e1);
-        const double b1 = g == 2 ? xi1 : e1;
-        const double bw = g == 0 ? wa : wb;
+    const int CI = edge_chunk<PtK>(n, NT);
+    PtK* tab = reinterpret_cast<PtK*>(lds);
+    for (int i0 = 0; i0 < nq; i0 += CI) {
+      const int cn = nq - i0 < CI ? nq - i0 : CI;
+      const int per = cn * n;  // entries of one table kind
+      __syncthreads();
+      for (int e = tid; e < 8 * per; e += NT) {
+        const int kind = e / per, rem = e - kind * per;
+        const int ci = rem / n, jx = rem - ci * n;
+        const int i = i0 + ci;
+        const double xi0 = gx[i / n], xi1 = gx[i % n], xj = gx[jx];
+        const bool odd = kind & 1;
         double u, v;
-        tau(odd ? 1 - a0 : a0, a1, r1, u, v);
-        const PtK a = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, w);
-        tau(odd ? 1 - b0 : b0, b1, r2, u, v);
-        const PtK b = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, bw);
-        acc.add(a, b, prm, rc, cc, valid, lane);
+        if (kind < 2) {  // g = 0, a: (t1, e0) on element 1
+          const double t1 = xi0 * (1 - xi1);
+          tau(odd ? 1 - t1 : t1, xi1 * xj, r1, u, v);
+          tab[e] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, 1.0);
+        } else if (kind < 4) {  // g = 1, a: (t3, xi1) on element 1
+          const double t3 = xi0 * (1 - xi1 * xj);
+          tau(odd ? 1 - t3 : t3, xi1, r1, u, v);
+          tab[e] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, 1.0);
+        } else if (kind < 6) {  // g = 2, b: (t3, xi1) on element 2
+          const double t3 = xi0 * (1 - xi1 * xj);
+          tau(odd ? 1 - t3 : t3, xi1, r2, u, v);
+          tab[e] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, 1.0);
+        } else {  // g = 0, b: (t2, e1) on element 2 (xj = xj1)
+          const double t2 = (1 - xi0) * (1 - xi1);
+          tau(odd ? 1 - t2 : t2, xi1 * xj, r2, u, v);
+          tab[e] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, 1.0);
+        }
+      }
+      __syncthreads();
+      const int total = cn * nq;
+      for (int base = 0; base < total; base += NT) {
+        const bool valid = base + tid < total;
+        const int idx = valid ? base + tid : total - 1;
+        const int ci = idx / nq, j = idx - (idx / nq) * nq;
+        const int i = i0 + ci;
+        const int j0 = j / n, j1 = j - (j / n) * n;
+        const double xi0 = gx[i / n], xi1 = gx[i % n], wi = gw[i / n] * gw[i % n];
+        const double xj0 = gx[j0], xj1 = gx[j1], wj = gw[j0] * gw[j1];
+        const double w = h * h * wi * xi1 * xi1;
+        const double e0 = xi1 * xj0, e1 = xi1 * xj1;
+        const double t4 = (1 - xi0) * (1 - e0);
+        const double wa = wj * (1 - xi1), wb = wj * (1 - e0);
+        const int ta = ci * n + j0, tb = ci * n + j1;
+#pragma unroll 1
+        for (int k = 0; k < 6; ++k) {
+          const int g = k >> 1;
+          const bool odd = k & 1;
+          PtK a, b;
+          if (g == 0) {
+            a = tab[(0 + odd) * per + ta];
+            b = tab[(6 + odd) * per + tb];
+          } else if (g == 1) {
+            double u, v;
+            a = tab[(2 + odd) * per + ta];
+            tau(odd ? 1 - t4 : t4, e1, r2, u, v);
+            b = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, 1.0);
+          } else {
+            double u, v;
+            tau(odd ? 1 - t4 : t4, e1, r1, u, v);
+            a = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, 1.0);
+            b = tab[(4 + odd) * per + ta];
+          }
+          a.W *= w;
+          b.W *= g == 0 ? wa : wb;
+          acc.add(a, b, prm, rc, cc, valid, lane);
+        }
       }
     }
+    __syncthreads();
   } else {
-    // integrate4: common vertex; the four outer points per i are staged in LDS
+    // integrate4: common vertex.  The four outer points per i are staged in LDS; the two inner
+    // points p5 = tau(e0, e1; r2) on element 2 and p6 = tau(e0, e1; r1) on element 1, e = xi0 xj,
+    // depend on the first outer coordinate and j only and are tabulated per chunk of xi0 values
     const int r1 = pr.rot1, r2 = pr.rot2;
     PtK* q = reinterpret_cast<PtK*>(lds);  // 4 * nq
+    PtK* tin = q + 4 * nq;                 // 2 * nq per xi0 of the chunk
     for (int i = tid; i < nq; i += NT) {
       const double xi0 = gx[i / n], wi = gw[i / n] * gw[i % n];
       const double xi1 = gx[i % n] * xi0;
@@ -452,23 +533,41 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       tau(xi1, xi0, r2, u, v);
       q[4 * i + 3] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, w);
     }
-    __syncthreads();
-    const int total = nq * nq;
-    for (int base = 0; base < total; base += NT) {
-      const bool valid = base + tid < total;
-      const int idx = valid ? base + tid : total - 1;
-      const int i = idx / nq, j = idx - (idx / nq) * nq;
-      const double xi0 = gx[i / n];
-      const double e0 = xi0 * gx[j / n], e1 = xi0 * gx[j % n], wj = gw[j / n] * gw[j % n];
-      double u, v;
-      tau(e0, e1, r2, u, v);
-      const PtK p5 = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, wj);
-      tau(e0, e1, r1, u, v);
-      const PtK p6 = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, wj);
-      acc.add(q[4 * i + 0], p5, prm, rc, cc, valid, lane);
-      acc.add(q[4 * i + 1], p5, prm, rc, cc, valid, lane);
-      acc.add(p6, q[4 * i + 2], prm, rc, cc, valid, lane);
-      acc.add(p6, q[4 * i + 3], prm, rc, cc, valid, lane);
+    const int C0 = vertex_chunk<PtK>(n, NT);
+    for (int a0 = 0; a0 < n; a0 += C0) {
+      const int cn = n - a0 < C0 ? n - a0 : C0;
+      __syncthreads();
+      for (int e = tid; e < 2 * cn * nq; e += NT) {
+        const int which = e / (cn * nq), rem = e - which * (cn * nq);
+        const int c0 = rem / nq, j = rem - c0 * nq;
+        const double xi0 = gx[a0 + c0];
+        const double e0 = xi0 * gx[j / n], e1 = xi0 * gx[j % n];
+        double u, v;
+        if (which == 0) {
+          tau(e0, e1, r2, u, v);
+          tin[e] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, 1.0);
+        } else {
+          tau(e0, e1, r1, u, v);
+          tin[e] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, 1.0);
+        }
+      }
+      __syncthreads();
+      const int total = cn * n * nq;  // (xi0 of the chunk, xi1, j)
+      for (int base = 0; base < total; base += NT) {
+        const bool valid = base + tid < total;
+        const int idx = valid ? base + tid : total - 1;
+        const int c0 = idx / (n * nq), rem = idx - c0 * (n * nq);
+        const int i1 = rem / nq, j = rem - i1 * nq;
+        const int i = (a0 + c0) * n + i1;
+        const double wj = gw[j / n] * gw[j % n];
+        PtK p5 = tin[c0 * nq + j], p6 = tin[(cn + c0) * nq + j];
+        p5.W *= wj;
+        p6.W *= wj;
+        acc.add(q[4 * i + 0], p5, prm, rc, cc, valid, lane);
+        acc.add(q[4 * i + 1], p5, prm, rc, cc, valid, lane);
+        acc.add(p6, q[4 * i + 2], prm, rc, cc, valid, lane);
+        acc.add(p6, q[4 * i + 3], prm, rc, cc, valid, lane);
+      }
     }
     __syncthreads();
   }
@@ -551,10 +650,13 @@ static void launch_class(const NFPair* d_pairs, int count, int deg, NFParams prm
   constexpr int NT = (CLS == 0) ? 64 : 256;
   const int nq = (deg + 1) * (deg + 1);
   size_t lds = sizeof(double) * ((NT / 64) + 1) * BlockAcc<P, KIND>::TOT;
+  const int n = deg + 1;
   if (CLS == 1) lds = std::max(lds, 2 * nq * sizeof(Pt<KIND>));
-  if (CLS == 4) lds = std::max(lds, 4 * nq * sizeof(Pt<KIND>));
+  if (CLS == 3) lds = std::max(lds, static_cast<size_t>(8 * n * edge_chunk<Pt<KIND>>(n, NT)) * sizeof(Pt<KIND>));
+  if (CLS == 4)
+    lds = std::max(lds, static_cast<size_t>(4 * nq + 2 * nq * vertex_chunk<Pt<KIND>>(n, NT)) * sizeof(Pt<KIND>));
   BMI_REQUIRE(lds <= 160 * 1024, BMI_UNSUPPORTED, "quadrature degree too large for LDS staging");
-  const char* name = CLS >= 2 ? "nf_duffy" : "nf_regular";
+  const char* name = CLS == 2 ? "nf_identical" : CLS == 3 ? "nf_edge" : CLS == 4 ? "nf_vertex" : "nf_regular";
   TimedScope ts(name, st);
   const int ncomp = KIND == BMI_MAXWELL_SL ? 4 : 1;
   hipLaunchKernelGGL((k_nearfield<P, KIND, PD, CLS, NT>), dim3(count, ncomp), dim3(NT), lds, st, d_pairs,

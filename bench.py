@@ -37,20 +37,67 @@ def pct(a, q):
 
 
 def nearfield_flops(classes_deg, P, pd):
-    """algorithmic FP64 flops of the Duffy pairs: evaluations * F_e + surface points * F_g"""
+    """FP64 flops of the Duffy pairs as computed: kernel evaluations * F_e + surface points * F_g.
+    Surface points per pair with n points per direction: identical 8 n^4; edge 4 n^4 + 8 n^3 (the
+    points that depend on one inner coordinate only are tabulated per outer index, instead of the
+    reference schedule's 12 n^4); vertex 4 n^2 + 2 n^3 (inner points tabulated per first outer
+    coordinate, instead of 4 n^2 + 2 n^4)"""
     Fe = 2 * (P + 1) ** 4 + 20
     Fg = 24 * pd * pd + 40
     tot = 0.0
     for cls, n, cnt in classes_deg:
         q = float(n) ** 4
+        c = float(n) ** 3
         if cls == 2:
             e, g = 8 * q, 8 * q
         elif cls == 3:
-            e, g = 6 * q, 12 * q
+            e, g = 6 * q, 4 * q + 8 * c
         else:
-            e, g = 4 * q, 4 * n * n + 2 * q
+            e, g = 4 * q, 4 * n * n + 2 * c
         tot += cnt * (e * Fe + g * Fg)
     return tot
+
+
+def measured_traffic(M, P, npts):
+    """HBM bytes per launch of the product sweep (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, scaled by a
+    1 GiB copy), as recorded under profiles/ for this workload; None for other sizes"""
+    f = os.path.join(ROOT, "profiles", "k_sweep_fused_laplace_M%d_P%d_np%d_traffic.json" % (M, P, npts))
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        d = json.load(fh)
+    return d["fetch_bytes"] + d["write_bytes"]
+
+
+def run_helmholtz_profile(B, torch):
+    """Helmholtz kappa 4, 24,576 elements, P 2, 5x5 Chebyshev nodes: per precision an assembly, 23
+    products and 300 GMRES(30) steps (the workload of the Helmholtz kernel statistics)"""
+    op = B.HelmholtzSingleLayerOperator()
+    op.set_wavenumber(4.0)
+    space = B.AnsatzSpace(B.Geometry(), 6, 2, op)
+    n = space.number_of_dofs
+    for name, prec in (("fp64", B.FP64), ("mixed", B.MIXED)):
+        t0 = time.perf_counter()
+        H = B.H2Matrix(space, op, n_cheb=5, precision=prec)
+        torch.cuda.synchronize()
+        log(name, "assembly %.3f s" % (time.perf_counter() - t0))
+        x = torch.rand(n, dtype=torch.float64, device="cuda") + 1j * torch.rand(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(3):
+            H.matvec(x, sync=False)
+        B.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            H.matvec(x, sync=False)
+        B.synchronize()
+        log(name, "matvec %.3f ms" % ((time.perf_counter() - t0) / 20 * 1e3))
+        b = B.DiscreteLinearForm(B.PlaneWaves([[0.0, 0.0, 1.0]], 4.0), space).compute(
+            device=True).get_discrete_linear_form()
+        t0 = time.perf_counter()
+        gmres_solve(B, H, b, maxit=300)
+        torch.cuda.synchronize()
+        log(name, "300 GMRES steps %.3f s" % (time.perf_counter() - t0))
+        del H
 
 
 def run_solves(B, torch, H, space, op, npts, b, nsolves):
@@ -103,7 +150,7 @@ def timed(torch, fn, reps=1):
     return (time.perf_counter() - t0) / reps, out
 
 
-def gmres_solve(B, H, b, tol=1e-10, restart=30, maxit=5000):
+def gmres_solve(B, H, b, tol=1e-10, restart=30, maxit=20000):
     gm = B.GMRES()
     gm.set_tolerance(tol)
     gm.set_restart(restart)
@@ -134,12 +181,15 @@ def run_batched(B, torch, rng, steps):
         out[name + "_matvec_ms"] = t * 1e3
         out[name + "_matvec_dof_rhs_per_s"] = n * nrhs / t
         r1 = rhs[:, :1].contiguous()
+        HH.matvec(r1)  # first single-RHS product of the handle (graph capture)
         t1, _ = timed(torch, lambda: HH.matvec(r1, sync=False), max(3, steps // 4))
         out[name + "_matvec_1rhs_ms"] = t1 * 1e3
-    t, (x64, gm) = timed(torch, lambda: gmres_solve(B, H, rhs))
+    log("batched 64 RHS: matvec fp64 %.2f ms, mixed %.2f ms" % (out["fp64_matvec_ms"], out["mixed_matvec_ms"]))
+    t, (x64, gm) = timed(torch, lambda: gmres_solve(B, H, rhs, maxit=8000))
     out["fp64_gmres_s"] = t
     out["fp64_gmres_iterations_max"] = int(np.max(gm.iterations()))
-    t, (xm, gmm) = timed(torch, lambda: gmres_solve(B, Hm, rhs))
+    log("batched 64 RHS: GMRES fp64 %.2f s (%d it)" % (t, out["fp64_gmres_iterations_max"]))
+    t, (xm, gmm) = timed(torch, lambda: gmres_solve(B, Hm, rhs, maxit=8000))
     out["mixed_gmres_s"] = t
     out["mixed_gmres_iterations_max"] = int(np.max(gmm.iterations()))
     out["mixed_rel_err_vs_fp64_max"] = float(torch.max(torch.linalg.norm(xm - x64, dim=0) /
@@ -171,6 +221,7 @@ def run_headline(B, torch, rng, cpu_threads, with_cpu):
         sols[name] = x
         out[name] = {"assembly_s": t_asm, "gmres_s": t_sol, "total_s": t_asm + t_sol,
                      "iterations": int(gm.iterations()[0]), "residual": float(gm.error()[0])}
+        log("Helmholtz M6 %s: assembly %.2f s, GMRES %.2f s (%d it)" % (name, t_asm, t_sol, out[name]["iterations"]))
         if name == "fp64":
             keep = H
         else:
@@ -296,6 +347,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-RHS Helmholtz workload")
     ap.add_argument("--no-headline", action="store_true", help="skip the Helmholtz 24,576-element assembly + GMRES")
     ap.add_argument("--serving", type=int, default=16, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--helmholtz-profile", action="store_true",
+                    help="only the Helmholtz 24,576-element products and GMRES steps (profiling workload)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the H2 path on a single MI355X "
@@ -306,6 +359,9 @@ def main():
     import bembel_amd as B
     torch.cuda.set_device(0)
     torch.cuda.init()
+    if a.helmholtz_profile:
+        run_helmholtz_profile(B, torch)
+        return
     M, P, npts = a.level, 2, 4
     geo = B.Geometry()
     op = B.LaplaceSingleLayerOperator()
@@ -333,17 +389,20 @@ def main():
     # per-kernel breakdown of the assembly (second assembly, events around every launch)
     duffy_ms = reg_ms = cpl_ms = 0.0
     duffy_n = reg_n = cpl_n = 0
+    cls_ms = {}
     if not a.no_breakdown:
         B.timing.reset()
         B.timing.enable(True)
         H2b = B.H2Matrix(space, op, n_cheb=npts)
         B.timing.enable(False)
-        duffy_ms, duffy_n = B.timing.get("nf_duffy")
+        cls_ms = {k: B.timing.get("nf_" + k)[0] for k in ("identical", "edge", "vertex")}
+        duffy_ms = sum(cls_ms.values())
+        duffy_n = sum(B.timing.get("nf_" + k)[1] for k in ("identical", "edge", "vertex"))
         reg_ms, reg_n = B.timing.get("nf_regular")
         cpl_ms, cpl_n = B.timing.get("coupling_assemble")
         del H2b
-    log("assembly kernels: nf_duffy %.1f ms (%d launches), nf_regular %.1f ms (%d), coupling %.1f ms" % (
-        duffy_ms, duffy_n, reg_ms, reg_n, cpl_ms))
+    log("assembly kernels: nf_duffy %.1f ms (%d launches: %s), nf_regular %.1f ms (%d), coupling %.1f ms" % (
+        duffy_ms, duffy_n, ", ".join("%s %.1f" % kv for kv in cls_ms.items()), reg_ms, reg_n, cpl_ms))
 
     # ---------------------------------------------------------------- matvec (the step)
     g = torch.Generator(device="cuda")
@@ -406,6 +465,13 @@ def main():
     fl = nearfield_flops([(2, ntouch, pairs_by_class[2]), (3, ntouch, pairs_by_class[3]),
                           (4, ntouch, pairs_by_class[4])], P, 5)
     duffy_tflops = fl / (duffy_ms * 1e-3) / 1e12 if duffy_ms > 0 else 0.0
+    cls_tflops = {}
+    for c, name in ((2, "identical"), (3, "edge"), (4, "vertex")):
+        t = cls_ms.get(name, 0.0)
+        if t > 0:
+            cls_tflops[name] = nearfield_flops([(c, ntouch, pairs_by_class[c])], P, 5) / (t * 1e-3) / 1e12
+    log("Duffy FP64: %.1f TFLOP/s overall (%s)" % (
+        duffy_tflops, ", ".join("%s %.1f" % kv for kv in cls_tflops.items())))
 
     # ---------------------------------------------------------------- solves
     rng = np.random.default_rng(2024)
@@ -479,21 +545,24 @@ def main():
                    "elements": space.number_of_elements, "dofs": n, "polynomial_degree": P, "n_cheb": npts,
                    "eta": 1.6, "min_cluster_level": 1, "nrhs": 1},
         "mixed_matvec": {"ms_per_step": t_mv_mixed * 1e3, "dof_per_s": n / t_mv_mixed, "rel_err_vs_fp64": mixed_err,
-                         "storage": "coupling blocks FP32, near field FP64, sums in FP64"},
+                         "storage": "near-field blocks FP32, coupling blocks FP32 (recomputed from the Chebyshev "
+                                    "surface points or read, whichever measured faster), sums in FP64"},
         "h2_assembly_s": t_asm,
         "space_setup_s": t_space,
-        "assembly_kernels_ms": {"nf_duffy": duffy_ms, "nf_regular": reg_ms, "coupling_assemble": cpl_ms},
+        "assembly_kernels_ms": {"nf_duffy": duffy_ms, "nf_duffy_by_class": cls_ms, "nf_regular": reg_ms,
+                                "coupling_assemble": cpl_ms},
         "h2": {k: info[k] for k in ("dense", "lowrank", "pairs", "pair_classes", "device_bytes")},
         "solve": {"solver": "CG", "tol": 1e-10, "fp64_iterations": it64, "fp64_p50_s": pct(lat64, 50),
                   "fp64_p99_s": pct(lat64, 99), "mixed_p50_s": pct(latm, 50), "mixed_p99_s": pct(latm, 99),
                   "mixed_rel_err_vs_fp64": rel_err_mixed, "solves": a.solves},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(M, P, npts),
                      "kernel": "k_sweep_fused (near-field leaves + coupling blocks of the matvec)",
                      "bytes_per_launch": sw_bytes, "avg_launch_ms": sw_avg},
         "roofline_assembly": {"bound": "fp64", "achieved": duffy_tflops, "peak": FP64_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": duffy_tflops / FP64_PEAK_TFLOPS,
-                              "kernel": "k_nearfield identical/edge/vertex (Duffy)", "flops": fl},
+                              "kernel": "k_nearfield identical/edge/vertex (Duffy)", "flops": fl,
+                              "by_class_tflops": cls_tflops},
         "cpu_baseline": cpu,
         "batched_64rhs": batched,
         "helmholtz_assembly_gmres": headline,

@@ -46,9 +46,10 @@ typedef enum {
 typedef enum {
   BMI_PRECISION_FP64 = 0,       /* everything FP64 */
   BMI_PRECISION_MIXED = 1,      /* reduced operator for products and inner solves: coupling blocks in FP32
-                                   (one RHS: recomputed on the fly in FP32 from the Chebyshev surface
-                                   points and combined with an FP32 copy of the near-field blocks;
-                                   several RHS: stored FP32 blocks with the FP64 near field), sums in
+                                   (one RHS: an FP32 copy of the near-field blocks with the coupling
+                                   blocks recomputed in FP32 from the Chebyshev surface points or read
+                                   from their FP32 copy, whichever the first product measures faster;
+                                   several RHS: FP32 coupling blocks with the FP64 near field), sums in
                                    FP64; the FP64 operator is kept and solvers refine with it */
   BMI_PRECISION_MIXED_BF16 = 2  /* as MIXED, and the coupling blocks of the two coarsest levels in BF16 */
 } bmi_precision;
