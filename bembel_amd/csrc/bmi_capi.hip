@@ -104,11 +104,13 @@ struct bmi_h2 {
   // leaf storage
   DevBuf<double> d_dense, d_coup;
   DevBuf<float> d_coup32;
+  int ld32 = 0;  // column stride of the FP32 coupling blocks
   DevBuf<unsigned short> d_coup16;  // BF16 copies of the coarsest coupling blocks (MIXED_BF16)
   int ncoarse = 0;
-  // MIXED single-component operators: FP32 copy of the near-field blocks and the FP32 node
-  // tables from which the single-RHS product evaluates the coupling blocks on the fly
-  bool fly = false;
+  // MIXED single-component operators (red1): FP32 copy of the near-field blocks for the reduced
+  // single-RHS product; for real kernels (fly) also the FP32 node tables from which that product
+  // may evaluate the coupling blocks on the fly
+  bool red1 = false, fly = false;
   DevBuf<float> d_dense32;
   DevBuf<float4> d_fly_pts;
   DevBuf<double4> d_fly_org;
@@ -171,11 +173,14 @@ struct CouplingOperand {
   const void* F;
   const unsigned short* Fc;
   int ncoarse;
+  int ld;       // column stride of a block
+  int64_t blk;  // entries per component block
 };
 static CouplingOperand coupling_operand(const bmi_h2* h, bool reduced) {
   if (reduced && h->op.precision != BMI_PRECISION_FP64)
-    return {true, h->d_coup32.ptr, h->ncoarse > 0 ? h->d_coup16.ptr : nullptr, h->ncoarse};
-  return {false, h->d_coup.ptr, nullptr, 0};
+    return {true, h->d_coup32.ptr, h->ncoarse > 0 ? h->d_coup16.ptr : nullptr, h->ncoarse, h->ld32,
+            static_cast<int64_t>(h->ld32) * h->cd};
+  return {false, h->d_coup.ptr, nullptr, 0, h->cd, static_cast<int64_t>(h->cd) * h->cd};
 }
 
 static void ensure_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
@@ -203,7 +208,7 @@ static void apply_far(bmi_h2* h, int nrhs, const double* xs, double* yf, bool re
   const int rows = h->cd;
   const int64_t vcs = static_cast<int64_t>(s->nel) * s->I2;  // super DOFs per component
   const CouplingOperand co = coupling_operand(h, reduced);
-  const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
+  const int64_t cblk = co.blk;
   for (int cc = 0; cc < h->vd; ++cc) {
     TimedScope ts("matvec_forward", st);
     launch_forward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[cc].ptr, h->d_T.ptr, xs, N, cc * vcs,
@@ -214,7 +219,7 @@ static void apply_far(bmi_h2* h, int nrhs, const double* xs, double* yf, bool re
     for (int cc = 0; cc < h->vd; ++cc)
       launch_coupling_apply(C, co.fp32, std::max(h->cpl_variant, 0), h->ntgt, nrhs, rows, h->d_tgt.ptr,
                             h->d_crp.ptr, h->d_csrc.ptr, co.F, h->vd * h->vd * cblk, (rc * h->vd + cc) * cblk,
-                            h->d_lvl_off.ptr, h->d_fw[cc].ptr, h->d_bw.ptr, 1, st, co.Fc, co.ncoarse);
+                            h->d_lvl_off.ptr, h->d_fw[cc].ptr, h->d_bw.ptr, 1, st, co.Fc, co.ncoarse, co.ld);
     TimedScope ts("matvec_backward", st);
     launch_backward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[rc].ptr, h->d_T.ptr, h->d_bw.ptr,
                     h->d_lvl_off.ptr, yf, N, rc * vcs, st);
@@ -264,14 +269,14 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
     const CouplingOperand co = coupling_operand(h, reduced);
     const bool fp32 = co.fp32;
     const void* coupF = co.F;
-    const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
+    const int64_t cblk = co.blk;
     device_fill(h->d_fw[0].ptr, S * h->lvl_total, 0.0, st);
     pick = 0;
     for (int v = 0; v < 2; ++v) {
       const float t = time_it([&] {
         launch_coupling_apply(h->cplx, fp32, v, h->ntgt, 1, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, coupF,
                               h->vd * h->vd * cblk, 0, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, 0, st, co.Fc,
-                              co.ncoarse);
+                              co.ncoarse, co.ld);
       });
       if (v == 0 || t < best) best = t, pick = v;
     }
@@ -286,19 +291,21 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
             ok = launch_sweep_fused(h->cplx, fp32, vl, vc, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr,
                                     h->d_dense.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
                                     h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
-                                    co.ncoarse);
+                                    co.ncoarse, false, co.ld);
           });
           if (ok && (pl < 0 || t < best)) best = t, pl = vl, pc = vc;
         }
       h->fused_leaf_variant = pl;
       h->fused_cpl_variant = pc;
     }
-    if (h->fly) {
-      // reduced single-RHS sweep: coupling recomputed on the fly (variants 0, 1) or read from the
-      // FP32 copy (variants 2 + 2 * near blocks per iteration + coupling blocks per iteration),
-      // both with the FP32 near-field blocks; whichever is faster (arithmetic vs bytes)
+    if (h->red1) {
+      // reduced single-RHS sweep: coupling recomputed on the fly (variants 0, 1; real kernels:
+      // an oscillatory kernel's FP32 phase is less accurate than the FP32-rounded FP64 entries and
+      // costs Krylov steps) or read from the FP32 copy (variants 2 + 2 * near blocks per
+      // iteration + coupling blocks per iteration), both with the FP32 near-field blocks;
+      // whichever is faster (arithmetic vs bytes)
       int pv = -1;
-      for (int v = 0; v < 6; ++v) {
+      for (int v = h->fly ? 0 : 2; v < 6; ++v) {
         bool ok = true;
         const float t = time_it([&] {
           if (v < 2)
@@ -310,7 +317,7 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
             ok = launch_sweep_fused(h->cplx, true, (v - 2) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd, h->d_drp.ptr,
                                     h->d_dsrc.ptr, h->d_dense32.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr,
                                     h->d_crp.ptr, h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, h->d_fw[0].ptr,
-                                    h->d_bw.ptr, st, nullptr, 0, true);
+                                    h->d_bw.ptr, st, nullptr, 0, true, h->ld32);
         });
         if (ok && (pv < 0 || t < best)) best = t, pv = v;
       }
@@ -328,7 +335,7 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
 static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, double* yf, bool reduced,
                         hipStream_t st) {
   const bool far = has_far(h);
-  if (far && reduced && h->fly && nrhs == 1) {
+  if (far && reduced && h->red1 && nrhs == 1) {
     // reduced operator, one RHS: FP32 near-field blocks and the coupling evaluated on the fly
     const bmi_space* s = h->s;
     const int64_t N = s->nsuper;
@@ -346,7 +353,7 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
               : launch_sweep_fused(h->cplx, true, (v - 2) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd, h->d_drp.ptr,
                                    h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
                                    h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st,
-                                   nullptr, 0, true);
+                                   nullptr, 0, true, h->ld32);
     if (ok) {
       TimedScope ts("matvec_backward", st);
       launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
@@ -367,7 +374,7 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
     if (launch_sweep_fused(h->cplx, co.fp32, h->fused_leaf_variant, h->fused_cpl_variant, h->ncl_leaf, h->dd,
                            h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr,
                            h->d_crp.ptr, h->d_csrc.ptr, co.F, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
-                           co.ncoarse)) {
+                           co.ncoarse, false, co.ld)) {
       TimedScope ts("matvec_backward", st);
       launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
                       h->d_lvl_off.ptr, yf, N, 0, st);
@@ -532,7 +539,7 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
     DevBuf<int4> d_lv;
     d_lv.upload(lv, st);
     launch_coupling_assemble(op.kind, d_lv.ptr, nl, h->np2, d_ptrs.ptr, op.kappa_re, op.kappa_im, h->d_coup.ptr, st);
-    if (op.precision == BMI_PRECISION_MIXED && vd == 1) {
+    if (op.precision == BMI_PRECISION_MIXED && vd == 1 && !h->cplx) {
       // node tables of every level at the cluster offsets of the multipole arrays
       h->d_fly_pts.alloc(static_cast<size_t>(h->lvl_total));
       h->d_fly_org.alloc(static_cast<size_t>(h->lvl_total / h->cd));
@@ -559,10 +566,17 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   h->d_tgt.upload(tgt, st);
   h->d_crp.upload(crp, st);
   h->d_csrc.upload(csrc, st);
-  if (op.precision != BMI_PRECISION_FP64) h->d_coup32.alloc(h->d_coup.n);
+  // FP32 coupling copy: MIXED pads the columns to whole 16-byte row vectors (the sweeps read rows
+  // as vectors); MIXED_BF16 keeps the FP64 layout, shared with its BF16 blocks
+  h->ld32 = op.precision == BMI_PRECISION_MIXED ? fp32_coupling_ld(h->cplx, h->cd) : h->cd;
+  if (op.precision != BMI_PRECISION_FP64)
+    h->d_coup32.alloc(static_cast<size_t>(S) * vd * vd * h->ld32 * h->cd * nl);
   BMI_CHECK_HIP(hipStreamSynchronize(st));
-  if (op.precision != BMI_PRECISION_FP64 && h->d_coup.n) convert_to_fp32(h->d_coup.ptr, h->d_coup32.ptr, h->d_coup.n, st);
-  if (h->fly) {
+  if (op.precision != BMI_PRECISION_FP64 && h->d_coup.n)
+    convert_blocks_to_fp32(h->d_coup.ptr, h->d_coup32.ptr, static_cast<int64_t>(nl) * vd * vd, h->cd, h->ld32,
+                           h->cplx, st);
+  h->red1 = op.precision == BMI_PRECISION_MIXED && vd == 1;
+  if (h->red1) {
     h->d_dense32.alloc(h->d_dense.n);
     if (h->d_dense.n) convert_to_fp32(h->d_dense.ptr, h->d_dense32.ptr, h->d_dense.n, st);
   }
@@ -969,9 +983,12 @@ bmi_status bmi_h2_estimate_bytes(const bmi_space* s, const bmi_op_desc* op, int 
   const int L = s->M - mcl;
   const int64_t lvl_total = int64_t(s->npatch) * (((int64_t(1) << (2 * (L + 1))) - 1) / 3) * cd;
   int64_t b = S * vd * vd * (int64_t(dense.size()) * dd * dd + int64_t(lowrank.size()) * cd * cd) * 8;
-  if (op->precision != BMI_PRECISION_FP64) b += S * vd * vd * int64_t(lowrank.size()) * cd * cd * 4;
-  if (op->precision == BMI_PRECISION_MIXED && vd == 1)
-    b += S * int64_t(dense.size()) * dd * dd * 4 + lvl_total * 16 + lvl_total / cd * 32;  // FP32 near field, nodes
+  const int64_t ld32 = op->precision == BMI_PRECISION_MIXED ? fp32_coupling_ld(S == 2, static_cast<int>(cd)) : cd;
+  if (op->precision != BMI_PRECISION_FP64) b += S * vd * vd * int64_t(lowrank.size()) * ld32 * cd * 4;
+  if (op->precision == BMI_PRECISION_MIXED && vd == 1) {
+    b += S * int64_t(dense.size()) * dd * dd * 4;                     // FP32 near field
+    if (S == 1) b += lvl_total * 16 + lvl_total / cd * 32;            // node tables
+  }
   b += (int64_t(dense.size()) + int64_t(lowrank.size())) * 8;                           // CSR indices
   b += std::max(nrhs, 1) * S * (3 * int64_t(s->nsuper) + (vd + 1) * lvl_total) * 8;   // workspace
   b += int64_t(s->nel) * 25 * 16 * 8 + int64_t(s->val.size()) * 24;                    // tables, T

@@ -24,11 +24,14 @@ void launch_moment(bool cplx, int ncl, int nrhs, int rows, int mc, const double*
 void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T, const double* child,
                 double* parent, hipStream_t st);
 // variant: 0 = one block per lane group per iteration, 1 = two (chosen by the autotuner)
-// Fc/ncoarse: BF16 copies of the first ncoarse blocks (fp32 only)
+// Fc/ncoarse: BF16 copies of the first ncoarse blocks (fp32 only); ld: column stride of F (0: cd)
 void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs, int cd, const int2* tgt,
                            const int* rp, const int* src, const void* F, int64_t fstride, int64_t fcomp,
                            const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st,
-                           const unsigned short* Fc = nullptr, int ncoarse = 0);
+                           const unsigned short* Fc = nullptr, int ncoarse = 0, int ld = 0);
+// column stride of the FP32 coupling copy: cd rounded up to whole 16-byte row vectors
+int fp32_coupling_ld(bool cplx, int cd);
+void convert_blocks_to_fp32(const double* src, float* dst, int64_t nb, int cd, int ld, bool cplx, hipStream_t st);
 void convert_to_bf16(const double* src, unsigned short* dst, int64_t n, hipStream_t st);
 // forward transformation of all levels (moments at level L, transfers up to level 0)
 void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
@@ -50,7 +53,7 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
                         const int* dsrc, const void* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
                         const double* fw, double* bw, hipStream_t st, const unsigned short* Fc = nullptr,
-                        int ncoarse = 0, bool near32 = false);
+                        int ncoarse = 0, bool near32 = false, int ld = 0);
 // FP32 node tables (offsets from each cluster's node 0, weights; node 0 in FP64) of one level
 void launch_fly_points(const double* pts, int ncl, int np2, float4* out, double4* org, hipStream_t st);
 // single-RHS sweep of the reduced operator: FP32 near-field blocks, coupling evaluated on the fly

@@ -497,7 +497,7 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
 // bw[t][k][i] += sum over blocks (t, s): F (cd x cd) * fw[s][k][:]  for all levels at once;
 // targets are (level, cluster) pairs listed in `tgt`, their blocks in CSR `rp`/`src`.
 template <bool C, typename FT>
-__global__ void k_coupling_apply(int ntgt, int nrhs, int cd, const int2* __restrict__ tgt,
+__global__ void k_coupling_apply(int ntgt, int nrhs, int cd, int ld, const int2* __restrict__ tgt,
                                  const int* __restrict__ rp, const int* __restrict__ src,
                                  const FT* __restrict__ F, int64_t fstride, int64_t fcomp,
                                  const int64_t* __restrict__ lvl_off, const double* __restrict__ fw,
@@ -516,10 +516,10 @@ __global__ void k_coupling_apply(int ntgt, int nrhs, int cd, const int2* __restr
     for (int j = 0; j < cd; ++j) {
       Val<C> a;
       if constexpr (C) {
-        a.r = static_cast<double>(f[2 * (static_cast<int64_t>(j) * cd + i)]);
-        a.i = static_cast<double>(f[2 * (static_cast<int64_t>(j) * cd + i) + 1]);
+        a.r = static_cast<double>(f[2 * (static_cast<int64_t>(j) * ld + i)]);
+        a.i = static_cast<double>(f[2 * (static_cast<int64_t>(j) * ld + i) + 1]);
       } else {
-        a.r = static_cast<double>(f[static_cast<int64_t>(j) * cd + i]);
+        a.r = static_cast<double>(f[static_cast<int64_t>(j) * ld + i]);
       }
       acc.fma(a, Val<C>::load(fw, xb + j));
     }
@@ -722,6 +722,7 @@ struct CplArgs {
   int accumulate;
   const unsigned short* Fc;  // BF16 copies of the first ncoarse blocks (coarsest levels), or null
   int ncoarse;
+  int ld;  // column stride of F blocks (cd, or cd padded to whole 16-byte row vectors for FP32)
 };
 
 __device__ __forceinline__ double bf16_to_double(unsigned short b) {
@@ -760,10 +761,11 @@ __device__ __forceinline__ void coupling_wave(int t, const CplArgs& a) {
           }
         } else {
           const FT* f = F + fo;
+          const int ld = std::is_same<FT, double>::value ? CD : a.ld;  // FP64 blocks are unpadded
 #pragma unroll
           for (int j = 0; j < CD; ++j) {
-            fr[u][j] = static_cast<double>(f[S * (j * CD + i)]);
-            if constexpr (C) fi[u][j] = static_cast<double>(f[S * (j * CD + i) + 1]);
+            fr[u][j] = static_cast<double>(f[S * (j * ld + i)]);
+            if constexpr (C) fi[u][j] = static_cast<double>(f[S * (j * ld + i) + 1]);
           }
         }
       }
@@ -821,18 +823,279 @@ __global__ __launch_bounds__(256) void k_coupling_w(CplArgs a) {
   coupling_wave<C, FT, CD, RT, UB>(t, a);
 }
 
+// Real FP64 blocks, one RHS: each lane loads two adjacent rows of a column (16 bytes, as the
+// complex kernels do) instead of one, so a load instruction moves up to 1 KiB.  Near field: the
+// DD/2 lanes of a group cover a column, the NG = 64/(DD/2) groups split the columns of the same
+// block (x of a column is read per lane, broadcast within the group); coupling: a group of CD/2
+// lanes owns a block, NG groups take every NG-th block.  Group sums are combined with shuffles.
+// vector of RPL consecutive rows of one column as one 16-byte load
+template <bool C, typename FT>
+struct RowVec {
+  static constexpr int RPL = 16 / ((C ? 2 : 1) * static_cast<int>(sizeof(FT)));  // rows per lane
+  FT r[RPL], i[C ? RPL : 1];  // kept in the storage type, widened where multiplied
+  __device__ __forceinline__ void load(const FT* p) {
+    if constexpr (std::is_same<FT, double>::value && !C) {
+      const double2 v = *reinterpret_cast<const double2*>(p);
+      r[0] = v.x, r[1] = v.y;
+    } else if constexpr (std::is_same<FT, double>::value) {
+      const double2 v = *reinterpret_cast<const double2*>(p);
+      r[0] = v.x, i[0] = v.y;
+    } else if constexpr (!C) {
+      const float4 v = *reinterpret_cast<const float4*>(p);
+      r[0] = v.x, r[1] = v.y, r[2] = v.z, r[3] = v.w;
+    } else {
+      const float4 v = *reinterpret_cast<const float4*>(p);
+      r[0] = v.x, i[0] = v.y, r[1] = v.z, i[1] = v.w;
+    }
+  }
+};
+
+template <bool C, typename FT, int DD, int UB>
+__device__ __forceinline__ void leaf_wave_v(int c, const LeafArgs& a) {
+  constexpr int S = C ? 2 : 1;
+  constexpr int RPL = RowVec<C, FT>::RPL;
+  constexpr int H = DD / RPL;
+  constexpr int NG = 64 / H;
+  constexpr int JC = (DD + NG - 1) / NG;
+  const int lane = threadIdx.x & 63;
+  const int g0 = lane / H;
+  const int g = g0 < NG ? g0 : NG - 1;  // spare lanes repeat the last group and store nothing
+  const int l = lane - g0 * H < H ? lane - g0 * H : 0;
+  const int p0 = a.rp[c], p1 = a.rp[c + 1];
+  const FT* __restrict__ F = static_cast<const FT*>(a.F);
+  const double* __restrict__ xs = a.xs;
+  double yr[RPL], yi[C ? RPL : 1];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    yr[q] = 0.0;
+    if constexpr (C) yi[q] = 0.0;
+  }
+  for (int p = p0; p < p1; p += UB) {
+    RowVec<C, FT> fv[UB][JC];
+    double xr[UB][JC], xi[UB][C ? JC : 1];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int pu = p + u < p1 ? p + u : p1 - 1;
+      const FT* f = F + S * (static_cast<int64_t>(pu) * a.fstride + a.fcomp);
+      const double* x = xs + S * (a.xseg + static_cast<int64_t>(a.src[pu]) * DD);
+#pragma unroll
+      for (int jj = 0; jj < JC; ++jj) {
+        const bool in = g * JC + jj < DD && p + u < p1;
+        const int j = g * JC + jj < DD ? g * JC + jj : DD - 1;
+        fv[u][jj].load(f + S * (j * DD + RPL * l));
+        xr[u][jj] = in ? x[S * j] : 0.0;
+        if constexpr (C) xi[u][jj] = in ? x[S * j + 1] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u)
+#pragma unroll
+      for (int jj = 0; jj < JC; ++jj)
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+          const double fr = static_cast<double>(fv[u][jj].r[q]);
+          if constexpr (C) {
+            const double fi = static_cast<double>(fv[u][jj].i[q]);
+            yr[q] += fr * xr[u][jj] - fi * xi[u][jj];
+            yi[q] += fr * xi[u][jj] + fi * xr[u][jj];
+          } else {
+            yr[q] += fr * xr[u][jj];
+          }
+        }
+  }
+  // group 0 collects the other groups' sums
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    double sr = yr[q], si = C ? yi[q] : 0.0;
+#pragma unroll
+    for (int k = 1; k < NG; ++k) {
+      const int src = lane + k * H < 64 ? lane + k * H : lane;
+      const double tr = __shfl(yr[q], src, 64);
+      const double ti = C ? __shfl(yi[q], src, 64) : 0.0;
+      if (lane + k * H < 64) sr += tr, si += ti;
+    }
+    yr[q] = sr;
+    if constexpr (C) yi[q] = si;
+  }
+  if (g0 == 0) {
+    double* y = a.ys + S * (a.yseg + static_cast<int64_t>(c) * DD + RPL * l);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      double vr = yr[q], vi = C ? yi[q] : 0.0;
+      if (a.accumulate) {
+        vr += y[S * q];
+        if constexpr (C) vi += y[S * q + 1];
+      }
+      y[S * q] = vr;
+      if constexpr (C) y[S * q + 1] = vi;
+    }
+  }
+}
+
+// FP32 coupling blocks with columns padded to whole 16-byte row vectors (LD rows): a group of
+// LD / RPL lanes owns a block, NG groups take every NG-th block; padding rows are zero.
+template <bool C, int CD, int LD, int UB>
+__device__ __forceinline__ void coupling_wave_v(int t, const CplArgs& a) {
+  constexpr int S = C ? 2 : 1;
+  constexpr int RPL = RowVec<C, float>::RPL;
+  constexpr int H = LD / RPL;
+  constexpr int NG = 64 / H;
+  const int lane = threadIdx.x & 63;
+  const int g0 = lane / H;
+  const int g = g0 < NG ? g0 : NG - 1;
+  const int l = lane - g0 * H < H ? lane - g0 * H : 0;
+  const int2 T = a.tgt[t];
+  const int64_t off = a.lvl_off[T.x];
+  const int p0 = a.rp[t], p1 = a.rp[t + 1];
+  const float* __restrict__ F = static_cast<const float*>(a.F);
+  double yr[RPL], yi[C ? RPL : 1];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    yr[q] = 0.0;
+    if constexpr (C) yi[q] = 0.0;
+  }
+  for (int p = p0 + g; p < p1; p += NG * UB) {
+    RowVec<C, float> fv[UB][CD];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int pu = p + NG * u < p1 ? p + NG * u : p;
+      const float* f = F + S * (static_cast<int64_t>(pu) * a.fstride + a.fcomp);
+#pragma unroll
+      for (int j = 0; j < CD; ++j) fv[u][j].load(f + S * (j * LD + RPL * l));
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const bool ok = p + NG * u < p1;
+      const int pu = ok ? p + NG * u : p;
+      const double* x = a.fw + S * (off + static_cast<int64_t>(a.src[pu]) * CD);
+      // FP32 entries, FP64 sums (FP64 multiply-adds issue at the FP32 rate here, and the reduced
+      // operator's accuracy sets the Krylov step count)
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        const double xr = ok ? x[S * j] : 0.0;
+        const double xi = (C && ok) ? x[S * j + 1] : 0.0;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+          const double fr = static_cast<double>(fv[u][j].r[q]);
+          if constexpr (C) {
+            const double fi = static_cast<double>(fv[u][j].i[q]);
+            yr[q] += fr * xr - fi * xi;
+            yi[q] += fr * xi + fi * xr;
+          } else {
+            yr[q] += fr * xr;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    double sr = yr[q], si = C ? yi[q] : 0.0;
+#pragma unroll
+    for (int k = 1; k < NG; ++k) {
+      const int src = lane + k * H < 64 ? lane + k * H : lane;
+      const double tr = __shfl(yr[q], src, 64);
+      const double ti = C ? __shfl(yi[q], src, 64) : 0.0;
+      if (lane + k * H < 64) sr += tr, si += ti;
+    }
+    yr[q] = sr;
+    if constexpr (C) yi[q] = si;
+  }
+  if (g0 == 0) {
+    double* y = a.bw + S * (off + static_cast<int64_t>(T.y) * CD);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = RPL * l + q;
+      if (row >= CD) continue;
+      y[S * row] = yr[q];
+      if constexpr (C) y[S * row + 1] = yi[q];
+    }
+  }
+}
+
+template <int CD, int UB>
+__device__ __forceinline__ void coupling_wave_r2(int t, const CplArgs& a) {
+  constexpr int H = CD / 2;
+  constexpr int NG = 64 / H;  // H is a power of two here
+  const int lane = threadIdx.x & 63;
+  const int g = lane / H, l = lane % H;
+  const int2 T = a.tgt[t];
+  const int64_t off = a.lvl_off[T.x];
+  const int p0 = a.rp[t], p1 = a.rp[t + 1];
+  const double* __restrict__ F = static_cast<const double*>(a.F);
+  double y0 = 0.0, y1 = 0.0;
+  for (int p = p0 + g; p < p1; p += NG * UB) {
+    double2 fv[UB][CD];
+    double xv[UB][CD];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const bool ok = p + NG * u < p1;
+      const int pu = ok ? p + NG * u : p;
+      const double* f = F + static_cast<int64_t>(pu) * a.fstride + a.fcomp;
+      const double* x = a.fw + off + static_cast<int64_t>(a.src[pu]) * CD;
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        fv[u][j] = *reinterpret_cast<const double2*>(f + j * CD + 2 * l);
+        xv[u][j] = ok ? x[j] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u)
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        y0 += fv[u][j].x * xv[u][j];
+        y1 += fv[u][j].y * xv[u][j];
+      }
+  }
+#pragma unroll
+  for (int m = H; m < 64; m <<= 1) {
+    y0 += __shfl_xor(y0, m, 64);
+    y1 += __shfl_xor(y1, m, 64);
+  }
+  if (g == 0) {
+    double* y = a.bw + off + static_cast<int64_t>(T.y) * CD + 2 * l;
+    y[0] = y0;
+    y[1] = y1;
+  }
+}
+
 // the two sweeps of a single-RHS product in one launch: workgroups [0, nb_near) run the near
 // field, the rest the coupling blocks, so both streams of leaves share the memory system
 template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT = double>
 __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, int nb_near) {
   const int b = static_cast<int>(blockIdx.x);
   const int w = static_cast<int>(threadIdx.x >> 6);
+  // real FP64 blocks: two rows per lane (16-byte loads)
+  // near-field rows of 16-byte vectors: real FP64 (2 rows per lane), FP32 (4 real, 2 complex)
+  constexpr bool R2N = (!C || std::is_same<NFT, float>::value) && DD % RowVec<C, NFT>::RPL == 0;
+  constexpr bool R2C = !C && std::is_same<FT, double>::value && (CD == 16 || CD == 32);
+  // FP32 coupling copies are stored with columns padded to LDV rows (see the FP32 conversion)
+  constexpr int RPLC = RowVec<C, float>::RPL;
+  constexpr int LDV = (CD + RPLC - 1) / RPLC * RPLC;
+  constexpr bool VC = std::is_same<FT, float>::value;
   if (b < nb_near) {
     const int c = __builtin_amdgcn_readfirstlane(b * 4 + w);
-    if (c < la.ncl) leaf_wave<C, DD, 1, UBN, NFT>(c, la);
+    if (c < la.ncl) {
+      if constexpr (R2N)
+        leaf_wave_v<C, NFT, DD, UBN>(c, la);
+      else
+        leaf_wave<C, DD, 1, UBN, NFT>(c, la);
+    }
   } else {
     const int t = __builtin_amdgcn_readfirstlane((b - nb_near) * 4 + w);
-    if (t < ca.ntgt) coupling_wave<C, FT, CD, 1, UBC>(t, ca);
+    if (t < ca.ntgt) {
+      if constexpr (R2C)
+        coupling_wave_r2<CD, UBC>(t, ca);
+      else if constexpr (VC) {
+        if (ca.ld == LDV && ca.Fc == nullptr) {
+          coupling_wave_v<C, CD, LDV, UBC>(t, ca);
+        } else {
+          coupling_wave<C, FT, CD, 1, UBC>(t, ca);
+        }
+      } else {
+        coupling_wave<C, FT, CD, 1, UBC>(t, ca);
+      }
+    }
   }
 }
 
@@ -851,6 +1114,7 @@ struct MultiArgs {
   const int* src;
   const void* F;
   int64_t fstride, fcomp;
+  int ldf;  // column stride of the blocks
   const double* x;
   int64_t xoff, xcs, xks;
   double* y;
@@ -890,40 +1154,29 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
     const FT* f = F + S * (static_cast<int64_t>(p) * a.fstride + a.fcomp + i0);
     const double* x = a.x + S * (xo + static_cast<int64_t>(a.src[p]) * a.xcs + static_cast<int64_t>(kk) * a.xks);
     if constexpr (F32) {
-      // FP32 blocks: FP32 products summed per block, the block sums added in FP64
-      float br[RPW], bi[C ? RPW : 1];
-#pragma unroll
-      for (int i = 0; i < RPW; ++i) {
-        br[i] = 0.f;
-        if constexpr (C) bi[i] = 0.f;
-      }
+      // FP32 blocks, FP64 sums
       for (int j = 0; j < ROWS; ++j) {
-        const float xr = static_cast<float>(x[S * j]);
-        const float xi = C ? static_cast<float>(x[S * j + 1]) : 0.f;
-        const float* fj = reinterpret_cast<const float*>(f) + S * j * ROWS;
+        const double xr = x[S * j];
+        const double xi = C ? x[S * j + 1] : 0.0;
+        const float* fj = reinterpret_cast<const float*>(f) + S * j * a.ldf;
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
-          const int ii = RPW == ROWS ? i : (i0 + i < ROWS ? i : ROWS - 1 - i0);  // rows past the end repeat the last
-          const float fr = fj[S * ii];
+          const int ii = RPW == ROWS ? i : (i0 + i < ROWS ? i : ROWS - 1 - i0);
+          const double fr = static_cast<double>(fj[S * ii]);
           if constexpr (C) {
-            const float fi = fj[S * ii + 1];
-            br[i] += fr * xr - fi * xi;
-            bi[i] += fr * xi + fi * xr;
+            const double fi = static_cast<double>(fj[S * ii + 1]);
+            ar[i] += fr * xr - fi * xi;
+            ai[i] += fr * xi + fi * xr;
           } else {
-            br[i] += fr * xr;
+            ar[i] += fr * xr;
           }
         }
-      }
-#pragma unroll
-      for (int i = 0; i < RPW; ++i) {
-        ar[i] += static_cast<double>(br[i]);
-        if constexpr (C) ai[i] += static_cast<double>(bi[i]);
       }
     } else {
       for (int j = 0; j < ROWS; ++j) {
         const double xr = x[S * j];
         const double xi = C ? x[S * j + 1] : 0.0;
-        const double* fj = reinterpret_cast<const double*>(f) + S * j * ROWS;
+        const double* fj = reinterpret_cast<const double*>(f) + S * j * a.ldf;
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
           const int ii = RPW == ROWS ? i : (i0 + i < ROWS ? i : ROWS - 1 - i0);
@@ -1049,7 +1302,12 @@ __global__ __launch_bounds__(256) void k_sweep_fly(LeafArgs la, FlyArgs fa, int 
     if (t < fa.ntgt) coupling_fly_wave<C, CD>(t, fa);
   } else {
     const int c = __builtin_amdgcn_readfirstlane(static_cast<int>(b - cb) * 4 + w);
-    if (c < la.ncl) leaf_wave<C, DD, 1, UBN, float>(c, la);
+    if (c < la.ncl) {
+      if constexpr (DD % RowVec<C, float>::RPL == 0)
+        leaf_wave_v<C, float, DD, UBN>(c, la);
+      else
+        leaf_wave<C, DD, 1, UBN, float>(c, la);
+    }
   }
 }
 
@@ -1065,6 +1323,21 @@ __global__ void k_fly_points(const double* __restrict__ pts, int ncl, int np2, f
   out[gid] = make_float4(static_cast<float>(p[0] - o[0]), static_cast<float>(p[1] - o[1]),
                          static_cast<float>(p[2] - o[2]), static_cast<float>(p[3]));
   if (gid % np2 == 0) org[c] = make_double4(o[0], o[1], o[2], 0.0);
+}
+
+// FP32 copy of nb coupling blocks (cd x cd, column-major, S doubles per entry) with the columns
+// padded to ld rows of zeros, so that rows can be read as 16-byte vectors
+__global__ void k_to_fp32_blocks(const double* __restrict__ src, float* __restrict__ dst, int64_t nb, int cd, int ld,
+                                 int S) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = static_cast<int64_t>(ld) * cd * S;
+  if (i >= nb * per) return;
+  const int64_t b = i / per;
+  const int64_t r = i - b * per;
+  const int s = static_cast<int>(r % S);
+  const int64_t e = r / S;
+  const int row = static_cast<int>(e % ld), col = static_cast<int>(e / ld);
+  dst[i] = row < cd ? static_cast<float>(src[((b * cd + col) * cd + row) * S + s]) : 0.f;
 }
 
 // BF16 copy (round to nearest even of the FP32 value) of n doubles
@@ -1132,12 +1405,13 @@ void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T
 void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs, int cd, const int2* tgt,
                            const int* rp, const int* src, const void* F, int64_t fstride, int64_t fcomp,
                            const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st,
-                           const unsigned short* Fc, int ncoarse) {
+                           const unsigned short* Fc, int ncoarse, int ld) {
+  if (ld <= 0) ld = cd;
   const int64_t n = static_cast<int64_t>(ntgt) * nrhs * cd;
   if (n == 0) return;
   TimedScope ts("coupling_apply", st);
   if (nrhs >= MULTI_MIN_RHS && (!Fc || ncoarse == 0)) {
-    const MultiArgs ma{ntgt, nrhs, tgt, lvl_off, rp, src, F, fstride, fcomp, fw, 0,
+    const MultiArgs ma{ntgt, nrhs, tgt, lvl_off, rp, src, F, fstride, fcomp, ld, fw, 0,
                        static_cast<int64_t>(nrhs) * cd, cd, bw, 0, static_cast<int64_t>(nrhs) * cd, cd, accumulate};
 #define BMI_CPL_M(CC, FTT, CDD)                                                                       \
   if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                           \
@@ -1164,7 +1438,7 @@ void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs
   // wave-per-target kernels for the block sizes of the configurations (np 4, 5; Maxwell np 4)
 #define BMI_CPL_W(CC, FTT, CDD)                                                                              \
   if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                                  \
-    const CplArgs ca{ntgt, nrhs, tgt, rp, src, F, fstride, fcomp, lvl_off, fw, bw, accumulate, Fc, ncoarse};   \
+    const CplArgs ca{ntgt, nrhs, tgt, rp, src, F, fstride, fcomp, lvl_off, fw, bw, accumulate, Fc, ncoarse, ld}; \
     if (nrhs == 1 && variant == 1)                                                                            \
       hipLaunchKernelGGL((k_coupling_w<CC, FTT, CDD, 1, 2>), wgrid, dim3(256), 0, st, ca);                    \
     else if (nrhs == 1)                                                                                       \
@@ -1186,16 +1460,16 @@ void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs
   BMI_CPL_W(true, float, 32)
 #undef BMI_CPL_W
   if (cplx && !fp32)
-    hipLaunchKernelGGL((k_coupling_apply<true, double>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt, rp,
+    hipLaunchKernelGGL((k_coupling_apply<true, double>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, ld, tgt, rp,
                        src, static_cast<const double*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
   else if (!cplx && !fp32)
-    hipLaunchKernelGGL((k_coupling_apply<false, double>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt,
+    hipLaunchKernelGGL((k_coupling_apply<false, double>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, ld, tgt,
                        rp, src, static_cast<const double*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
   else if (cplx)
-    hipLaunchKernelGGL((k_coupling_apply<true, float>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt, rp,
+    hipLaunchKernelGGL((k_coupling_apply<true, float>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, ld, tgt, rp,
                        src, static_cast<const float*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
   else
-    hipLaunchKernelGGL((k_coupling_apply<false, float>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, tgt, rp,
+    hipLaunchKernelGGL((k_coupling_apply<false, float>), dim3(nblk(n)), dim3(256), 0, st, ntgt, nrhs, cd, ld, tgt, rp,
                        src, static_cast<const float*>(F), fstride, fcomp, lvl_off, fw, bw, accumulate);
   BMI_CHECK_HIP(hipGetLastError());
 }
@@ -1298,8 +1572,8 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
   const int64_t n = static_cast<int64_t>(ncl) * nrhs * dd;
   TimedScope ts("matvec_nearfield", st);
   if (nrhs >= MULTI_MIN_RHS) {
-    const MultiArgs ma{ncl, nrhs, nullptr, nullptr, rp, src, F, fstride, fcomp, xs, xseg, dd, ldxs, ys, yseg, dd, ldys,
-                       accumulate};
+    const MultiArgs ma{ncl, nrhs, nullptr, nullptr, rp, src, F, fstride, fcomp, dd, xs, xseg, dd, ldxs, ys, yseg, dd,
+                       ldys, accumulate};
 #define BMI_LEAF_M(CC, DDD)                                                                           \
   if (cplx == CC && dd == DDD) {                                                                      \
     constexpr int RPW = DDD;                                                                          \
@@ -1347,9 +1621,10 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
                         const int* dsrc, const void* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
                         const double* fw, double* bw, hipStream_t st, const unsigned short* Fc, int ncoarse,
-                        bool near32) {
+                        bool near32, int ld) {
   const LeafArgs la{ncl, 1, drp, dsrc, dF, static_cast<int64_t>(dd) * dd, 0, xs, 0, 0, ys, 0, 0, 0};
-  const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(cd) * cd, 0, lvl_off, fw, bw, 0, Fc, ncoarse};
+  if (ld <= 0) ld = cd;
+  const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(ld) * cd, 0, lvl_off, fw, bw, 0, Fc, ncoarse, ld};
   const int nbn = (ncl + 3) / 4, nbc = (ntgt + 3) / 4;
   const dim3 grid(static_cast<unsigned>(nbn + nbc));
   TimedScope ts("matvec_sweep", st);
@@ -1411,12 +1686,21 @@ bool launch_sweep_fly(bool cplx, int variant, int ncl, int dd, const int* drp, c
   }
   BMI_FLY(false, 36, 16)
   BMI_FLY(false, 36, 25)
-  BMI_FLY(true, 36, 25)
-  BMI_FLY(true, 36, 16)
   BMI_FLY(false, 16, 16)
-  BMI_FLY(true, 16, 16)
 #undef BMI_FLY
   return false;
+}
+
+int fp32_coupling_ld(bool cplx, int cd) {
+  const int rpl = cplx ? 2 : 4;
+  return (cd + rpl - 1) / rpl * rpl;
+}
+
+void convert_blocks_to_fp32(const double* src, float* dst, int64_t nb, int cd, int ld, bool cplx, hipStream_t st) {
+  const int64_t n = nb * ld * cd * (cplx ? 2 : 1);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_fp32_blocks, dim3(nblk(n)), dim3(256), 0, st, src, dst, nb, cd, ld, cplx ? 2 : 1);
+  BMI_CHECK_HIP(hipGetLastError());
 }
 
 void convert_to_bf16(const double* src, unsigned short* dst, int64_t n, hipStream_t st) {

@@ -47,8 +47,9 @@ typedef enum {
   BMI_PRECISION_FP64 = 0,       /* everything FP64 */
   BMI_PRECISION_MIXED = 1,      /* reduced operator for products and inner solves: coupling blocks in FP32
                                    (one RHS: an FP32 copy of the near-field blocks with the coupling
-                                   blocks recomputed in FP32 from the Chebyshev surface points or read
-                                   from their FP32 copy, whichever the first product measures faster;
+                                   blocks read from their FP32 copy or, for the real Laplace kernel,
+                                   recomputed in FP32 from the Chebyshev surface points, whichever the
+                                   first product measures faster;
                                    several RHS: FP32 coupling blocks with the FP64 near field), sums in
                                    FP64; the FP64 operator is kept and solvers refine with it */
   BMI_PRECISION_MIXED_BF16 = 2  /* as MIXED, and the coupling blocks of the two coarsest levels in BF16 */
