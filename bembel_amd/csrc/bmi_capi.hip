@@ -126,6 +126,7 @@ struct bmi_h2 {
   int64_t lvl_total = 0;
   // FMM constants
   DevBuf<double> d_T, d_mom[2];
+  DevBuf<double> d_Tt, d_momt[2];  // transposed copies for the RHS-fastest backward steps
   // workspace (grown on demand)
   int ws_rhs = 0;
   DevBuf<double> d_xs, d_ys, d_fw[2], d_bw, d_xin, d_yout;
@@ -158,7 +159,7 @@ struct bmi_h2 {
   size_t bytes() const {
     return d_dense.bytes() + d_coup.bytes() + d_coup32.bytes() + d_coup16.bytes() + d_dense32.bytes() +
            d_fly_pts.bytes() + d_fly_org.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() +
-           d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_xs.bytes() +
+           d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_Tt.bytes() + d_momt[0].bytes() + d_momt[1].bytes() + d_xs.bytes() +
            d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes() + d_yf.bytes();
   }
 };
@@ -394,23 +395,83 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
   if (far) BMI_CHECK_HIP(hipStreamWaitEvent(st, h->ev_join, 0));
 }
 
+// many right-hand sides, internal vectors RHS-fastest (xs, ys, yf: [super row][rhs]; multipole
+// vectors: [level offset + cluster * rows + row][rhs]): each wave's lanes are 64 consecutive
+// right-hand sides, so every vector access of the block sweeps is one contiguous row segment
+static bool use_rhs_fastest(const bmi_h2* h, int nrhs, bool reduced) {
+  if (nrhs < MULTI_MIN_RHS || !multi_t_supported(h->cplx, h->dd, h->cd)) return false;
+  return !(reduced && h->op.precision != BMI_PRECISION_FP64 && h->ncoarse > 0);  // no BF16 levels here
+}
+
+static void apply_super_t(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
+  const bmi_space* s = h->s;
+  const bool C = h->cplx;
+  const int S = C ? 2 : 1;
+  const int vd = h->vd;
+  const int64_t vcs = static_cast<int64_t>(s->nel) * s->I2;
+  const bool far = has_far(h);
+  if (far) {
+    if (!h->side_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking));
+    if (!h->ev_fork) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    if (!h->ev_join) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    BMI_CHECK_HIP(hipEventRecord(h->ev_fork, st));
+    BMI_CHECK_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
+    hipStream_t fs = h->side_stream;
+    const CouplingOperand co = coupling_operand(h, reduced);
+    for (int cc = 0; cc < vd; ++cc) {
+      TimedScope ts("matvec_forward", fs);
+      launch_forward_t(C, s->npatch, h->L, h->cd, h->np2, h->dd, nrhs, h->d_mom[cc].ptr, h->d_T.ptr, h->d_xs.ptr,
+                       cc * vcs, h->d_fw[cc].ptr, h->lvl_off.data(), fs);
+    }
+    for (int rc = 0; rc < vd; ++rc) {
+      device_fill(h->d_bw.ptr, S * h->lvl_total * nrhs, 0.0, fs);
+      for (int cc = 0; cc < vd; ++cc)
+        BMI_REQUIRE(launch_multi_cpl_t(C, co.fp32, h->ntgt, nrhs, h->cd, co.ld, h->d_tgt.ptr, h->d_crp.ptr,
+                                       h->d_csrc.ptr, co.F, vd * vd * co.blk, (rc * vd + cc) * co.blk,
+                                       h->d_lvl_off.ptr, h->d_fw[cc].ptr, h->d_bw.ptr, cc > 0 ? 1 : 0, fs),
+                    BMI_UNSUPPORTED, "coupling block size without an RHS-fastest sweep");
+      TimedScope ts("matvec_backward", fs);
+      launch_backward_t(C, s->npatch, h->L, h->cd, h->np2, h->dd, nrhs, h->d_momt[rc].ptr, h->d_Tt.ptr, h->d_bw.ptr,
+                        h->lvl_off.data(), h->d_yf.ptr, rc * vcs, fs);
+    }
+    BMI_CHECK_HIP(hipEventRecord(h->ev_join, fs));
+  }
+  // FP32 near-field blocks with the reduced operator of a mixed handle
+  const bool n32 = reduced && h->red1;
+  const int64_t dblk = static_cast<int64_t>(h->dd) * h->dd;
+  const void* F = n32 ? static_cast<const void*>(h->d_dense32.ptr) : static_cast<const void*>(h->d_dense.ptr);
+  for (int rc = 0; rc < vd; ++rc)
+    for (int cc = 0; cc < vd; ++cc)
+      BMI_REQUIRE(launch_multi_near_t(C, n32, h->ncl_leaf, nrhs, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, F,
+                                      vd * vd * dblk, (rc * vd + cc) * dblk, h->d_xs.ptr, cc * vcs, h->d_ys.ptr,
+                                      rc * vcs, cc > 0 ? 1 : 0, st),
+                  BMI_UNSUPPORTED, "near-field block size without an RHS-fastest sweep");
+  if (far) BMI_CHECK_HIP(hipStreamWaitEvent(st, h->ev_join, 0));
+}
+
 // ys = A_super xs on the workspace, through a captured graph when possible.  The graph holds
 // only kernel nodes (buffers are cleared by a fill kernel, not a memset node); in-library
 // timing needs host synchronisation per kernel and therefore runs eagerly.
-static void apply_super_workspace(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
-  autotune(h, h->d_xs.ptr, reduced, st);
+static void apply_super_workspace(bmi_h2* h, int nrhs, bool reduced, bool tl, hipStream_t st) {
+  auto run = [&](hipStream_t rs) {
+    if (tl)
+      apply_super_t(h, nrhs, reduced, rs);
+    else
+      apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, reduced, rs);
+  };
+  if (!tl) autotune(h, h->d_xs.ptr, reduced, st);
   if (!h->use_graphs || timing_enabled()) {
-    apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, reduced, st);
+    run(st);
     return;
   }
-  const int key = 2 * nrhs + (reduced ? 1 : 0);
+  const int key = 4 * nrhs + (tl ? 2 : 0) + (reduced ? 1 : 0);
   auto it = h->graphs.find(key);
   if (it == h->graphs.end()) {
     if (!h->cap_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
     hipGraph_t g = nullptr;
     BMI_CHECK_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
     try {
-      apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, reduced, h->cap_stream);
+      run(h->cap_stream);
     } catch (...) {
       (void)hipStreamEndCapture(h->cap_stream, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -431,9 +492,16 @@ void h2_apply_device(bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* 
   const bmi_space* s = h->s;
   ensure_workspace(h, nrhs, st);
   const int64_t N = s->nsuper;
+  if (use_rhs_fastest(h, nrhs, reduced)) {
+    launch_spmv_tin(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, h->d_xs.ptr, st);
+    apply_super_workspace(h, nrhs, reduced, true, st);
+    launch_spmv_tout(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, h->d_ys.ptr,
+                     has_far(h) ? h->d_yf.ptr : nullptr, Y, ldy, alpha, beta, st);
+    return;
+  }
   launch_spmv(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, h->d_xs.ptr, N, nullptr,
               nullptr, st);
-  apply_super_workspace(h, nrhs, reduced, st);
+  apply_super_workspace(h, nrhs, reduced, false, st);
   launch_spmv(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, h->d_ys.ptr, N, Y, ldy, alpha,
               beta, st, has_far(h) ? h->d_yf.ptr : nullptr, true);
 }
@@ -505,11 +573,25 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   std::vector<double> T, mom;
   transfer_matrix(h->np, T);
   h->d_T.upload(T, st);
+  {
+    // T_q(i, j) = T[(q np2 + j) np2 + i]; Tt holds T_q^T in the same layout
+    const int n2 = h->np2;
+    std::vector<double> Tt(T.size());
+    for (int q = 0; q < 4; ++q)
+      for (int j = 0; j < n2; ++j)
+        for (int i = 0; i < n2; ++i) Tt[(q * n2 + i) * n2 + j] = T[(q * n2 + j) * n2 + i];
+    h->d_Tt.upload(Tt, st);
+  }
   for (int c = 0; c < vd; ++c) {
     int rows, cols;
     moment_matrix(s->P, h->L, h->mcl, h->np, vd == 2 ? c + 1 : 0, mom, &rows, &cols);
     BMI_REQUIRE(rows == h->cd && cols == h->dd, BMI_DEVICE_ERROR, "moment matrix shape");
     h->d_mom[c].upload(mom, st);
+    // Mom is rows x cols column-major; momt = Mom^T, cols x rows column-major
+    std::vector<double> momt(mom.size());
+    for (int j = 0; j < cols; ++j)
+      for (int i = 0; i < rows; ++i) momt[static_cast<size_t>(i) * cols + j] = mom[static_cast<size_t>(j) * rows + i];
+    h->d_momt[c].upload(momt, st);
   }
   // level offsets of the multipole arrays
   h->lvl_off.assign(h->L + 2, 0);

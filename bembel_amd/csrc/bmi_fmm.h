@@ -8,6 +8,9 @@ struct bmi_space;
 
 namespace bmi {
 
+// from this many right-hand sides on, block rows run one wave per owner and 64 columns
+constexpr int MULTI_MIN_RHS = 8;
+
 constexpr int CP_STRIDE = 12;  // doubles per cluster node: x(3) kappa ds(3) dt(3) pad(2)
 
 void launch_cluster_points(const bmi_space* s, int level, int np, const double* d_cheb, double* out,
@@ -29,6 +32,29 @@ void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs
                            const int* rp, const int* src, const void* F, int64_t fstride, int64_t fcomp,
                            const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st,
                            const unsigned short* Fc = nullptr, int ncoarse = 0, int ld = 0);
+// many right-hand sides: internal vectors with the RHS index fastest (see bmi_fmm.hip)
+void launch_spmv_tin(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* X,
+                     int64_t ldx, double* Yt, hipStream_t st);
+void launch_spmv_tout(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* At,
+                      const double* Bt, double* Y, int64_t ldy, const double* alpha, const double* beta,
+                      hipStream_t st);
+// lvl_off: host array of the level offsets
+void launch_forward_t(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
+                      const double* T, const double* xs_t, int64_t xseg, double* fw_t, const int64_t* lvl_off,
+                      hipStream_t st);
+// momt: Mom^T (dd x rows, column-major); Tt: the four T_q transposed
+void launch_backward_t(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* momt,
+                       const double* Tt, double* bw_t, const int64_t* lvl_off, double* yf_t, int64_t yseg,
+                       hipStream_t st);
+// block sizes the RHS-fastest sweeps are instantiated for
+bool multi_t_supported(bool cplx, int dd, int cd);
+bool launch_multi_near_t(bool cplx, bool fp32, int ncl, int nrhs, int dd, const int* rp, const int* src,
+                         const void* F, int64_t fstride, int64_t fcomp, const double* xs_t, int64_t xseg,
+                         double* ys_t, int64_t yseg, int accumulate, hipStream_t st);
+// lvl_off: device array
+bool launch_multi_cpl_t(bool cplx, bool fp32, int ntgt, int nrhs, int cd, int ld, const int2* tgt, const int* rp,
+                        const int* src, const void* F, int64_t fstride, int64_t fcomp, const int64_t* lvl_off,
+                        const double* fw_t, double* bw_t, int accumulate, hipStream_t st);
 // column stride of the FP32 coupling copy: cd rounded up to whole 16-byte row vectors
 int fp32_coupling_ld(bool cplx, int cd);
 void convert_blocks_to_fp32(const double* src, float* dst, int64_t nb, int cd, int ld, bool cplx, hipStream_t st);

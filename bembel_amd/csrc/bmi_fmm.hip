@@ -1116,10 +1116,15 @@ struct MultiArgs {
   int64_t fstride, fcomp;
   int ldf;  // column stride of the blocks
   const double* x;
-  int64_t xoff, xcs, xks;
+  int64_t xoff, xcs, xks, xrs;  // offset, cluster, column (RHS) and row strides of x
   double* y;
-  int64_t yoff, ycs, yks;
+  int64_t yoff, ycs, yks, yrs;
   int accumulate;
+  // block list of owner r: 0 = CSR (rp, src; block p); implicit lists of the tree steps:
+  // 1 = blocks 4r..4r+3 on sources 4r..4r+3, block index p % 4 (children to parent);
+  // 2 = one block on source r / 4, block index r % 4 (parent to child);
+  // 3 = one block on source r, block index 0 (leaf moments and leaf rows)
+  int mode;
 };
 
 // One wave covers RPW of the ROWS rows (blockIdx.z picks which): fewer accumulators per lane and
@@ -1156,8 +1161,8 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
     if constexpr (F32) {
       // FP32 blocks, FP64 sums
       for (int j = 0; j < ROWS; ++j) {
-        const double xr = x[S * j];
-        const double xi = C ? x[S * j + 1] : 0.0;
+        const double xr = x[S * j * a.xrs];
+        const double xi = C ? x[S * j * a.xrs + 1] : 0.0;
         const float* fj = reinterpret_cast<const float*>(f) + S * j * a.ldf;
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
@@ -1174,8 +1179,8 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
       }
     } else {
       for (int j = 0; j < ROWS; ++j) {
-        const double xr = x[S * j];
-        const double xi = C ? x[S * j + 1] : 0.0;
+        const double xr = x[S * j * a.xrs];
+        const double xi = C ? x[S * j * a.xrs + 1] : 0.0;
         const double* fj = reinterpret_cast<const double*>(f) + S * j * a.ldf;
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
@@ -1193,17 +1198,215 @@ __global__ __launch_bounds__(64) void k_multi_rows(MultiArgs a) {
     }
   }
   if (!valid) return;
-  double* y = a.y + S * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k) * a.yks + i0);
+  double* y = a.y + S * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k) * a.yks + i0 * a.yrs);
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
     if (RPW != ROWS && i0 + i >= ROWS) continue;
     double vr = ar[i], vi = C ? ai[i] : 0.0;
+    double* yi_ = y + S * i * a.yrs;
     if (a.accumulate) {
-      vr += y[S * i];
-      if constexpr (C) vi += y[S * i + 1];
+      vr += yi_[0];
+      if constexpr (C) vi += yi_[1];
     }
-    y[S * i] = vr;
-    if constexpr (C) y[S * i + 1] = vi;
+    yi_[0] = vr;
+    if constexpr (C) yi_[1] = vi;
+  }
+}
+
+// ------------------------------------------------------------------ block rows staged in LDS (many RHS)
+// One workgroup (4 waves) per owner and 64 right-hand sides: y_owner (ROUT x 64) = sum over the
+// owner's blocks of F_p (ROUT x RIN) x_src(p) (RIN x 64). Each block and the x tile it multiplies
+// are copied to LDS with coalesced 8-byte loads by all 256 threads; the next block's copy is in
+// flight (registers) while the current one is multiplied. Wave w owns output rows
+// [w*RPW, w*RPW + RPW); its lanes are the 64 right-hand sides, reading the x tile row by row
+// (consecutive lanes, consecutive LDS words) and the block entries as LDS broadcasts. Sums in
+// FP64 whatever the block type. XC: complex vectors; FC: complex blocks (real blocks with
+// complex vectors are the tree steps of an oscillatory kernel).
+template <bool XC, bool FC, typename FT, int ROUT, int RIN>
+__global__ __launch_bounds__(256) void k_multi_lds(MultiArgs a) {
+  static_assert(XC || !FC, "complex blocks need complex vectors");
+  constexpr int SX = XC ? 2 : 1, SF = FC ? 2 : 1;
+  constexpr int RPW = (ROUT + 3) / 4;
+  constexpr int LDMAX = (ROUT + 3) / 4 * 4;
+  constexpr int FU_MAX = SF * LDMAX * RIN;  // 8-byte units of the largest (FP64) block
+  constexpr int XU = RIN * 64 * SX;         // 8-byte units of an x tile
+  constexpr int NFU = (FU_MAX + 255) / 256, NXU = (XU + 255) / 256;
+  __shared__ uint2 Fs[FU_MAX];
+  __shared__ double Xs[XU];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = blockIdx.x;
+  const int k0 = static_cast<int>(blockIdx.y) * 64;
+  const int kmax = a.nrhs - 1 - k0;  // last valid lane of this tile
+  int cl = r;
+  int64_t xo = a.xoff, yo = a.yoff;
+  if (a.tgt) {
+    const int2 T = a.tgt[r];
+    cl = T.y;
+    const int64_t lo = a.lvl_off[T.x] * a.nrhs;
+    xo += lo;
+    yo += lo;
+  }
+  const int mode = a.mode;
+  int p0, p1;
+  if (mode == 0) {
+    p0 = a.rp[r];
+    p1 = a.rp[r + 1];
+  } else if (mode == 1) {
+    p0 = 4 * r;
+    p1 = p0 + 4;
+  } else {
+    p0 = r;
+    p1 = r + 1;
+  }
+  const int fu = static_cast<int>((static_cast<int64_t>(SF) * a.ldf * RIN * sizeof(FT)) / 8);
+  const uint2* __restrict__ F8 = static_cast<const uint2*>(a.F);
+  const int64_t fblk = static_cast<int64_t>(SF) * a.fstride * sizeof(FT) / 8;  // 8-byte units per block index
+  const int64_t fc8 = static_cast<int64_t>(SF) * a.fcomp * sizeof(FT) / 8;
+  uint2 pf[NFU];
+  double px[NXU];
+  auto fetch = [&](int p) {
+    const int bi = mode == 0 ? p : (mode == 3 ? 0 : (p & 3));
+    const int sc = mode == 0 ? a.src[p] : (mode == 2 ? (p >> 2) : p);
+    const uint2* f = F8 + static_cast<int64_t>(bi) * fblk + fc8;
+#pragma unroll
+    for (int q = 0; q < NFU; ++q) {
+      const int u = tid + 256 * q;
+      if (u < fu) pf[q] = f[u];
+    }
+    const double* x = a.x + SX * (xo + static_cast<int64_t>(sc) * a.xcs);
+#pragma unroll
+    for (int q = 0; q < NXU; ++q) {
+      const int u = tid + 256 * q;
+      if (XU % 256 == 0 || u < XU) {
+        const int j = u / (64 * SX), rem = u - j * (64 * SX);
+        int kl = rem / SX;
+        kl = kl <= kmax ? kl : kmax;
+        px[q] = x[SX * (static_cast<int64_t>(j) * a.xrs + static_cast<int64_t>(k0 + kl) * a.xks) + (rem % SX)];
+      }
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int q = 0; q < NFU; ++q) {
+      const int u = tid + 256 * q;
+      if (u < fu) Fs[u] = pf[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NXU; ++q) {
+      const int u = tid + 256 * q;
+      if (XU % 256 == 0 || u < XU) Xs[u] = px[q];
+    }
+  };
+  double ar[RPW], ai[XC ? RPW : 1];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    ar[i] = 0.0;
+    if constexpr (XC) ai[i] = 0.0;
+  }
+  const int i0 = w * RPW;
+  const FT* Fl = reinterpret_cast<const FT*>(Fs);
+  if (p0 < p1) fetch(p0);
+  for (int p = p0; p < p1; ++p) {
+    __syncthreads();  // the previous block is no longer read
+    commit();
+    __syncthreads();
+    if (p + 1 < p1) fetch(p + 1);
+#pragma unroll 2
+    for (int j = 0; j < RIN; ++j) {
+      const double xr = Xs[SX * (j * 64 + lane)];
+      const double xi = XC ? Xs[SX * (j * 64 + lane) + 1] : 0.0;
+      const FT* fj = Fl + SF * (j * a.ldf);
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) {
+        const int ii = (ROUT % 4 == 0 || i0 + i < ROUT) ? i0 + i : ROUT - 1;
+        const double fr = static_cast<double>(fj[SF * ii]);
+        if constexpr (FC) {
+          const double fi = static_cast<double>(fj[SF * ii + 1]);
+          ar[i] += fr * xr - fi * xi;
+          ai[i] += fr * xi + fi * xr;
+        } else if constexpr (XC) {
+          ar[i] += fr * xr;
+          ai[i] += fr * xi;
+        } else {
+          ar[i] += fr * xr;
+        }
+      }
+    }
+  }
+  if (lane > kmax) return;
+  double* y = a.y + SX * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k0 + lane) * a.yks);
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    if (ROUT % 4 != 0 && i0 + i >= ROUT) continue;
+    double* yi_ = y + SX * static_cast<int64_t>(i0 + i) * a.yrs;
+    double vr = ar[i], vi = XC ? ai[i] : 0.0;
+    if (a.accumulate) {
+      vr += yi_[0];
+      if constexpr (XC) vi += yi_[1];
+    }
+    yi_[0] = vr;
+    if constexpr (XC) yi_[1] = vi;
+  }
+}
+
+// ------------------------------------------------------------------ RHS-fastest layout (many RHS)
+// With many right-hand sides the internal vectors keep the RHS index fastest, so that a wave's 64
+// lanes (64 right-hand sides) read and write 64 consecutive entries:
+//   super vectors  v_t[(row * nrhs + k)]            (S doubles per entry)
+//   multipole      m_t[((lvl_off[l] + c * rows + r) * nrhs + k)]
+// T * X reads the caller's column-major X; T^T * (ys + yf) writes the caller's column-major Y.
+template <bool C>
+__global__ void k_spmv_tin(int nrows, int nrhs, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ v, const double* __restrict__ X, int64_t ldx,
+                           double* __restrict__ Yt) {
+  constexpr int S = C ? 2 : 1;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(nrows) * nrhs) return;
+  const int k = static_cast<int>(gid % nrhs), row = static_cast<int>(gid / nrhs);
+  const double* xk = X + S * static_cast<int64_t>(k) * ldx;
+  double ar = 0.0, ai = 0.0;
+  for (int p = rp[row]; p < rp[row + 1]; ++p) {
+    const int c = ci[p];
+    ar += v[p] * xk[S * c];
+    if constexpr (C) ai += v[p] * xk[S * c + 1];
+  }
+  Yt[S * gid] = ar;
+  if constexpr (C) Yt[S * gid + 1] = ai;
+}
+
+template <bool C>
+__global__ void k_spmv_tout(int nrows, int nrhs, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const double* __restrict__ v, const double* __restrict__ At, const double* __restrict__ Bt,
+                            double* Y, int64_t ldy, double ar_, double ai_, double br_, double bi_, int use_beta) {
+  constexpr int S = C ? 2 : 1;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(nrows) * nrhs) return;
+  const int k = static_cast<int>(gid % nrhs), row = static_cast<int>(gid / nrhs);
+  double sr = 0.0, si = 0.0;
+  for (int p = rp[row]; p < rp[row + 1]; ++p) {
+    const int64_t q = S * (static_cast<int64_t>(ci[p]) * nrhs + k);
+    double xr = At[q], xi = C ? At[q + 1] : 0.0;
+    if (Bt) {
+      xr += Bt[q];
+      if constexpr (C) xi += Bt[q + 1];
+    }
+    sr += v[p] * xr;
+    if constexpr (C) si += v[p] * xi;
+  }
+  double* y = Y + S * (static_cast<int64_t>(k) * ldy + row);
+  if constexpr (C) {
+    double r = ar_ * sr - ai_ * si, im = ar_ * si + ai_ * sr;
+    if (use_beta) {
+      const double yr = y[0], yi = y[1];
+      r += br_ * yr - bi_ * yi;
+      im += br_ * yi + bi_ * yr;
+    }
+    y[0] = r;
+    y[1] = im;
+  } else {
+    double r = ar_ * sr;
+    if (use_beta) r += br_ * y[0];
+    y[0] = r;
   }
 }
 
@@ -1351,8 +1554,6 @@ __global__ void k_to_bf16(const double* __restrict__ src, unsigned short* __rest
 
 // ------------------------------------------------------------------ host launchers
 static inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
-// from this many right-hand sides on, block rows run one wave per owner and 64 columns
-constexpr int MULTI_MIN_RHS = 8;
 
 void launch_spmv(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* x,
                  int64_t ldx, double* y, int64_t ldy, const double* alpha, const double* beta, hipStream_t st,
@@ -1412,7 +1613,7 @@ void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs
   TimedScope ts("coupling_apply", st);
   if (nrhs >= MULTI_MIN_RHS && (!Fc || ncoarse == 0)) {
     const MultiArgs ma{ntgt, nrhs, tgt, lvl_off, rp, src, F, fstride, fcomp, ld, fw, 0,
-                       static_cast<int64_t>(nrhs) * cd, cd, bw, 0, static_cast<int64_t>(nrhs) * cd, cd, accumulate};
+                       static_cast<int64_t>(nrhs) * cd, cd, 1, bw, 0, static_cast<int64_t>(nrhs) * cd, cd, 1, accumulate};
 #define BMI_CPL_M(CC, FTT, CDD)                                                                       \
   if (cplx == CC && fp32 == std::is_same<FTT, float>::value && cd == CDD) {                           \
     constexpr int RPW = CDD;                                                                          \
@@ -1572,8 +1773,8 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
   const int64_t n = static_cast<int64_t>(ncl) * nrhs * dd;
   TimedScope ts("matvec_nearfield", st);
   if (nrhs >= MULTI_MIN_RHS) {
-    const MultiArgs ma{ncl, nrhs, nullptr, nullptr, rp, src, F, fstride, fcomp, dd, xs, xseg, dd, ldxs, ys, yseg, dd,
-                       ldys, accumulate};
+    const MultiArgs ma{ncl, nrhs, nullptr, nullptr, rp, src, F, fstride, fcomp, dd, xs, xseg, dd, ldxs, 1, ys, yseg, dd,
+                       ldys, 1, accumulate};
 #define BMI_LEAF_M(CC, DDD)                                                                           \
   if (cplx == CC && dd == DDD) {                                                                      \
     constexpr int RPW = DDD;                                                                          \
@@ -1689,6 +1890,171 @@ bool launch_sweep_fly(bool cplx, int variant, int ncl, int dd, const int* drp, c
   BMI_FLY(false, 16, 16)
 #undef BMI_FLY
   return false;
+}
+
+void launch_spmv_tin(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* X,
+                     int64_t ldx, double* Yt, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(nrows) * nrhs;
+  if (n == 0) return;
+  if (cplx)
+    hipLaunchKernelGGL(k_spmv_tin<true>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, X, ldx, Yt);
+  else
+    hipLaunchKernelGGL(k_spmv_tin<false>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, X, ldx, Yt);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+void launch_spmv_tout(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* At,
+                      const double* Bt, double* Y, int64_t ldy, const double* alpha, const double* beta,
+                      hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(nrows) * nrhs;
+  if (n == 0) return;
+  const double ar = alpha ? alpha[0] : 1.0, ai = alpha ? alpha[1] : 0.0;
+  const double br = beta ? beta[0] : 0.0, bi = beta ? beta[1] : 0.0;
+  const int ub = (beta && (br != 0.0 || bi != 0.0)) ? 1 : 0;
+  if (cplx)
+    hipLaunchKernelGGL(k_spmv_tout<true>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, At, Bt, Y, ldy, ar,
+                       ai, br, bi, ub);
+  else
+    hipLaunchKernelGGL(k_spmv_tout<false>, dim3(nblk(n)), dim3(256), 0, st, nrows, nrhs, rp, ci, v, At, Bt, Y, ldy, ar,
+                       ai, br, bi, ub);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+// instantiations of k_multi_lds: block products (XC = FC) and the tree steps (real blocks)
+static bool launch_lds(bool xc, bool fc, bool fp32, int rout, int rin, const MultiArgs& ma, dim3 grid,
+                       hipStream_t st) {
+#define BMI_LDS(X, FCC, FTT, RO, RI)                                                                \
+  if (xc == X && fc == FCC && fp32 == std::is_same<FTT, float>::value && rout == RO && rin == RI) { \
+    hipLaunchKernelGGL((k_multi_lds<X, FCC, FTT, RO, RI>), grid, dim3(256), 0, st, ma);            \
+    BMI_CHECK_HIP(hipGetLastError());                                                               \
+    return true;                                                                                    \
+  }
+#define BMI_LDS_BLK(X, R) BMI_LDS(X, X, double, R, R) BMI_LDS(X, X, float, R, R)
+#define BMI_LDS_TREE(X, A, B) BMI_LDS(X, false, double, A, B) BMI_LDS(X, false, double, B, A)
+  BMI_LDS_BLK(false, 16)
+  BMI_LDS_BLK(false, 25)
+  BMI_LDS_BLK(false, 36)
+  BMI_LDS_BLK(true, 16)
+  BMI_LDS_BLK(true, 25)
+  BMI_LDS_BLK(true, 32)
+  BMI_LDS_BLK(true, 36)
+  BMI_LDS(true, false, double, 16, 16)
+  BMI_LDS(true, false, double, 25, 25)
+  BMI_LDS_TREE(false, 16, 36)
+  BMI_LDS_TREE(false, 25, 16)
+  BMI_LDS_TREE(false, 25, 36)
+  BMI_LDS_TREE(true, 16, 36)
+  BMI_LDS_TREE(true, 25, 16)
+  BMI_LDS_TREE(true, 25, 36)
+  BMI_LDS_TREE(true, 32, 16)
+  BMI_LDS_TREE(true, 32, 36)
+#undef BMI_LDS_TREE
+#undef BMI_LDS_BLK
+#undef BMI_LDS
+  return false;
+}
+
+// the LDS copy moves whole 8-byte units: blocks must start and end on them and fit the tile
+static bool multi_lds_fits(bool fc, bool fp32, int rout, int rin, int ld, int64_t fstride, int64_t fcomp) {
+  const int64_t es = (fc ? 2 : 1) * (fp32 ? 4 : 8);
+  const int ldmax = (rout + 3) / 4 * 4;
+  return ld >= rout && ld <= ldmax && (es * fstride) % 8 == 0 && (es * fcomp) % 8 == 0 &&
+         (es * ld * rin) % 8 == 0;
+}
+
+static MultiArgs tree_args(int nown, int nrhs, int mode, const double* F, int64_t fstride, int ld, const double* x,
+                           int64_t xoff, int64_t xcs, double* y, int64_t yoff, int64_t ycs, int accumulate) {
+  MultiArgs ma{};
+  ma.nown = nown;
+  ma.nrhs = nrhs;
+  ma.F = F;
+  ma.fstride = fstride;
+  ma.ldf = ld;
+  ma.x = x;
+  ma.xoff = xoff;
+  ma.xcs = xcs;
+  ma.xks = 1;
+  ma.xrs = nrhs;
+  ma.y = y;
+  ma.yoff = yoff;
+  ma.ycs = ycs;
+  ma.yks = 1;
+  ma.yrs = nrhs;
+  ma.accumulate = accumulate;
+  ma.mode = mode;
+  return ma;
+}
+
+static void tree_step(bool cplx, int rout, int rin, const MultiArgs& ma, hipStream_t st) {
+  const dim3 grid(static_cast<unsigned>(ma.nown), static_cast<unsigned>((ma.nrhs + 63) / 64), 1);
+  BMI_REQUIRE(launch_lds(cplx, false, false, rout, rin, ma, grid, st), BMI_UNSUPPORTED,
+              "tree step without an RHS-fastest instantiation");
+}
+
+void launch_forward_t(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
+                      const double* T, const double* xs_t, int64_t xseg, double* fw_t, const int64_t* lvl_off,
+                      hipStream_t st) {
+  const int64_t R = nrhs;
+  // leaf moments: fw_L[c] = Mom (rows x dd) xs[c]
+  tree_step(cplx, rows, dd,
+            tree_args(npatch << (2 * L), nrhs, 3, mom, 0, rows, xs_t, xseg * R, dd * R, fw_t, lvl_off[L] * R,
+                      rows * R, 0),
+            st);
+  // parents from children, per interpolation component: fw_l[c] = sum_q T_q fw_{l+1}[4c + q]
+  for (int l = L - 1; l >= 0; --l)
+    for (int comp = 0; comp * np2 < rows; ++comp)
+      tree_step(cplx, np2, np2,
+                tree_args(npatch << (2 * l), nrhs, 1, T, static_cast<int64_t>(np2) * np2, np2, fw_t,
+                          (lvl_off[l + 1] + comp * np2) * R, rows * R, fw_t, (lvl_off[l] + comp * np2) * R,
+                          rows * R, 0),
+                st);
+}
+
+void launch_backward_t(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* momt,
+                       const double* Tt, double* bw_t, const int64_t* lvl_off, double* yf_t, int64_t yseg,
+                       hipStream_t st) {
+  const int64_t R = nrhs;
+  // children += T_q^T parent
+  for (int l = 1; l <= L; ++l)
+    for (int comp = 0; comp * np2 < rows; ++comp)
+      tree_step(cplx, np2, np2,
+                tree_args(npatch << (2 * l), nrhs, 2, Tt, static_cast<int64_t>(np2) * np2, np2, bw_t,
+                          (lvl_off[l - 1] + comp * np2) * R, rows * R, bw_t, (lvl_off[l] + comp * np2) * R,
+                          rows * R, 1),
+                st);
+  // leaf rows: yf[c] = Mom^T (dd x rows) bw_L[c]
+  tree_step(cplx, dd, rows,
+            tree_args(npatch << (2 * L), nrhs, 3, momt, 0, dd, bw_t, lvl_off[L] * R, rows * R, yf_t, yseg * R,
+                      dd * R, 0),
+            st);
+}
+
+bool multi_t_supported(bool cplx, int dd, int cd) {
+  return (dd == 16 || dd == 36) && (cd == 16 || cd == 25 || (cplx && cd == 32));
+}
+
+bool launch_multi_near_t(bool cplx, bool fp32, int ncl, int nrhs, int dd, const int* rp, const int* src,
+                         const void* F, int64_t fstride, int64_t fcomp, const double* xs_t, int64_t xseg,
+                         double* ys_t, int64_t yseg, int accumulate, hipStream_t st) {
+  const MultiArgs ma{ncl, nrhs, nullptr, nullptr, rp, src, F, fstride, fcomp, dd,
+                     xs_t, xseg * nrhs, static_cast<int64_t>(dd) * nrhs, 1, nrhs,
+                     ys_t, yseg * nrhs, static_cast<int64_t>(dd) * nrhs, 1, nrhs, accumulate, 0};
+  TimedScope ts("matvec_nearfield", st);
+  const dim3 grid(static_cast<unsigned>(ncl), static_cast<unsigned>((nrhs + 63) / 64), 1);
+  if (!multi_lds_fits(cplx, fp32, dd, dd, dd, fstride, fcomp)) return false;
+  return launch_lds(cplx, cplx, fp32, dd, dd, ma, grid, st);
+}
+
+bool launch_multi_cpl_t(bool cplx, bool fp32, int ntgt, int nrhs, int cd, int ld, const int2* tgt, const int* rp,
+                        const int* src, const void* F, int64_t fstride, int64_t fcomp, const int64_t* lvl_off,
+                        const double* fw_t, double* bw_t, int accumulate, hipStream_t st) {
+  const MultiArgs ma{ntgt, nrhs, tgt, lvl_off, rp, src, F, fstride, fcomp, ld,
+                     fw_t, 0, static_cast<int64_t>(cd) * nrhs, 1, nrhs,
+                     bw_t, 0, static_cast<int64_t>(cd) * nrhs, 1, nrhs, accumulate, 0};
+  TimedScope ts("coupling_apply", st);
+  const dim3 grid(static_cast<unsigned>(ntgt), static_cast<unsigned>((nrhs + 63) / 64), 1);
+  if (!multi_lds_fits(cplx, fp32, cd, cd, ld, fstride, fcomp)) return false;
+  return launch_lds(cplx, cplx, fp32, cd, cd, ma, grid, st);
 }
 
 int fp32_coupling_ld(bool cplx, int cd) {
