@@ -13,6 +13,8 @@
 //   multipole levels fw/bw: [cluster][rhs][row], row = comp*np2 + node.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <type_traits>
 #include <vector>
@@ -1349,6 +1351,171 @@ __global__ __launch_bounds__(256) void k_multi_lds(MultiArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ block rows on the matrix cores (many RHS)
+// Same product as k_multi_lds, on FP64 MFMA (v_mfma_f64_16x16x4f64): y_owner (ROUT x 64) = sum over
+// the owner's blocks of F_p (ROUT x RIN) x_src(p) (RIN x 64). The block (widened to FP64) and its
+// x tile are staged in LDS, zero-padded to MT*16 rows and KS*4 columns; wave w computes the
+// 16 right-hand sides [16w, 16w + 16) for all MT row tiles. Operand layout of the instruction:
+// A(i, k) on lane i + 16k, B(k, j) on lane j + 16k, D(i, j) on lane j + 16 (i % 4), element i / 4.
+// Complex products are four real ones (Re += Fr Xr - Fi Xi, Im += Fr Xi + Fi Xr).
+typedef double bmi_d4 __attribute__((ext_vector_type(4)));
+
+template <bool XC, bool FC, typename FT, int ROUT, int RIN>
+__global__ __launch_bounds__(256) void k_multi_mfma(MultiArgs a) {
+  static_assert(XC || !FC, "complex blocks need complex vectors");
+  constexpr int SX = XC ? 2 : 1, SF = FC ? 2 : 1;
+  constexpr int MT = (ROUT + 15) / 16, KS = (RIN + 3) / 4;
+  constexpr int FLD = MT * 16;   // LDS column stride of the block
+  constexpr int NP = MT == 1 ? 2 : 1;  // independent accumulator chains per tile
+  constexpr int LDMAX = (ROUT + 3) / 4 * 4;
+  constexpr int NFE = (LDMAX * RIN + 255) / 256;  // block elements per thread (bound)
+  constexpr int NXE = (RIN * 64 + 255) / 256;     // x tile elements per thread
+  __shared__ double Fs[SF * KS * 4 * FLD];
+  __shared__ double Xs[SX * KS * 4 * 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = blockIdx.x;
+  const int k0 = static_cast<int>(blockIdx.y) * 64;
+  const int kmax = a.nrhs - 1 - k0;
+  int cl = r;
+  int64_t xo = a.xoff, yo = a.yoff;
+  if (a.tgt) {
+    const int2 T = a.tgt[r];
+    cl = T.y;
+    const int64_t lo = a.lvl_off[T.x] * a.nrhs;
+    xo += lo;
+    yo += lo;
+  }
+  const int mode = a.mode;
+  int p0, p1;
+  if (mode == 0) {
+    p0 = a.rp[r];
+    p1 = a.rp[r + 1];
+  } else if (mode == 1) {
+    p0 = 4 * r;
+    p1 = p0 + 4;
+  } else {
+    p0 = r;
+    p1 = r + 1;
+  }
+  // zero padding (rows >= ROUT, columns >= RIN) stays zero: the copies below write inside only
+  for (int u = tid; u < SF * KS * 4 * FLD; u += 256) Fs[u] = 0.0;
+  for (int u = tid; u < SX * KS * 4 * 64; u += 256) Xs[u] = 0.0;
+  const int ldf = a.ldf;
+  const int fe = ldf * RIN;
+  const FT* __restrict__ F = static_cast<const FT*>(a.F);
+  FT pfr[NFE], pfi[FC ? NFE : 1];
+  double pxr[NXE], pxi[XC ? NXE : 1];
+  auto fetch = [&](int p) {
+    const int bi = mode == 0 ? p : (mode == 3 ? 0 : (p & 3));
+    const int sc = mode == 0 ? a.src[p] : (mode == 2 ? (p >> 2) : p);
+    const FT* f = F + SF * (static_cast<int64_t>(bi) * a.fstride + a.fcomp);
+#pragma unroll
+    for (int q = 0; q < NFE; ++q) {
+      const int e = tid + 256 * q;
+      if (e < fe) {
+        pfr[q] = f[SF * e];
+        if constexpr (FC) pfi[q] = f[SF * e + 1];
+      }
+    }
+    const double* x = a.x + SX * (xo + static_cast<int64_t>(sc) * a.xcs);
+#pragma unroll
+    for (int q = 0; q < NXE; ++q) {
+      const int e = tid + 256 * q;
+      if ((RIN * 64) % 256 == 0 || e < RIN * 64) {
+        const int j = e >> 6;
+        int kl = e & 63;
+        kl = kl <= kmax ? kl : kmax;
+        const double* xe = x + SX * (static_cast<int64_t>(j) * a.xrs + static_cast<int64_t>(k0 + kl) * a.xks);
+        pxr[q] = xe[0];
+        if constexpr (XC) pxi[q] = xe[1];
+      }
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int q = 0; q < NFE; ++q) {
+      const int e = tid + 256 * q;
+      if (e < fe) {
+        const int c = e / ldf, i = e - c * ldf;
+        if (i < ROUT) {
+          Fs[SF * (c * FLD + i)] = static_cast<double>(pfr[q]);
+          if constexpr (FC) Fs[SF * (c * FLD + i) + 1] = static_cast<double>(pfi[q]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NXE; ++q) {
+      const int e = tid + 256 * q;
+      if ((RIN * 64) % 256 == 0 || e < RIN * 64) {
+        Xs[SX * e] = pxr[q];
+        if constexpr (XC) Xs[SX * e + 1] = pxi[q];
+      }
+    }
+  };
+  bmi_d4 accr[MT][NP], acci[XC ? MT : 1][NP];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int h = 0; h < NP; ++h) {
+      accr[m][h] = bmi_d4{0.0, 0.0, 0.0, 0.0};
+      if constexpr (XC) acci[m][h] = bmi_d4{0.0, 0.0, 0.0, 0.0};
+    }
+  const int li = lane & 15, lk = lane >> 4;
+  const int n = w * 16 + li;  // this lane's column of the x tile
+  if (p0 < p1) fetch(p0);
+  for (int p = p0; p < p1; ++p) {
+    __syncthreads();  // previous block no longer read (first pass: zero fill done)
+    commit();
+    __syncthreads();
+    if (p + 1 < p1) fetch(p + 1);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int kr = ks * 4 + lk;
+      const int h = NP == 2 ? (ks & 1) : 0;
+      const double br = Xs[SX * (kr * 64 + n)];
+      const double bi = XC ? Xs[SX * (kr * 64 + n) + 1] : 0.0;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const double fr = Fs[SF * (kr * FLD + m * 16 + li)];
+        if constexpr (FC) {
+          const double fi = Fs[SF * (kr * FLD + m * 16 + li) + 1];
+          accr[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, br, accr[m][h], 0, 0, 0);
+          acci[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bi, acci[m][h], 0, 0, 0);
+          accr[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fi, bi, accr[m][h], 0, 0, 0);
+          acci[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi, br, acci[m][h], 0, 0, 0);
+        } else if constexpr (XC) {
+          accr[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, br, accr[m][h], 0, 0, 0);
+          acci[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bi, acci[m][h], 0, 0, 0);
+        } else {
+          accr[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, br, accr[m][h], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (n > kmax) return;
+  double* y = a.y + SX * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k0 + n) * a.yks);
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = m * 16 + 4 * v + lk;
+      if (ROUT % 16 != 0 && row >= ROUT) continue;
+      double vr = accr[m][0][v], vi = 0.0;
+      if constexpr (NP == 2) vr += accr[m][1][v];
+      if constexpr (XC) {
+        vi = acci[m][0][v];
+        if constexpr (NP == 2) vi += acci[m][1][v];
+      }
+      double* yr = y + SX * static_cast<int64_t>(row) * a.yrs;
+      if (a.accumulate) {
+        vr += yr[0];
+        if constexpr (XC) vi += yr[1];
+      }
+      yr[0] = vr;
+      if constexpr (XC) yr[1] = vi;
+    }
+}
+
 // ------------------------------------------------------------------ RHS-fastest layout (many RHS)
 // With many right-hand sides the internal vectors keep the RHS index fastest, so that a wave's 64
 // lanes (64 right-hand sides) read and write 64 consecutive entries:
@@ -1954,6 +2121,36 @@ static bool launch_lds(bool xc, bool fc, bool fp32, int rout, int rin, const Mul
   return false;
 }
 
+// block products on the matrix cores
+static bool launch_mfma(bool cplx, bool fp32, int rows, const MultiArgs& ma, dim3 grid, hipStream_t st) {
+#define BMI_MF(CC, FTT, R)                                                                \
+  if (cplx == CC && fp32 == std::is_same<FTT, float>::value && rows == R) {               \
+    hipLaunchKernelGGL((k_multi_mfma<CC, CC, FTT, R, R>), grid, dim3(256), 0, st, ma);    \
+    BMI_CHECK_HIP(hipGetLastError());                                                     \
+    return true;                                                                          \
+  }
+#define BMI_MF2(CC, R) BMI_MF(CC, double, R) BMI_MF(CC, float, R)
+  BMI_MF2(false, 16)
+  BMI_MF2(false, 25)
+  BMI_MF2(false, 36)
+  BMI_MF2(true, 16)
+  BMI_MF2(true, 25)
+  BMI_MF2(true, 32)
+  BMI_MF2(true, 36)
+#undef BMI_MF2
+#undef BMI_MF
+  return false;
+}
+
+// block products of the RHS-fastest sweeps: 1 = matrix cores, 0 = vector units (BMI_MULTI_VECTOR=1)
+static int multi_engine() {
+  static const int e = [] {
+    const char* v = std::getenv("BMI_MULTI_VECTOR");
+    return (v && v[0] == '1') ? 0 : 1;
+  }();
+  return e;
+}
+
 // the LDS copy moves whole 8-byte units: blocks must start and end on them and fit the tile
 static bool multi_lds_fits(bool fc, bool fp32, int rout, int rin, int ld, int64_t fstride, int64_t fcomp) {
   const int64_t es = (fc ? 2 : 1) * (fp32 ? 4 : 8);
@@ -2041,6 +2238,7 @@ bool launch_multi_near_t(bool cplx, bool fp32, int ncl, int nrhs, int dd, const 
                      ys_t, yseg * nrhs, static_cast<int64_t>(dd) * nrhs, 1, nrhs, accumulate, 0};
   TimedScope ts("matvec_nearfield", st);
   const dim3 grid(static_cast<unsigned>(ncl), static_cast<unsigned>((nrhs + 63) / 64), 1);
+  if (multi_engine() == 1) return launch_mfma(cplx, fp32, dd, ma, grid, st);
   if (!multi_lds_fits(cplx, fp32, dd, dd, dd, fstride, fcomp)) return false;
   return launch_lds(cplx, cplx, fp32, dd, dd, ma, grid, st);
 }
@@ -2053,6 +2251,7 @@ bool launch_multi_cpl_t(bool cplx, bool fp32, int ntgt, int nrhs, int cd, int ld
                      bw_t, 0, static_cast<int64_t>(cd) * nrhs, 1, nrhs, accumulate, 0};
   TimedScope ts("coupling_apply", st);
   const dim3 grid(static_cast<unsigned>(ntgt), static_cast<unsigned>((nrhs + 63) / 64), 1);
+  if (multi_engine() == 1 && ld >= cd && ld <= (cd + 3) / 4 * 4) return launch_mfma(cplx, fp32, cd, ma, grid, st);
   if (!multi_lds_fits(cplx, fp32, cd, cd, ld, fstride, fcomp)) return false;
   return launch_lds(cplx, cplx, fp32, cd, cd, ma, grid, st);
 }
