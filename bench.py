@@ -185,6 +185,27 @@ def run_batched(B, torch, rng, steps):
         t1, _ = timed(torch, lambda: HH.matvec(r1, sync=False), max(3, steps // 4))
         out[name + "_matvec_1rhs_ms"] = t1 * 1e3
     log("batched 64 RHS: matvec fp64 %.2f ms, mixed %.2f ms" % (out["fp64_matvec_ms"], out["mixed_matvec_ms"]))
+    # matrix-core roofline of the FP64 64-RHS block sweeps: algorithmic flops (8 per complex
+    # multiply-add, per right-hand side) over the average launch time (HIP events in the library)
+    B.timing.reset()
+    B.timing.enable(True)
+    for _ in range(3):
+        H.matvec(rhs)
+    B.timing.enable(False)
+    info = H.info
+    sweeps = {}
+    for key, name, blocks, dim in (("coupling_apply", "coupling", info["lowrank"], info["coupling_dim"]),
+                                   ("matvec_nearfield", "near_field", info["dense"], info["dense_dim"])):
+        ms, cnt = B.timing.get(key)
+        flops = 8.0 * blocks * dim * dim * nrhs
+        avg = ms / max(cnt, 1)
+        sweeps[name] = {"bound": "mfma", "achieved": flops / (avg * 1e-3) / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                        "frac": flops / (avg * 1e-3) / 78.6e12, "flops_per_launch": flops, "avg_launch_ms": avg,
+                        "kernel": "k_multi_mfma<complex, FP64, %d> (v_mfma_f64_16x16x4f64)" % dim}
+    out["roofline_64rhs"] = sweeps
+    log("batched 64 RHS sweeps on the matrix cores: coupling %.1f TFLOP/s (%.3f ms), near field %.1f TFLOP/s "
+        "(%.3f ms)" % (sweeps["coupling"]["achieved"], sweeps["coupling"]["avg_launch_ms"],
+                       sweeps["near_field"]["achieved"], sweeps["near_field"]["avg_launch_ms"]))
     t, (x64, gm) = timed(torch, lambda: gmres_solve(B, H, rhs, maxit=8000))
     out["fp64_gmres_s"] = t
     out["fp64_gmres_iterations_max"] = int(np.max(gm.iterations()))
