@@ -69,6 +69,35 @@ def measured_traffic(M, P, npts):
     return d["fetch_bytes"] + d["write_bytes"]
 
 
+def run_batched_profile(B, torch):
+    """Helmholtz kappa 4, 6,144 elements, P 2, 5x5 Chebyshev nodes, 64 plane-wave right-hand sides:
+    per precision an assembly, 8 batched products and 60 batched GMRES(30) steps (the workload of the
+    64-RHS kernel statistics and matrix-core counters)"""
+    op = B.HelmholtzSingleLayerOperator()
+    op.set_wavenumber(4.0)
+    space = B.AnsatzSpace(B.Geometry(), 5, 2, op)
+    rng = np.random.default_rng(3)
+    rhs = None
+    for name, prec in (("fp64", B.FP64), ("mixed", B.MIXED)):
+        H = B.H2Matrix(space, op, n_cheb=5, precision=prec)
+        if rhs is None:
+            rhs = B.DiscreteLinearForm(B.PlaneWaves(unit_directions(rng, 64), 4.0), space).compute(
+                device=True).get_discrete_linear_form()
+        for _ in range(3):
+            H.matvec(rhs)
+        B.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            H.matvec(rhs, sync=False)
+        B.synchronize()
+        log(name, "64-RHS matvec %.3f ms" % ((time.perf_counter() - t0) / 5 * 1e3))
+        t0 = time.perf_counter()
+        gmres_solve(B, H, rhs, tol=1e-12, maxit=60)
+        torch.cuda.synchronize()
+        log(name, "60 batched GMRES steps %.3f s" % (time.perf_counter() - t0))
+        del H
+
+
 def run_helmholtz_profile(B, torch):
     """Helmholtz kappa 4, 24,576 elements, P 2, 5x5 Chebyshev nodes: per precision an assembly, 23
     products and 300 GMRES(30) steps (the workload of the Helmholtz kernel statistics)"""
@@ -368,6 +397,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-RHS Helmholtz workload")
     ap.add_argument("--no-headline", action="store_true", help="skip the Helmholtz 24,576-element assembly + GMRES")
     ap.add_argument("--serving", type=int, default=16, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--batched-profile", action="store_true",
+                    help="only the Helmholtz 6,144-element 64-RHS products and GMRES steps (profiling workload)")
     ap.add_argument("--helmholtz-profile", action="store_true",
                     help="only the Helmholtz 24,576-element products and GMRES steps (profiling workload)")
     a = ap.parse_args()
@@ -382,6 +413,9 @@ def main():
     torch.cuda.init()
     if a.helmholtz_profile:
         run_helmholtz_profile(B, torch)
+        return
+    if a.batched_profile:
+        run_batched_profile(B, torch)
         return
     M, P, npts = a.level, 2, 4
     geo = B.Geometry()
