@@ -2121,23 +2121,36 @@ static bool launch_lds(bool xc, bool fc, bool fp32, int rout, int rin, const Mul
   return false;
 }
 
-// block products on the matrix cores
-static bool launch_mfma(bool cplx, bool fp32, int rows, const MultiArgs& ma, dim3 grid, hipStream_t st) {
-#define BMI_MF(CC, FTT, R)                                                                \
-  if (cplx == CC && fp32 == std::is_same<FTT, float>::value && rows == R) {               \
-    hipLaunchKernelGGL((k_multi_mfma<CC, CC, FTT, R, R>), grid, dim3(256), 0, st, ma);    \
-    BMI_CHECK_HIP(hipGetLastError());                                                     \
-    return true;                                                                          \
+// block products and tree steps on the matrix cores
+static bool launch_mfma(bool xc, bool fc, bool fp32, int rout, int rin, const MultiArgs& ma, dim3 grid,
+                        hipStream_t st) {
+#define BMI_MF(X, FCC, FTT, RO, RI)                                                                 \
+  if (xc == X && fc == FCC && fp32 == std::is_same<FTT, float>::value && rout == RO && rin == RI) { \
+    hipLaunchKernelGGL((k_multi_mfma<X, FCC, FTT, RO, RI>), grid, dim3(256), 0, st, ma);           \
+    BMI_CHECK_HIP(hipGetLastError());                                                               \
+    return true;                                                                                    \
   }
-#define BMI_MF2(CC, R) BMI_MF(CC, double, R) BMI_MF(CC, float, R)
-  BMI_MF2(false, 16)
-  BMI_MF2(false, 25)
-  BMI_MF2(false, 36)
-  BMI_MF2(true, 16)
-  BMI_MF2(true, 25)
-  BMI_MF2(true, 32)
-  BMI_MF2(true, 36)
-#undef BMI_MF2
+#define BMI_MF_BLK(X, R) BMI_MF(X, X, double, R, R) BMI_MF(X, X, float, R, R)
+#define BMI_MF_TREE(X, A, B) BMI_MF(X, false, double, A, B) BMI_MF(X, false, double, B, A)
+  BMI_MF_BLK(false, 16)
+  BMI_MF_BLK(false, 25)
+  BMI_MF_BLK(false, 36)
+  BMI_MF_BLK(true, 16)
+  BMI_MF_BLK(true, 25)
+  BMI_MF_BLK(true, 32)
+  BMI_MF_BLK(true, 36)
+  BMI_MF(true, false, double, 16, 16)
+  BMI_MF(true, false, double, 25, 25)
+  BMI_MF_TREE(false, 16, 36)
+  BMI_MF_TREE(false, 25, 16)
+  BMI_MF_TREE(false, 25, 36)
+  BMI_MF_TREE(true, 16, 36)
+  BMI_MF_TREE(true, 25, 16)
+  BMI_MF_TREE(true, 25, 36)
+  BMI_MF_TREE(true, 32, 16)
+  BMI_MF_TREE(true, 32, 36)
+#undef BMI_MF_TREE
+#undef BMI_MF_BLK
 #undef BMI_MF
   return false;
 }
@@ -2184,8 +2197,9 @@ static MultiArgs tree_args(int nown, int nrhs, int mode, const double* F, int64_
 
 static void tree_step(bool cplx, int rout, int rin, const MultiArgs& ma, hipStream_t st) {
   const dim3 grid(static_cast<unsigned>(ma.nown), static_cast<unsigned>((ma.nrhs + 63) / 64), 1);
-  BMI_REQUIRE(launch_lds(cplx, false, false, rout, rin, ma, grid, st), BMI_UNSUPPORTED,
-              "tree step without an RHS-fastest instantiation");
+  const bool ok = multi_engine() == 1 ? launch_mfma(cplx, false, false, rout, rin, ma, grid, st)
+                                      : launch_lds(cplx, false, false, rout, rin, ma, grid, st);
+  BMI_REQUIRE(ok, BMI_UNSUPPORTED, "tree step without an RHS-fastest instantiation");
 }
 
 void launch_forward_t(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
@@ -2238,7 +2252,7 @@ bool launch_multi_near_t(bool cplx, bool fp32, int ncl, int nrhs, int dd, const 
                      ys_t, yseg * nrhs, static_cast<int64_t>(dd) * nrhs, 1, nrhs, accumulate, 0};
   TimedScope ts("matvec_nearfield", st);
   const dim3 grid(static_cast<unsigned>(ncl), static_cast<unsigned>((nrhs + 63) / 64), 1);
-  if (multi_engine() == 1) return launch_mfma(cplx, fp32, dd, ma, grid, st);
+  if (multi_engine() == 1) return launch_mfma(cplx, cplx, fp32, dd, dd, ma, grid, st);
   if (!multi_lds_fits(cplx, fp32, dd, dd, dd, fstride, fcomp)) return false;
   return launch_lds(cplx, cplx, fp32, dd, dd, ma, grid, st);
 }
@@ -2251,7 +2265,8 @@ bool launch_multi_cpl_t(bool cplx, bool fp32, int ntgt, int nrhs, int cd, int ld
                      bw_t, 0, static_cast<int64_t>(cd) * nrhs, 1, nrhs, accumulate, 0};
   TimedScope ts("coupling_apply", st);
   const dim3 grid(static_cast<unsigned>(ntgt), static_cast<unsigned>((nrhs + 63) / 64), 1);
-  if (multi_engine() == 1 && ld >= cd && ld <= (cd + 3) / 4 * 4) return launch_mfma(cplx, fp32, cd, ma, grid, st);
+  if (multi_engine() == 1 && ld >= cd && ld <= (cd + 3) / 4 * 4)
+    return launch_mfma(cplx, cplx, fp32, cd, cd, ma, grid, st);
   if (!multi_lds_fits(cplx, fp32, cd, cd, ld, fstride, fcomp)) return false;
   return launch_lds(cplx, cplx, fp32, cd, cd, ma, grid, st);
 }

@@ -667,11 +667,13 @@ static void launch_class(const NFPair* d_pairs, int count, int deg, NFParams prm
 template <int P, int KIND, int PD>
 static void launch_all(const std::vector<NFPair>& pairs, const NFPair* d_pairs, NFParams prm,
                        hipStream_t st) {
-  // pairs are sorted by (class, degree); one launch per group
+  // pairs are sorted by class, then by falling degree: one launch per class holds every
+  // quadrature degree of the class (each workgroup reads its pair's degree; LDS is sized for the
+  // first, largest one), the most expensive pairs first so the tail of the launch is short
   size_t a = 0;
   while (a < pairs.size()) {
     size_t b = a;
-    while (b < pairs.size() && pairs[b].cls == pairs[a].cls && pairs[b].deg == pairs[a].deg) ++b;
+    while (b < pairs.size() && pairs[b].cls == pairs[a].cls) ++b;
     const int cnt = static_cast<int>(b - a), deg = pairs[a].deg;
     switch (pairs[a].cls) {
       case 0: launch_class<P, KIND, PD, 0>(d_pairs + a, cnt, deg, prm, st); break;
