@@ -111,6 +111,7 @@ def library():
         "bmi_h2_matvec": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, C.c_int, vp]),
         "bmi_h2_matvec_super": (C.c_int, [vp, C.c_int, dp, dp, vp]),
         "bmi_h2_use_graphs": (C.c_int, [vp, C.c_int]),
+        "bmi_h2_set_sweep": (C.c_int, [vp, C.c_int]),
         "bmi_synchronize": (C.c_int, [vp]),
         "bmi_rhs_assemble": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp, C.c_int, vp]),
         "bmi_solve_gmres": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, ip, dp,
@@ -517,6 +518,13 @@ class H2Matrix:
     def use_graphs(self, enable=True):
         """Replay the product from captured HIP graphs (default) or launch kernel by kernel."""
         _check(library().bmi_h2_use_graphs(self._h, 1 if enable else 0))
+
+    def set_sweep(self, mode):
+        """Coupling sweep of single-RHS products: None (timed choice, default), "full" (every
+        coupling block) or "upper" (upper blocks of a symmetric kernel with the transposed
+        products formed from them)."""
+        m = {None: -1, "full": 0, "upper": 1}[mode]
+        _check(library().bmi_h2_set_sweep(self._h, m))
 
     def get_dense(self):
         """Dense matrix by applying H to the identity (H2Matrix::get_dense, H2Matrix.hpp:254-264)."""

@@ -127,6 +127,14 @@ struct bmi_h2 {
   // FMM constants
   DevBuf<double> d_T, d_mom[2];
   DevBuf<double> d_Tt, d_momt[2];  // transposed copies for the RHS-fastest backward steps
+  // upper half of the coupling blocks (symmetric kernels) for single-RHS sweeps
+  bool sym = false;
+  int sym_fused = 0;  // chosen by the autotuner: FP64 single-RHS sweeps read only the upper blocks
+  int sweep_mode = -1;  // -1: timed choice, 0: every coupling block, 1: upper blocks
+  DevBuf<int2> d_stgt, d_sgtgt;
+  DevBuf<int> d_srp, d_ssrc, d_sblk, d_sgrp, d_sgslot;
+  DevBuf<double> d_part;
+  bmi::SymList sl{};
   // workspace (grown on demand)
   int ws_rhs = 0;
   DevBuf<double> d_xs, d_ys, d_fw[2], d_bw, d_xin, d_yout;
@@ -159,7 +167,8 @@ struct bmi_h2 {
   size_t bytes() const {
     return d_dense.bytes() + d_coup.bytes() + d_coup32.bytes() + d_coup16.bytes() + d_dense32.bytes() +
            d_fly_pts.bytes() + d_fly_org.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() +
-           d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_Tt.bytes() + d_momt[0].bytes() + d_momt[1].bytes() + d_xs.bytes() +
+           d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_Tt.bytes() + d_momt[0].bytes() + d_momt[1].bytes() + d_stgt.bytes() + d_sgtgt.bytes() + d_srp.bytes() +
+           d_ssrc.bytes() + d_sblk.bytes() + d_sgrp.bytes() + d_sgslot.bytes() + d_part.bytes() + d_xs.bytes() +
            d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes() + d_yf.bytes();
   }
 };
@@ -284,20 +293,25 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
     h->cpl_variant = pick;
     // the fused sweep: all four combinations (register use, and so occupancy, differs)
     if (h->vd == 1) {
-      int pl = -1, pc = -1;
-      for (int vl = 0; vl < 2; ++vl)
-        for (int vc = 0; vc < 2; ++vc) {
-          bool ok = true;
-          const float t = time_it([&] {
-            ok = launch_sweep_fused(h->cplx, fp32, vl, vc, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr,
-                                    h->d_dense.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
-                                    h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
-                                    co.ncoarse, false, co.ld);
-          });
-          if (ok && (pl < 0 || t < best)) best = t, pl = vl, pc = vc;
-        }
+      // full block lists, and (symmetric kernels, FP64 blocks) the upper blocks with transposed
+      // partials: the upper form reads half the coupling bytes but adds the column sums
+      int pl = -1, pc = -1, ps = 0;
+      const int nsym = (h->sym && !fp32) ? 2 : 1;
+      for (int sy = h->sweep_mode == 1 && nsym == 2 ? 1 : 0; sy < (h->sweep_mode == 0 ? 1 : nsym); ++sy)
+        for (int vl = 0; vl < 2; ++vl)
+          for (int vc = 0; vc < 2; ++vc) {
+            bool ok = true;
+            const float t = time_it([&] {
+              ok = launch_sweep_fused(h->cplx, fp32, vl, vc, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr,
+                                      h->d_dense.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
+                                      h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
+                                      co.ncoarse, false, co.ld, sy ? &h->sl : nullptr);
+            });
+            if (ok && (pl < 0 || t < best)) best = t, pl = vl, pc = vc, ps = sy;
+          }
       h->fused_leaf_variant = pl;
       h->fused_cpl_variant = pc;
+      h->sym_fused = ps;
     }
     if (h->red1) {
       // reduced single-RHS sweep: coupling recomputed on the fly (variants 0, 1; real kernels:
@@ -305,8 +319,11 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
       // costs Krylov steps) or read from the FP32 copy (variants 2 + 2 * near blocks per
       // iteration + coupling blocks per iteration), both with the FP32 near-field blocks;
       // whichever is faster (arithmetic vs bytes)
+      // (variants 6-9: the FP32 copy's upper blocks with transposed partials)
       int pv = -1;
-      for (int v = h->fly ? 0 : 2; v < 6; ++v) {
+      const int v0 = (h->sym && h->sweep_mode == 1) ? 6 : (h->fly ? 0 : 2);
+      const int v1 = (h->sym && h->sweep_mode != 0) ? 10 : 6;
+      for (int v = v0; v < v1; ++v) {
         bool ok = true;
         const float t = time_it([&] {
           if (v < 2)
@@ -315,10 +332,11 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
                                   h->d_lvl_off.ptr, h->d_fly_pts.ptr, h->d_fly_org.ptr, h->op.kappa_re,
                                   h->op.kappa_im, h->d_fw[0].ptr, h->d_bw.ptr, st);
           else
-            ok = launch_sweep_fused(h->cplx, true, (v - 2) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd, h->d_drp.ptr,
-                                    h->d_dsrc.ptr, h->d_dense32.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr,
-                                    h->d_crp.ptr, h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, h->d_fw[0].ptr,
-                                    h->d_bw.ptr, st, nullptr, 0, true, h->ld32);
+            ok = launch_sweep_fused(h->cplx, true, ((v - 2) & 3) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd,
+                                    h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd,
+                                    h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr,
+                                    h->d_fw[0].ptr, h->d_bw.ptr, st, nullptr, 0, true, h->ld32,
+                                    v >= 6 ? &h->sl : nullptr);
         });
         if (ok && (pv < 0 || t < best)) best = t, pv = v;
       }
@@ -351,10 +369,10 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
                                  h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, h->d_lvl_off.ptr,
                                  h->d_fly_pts.ptr, h->d_fly_org.ptr, h->op.kappa_re, h->op.kappa_im, h->d_fw[0].ptr,
                                  h->d_bw.ptr, st)
-              : launch_sweep_fused(h->cplx, true, (v - 2) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd, h->d_drp.ptr,
-                                   h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
-                                   h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st,
-                                   nullptr, 0, true, h->ld32);
+              : launch_sweep_fused(h->cplx, true, ((v - 2) & 3) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd,
+                                   h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr,
+                                   h->d_crp.ptr, h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, h->d_fw[0].ptr,
+                                   h->d_bw.ptr, st, nullptr, 0, true, h->ld32, v >= 6 ? &h->sl : nullptr);
     if (ok) {
       TimedScope ts("matvec_backward", st);
       launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
@@ -372,10 +390,11 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
       launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
                      h->d_fw[0].ptr, h->d_lvl_off.ptr, st, h->d_bw.ptr);
     }
+    const bool use_sym = h->sym && h->sym_fused && !co.fp32;
     if (launch_sweep_fused(h->cplx, co.fp32, h->fused_leaf_variant, h->fused_cpl_variant, h->ncl_leaf, h->dd,
                            h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr,
                            h->d_crp.ptr, h->d_csrc.ptr, co.F, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
-                           co.ncoarse, false, co.ld)) {
+                           co.ncoarse, false, co.ld, use_sym ? &h->sl : nullptr)) {
       TimedScope ts("matvec_backward", st);
       launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
                       h->d_lvl_off.ptr, yf, N, 0, st);
@@ -512,6 +531,64 @@ bool h2_is_symmetric_real(const bmi_h2* h) { return h->op.kind == BMI_LAPLACE_SL
 bool h2_is_mixed(const bmi_h2* h) { return h->op.precision != BMI_PRECISION_FP64; }
 
 // ------------------------------------------------------------------ assembly
+// Coupling blocks of a symmetric kernel pair up: (level, c2, c1) is the transpose of (level, c1,
+// c2) (same kernel values; the two weights multiply in the other order). For single-RHS sweeps
+// list the upper blocks (c1 < c2) per target, and per column cluster the upper blocks whose
+// transposed products it receives. Leaves the handle unchanged when a block has no partner.
+static void build_symmetric_lists(bmi_h2* h, const std::vector<int2>& tgt, const std::vector<int>& crp,
+                                  const std::vector<int>& csrc, hipStream_t st) {
+  const int nt = static_cast<int>(tgt.size());
+  auto find_target = [&](int level, int c) {
+    const auto it = std::lower_bound(tgt.begin(), tgt.end(), make_int2(level, c), [](const int2& a, const int2& b) {
+      return a.x != b.x ? a.x < b.x : a.y < b.y;
+    });
+    return (it != tgt.end() && it->x == level && it->y == c) ? static_cast<int>(it - tgt.begin()) : -1;
+  };
+  std::vector<int2> utgt;
+  std::vector<int> urp(1, 0), usrc, ublk;
+  std::vector<int2> ucol;  // (target index of the column cluster, upper entry)
+  for (int t = 0; t < nt; ++t) {
+    const int level = tgt[t].x, c1 = tgt[t].y;
+    bool any = false;
+    for (int p = crp[t]; p < crp[t + 1]; ++p) {
+      const int c2 = csrc[p];
+      const int t2 = find_target(level, c2);
+      if (c2 == c1 || t2 < 0 || !std::binary_search(csrc.begin() + crp[t2], csrc.begin() + crp[t2 + 1], c1)) return;
+      if (c2 < c1) continue;
+      if (!any) utgt.push_back(tgt[t]), urp.push_back(urp.back()), any = true;
+      ucol.push_back(make_int2(t2, static_cast<int>(usrc.size())));
+      usrc.push_back(c2);
+      ublk.push_back(p);
+      urp.back()++;
+    }
+  }
+  // gather lists, column targets in (level, cluster) order, entries in list order
+  std::stable_sort(ucol.begin(), ucol.end(), [](const int2& a, const int2& b) { return a.x < b.x; });
+  std::vector<int2> gtgt;
+  std::vector<int> grp(1, 0), gslot;
+  for (const int2& e : ucol) {
+    if (gtgt.empty() || gtgt.back().x != tgt[e.x].x || gtgt.back().y != tgt[e.x].y) {
+      gtgt.push_back(tgt[e.x]);
+      grp.push_back(grp.back());
+    }
+    grp.back()++;
+    gslot.push_back(e.y);
+  }
+  const int S = h->cplx ? 2 : 1;
+  h->d_stgt.upload(utgt, st);
+  h->d_srp.upload(urp, st);
+  h->d_ssrc.upload(usrc, st);
+  h->d_sblk.upload(ublk, st);
+  h->d_sgtgt.upload(gtgt, st);
+  h->d_sgrp.upload(grp, st);
+  h->d_sgslot.upload(gslot, st);
+  h->d_part.alloc(static_cast<size_t>(S) * usrc.size() * h->cd);
+  h->sl = SymList{static_cast<int>(utgt.size()), h->d_stgt.ptr, h->d_srp.ptr, h->d_ssrc.ptr, h->d_sblk.ptr,
+                  h->d_part.ptr, static_cast<int>(gtgt.size()), h->d_sgtgt.ptr, h->d_sgrp.ptr, h->d_sgslot.ptr};
+  h->sym = true;
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+}
+
 static void assemble_h2(bmi_h2* h, hipStream_t st) {
   const bmi_space* s = h->s;
   const bmi_op_desc& op = h->op;
@@ -648,6 +725,8 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   h->d_tgt.upload(tgt, st);
   h->d_crp.upload(crp, st);
   h->d_csrc.upload(csrc, st);
+  if (vd == 1 && (op.kind == BMI_LAPLACE_SL || op.kind == BMI_HELMHOLTZ_SL) && sym_sweep_supported(h->cplx, h->dd, h->cd))
+    build_symmetric_lists(h, tgt, crp, csrc, st);
   // FP32 coupling copy: MIXED pads the columns to whole 16-byte row vectors (the sweeps read rows
   // as vectors); MIXED_BF16 keeps the FP64 layout, shared with its BF16 blocks
   h->ld32 = op.precision == BMI_PRECISION_MIXED ? fp32_coupling_ld(h->cplx, h->cd) : h->cd;
@@ -1196,6 +1275,21 @@ bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable) {
   BMI_REQUIRE(h, BMI_INVALID_ARG, "null handle");
   h->use_graphs = enable != 0;
   if (!h->use_graphs) h->drop_graphs();
+  BMI_API_END
+}
+
+bmi_status bmi_h2_set_sweep(bmi_h2* h, int mode) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(h, BMI_INVALID_ARG, "null handle");
+  BMI_REQUIRE(mode >= -1 && mode <= 1, BMI_INVALID_ARG, "sweep mode must be -1, 0 or 1");
+  BMI_REQUIRE(mode != 1 || h->sym, BMI_UNSUPPORTED, "no upper-block sweep for this operator or block size");
+  h->sweep_mode = mode;
+  // choose again at the next product
+  h->leaf_variant = h->cpl_variant = -1;
+  h->fused_leaf_variant = h->fused_cpl_variant = -1;
+  h->fly_variant = -1;
+  h->sym_fused = 0;
+  h->drop_graphs();
   BMI_API_END
 }
 

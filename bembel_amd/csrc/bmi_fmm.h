@@ -75,11 +75,28 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
 // near-field sweep and coupling sweep of a single-RHS, single-component product in one launch
 // (ys written, bw written); false when no fused kernel exists for these sizes
 // near32: the near-field blocks dF are the FP32 copy (with FP32 coupling blocks)
+// block sizes with symmetric single-RHS sweeps
+bool sym_sweep_supported(bool cplx, int dd, int cd);
+// upper half of the coupling blocks of a symmetric kernel, for single-RHS sweeps
+struct SymList {
+  int ntgt;  // targets with upper blocks, their CSR (sources) and the block index of each entry
+  const int2* tgt;
+  const int* rp;
+  const int* src;
+  const int* blk;
+  double* part;  // per entry: B^T x_t (S * cd doubles)
+  int ngather;   // targets receiving partials, their CSR over entries
+  const int2* gtgt;
+  const int* grp;
+  const int* gslot;
+};
+
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
                         const int* dsrc, const void* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
                         const double* fw, double* bw, hipStream_t st, const unsigned short* Fc = nullptr,
-                        int ncoarse = 0, bool near32 = false, int ld = 0);
+                        int ncoarse = 0, bool near32 = false, int ld = 0,
+                        const SymList* sym = nullptr);
 // FP32 node tables (offsets from each cluster's node 0, weights; node 0 in FP64) of one level
 void launch_fly_points(const double* pts, int ncl, int np2, float4* out, double4* org, hipStream_t st);
 // single-RHS sweep of the reduced operator: FP32 near-field blocks, coupling evaluated on the fly

@@ -725,6 +725,9 @@ struct CplArgs {
   const unsigned short* Fc;  // BF16 copies of the first ncoarse blocks (coarsest levels), or null
   int ncoarse;
   int ld;  // column stride of F blocks (cd, or cd padded to whole 16-byte row vectors for FP32)
+  // symmetric sweeps: block index of each list entry and the per-entry partial vectors B^T x_t
+  const int* blk;
+  double* part;
 };
 
 __device__ __forceinline__ double bf16_to_double(unsigned short b) {
@@ -1061,9 +1064,364 @@ __device__ __forceinline__ void coupling_wave_r2(int t, const CplArgs& a) {
   }
 }
 
+// ------------------------------------------------------------------ symmetric coupling (single RHS)
+// Coupling blocks of the scalar kernels come in transposed pairs: the block of (level, s, t) is the
+// transpose of (level, t, s) (the kernel is symmetric in its arguments, the node sets are per
+// level). A single-RHS sweep can therefore read only the upper blocks (t < s): the wave owning t
+// adds B x_s to its own rows and leaves u = B^T x_t, which belongs to s, in a per-block partial
+// vector; k_sym_gather then adds each target's partials in a fixed order (no atomics, the same
+// bits on every run). Each upper block is read once for both products.
+
+// Column sums of a W-lane group (W a power of two, groups W-aligned within the wave): lane li
+// supplies its partial value v(j) of every column j < W*R; on return out[r] holds the group's sum
+// of column R*li + r. Recursive halving: under mask m each lane keeps the half of its columns
+// selected by (li & m) and adds its partner's copy of that half.
+template <int W, int R, class V>
+__device__ __forceinline__ void group_column_sums(int li, V v, double (&out)[R]) {
+  constexpr int N = W * R;
+  if constexpr (W == 1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[r] = v(r);
+  } else {
+    constexpr int M0 = W / 2;
+    double h[N / 2];
+    const bool up0 = (li & M0) != 0;
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+      const double lo = v(k), hi = v(k + N / 2);
+      h[k] = (up0 ? hi : lo) + __shfl_xor(up0 ? lo : hi, M0, 64);
+    }
+#pragma unroll
+    for (int m = M0 / 2; m >= 1; m >>= 1) {
+      const bool up = (li & m) != 0;
+#pragma unroll
+      for (int k = 0; k < m * R; ++k) {
+        const double lo = h[k], hi = h[k + m * R];
+        h[k] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, m, 64);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[r] = h[r];
+  }
+}
+
+__host__ __device__ constexpr int pow2_at_least(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// FP64 blocks, one row per lane (complex, or real sizes without the two-row form)
+template <bool C, int CD, int UB>
+__device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
+  constexpr int S = C ? 2 : 1;
+  constexpr int W = pow2_at_least(CD);
+  static_assert(W <= 64, "block rows exceed a wave");
+  constexpr int G = 64 / W;
+  const int lane = threadIdx.x & 63;
+  const int g = lane / W, li = lane % W;
+  const bool act = li < CD;
+  const int i = act ? li : 0;
+  const int2 T = a.tgt[t];
+  const int64_t off = a.lvl_off[T.x];
+  const int p0 = a.rp[t], p1 = a.rp[t + 1];
+  const double* __restrict__ F = static_cast<const double*>(a.F);
+  const double* xt = a.fw + S * (off + static_cast<int64_t>(T.y) * CD + i);
+  const double tr = act ? xt[0] : 0.0, ti = (C && act) ? xt[1] : 0.0;
+  double ar = 0.0, ai = 0.0;
+  for (int p = p0 + g; p < p1; p += G * UB) {
+    double fr[UB][CD], fi[UB][C ? CD : 1];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int pu = p + G * u < p1 ? p + G * u : p;
+      const double* f = F + S * (static_cast<int64_t>(a.blk[pu]) * a.fstride + a.fcomp);
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        fr[u][j] = f[S * (j * CD + i)];
+        if constexpr (C) fi[u][j] = f[S * (j * CD + i) + 1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const bool ok = p + G * u < p1;
+      const int pu = ok ? p + G * u : p;
+      const double* x = a.fw + S * (off + static_cast<int64_t>(a.src[pu]) * CD);
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        const double xr = ok ? x[S * j] : 0.0;
+        const double xi = (C && ok) ? x[S * j + 1] : 0.0;
+        if constexpr (C) {
+          ar += fr[u][j] * xr - fi[u][j] * xi;
+          ai += fr[u][j] * xi + fi[u][j] * xr;
+        } else {
+          ar += fr[u][j] * xr;
+        }
+      }
+      double cr[1], ci[1];
+      group_column_sums<W, 1>(
+          li,
+          [&](int j) {
+            const int jj = j < CD ? j : 0;
+            return j < CD ? (C ? fr[u][jj] * tr - fi[u][C ? jj : 0] * ti : fr[u][jj] * tr) : 0.0;
+          },
+          cr);
+      if constexpr (C)
+        group_column_sums<W, 1>(
+            li,
+            [&](int j) {
+              const int jj = j < CD ? j : 0;
+              return j < CD ? fr[u][jj] * ti + fi[u][jj] * tr : 0.0;
+            },
+            ci);
+      if (ok && act) {
+        double* pp = a.part + S * (static_cast<int64_t>(pu) * CD + li);
+        pp[0] = cr[0];
+        if constexpr (C) pp[1] = ci[0];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = W; m < 64; m <<= 1) {
+    ar += __shfl_xor(ar, m, 64);
+    if constexpr (C) ai += __shfl_xor(ai, m, 64);
+  }
+  if (g == 0 && act) {
+    double* y = a.bw + S * (off + static_cast<int64_t>(T.y) * CD + li);
+    y[0] = ar;
+    if constexpr (C) y[1] = ai;
+  }
+}
+
+// FP32 blocks with columns padded to whole 16-byte row vectors (LD rows): a group of W lanes
+// (W = LD / RPL rounded up to a power of two; lanes past LD / RPL idle) owns a block
+template <bool C, int CD, int LD, int UB>
+__device__ __forceinline__ void coupling_wave_v_sym(int t, const CplArgs& a) {
+  constexpr int S = C ? 2 : 1;
+  constexpr int RPL = RowVec<C, float>::RPL;
+  constexpr int H = LD / RPL;
+  constexpr int W = pow2_at_least(H);
+  constexpr int NG = 64 / W;
+  constexpr int R = (CD + W - 1) / W;  // columns per lane after the column sums
+  const int lane = threadIdx.x & 63;
+  const int g = lane / W, l = lane % W;
+  const bool act = l < H;
+  const int lc = act ? l : 0;
+  const int2 T = a.tgt[t];
+  const int64_t off = a.lvl_off[T.x];
+  const int p0 = a.rp[t], p1 = a.rp[t + 1];
+  const float* __restrict__ F = static_cast<const float*>(a.F);
+  // x_t at this lane's rows
+  double tr[RPL], ti[RPL];
+  {
+    const double* xt = a.fw + S * (off + static_cast<int64_t>(T.y) * CD);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = RPL * l + q;
+      const bool in = act && row < CD;
+      tr[q] = in ? xt[S * (in ? row : 0)] : 0.0;
+      ti[q] = (C && in) ? xt[S * (in ? row : 0) + 1] : 0.0;
+    }
+  }
+  double yr[RPL], yi[C ? RPL : 1];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    yr[q] = 0.0;
+    if constexpr (C) yi[q] = 0.0;
+  }
+#pragma unroll 1
+  for (int p = p0 + g; p < p1; p += NG * UB) {
+    RowVec<C, float> fv[UB][CD];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int pu = p + NG * u < p1 ? p + NG * u : p;
+      const float* f = F + S * (static_cast<int64_t>(a.blk[pu]) * a.fstride + a.fcomp);
+#pragma unroll
+      for (int j = 0; j < CD; ++j) fv[u][j].load(f + S * (j * LD + RPL * lc));
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const bool ok = p + NG * u < p1;
+      const int pu = ok ? p + NG * u : p;
+      // F^T x_t first: this lane's rows of column j, summed over the group right away (xor
+      // all-reduce: every lane ends with the same bits); lane j % W keeps column j. The partials
+      // are stored before the row products (fewer registers held across the stores)
+      double cr[R], ci[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) cr[r] = 0.0, ci[r] = 0.0;
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+          const double fr = static_cast<double>(fv[u][j].r[q]);
+          if constexpr (C) {
+            const double fi = static_cast<double>(fv[u][j].i[q]);
+            sr += fr * tr[q] - fi * ti[q];
+            si += fr * ti[q] + fi * tr[q];
+          } else {
+            sr += fr * tr[q];
+          }
+        }
+#pragma unroll
+        for (int m = W / 2; m >= 1; m >>= 1) {
+          sr += __shfl_xor(sr, m, 64);
+          if constexpr (C) si += __shfl_xor(si, m, 64);
+        }
+        if (l == j % W) {
+          cr[j / W] = sr;
+          if constexpr (C) ci[j / W] = si;
+        }
+      }
+      if (ok) {
+        double* pp = a.part + S * static_cast<int64_t>(pu) * CD;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int col = r * W + l;
+          if (col < CD) {
+            pp[S * col] = cr[r];
+            if constexpr (C) pp[S * col + 1] = ci[r];
+          }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      const double* x = a.fw + S * (off + static_cast<int64_t>(a.src[pu]) * CD);
+      // row products in two column halves (half of the x values live at a time)
+      constexpr int JH = (CD + 1) / 2;
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        if (half) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = half * JH; j < (half ? CD : JH); ++j) {
+          const double xr = ok ? x[S * j] : 0.0;
+          const double xi = (C && ok) ? x[S * j + 1] : 0.0;
+#pragma unroll
+          for (int q = 0; q < RPL; ++q) {
+            const double fr = static_cast<double>(fv[u][j].r[q]);
+            if constexpr (C) {
+              const double fi = static_cast<double>(fv[u][j].i[q]);
+              yr[q] += fr * xr - fi * xi;
+              yi[q] += fr * xi + fi * xr;
+            } else {
+              yr[q] += fr * xr;
+            }
+          }
+        }
+      }
+    }
+  }
+  // the NG groups' row sums
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+#pragma unroll
+    for (int m = W; m < 64; m <<= 1) {
+      yr[q] += __shfl_xor(yr[q], m, 64);
+      if constexpr (C) yi[q] += __shfl_xor(yi[q], m, 64);
+    }
+  }
+  if (g == 0 && act) {
+    double* y = a.bw + S * (off + static_cast<int64_t>(T.y) * CD);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = RPL * l + q;
+      if (row >= CD) continue;
+      y[S * row] = yr[q];
+      if constexpr (C) y[S * row + 1] = yi[q];
+    }
+  }
+}
+
+// real FP64 blocks, two rows per lane
+template <int CD, int UB>
+__device__ __forceinline__ void coupling_wave_r2_sym(int t, const CplArgs& a) {
+  constexpr int H = CD / 2;
+  constexpr int NG = 64 / H;  // H is a power of two here
+  const int lane = threadIdx.x & 63;
+  const int g = lane / H, l = lane % H;
+  const int2 T = a.tgt[t];
+  const int64_t off = a.lvl_off[T.x];
+  const int p0 = a.rp[t], p1 = a.rp[t + 1];
+  const double* __restrict__ F = static_cast<const double*>(a.F);
+  const double* xt = a.fw + off + static_cast<int64_t>(T.y) * CD + 2 * l;
+  const double t0 = xt[0], t1 = xt[1];
+  double y0 = 0.0, y1 = 0.0;
+  for (int p = p0 + g; p < p1; p += NG * UB) {
+    double2 fv[UB][CD];
+    double xv[UB][CD];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const bool ok = p + NG * u < p1;
+      const int pu = ok ? p + NG * u : p;
+      const double* f = F + static_cast<int64_t>(a.blk[pu]) * a.fstride + a.fcomp;
+      const double* x = a.fw + off + static_cast<int64_t>(a.src[pu]) * CD;
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        fv[u][j] = *reinterpret_cast<const double2*>(f + j * CD + 2 * l);
+        xv[u][j] = ok ? x[j] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+#pragma unroll
+      for (int j = 0; j < CD; ++j) {
+        y0 += fv[u][j].x * xv[u][j];
+        y1 += fv[u][j].y * xv[u][j];
+      }
+      const bool ok = p + NG * u < p1;
+      const int pu = ok ? p + NG * u : p;
+      double c[2];
+      group_column_sums<H, 2>(l, [&](int j) { return fv[u][j].x * t0 + fv[u][j].y * t1; }, c);
+      if (ok) {
+        double* pp = a.part + static_cast<int64_t>(pu) * CD + 2 * l;
+        pp[0] = c[0];
+        pp[1] = c[1];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = H; m < 64; m <<= 1) {
+    y0 += __shfl_xor(y0, m, 64);
+    y1 += __shfl_xor(y1, m, 64);
+  }
+  if (g == 0) {
+    double* y = a.bw + off + static_cast<int64_t>(T.y) * CD + 2 * l;
+    y[0] = y0;
+    y[1] = y1;
+  }
+}
+
+// bw[s] += the partials of the upper blocks (t, s), in their list order: one thread per
+// (target, entry of its S*CD doubles); four partial loads in flight, added in list order
+template <bool C, int CD>
+__global__ __launch_bounds__(256) void k_sym_gather(int ng, const int2* __restrict__ gtgt, const int* __restrict__ grp,
+                                                    const int* __restrict__ gslot, const double* __restrict__ part,
+                                                    const int64_t* __restrict__ lvl_off, double* bw) {
+  constexpr int S = C ? 2 : 1;
+  constexpr int E = S * CD;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= static_cast<int64_t>(ng) * E) return;
+  const int w = static_cast<int>(gid / E), e = static_cast<int>(gid - static_cast<int64_t>(w) * E);
+  const int2 T = gtgt[w];
+  double* y = bw + S * (lvl_off[T.x] + static_cast<int64_t>(T.y) * CD) + e;
+  const int q0 = grp[w], q1 = grp[w + 1];
+  double s = *y;
+  int q = q0;
+  for (; q + 4 <= q1; q += 4) {
+    const double v0 = part[static_cast<int64_t>(gslot[q]) * E + e];
+    const double v1 = part[static_cast<int64_t>(gslot[q + 1]) * E + e];
+    const double v2 = part[static_cast<int64_t>(gslot[q + 2]) * E + e];
+    const double v3 = part[static_cast<int64_t>(gslot[q + 3]) * E + e];
+    s += v0;
+    s += v1;
+    s += v2;
+    s += v3;
+  }
+  for (; q < q1; ++q) s += part[static_cast<int64_t>(gslot[q]) * E + e];
+  *y = s;
+}
+
 // the two sweeps of a single-RHS product in one launch: workgroups [0, nb_near) run the near
 // field, the rest the coupling blocks, so both streams of leaves share the memory system
-template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT = double>
+template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT = double, bool SYM = false>
 __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, int nb_near) {
   const int b = static_cast<int>(blockIdx.x);
   const int w = static_cast<int>(threadIdx.x >> 6);
@@ -1086,7 +1444,15 @@ __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, in
   } else {
     const int t = __builtin_amdgcn_readfirstlane((b - nb_near) * 4 + w);
     if (t < ca.ntgt) {
-      if constexpr (R2C)
+      if constexpr (SYM) {
+        // upper blocks only (the launcher checks the padded FP32 layout)
+        if constexpr (R2C)
+          coupling_wave_r2_sym<CD, UBC>(t, ca);
+        else if constexpr (VC)
+          coupling_wave_v_sym<C, CD, LDV, UBC>(t, ca);
+        else
+          coupling_wave_sym<C, CD, UBC>(t, ca);
+      } else if constexpr (R2C)
         coupling_wave_r2<CD, UBC>(t, ca);
       else if constexpr (VC) {
         if (ca.ld == LDV && ca.Fc == nullptr) {
@@ -1985,13 +2351,66 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
   BMI_CHECK_HIP(hipGetLastError());
 }
 
+static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd,
+                                   const LeafArgs& la, int cd, const void* cF, int ld, const int64_t* lvl_off,
+                                   const double* fw, double* bw, bool near32, const SymList& sl, hipStream_t st) {
+  CplArgs ca{sl.ntgt, 1, sl.tgt, sl.rp, sl.src, cF, static_cast<int64_t>(ld) * cd, 0, lvl_off, fw, bw, 0, nullptr,
+             0, ld, sl.blk, sl.part};
+  const int nbn = (ncl + 3) / 4, nbc = (sl.ntgt + 3) / 4;
+  const dim3 grid(static_cast<unsigned>(nbn + nbc));
+  bool launched = false;
+  {
+    TimedScope ts("matvec_sweep", st);
+#define BMI_FUSED_S(CC, DDD, FTT, CDD, UN, UC, NFTT)                                                                 \
+  if (!launched && cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&               \
+      near32 == std::is_same<NFTT, float>::value && leaf_variant == UN - 1 && cpl_variant == UC - 1) {              \
+    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC, NFTT, true>), grid, dim3(256), 0, st, la, ca, nbn); \
+    BMI_CHECK_HIP(hipGetLastError());                                                                                \
+    launched = true;                                                                                                 \
+  }
+#define BMI_FUSED_SV(CC, DDD, FTT, CDD, NFTT)                                                   \
+  BMI_FUSED_S(CC, DDD, FTT, CDD, 1, 1, NFTT) BMI_FUSED_S(CC, DDD, FTT, CDD, 1, 2, NFTT)         \
+  BMI_FUSED_S(CC, DDD, FTT, CDD, 2, 1, NFTT) BMI_FUSED_S(CC, DDD, FTT, CDD, 2, 2, NFTT)
+    BMI_FUSED_SV(false, 36, double, 16, double)
+    BMI_FUSED_SV(true, 36, double, 25, double)
+    BMI_FUSED_SV(false, 36, float, 16, float)
+    BMI_FUSED_SV(true, 36, float, 25, float)
+#undef BMI_FUSED_SV
+#undef BMI_FUSED_S
+  }
+  if (!launched) return false;
+  if (sl.ngather > 0) {
+    TimedScope ts("matvec_sym_gather", st);
+    const dim3 gg(static_cast<unsigned>(nblk(static_cast<int64_t>(sl.ngather) * (cplx ? 2 : 1) * cd)));
+    if (cplx && cd == 25)
+      hipLaunchKernelGGL((k_sym_gather<true, 25>), gg, dim3(256), 0, st, sl.ngather, sl.gtgt, sl.grp, sl.gslot,
+                         sl.part, lvl_off, bw);
+    else if (!cplx && cd == 16)
+      hipLaunchKernelGGL((k_sym_gather<false, 16>), gg, dim3(256), 0, st, sl.ngather, sl.gtgt, sl.grp, sl.gslot,
+                         sl.part, lvl_off, bw);
+    else
+      throw Error(BMI_UNSUPPORTED, "symmetric gather not instantiated");
+    BMI_CHECK_HIP(hipGetLastError());
+  }
+  return true;
+}
+
+bool sym_sweep_supported(bool cplx, int dd, int cd) { return dd == 36 && ((cplx && cd == 25) || (!cplx && cd == 16)); }
+
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
                         const int* dsrc, const void* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
                         const double* fw, double* bw, hipStream_t st, const unsigned short* Fc, int ncoarse,
-                        bool near32, int ld) {
+                        bool near32, int ld, const SymList* sym) {
   const LeafArgs la{ncl, 1, drp, dsrc, dF, static_cast<int64_t>(dd) * dd, 0, xs, 0, 0, ys, 0, 0, 0};
   if (ld <= 0) ld = cd;
+  if (sym) {
+    // upper blocks only: FP64 blocks, or the FP32 copy padded to whole row vectors
+    const int rpl = fp32 ? (cplx ? 2 : 4) : 1;
+    if (Fc || (fp32 && ld != (cd + rpl - 1) / rpl * rpl)) return false;
+    return launch_sweep_fused_sym(cplx, fp32, leaf_variant, cpl_variant, ncl, dd, la, cd, cF, ld, lvl_off, fw, bw,
+                                  near32, *sym, st);
+  }
   const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(ld) * cd, 0, lvl_off, fw, bw, 0, Fc, ncoarse, ld};
   const int nbn = (ncl + 3) / 4, nbc = (ntgt + 3) / 4;
   const dim3 grid(static_cast<unsigned>(nbn + nbc));

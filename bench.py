@@ -506,12 +506,19 @@ def main():
     rows = info["coupling_dim"]
     L = M - 1
     lvl_total = 6 * (4 ** (L + 1) - 1) // 3 * rows
-    # algorithmic bytes of the fused sweep: every stored leaf once, x read and y written once,
-    # the multipole vectors read and the local expansions written once
-    sw_bytes = (info["dense"] * dd * dd + info["lowrank"] * rows * rows) * 8 + 2 * nsuper * 8 + 2 * lvl_total * 8
+    # algorithmic bytes of the fused sweep: every near-field leaf once, every coupling block once
+    # (or, when the autotuner chose the symmetric sweep, the upper half of the coupling blocks plus
+    # one transposed partial vector written per upper block), x read and y written once, the
+    # multipole vectors read and the local expansions written once
+    sg_ms, sg_n = B.timing.get("matvec_sym_gather")
+    sym_sweep = sg_n > 0
+    ncpl = info["lowrank"] // 2 if sym_sweep else info["lowrank"]
+    sw_bytes = ((info["dense"] * dd * dd + ncpl * rows * rows) * 8 + 2 * nsuper * 8 + 2 * lvl_total * 8 +
+                (ncpl * rows * 8 if sym_sweep else 0))
     achieved = sw_bytes / (sw_avg * 1e-3) / 1e9 if sw_avg > 0 else 0.0
-    log("sweep (near field + coupling) %.3f ms/launch, %.1f MB -> %.0f GB/s; forward %.3f ms, backward %.3f ms" % (
-        sw_avg, sw_bytes / 1e6, achieved, fw_ms / max(fw_n, 1), bw_ms / max(bw_n, 1)))
+    log("sweep (near field + coupling%s) %.3f ms/launch, %.1f MB -> %.0f GB/s; forward %.3f ms, backward %.3f ms" % (
+        ", upper coupling blocks" if sym_sweep else "", sw_avg, sw_bytes / 1e6, achieved, fw_ms / max(fw_n, 1),
+        bw_ms / max(bw_n, 1)))
 
     # near-field quadrature roofline (FP64)
     pairs_by_class = info["pair_classes"]
@@ -613,7 +620,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(M, P, npts),
                      "kernel": "k_sweep_fused (near-field leaves + coupling blocks of the matvec)",
-                     "bytes_per_launch": sw_bytes, "avg_launch_ms": sw_avg},
+                     "bytes_per_launch": sw_bytes, "avg_launch_ms": sw_avg, "symmetric_coupling": sym_sweep,
+                     "sym_gather_ms": sg_ms / max(sg_n, 1) if sym_sweep else 0.0},
         "roofline_assembly": {"bound": "fp64", "achieved": duffy_tflops, "peak": FP64_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": duffy_tflops / FP64_PEAK_TFLOPS,
                               "kernel": "k_nearfield identical/edge/vertex (Duffy)", "flops": fl,
