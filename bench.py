@@ -58,14 +58,15 @@ def nearfield_flops(classes_deg, P, pd):
     return tot
 
 
-def measured_traffic(M, P, npts):
+def measured_traffic(M, P, npts, upper):
     """HBM bytes per launch of the product sweep (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, scaled by a
-    1 GiB copy), as recorded under profiles/ for this workload; None for other sizes"""
+    1 GiB copy) of the chosen form (every coupling block, or the upper blocks), as recorded under
+    profiles/ for this workload; None for other sizes"""
     f = os.path.join(ROOT, "profiles", "k_sweep_fused_laplace_M%d_P%d_np%d_traffic.json" % (M, P, npts))
     if not os.path.exists(f):
         return None
     with open(f) as fh:
-        d = json.load(fh)
+        d = json.load(fh)["upper" if upper else "full"]
     return d["fetch_bytes"] + d["write_bytes"]
 
 
@@ -618,7 +619,7 @@ def main():
                   "fp64_p99_s": pct(lat64, 99), "mixed_p50_s": pct(latm, 50), "mixed_p99_s": pct(latm, 99),
                   "mixed_rel_err_vs_fp64": rel_err_mixed, "solves": a.solves},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(M, P, npts),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(M, P, npts, sym_sweep),
                      "kernel": "k_sweep_fused (near-field leaves + coupling blocks of the matvec)",
                      "bytes_per_launch": sw_bytes, "avg_launch_ms": sw_avg, "symmetric_coupling": sym_sweep,
                      "sym_gather_ms": sg_ms / max(sg_n, 1) if sym_sweep else 0.0},
