@@ -1111,8 +1111,10 @@ __host__ __device__ constexpr int pow2_at_least(int n) {
   return p;
 }
 
-// FP64 blocks, one row per lane (complex, or real sizes without the two-row form)
-template <bool C, int CD, int UB>
+// one row per lane: FP64 blocks (complex, or real sizes without the two-row form), or the
+// padded FP32 copy of complex blocks (column stride LD; 8-byte entries, half the registers of
+// the 16-byte row-vector form)
+template <bool C, int CD, int UB, typename FT = double, int LD = CD>
 __device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
   constexpr int S = C ? 2 : 1;
   constexpr int W = pow2_at_least(CD);
@@ -1125,20 +1127,26 @@ __device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
   const int2 T = a.tgt[t];
   const int64_t off = a.lvl_off[T.x];
   const int p0 = a.rp[t], p1 = a.rp[t + 1];
-  const double* __restrict__ F = static_cast<const double*>(a.F);
+  const FT* __restrict__ F = static_cast<const FT*>(a.F);
   const double* xt = a.fw + S * (off + static_cast<int64_t>(T.y) * CD + i);
   const double tr = act ? xt[0] : 0.0, ti = (C && act) ? xt[1] : 0.0;
   double ar = 0.0, ai = 0.0;
   for (int p = p0 + g; p < p1; p += G * UB) {
-    double fr[UB][CD], fi[UB][C ? CD : 1];
+    FT fr[UB][CD], fi[UB][C ? CD : 1];
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
       const int pu = p + G * u < p1 ? p + G * u : p;
-      const double* f = F + S * (static_cast<int64_t>(a.blk[pu]) * a.fstride + a.fcomp);
+      const FT* f = F + S * (static_cast<int64_t>(a.blk[pu]) * a.fstride + a.fcomp);
 #pragma unroll
       for (int j = 0; j < CD; ++j) {
-        fr[u][j] = f[S * (j * CD + i)];
-        if constexpr (C) fi[u][j] = f[S * (j * CD + i) + 1];
+        if constexpr (C && std::is_same<FT, float>::value) {
+          const float2 v = *reinterpret_cast<const float2*>(f + S * (j * LD + i));
+          fr[u][j] = v.x;
+          fi[u][j] = v.y;
+        } else {
+          fr[u][j] = f[S * (j * LD + i)];
+          if constexpr (C) fi[u][j] = f[S * (j * LD + i) + 1];
+        }
       }
     }
 #pragma unroll
@@ -1150,11 +1158,12 @@ __device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
       for (int j = 0; j < CD; ++j) {
         const double xr = ok ? x[S * j] : 0.0;
         const double xi = (C && ok) ? x[S * j + 1] : 0.0;
+        const double er = static_cast<double>(fr[u][j]), ei = C ? static_cast<double>(fi[u][C ? j : 0]) : 0.0;
         if constexpr (C) {
-          ar += fr[u][j] * xr - fi[u][j] * xi;
-          ai += fr[u][j] * xi + fi[u][j] * xr;
+          ar += er * xr - ei * xi;
+          ai += er * xi + ei * xr;
         } else {
-          ar += fr[u][j] * xr;
+          ar += er * xr;
         }
       }
       double cr[1], ci[1];
@@ -1162,7 +1171,8 @@ __device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
           li,
           [&](int j) {
             const int jj = j < CD ? j : 0;
-            return j < CD ? (C ? fr[u][jj] * tr - fi[u][C ? jj : 0] * ti : fr[u][jj] * tr) : 0.0;
+            const double er = static_cast<double>(fr[u][jj]), ei = C ? static_cast<double>(fi[u][C ? jj : 0]) : 0.0;
+            return j < CD ? (C ? er * tr - ei * ti : er * tr) : 0.0;
           },
           cr);
       if constexpr (C)
@@ -1170,7 +1180,7 @@ __device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
             li,
             [&](int j) {
               const int jj = j < CD ? j : 0;
-              return j < CD ? fr[u][jj] * ti + fi[u][jj] * tr : 0.0;
+              return j < CD ? static_cast<double>(fr[u][jj]) * ti + static_cast<double>(fi[u][jj]) * tr : 0.0;
             },
             ci);
       if (ok && act) {
@@ -1448,6 +1458,8 @@ __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, in
         // upper blocks only (the launcher checks the padded FP32 layout)
         if constexpr (R2C)
           coupling_wave_r2_sym<CD, UBC>(t, ca);
+        else if constexpr (VC && C)
+          coupling_wave_sym<C, CD, UBC, float, LDV>(t, ca);
         else if constexpr (VC)
           coupling_wave_v_sym<C, CD, LDV, UBC>(t, ca);
         else

@@ -30,59 +30,86 @@ void convert_to_fp32(const double* src, float* dst, size_t n, hipStream_t st) {
 }
 
 // Inner products of many vectors with one in two passes: block (chunk, column c) sums its rows
-// of conj(V_j) W for every j into part[((j * nrhs + c) * nchunk + chunk)], then one thread per
-// (j, c) adds the chunk sums in order (deterministic).  W's rows are read once per chunk for all j.
+// of conj(V_j) W for every j into part[((j * nrhs + c) * nchunk + chunk)], four vectors per pass
+// over the rows (independent loads in flight, one barrier per four), then one thread per (j, c)
+// adds the chunk sums in order (deterministic: the chunk count depends on the sizes only).
+// W's rows are read once per four vectors.
 constexpr int DOT_CHUNKS = 64;
+static inline int dot_chunks(int nrhs) { return nrhs <= 2 ? 2 * DOT_CHUNKS : DOT_CHUNKS; }
 template <bool C>
 __global__ __launch_bounds__(256) void k_dots_part(const double* __restrict__ V, int64_t vstride, int nv,
-                                                    const double* __restrict__ W, int64_t n, int nrhs,
+                                                    const double* __restrict__ W, int64_t n, int nrhs, int nchunk,
                                                     double* __restrict__ part) {
   constexpr int S = C ? 2 : 1;
-  __shared__ double red[2][4];
+  constexpr int JB = 4;
+  __shared__ double red[2][JB][4];
   const int chunk = blockIdx.x, c = blockIdx.y;
-  const int64_t per = (n + DOT_CHUNKS - 1) / DOT_CHUNKS;
+  const int64_t per = (n + nchunk - 1) / nchunk;
   const int64_t r0 = chunk * per, r1 = r0 + per < n ? r0 + per : n;
   const double* w = W + static_cast<int64_t>(S) * n * c;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = 0; j < nv; ++j) {
-    const double* v = V + j * vstride + static_cast<int64_t>(S) * n * c;
-    double ar = 0.0, ai = 0.0;
+  for (int j0 = 0; j0 < nv; j0 += JB) {
+    const int nj = nv - j0 < JB ? nv - j0 : JB;
+    double ar[JB], ai[JB];
+#pragma unroll
+    for (int q = 0; q < JB; ++q) ar[q] = 0.0, ai[q] = 0.0;
     for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-      if (C) {
-        const double vr = v[2 * i], vi = v[2 * i + 1], wr = w[2 * i], wi = w[2 * i + 1];
-        ar += vr * wr + vi * wi;
-        ai += vr * wi - vi * wr;
-      } else {
-        ar += v[i] * w[i];
+      const double wr = w[S * i], wi = C ? w[S * i + 1] : 0.0;
+#pragma unroll
+      for (int q = 0; q < JB; ++q) {
+        if (q < nj) {
+          const double* v = V + (j0 + q) * vstride + static_cast<int64_t>(S) * n * c;
+          if (C) {
+            const double vr = v[2 * i], vi = v[2 * i + 1];
+            ar[q] += vr * wr + vi * wi;
+            ai[q] += vr * wi - vi * wr;
+          } else {
+            ar[q] += v[i] * wr;
+          }
+        }
       }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      ar += __shfl_xor(ar, off, 64);
-      if (C) ai += __shfl_xor(ai, off, 64);
+    for (int q = 0; q < JB; ++q) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        ar[q] += __shfl_xor(ar[q], off, 64);
+        if (C) ai[q] += __shfl_xor(ai[q], off, 64);
+      }
+      if (lane == 0) red[0][q][wave] = ar[q], red[1][q][wave] = ai[q];
     }
-    if (lane == 0) red[0][wave] = ar, red[1][wave] = ai;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      const int64_t o = (static_cast<int64_t>(j) * nrhs + c) * DOT_CHUNKS + chunk;
-      part[S * o] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
-      if (C) part[S * o + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    if (threadIdx.x < nj) {
+      const int q = threadIdx.x;
+      const int64_t o = (static_cast<int64_t>(j0 + q) * nrhs + c) * nchunk + chunk;
+      part[S * o] = (red[0][q][0] + red[0][q][1]) + (red[0][q][2] + red[0][q][3]);
+      if (C) part[S * o + 1] = (red[1][q][0] + red[1][q][1]) + (red[1][q][2] + red[1][q][3]);
     }
     __syncthreads();
   }
 }
+// one wave per inner product: lane l adds chunks l, l + 64, ... in order, then a fixed shuffle tree
 template <bool C>
-__global__ void k_dots_final(const double* __restrict__ part, int count, double* out) {
+__global__ __launch_bounds__(256) void k_dots_final(const double* __restrict__ part, int count, int nchunk,
+                                                     double* out) {
   constexpr int S = C ? 2 : 1;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
   if (q >= count) return;
+  const int lane = threadIdx.x & 63;
   double ar = 0.0, ai = 0.0;
-  for (int k = 0; k < DOT_CHUNKS; ++k) {
-    ar += part[S * (static_cast<int64_t>(q) * DOT_CHUNKS + k)];
-    if (C) ai += part[S * (static_cast<int64_t>(q) * DOT_CHUNKS + k) + 1];
+  for (int k = lane; k < nchunk; k += 64) {
+    ar += part[S * (static_cast<int64_t>(q) * nchunk + k)];
+    if (C) ai += part[S * (static_cast<int64_t>(q) * nchunk + k) + 1];
   }
-  out[S * q] = ar;
-  if (C) out[S * q + 1] = ai;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    ar += __shfl_xor(ar, off, 64);
+    if (C) ai += __shfl_xor(ai, off, 64);
+  }
+  if (lane == 0) {
+    out[S * q] = ar;
+    if (C) out[S * q + 1] = ai;
+  }
 }
 
 // W[i, c] -= sum_{j < nv} V_j[i, c] * h[j, c]
@@ -167,19 +194,22 @@ struct Vecs {
     BMI_CHECK_HIP(hipStreamSynchronize(st));
   }
   void dots_async(const double* V, int nv, const double* W, double* d_out) {
-    const size_t need = static_cast<size_t>(S()) * nv * nrhs * DOT_CHUNKS;
+    const int nchunk = dot_chunks(nrhs);
+    const size_t need = static_cast<size_t>(S()) * nv * nrhs * nchunk;
     if (part.n < need) {
       BMI_CHECK_HIP(hipStreamSynchronize(st));  // earlier launches may still use the old buffer
-      part.alloc(std::max(need, std::max(2 * part.n, static_cast<size_t>(S()) * 32 * nrhs * DOT_CHUNKS)));
+      part.alloc(std::max(need, std::max(2 * part.n, static_cast<size_t>(S()) * 32 * nrhs * nchunk)));
     }
-    const dim3 grid(DOT_CHUNKS, static_cast<unsigned>(nrhs));
+    const dim3 grid(static_cast<unsigned>(nchunk), static_cast<unsigned>(nrhs));
     const int count = nv * nrhs;
     if (C) {
-      hipLaunchKernelGGL(k_dots_part<true>, grid, dim3(256), 0, st, V, block(), nv, W, n, nrhs, part.ptr);
-      hipLaunchKernelGGL(k_dots_final<true>, dim3((count + 63) / 64), dim3(64), 0, st, part.ptr, count, d_out);
+      hipLaunchKernelGGL(k_dots_part<true>, grid, dim3(256), 0, st, V, block(), nv, W, n, nrhs, nchunk, part.ptr);
+      hipLaunchKernelGGL(k_dots_final<true>, dim3((count + 3) / 4), dim3(256), 0, st, part.ptr, count, nchunk,
+                         d_out);
     } else {
-      hipLaunchKernelGGL(k_dots_part<false>, grid, dim3(256), 0, st, V, block(), nv, W, n, nrhs, part.ptr);
-      hipLaunchKernelGGL(k_dots_final<false>, dim3((count + 63) / 64), dim3(64), 0, st, part.ptr, count, d_out);
+      hipLaunchKernelGGL(k_dots_part<false>, grid, dim3(256), 0, st, V, block(), nv, W, n, nrhs, nchunk, part.ptr);
+      hipLaunchKernelGGL(k_dots_final<false>, dim3((count + 3) / 4), dim3(256), 0, st, part.ptr, count, nchunk,
+                         d_out);
     }
     BMI_CHECK_HIP(hipGetLastError());
   }
