@@ -562,17 +562,19 @@ static void build_symmetric_lists(bmi_h2* h, const std::vector<int2>& tgt, const
       urp.back()++;
     }
   }
-  // gather lists, column targets in (level, cluster) order, entries in list order
+  // gather order: column targets in (level, cluster) order, entries in list order; each entry's
+  // partial is stored at its position in this order
   std::stable_sort(ucol.begin(), ucol.end(), [](const int2& a, const int2& b) { return a.x < b.x; });
   std::vector<int2> gtgt;
-  std::vector<int> grp(1, 0), gslot;
-  for (const int2& e : ucol) {
+  std::vector<int> grp(1, 0), gslot(usrc.size());
+  for (size_t q = 0; q < ucol.size(); ++q) {
+    const int2& e = ucol[q];
     if (gtgt.empty() || gtgt.back().x != tgt[e.x].x || gtgt.back().y != tgt[e.x].y) {
       gtgt.push_back(tgt[e.x]);
       grp.push_back(grp.back());
     }
     grp.back()++;
-    gslot.push_back(e.y);
+    gslot[e.y] = static_cast<int>(q);
   }
   const int S = h->cplx ? 2 : 1;
   h->d_stgt.upload(utgt, st);
@@ -584,7 +586,7 @@ static void build_symmetric_lists(bmi_h2* h, const std::vector<int2>& tgt, const
   h->d_sgslot.upload(gslot, st);
   h->d_part.alloc(static_cast<size_t>(S) * usrc.size() * h->cd);
   h->sl = SymList{static_cast<int>(utgt.size()), h->d_stgt.ptr, h->d_srp.ptr, h->d_ssrc.ptr, h->d_sblk.ptr,
-                  h->d_part.ptr, static_cast<int>(gtgt.size()), h->d_sgtgt.ptr, h->d_sgrp.ptr, h->d_sgslot.ptr};
+                  h->d_part.ptr, h->d_sgslot.ptr, static_cast<int>(gtgt.size()), h->d_sgtgt.ptr, h->d_sgrp.ptr};
   h->sym = true;
   BMI_CHECK_HIP(hipStreamSynchronize(st));
 }

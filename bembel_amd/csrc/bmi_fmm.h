@@ -84,11 +84,11 @@ struct SymList {
   const int* rp;
   const int* src;
   const int* blk;
-  double* part;  // per entry: B^T x_t (S * cd doubles)
-  int ngather;   // targets receiving partials, their CSR over entries
+  double* part;      // B^T x_t of every entry (S * cd doubles), stored in gather order
+  const int* slot;   // per entry: its slot in part
+  int ngather;       // targets receiving partials and the ranges of their slots
   const int2* gtgt;
   const int* grp;
-  const int* gslot;
 };
 
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,

@@ -725,9 +725,11 @@ struct CplArgs {
   const unsigned short* Fc;  // BF16 copies of the first ncoarse blocks (coarsest levels), or null
   int ncoarse;
   int ld;  // column stride of F blocks (cd, or cd padded to whole 16-byte row vectors for FP32)
-  // symmetric sweeps: block index of each list entry and the per-entry partial vectors B^T x_t
+  // symmetric sweeps: block index of each list entry, the partial vectors B^T x_t and the slot of
+  // each entry's partial (its position in the gather order)
   const int* blk;
   double* part;
+  const int* slot;
 };
 
 __device__ __forceinline__ double bf16_to_double(unsigned short b) {
@@ -1184,7 +1186,7 @@ __device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
             },
             ci);
       if (ok && act) {
-        double* pp = a.part + S * (static_cast<int64_t>(pu) * CD + li);
+        double* pp = a.part + S * (static_cast<int64_t>(a.slot[pu]) * CD + li);
         pp[0] = cr[0];
         if constexpr (C) pp[1] = ci[0];
       }
@@ -1283,7 +1285,7 @@ __device__ __forceinline__ void coupling_wave_v_sym(int t, const CplArgs& a) {
         }
       }
       if (ok) {
-        double* pp = a.part + S * static_cast<int64_t>(pu) * CD;
+        double* pp = a.part + S * static_cast<int64_t>(a.slot[pu]) * CD;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int col = r * W + l;
@@ -1381,7 +1383,7 @@ __device__ __forceinline__ void coupling_wave_r2_sym(int t, const CplArgs& a) {
       double c[2];
       group_column_sums<H, 2>(l, [&](int j) { return fv[u][j].x * t0 + fv[u][j].y * t1; }, c);
       if (ok) {
-        double* pp = a.part + static_cast<int64_t>(pu) * CD + 2 * l;
+        double* pp = a.part + static_cast<int64_t>(a.slot[pu]) * CD + 2 * l;
         pp[0] = c[0];
         pp[1] = c[1];
       }
@@ -1399,11 +1401,11 @@ __device__ __forceinline__ void coupling_wave_r2_sym(int t, const CplArgs& a) {
   }
 }
 
-// bw[s] += the partials of the upper blocks (t, s), in their list order: one thread per
-// (target, entry of its S*CD doubles); four partial loads in flight, added in list order
+// bw[s] += the partials of the upper blocks (t, s), stored contiguously in gather order: one
+// thread per (target, entry of its S*CD doubles), independent loads, added in list order
 template <bool C, int CD>
 __global__ __launch_bounds__(256) void k_sym_gather(int ng, const int2* __restrict__ gtgt, const int* __restrict__ grp,
-                                                    const int* __restrict__ gslot, const double* __restrict__ part,
+                                                    const double* __restrict__ part,
                                                     const int64_t* __restrict__ lvl_off, double* bw) {
   constexpr int S = C ? 2 : 1;
   constexpr int E = S * CD;
@@ -1413,19 +1415,18 @@ __global__ __launch_bounds__(256) void k_sym_gather(int ng, const int2* __restri
   const int2 T = gtgt[w];
   double* y = bw + S * (lvl_off[T.x] + static_cast<int64_t>(T.y) * CD) + e;
   const int q0 = grp[w], q1 = grp[w + 1];
+  const double* pp = part + static_cast<int64_t>(q0) * E + e;
   double s = *y;
-  int q = q0;
-  for (; q + 4 <= q1; q += 4) {
-    const double v0 = part[static_cast<int64_t>(gslot[q]) * E + e];
-    const double v1 = part[static_cast<int64_t>(gslot[q + 1]) * E + e];
-    const double v2 = part[static_cast<int64_t>(gslot[q + 2]) * E + e];
-    const double v3 = part[static_cast<int64_t>(gslot[q + 3]) * E + e];
+  int q = 0;
+  for (; q + 4 <= q1 - q0; q += 4) {
+    const double v0 = pp[static_cast<int64_t>(q) * E], v1 = pp[static_cast<int64_t>(q + 1) * E];
+    const double v2 = pp[static_cast<int64_t>(q + 2) * E], v3 = pp[static_cast<int64_t>(q + 3) * E];
     s += v0;
     s += v1;
     s += v2;
     s += v3;
   }
-  for (; q < q1; ++q) s += part[static_cast<int64_t>(gslot[q]) * E + e];
+  for (; q < q1 - q0; ++q) s += pp[static_cast<int64_t>(q) * E];
   *y = s;
 }
 
@@ -2367,7 +2368,7 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
                                    const LeafArgs& la, int cd, const void* cF, int ld, const int64_t* lvl_off,
                                    const double* fw, double* bw, bool near32, const SymList& sl, hipStream_t st) {
   CplArgs ca{sl.ntgt, 1, sl.tgt, sl.rp, sl.src, cF, static_cast<int64_t>(ld) * cd, 0, lvl_off, fw, bw, 0, nullptr,
-             0, ld, sl.blk, sl.part};
+             0, ld, sl.blk, sl.part, sl.slot};
   const int nbn = (ncl + 3) / 4, nbc = (sl.ntgt + 3) / 4;
   const dim3 grid(static_cast<unsigned>(nbn + nbc));
   bool launched = false;
@@ -2395,11 +2396,11 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
     TimedScope ts("matvec_sym_gather", st);
     const dim3 gg(static_cast<unsigned>(nblk(static_cast<int64_t>(sl.ngather) * (cplx ? 2 : 1) * cd)));
     if (cplx && cd == 25)
-      hipLaunchKernelGGL((k_sym_gather<true, 25>), gg, dim3(256), 0, st, sl.ngather, sl.gtgt, sl.grp, sl.gslot,
-                         sl.part, lvl_off, bw);
+      hipLaunchKernelGGL((k_sym_gather<true, 25>), gg, dim3(256), 0, st, sl.ngather, sl.gtgt, sl.grp, sl.part,
+                         lvl_off, bw);
     else if (!cplx && cd == 16)
-      hipLaunchKernelGGL((k_sym_gather<false, 16>), gg, dim3(256), 0, st, sl.ngather, sl.gtgt, sl.grp, sl.gslot,
-                         sl.part, lvl_off, bw);
+      hipLaunchKernelGGL((k_sym_gather<false, 16>), gg, dim3(256), 0, st, sl.ngather, sl.gtgt, sl.grp, sl.part,
+                         lvl_off, bw);
     else
       throw Error(BMI_UNSUPPORTED, "symmetric gather not instantiated");
     BMI_CHECK_HIP(hipGetLastError());
