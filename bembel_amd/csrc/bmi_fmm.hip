@@ -1895,6 +1895,159 @@ __global__ __launch_bounds__(256) void k_multi_mfma(MultiArgs a) {
     }
 }
 
+// FP32 blocks of a reduced operator on the FP32 matrix cores (v_mfma_f32_16x16x4f32, twice the
+// FP64 rate): the block and its x tile (rounded to FP32) are staged in LDS; each block's product
+// is accumulated in FP32 from zero and then added to FP64 sums, so only the rows x columns of one
+// block (at most RIN terms) are summed in FP32 and the sum over the owner's blocks is FP64.
+// Operand layout: A(i, k) on lane i + 16k, B(k, j) on lane j + 16k, D(i, j) on lane
+// j + 16 (i / 4), element i % 4 (checked on the device).
+typedef float bmi_f4 __attribute__((ext_vector_type(4)));
+
+template <bool C, int ROUT, int RIN>
+__global__ __launch_bounds__(256) void k_multi_mfma32(MultiArgs a) {
+  constexpr int S = C ? 2 : 1;
+  constexpr int MT = (ROUT + 15) / 16, KS = (RIN + 3) / 4;
+  constexpr int FLD = MT * 16;
+  constexpr int LDMAX = (ROUT + 3) / 4 * 4;
+  constexpr int NFE = (LDMAX * RIN + 255) / 256;
+  constexpr int NXE = (RIN * 64 + 255) / 256;
+  __shared__ float Fs[S * KS * 4 * FLD];
+  __shared__ float Xs[S * KS * 4 * 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = blockIdx.x;
+  const int k0 = static_cast<int>(blockIdx.y) * 64;
+  const int kmax = a.nrhs - 1 - k0;
+  int cl = r;
+  int64_t xo = a.xoff, yo = a.yoff;
+  if (a.tgt) {
+    const int2 T = a.tgt[r];
+    cl = T.y;
+    const int64_t lo = a.lvl_off[T.x] * a.nrhs;
+    xo += lo;
+    yo += lo;
+  }
+  const int p0 = a.rp[r], p1 = a.rp[r + 1];
+  for (int u = tid; u < S * KS * 4 * FLD; u += 256) Fs[u] = 0.0f;
+  for (int u = tid; u < S * KS * 4 * 64; u += 256) Xs[u] = 0.0f;
+  const int ldf = a.ldf;
+  const int fe = ldf * RIN;
+  const float* __restrict__ F = static_cast<const float*>(a.F);
+  float pfr[NFE], pfi[C ? NFE : 1];
+  float pxr[NXE], pxi[C ? NXE : 1];
+  auto fetch = [&](int p) {
+    const float* f = F + S * (static_cast<int64_t>(p) * a.fstride + a.fcomp);
+#pragma unroll
+    for (int q = 0; q < NFE; ++q) {
+      const int e = tid + 256 * q;
+      if (e < fe) {
+        pfr[q] = f[S * e];
+        if constexpr (C) pfi[q] = f[S * e + 1];
+      }
+    }
+    const double* x = a.x + S * (xo + static_cast<int64_t>(a.src[p]) * a.xcs);
+#pragma unroll
+    for (int q = 0; q < NXE; ++q) {
+      const int e = tid + 256 * q;
+      if ((RIN * 64) % 256 == 0 || e < RIN * 64) {
+        const int j = e >> 6;
+        int kl = e & 63;
+        kl = kl <= kmax ? kl : kmax;
+        const double* xe = x + S * (static_cast<int64_t>(j) * a.xrs + static_cast<int64_t>(k0 + kl) * a.xks);
+        pxr[q] = static_cast<float>(xe[0]);
+        if constexpr (C) pxi[q] = static_cast<float>(xe[1]);
+      }
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int q = 0; q < NFE; ++q) {
+      const int e = tid + 256 * q;
+      if (e < fe) {
+        const int c = e / ldf, i = e - c * ldf;
+        if (i < ROUT) {
+          Fs[S * (c * FLD + i)] = pfr[q];
+          if constexpr (C) Fs[S * (c * FLD + i) + 1] = pfi[q];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NXE; ++q) {
+      const int e = tid + 256 * q;
+      if ((RIN * 64) % 256 == 0 || e < RIN * 64) {
+        Xs[S * e] = pxr[q];
+        if constexpr (C) Xs[S * e + 1] = pxi[q];
+      }
+    }
+  };
+  double sr[MT][4], si[C ? MT : 1][4];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      sr[m][v] = 0.0;
+      if constexpr (C) si[m][v] = 0.0;
+    }
+  const int li = lane & 15, lk = lane >> 4;
+  const int n = w * 16 + li;
+  const bmi_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (p0 < p1) fetch(p0);
+  for (int p = p0; p < p1; ++p) {
+    __syncthreads();
+    commit();
+    __syncthreads();
+    if (p + 1 < p1) fetch(p + 1);
+    bmi_f4 ar[MT], ai[C ? MT : 1];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      ar[m] = zero;
+      if constexpr (C) ai[m] = zero;
+    }
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int kr = ks * 4 + lk;
+      const float br = Xs[S * (kr * 64 + n)];
+      const float bi = C ? Xs[S * (kr * 64 + n) + 1] : 0.0f;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const float fr = Fs[S * (kr * FLD + m * 16 + li)];
+        if constexpr (C) {
+          const float fi = Fs[S * (kr * FLD + m * 16 + li) + 1];
+          ar[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr, br, ar[m], 0, 0, 0);
+          ai[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr, bi, ai[m], 0, 0, 0);
+          ar[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(-fi, bi, ar[m], 0, 0, 0);
+          ai[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(fi, br, ai[m], 0, 0, 0);
+        } else {
+          ar[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr, br, ar[m], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        sr[m][v] += static_cast<double>(ar[m][v]);
+        if constexpr (C) si[m][v] += static_cast<double>(ai[m][v]);
+      }
+  }
+  if (n > kmax) return;
+  double* y = a.y + S * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k0 + n) * a.yks);
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = m * 16 + 4 * lk + v;
+      if (ROUT % 16 != 0 && row >= ROUT) continue;
+      double vr = sr[m][v], vi = C ? si[m][v] : 0.0;
+      double* yr = y + S * static_cast<int64_t>(row) * a.yrs;
+      if (a.accumulate) {
+        vr += yr[0];
+        if constexpr (C) vi += yr[1];
+      }
+      yr[0] = vr;
+      if constexpr (C) yr[1] = vi;
+    }
+}
+
 // ------------------------------------------------------------------ RHS-fastest layout (many RHS)
 // With many right-hand sides the internal vectors keep the RHS index fastest, so that a wave's 64
 // lanes (64 right-hand sides) read and write 64 consecutive entries:
@@ -2587,6 +2740,35 @@ static bool launch_mfma(bool xc, bool fc, bool fp32, int rout, int rin, const Mu
   return false;
 }
 
+// FP32 blocks on the FP32 matrix cores (block sums FP32, sums over blocks FP64)
+static bool launch_mfma32(bool cplx, int rows, const MultiArgs& ma, dim3 grid, hipStream_t st) {
+#define BMI_MF32(CC, R)                                                             \
+  if (cplx == CC && rows == R) {                                                  \
+    hipLaunchKernelGGL((k_multi_mfma32<CC, R, R>), grid, dim3(256), 0, st, ma);   \
+    BMI_CHECK_HIP(hipGetLastError());                                             \
+    return true;                                                                  \
+  }
+  BMI_MF32(false, 16)
+  BMI_MF32(false, 25)
+  BMI_MF32(false, 36)
+  BMI_MF32(true, 16)
+  BMI_MF32(true, 25)
+  BMI_MF32(true, 32)
+  BMI_MF32(true, 36)
+#undef BMI_MF32
+  return false;
+}
+
+// FP32 blocks multiplied in FP32 on the matrix cores (default) or widened to FP64
+// (BMI_MULTI_MATH64=1, the accuracy of the single-RHS reduced products)
+static bool multi_math32() {
+  static const bool on = [] {
+    const char* v = std::getenv("BMI_MULTI_MATH64");
+    return !(v && v[0] == '1');
+  }();
+  return on;
+}
+
 // block products of the RHS-fastest sweeps: 1 = matrix cores, 0 = vector units (BMI_MULTI_VECTOR=1)
 static int multi_engine() {
   static const int e = [] {
@@ -2684,6 +2866,7 @@ bool launch_multi_near_t(bool cplx, bool fp32, int ncl, int nrhs, int dd, const 
                      ys_t, yseg * nrhs, static_cast<int64_t>(dd) * nrhs, 1, nrhs, accumulate, 0};
   TimedScope ts("matvec_nearfield", st);
   const dim3 grid(static_cast<unsigned>(ncl), static_cast<unsigned>((nrhs + 63) / 64), 1);
+  if (multi_engine() == 1 && fp32 && multi_math32()) return launch_mfma32(cplx, dd, ma, grid, st);
   if (multi_engine() == 1) return launch_mfma(cplx, cplx, fp32, dd, dd, ma, grid, st);
   if (!multi_lds_fits(cplx, fp32, dd, dd, dd, fstride, fcomp)) return false;
   return launch_lds(cplx, cplx, fp32, dd, dd, ma, grid, st);
@@ -2697,8 +2880,10 @@ bool launch_multi_cpl_t(bool cplx, bool fp32, int ntgt, int nrhs, int cd, int ld
                      bw_t, 0, static_cast<int64_t>(cd) * nrhs, 1, nrhs, accumulate, 0};
   TimedScope ts("coupling_apply", st);
   const dim3 grid(static_cast<unsigned>(ntgt), static_cast<unsigned>((nrhs + 63) / 64), 1);
-  if (multi_engine() == 1 && ld >= cd && ld <= (cd + 3) / 4 * 4)
+  if (multi_engine() == 1 && ld >= cd && ld <= (cd + 3) / 4 * 4) {
+    if (fp32 && multi_math32()) return launch_mfma32(cplx, cd, ma, grid, st);
     return launch_mfma(cplx, cplx, fp32, cd, cd, ma, grid, st);
+  }
   if (!multi_lds_fits(cplx, fp32, cd, cd, ld, fstride, fcomp)) return false;
   return launch_lds(cplx, cplx, fp32, cd, cd, ma, grid, st);
 }

@@ -208,6 +208,11 @@ def test_gpu_mixed_precision_matvec_and_refinement(gpu):
     e32, e16 = rel(H32 @ x, y64), rel(H16 @ x, y64)
     print("mixed matvec rel err fp32 %.2e bf16 %.2e" % (e32, e16))
     assert e32 <= 1e-6 and e16 <= 1e-2 and e16 >= e32
+    # 16 columns: the FP32 blocks on the FP32 matrix cores (per-block FP32 sums, FP64 across blocks)
+    X = np.random.default_rng(3).uniform(-1, 1, (ps.number_of_dofs, 16))
+    e32m = rel(H32 @ X, H64 @ X)
+    print("mixed matvec rel err, 16 columns: %.2e" % e32m)
+    assert e32m <= 1e-6
     b = B.DiscreteLinearForm(B.DirichletTrace([0.1, -0.2, 0.05]), ps).compute().get_discrete_linear_form()
     sols = {}
     for name, H in (("fp64", H64), ("mixed", H32), ("bf16", H16)):
@@ -228,8 +233,8 @@ def test_gpu_mixed_precision_matvec_and_refinement(gpu):
     ps3 = B.AnsatzSpace(B.Geometry(), 3, 2)
     Hh64 = B.H2Matrix(ps3, oph, n_cheb=5)
     Hh32 = B.H2Matrix(ps3, oph, n_cheb=5, precision=B.MIXED)
-    # one column: FP32 near field with the coupling evaluated on the fly; several columns: FP32
-    # stored coupling blocks with the FP64 near field (a few columns per pass, or 64 per wave)
+    # one column: FP32 near-field and coupling copies with FP64 sums; 3 columns: FP32 coupling
+    # blocks with the FP64 near field; 12 columns: FP32 blocks on the FP32 matrix cores
     rng = np.random.default_rng(5)
     nh = ps3.number_of_dofs
     xh = rng.uniform(-1, 1, (nh, 12)) + 1j * rng.uniform(-1, 1, (nh, 12))
