@@ -112,6 +112,12 @@ def library():
         "bmi_h2_matvec_super": (C.c_int, [vp, C.c_int, dp, dp, vp]),
         "bmi_h2_use_graphs": (C.c_int, [vp, C.c_int]),
         "bmi_h2_set_sweep": (C.c_int, [vp, C.c_int]),
+        "bmi_gmres_start": (C.c_int, [vp, C.c_int, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, vp,
+                                       C.POINTER(vp)]),
+        "bmi_gmres_poll": (C.c_int, [vp, C.POINTER(C.c_int)]),
+        "bmi_gmres_finish": (C.c_int, [vp, ip, dp]),
+        "bmi_stream_create": (C.c_int, [C.POINTER(vp)]),
+        "bmi_stream_destroy": (C.c_int, [vp]),
         "bmi_synchronize": (C.c_int, [vp]),
         "bmi_rhs_assemble": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp, C.c_int, vp]),
         "bmi_solve_gmres": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, ip, dp,
@@ -740,8 +746,9 @@ class ConjugateGradient(_Solver):
 # ---------------------------------------------------------------------------- serving
 class Server:
     """Request loop on the GPU with continuous batching (bmi_server_*): submit() returns at once,
-    result() waits and returns (solution, info).  Up to max_active batches are admitted and take
-    turns on the device in slices of slice_iterations GMRES steps."""
+    result() waits and returns (solution, info).  Up to max_active batches are admitted; their
+    GMRES solves run as tasks on separate streams driven by one device thread, so their work
+    overlaps on the device (slice_iterations=0: one batch at a time, each to the end)."""
 
     def __init__(self, hbm_budget_bytes=None, max_active=None, max_batch_rhs=64, slice_iterations=None):
         cfg = ServerConfig()

@@ -2759,12 +2759,14 @@ static bool launch_mfma32(bool cplx, int rows, const MultiArgs& ma, dim3 grid, h
   return false;
 }
 
-// FP32 blocks multiplied in FP32 on the matrix cores (default) or widened to FP64
-// (BMI_MULTI_MATH64=1, the accuracy of the single-RHS reduced products)
+// FP32 blocks widened to FP64 on the FP64 matrix cores (default: the accuracy of the single-RHS
+// reduced products, 1e-8, and the FP64 Krylov step count), or multiplied in FP32 on the FP32
+// matrix cores (BMI_MULTI_MATH32=1: 1.3x faster products at 1e-7, which cost restarted GMRES
+// about half as many steps again on the 64-RHS Helmholtz M = 5 problem)
 static bool multi_math32() {
   static const bool on = [] {
-    const char* v = std::getenv("BMI_MULTI_MATH64");
-    return !(v && v[0] == '1');
+    const char* v = std::getenv("BMI_MULTI_MATH32");
+    return v && v[0] == '1';
   }();
   return on;
 }

@@ -52,7 +52,7 @@ struct Operator {
   bmi_op_desc op{};
   int64_t bytes = 0;  // resident device bytes (estimate)
   int64_t ndof = 0, nsuper = 0;
-  bool busy = false;
+  int users = 0;  // admitted batches using the operator (not evicted while > 0)
   double last_used = 0.0;
   Operator() = default;
   Operator(const Operator&) = delete;
@@ -98,7 +98,8 @@ struct Job {
   hipStream_t stream = nullptr;
   double* dB = nullptr;
   double* dX = nullptr;
-  int rhs = 0, S = 1, used = 0, slices = 0, cache_hit = 0;
+  bmi_gmres_task* task = nullptr;  // GMRES in flight on `stream`
+  int rhs = 0, S = 1, slices = 0, cache_hit = 0;
   bool started = false;
   bmi_status st = BMI_OK;
   std::string err;
@@ -138,7 +139,9 @@ struct bmi_server {
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
   void run_scheduler();
   void run_device();
-  bool run_slice(Job& j);                 // device thread, without the lock; true when finished
+  // device thread, without the lock: 0 waiting, 1 finished, 2 work enqueued
+  int step(Job& j, const std::vector<std::shared_ptr<Job>>& live);
+  void release(Job& j);                   // device thread: solution to the host, buffers freed
   void finish(const std::shared_ptr<Job>& j);  // with the lock
   bool estimate(Request& r);              // called without the lock
 };
@@ -221,7 +224,7 @@ void bmi_server::run_scheduler() {
     for (size_t qi = 0; qi < queue.size() && active < cfg.max_active; ++qi) {
       std::shared_ptr<Request> r = queue[qi];
       auto it = ops.find(r->key);
-      if (it != ops.end() && it->second->busy) continue;  // its operator is in use: wait
+      if (it != ops.end() && it->second->users > 0) continue;  // its operator is in use: wait
       if (it == ops.end() && r->op_bytes < 0) continue;
       // batch: queued requests with the same operator, data kind and solver settings
       std::vector<std::shared_ptr<Request>> batch;
@@ -241,7 +244,8 @@ void bmi_server::run_scheduler() {
       while (used_bytes + need > cfg.hbm_budget_bytes) {
         std::shared_ptr<Operator> victim;
         for (auto& o : ops)
-          if (!o.second->busy && o.second->key != r->key && (!victim || o.second->last_used < victim->last_used))
+          if (o.second->users == 0 && o.second->key != r->key &&
+              (!victim || o.second->last_used < victim->last_used))
             victim = o.second;
         if (!victim) break;
         used_bytes -= victim->bytes;
@@ -258,7 +262,7 @@ void bmi_server::run_scheduler() {
         ops[r->key] = op;
         used_bytes += op->bytes;
       }
-      op->busy = true;
+      ++op->users;
       used_bytes += ws;
       ++active;
       const double t = now();
@@ -282,9 +286,10 @@ void bmi_server::run_scheduler() {
 }
 
 void bmi_server::run_device() {
+  std::vector<std::shared_ptr<Job>> live;  // admitted batches on the device (this thread only)
   std::unique_lock<std::mutex> lk(mu);
   while (true) {
-    cv.wait(lk, [&] { return stop || !ready.empty() || !graveyard.empty(); });
+    if (live.empty()) cv.wait(lk, [&] { return stop || !ready.empty() || !graveyard.empty(); });
     if (!graveyard.empty()) {
       std::vector<std::shared_ptr<Operator>> g;
       g.swap(graveyard);
@@ -293,32 +298,49 @@ void bmi_server::run_device() {
       lk.lock();
       continue;
     }
-    if (ready.empty()) {
+    while (!ready.empty()) {
+      live.push_back(ready.front());
+      ready.pop_front();
+    }
+    if (live.empty()) {
       if (stop) return;
       continue;
     }
-    std::shared_ptr<Job> j = ready.front();
-    ready.pop_front();
     lk.unlock();
-    const bool done = run_slice(*j);
+    // one round over the admitted batches: each enqueues its next work if it is ready, so the
+    // batches' kernels are in flight on the device together
+    bool progress = false;
+    std::vector<std::shared_ptr<Job>> done;
+    for (auto& j : live) {
+      const int r = step(*j, live);
+      if (r == 1) done.push_back(j);
+      progress = progress || r != 0;
+    }
     lk.lock();
-    if (done)
+    for (auto& j : done) {
       finish(j);
-    else
-      ready.push_back(j);  // the next admitted batch takes the device
-    cv.notify_all();
+      live.erase(std::find(live.begin(), live.end(), j));
+    }
+    if (!done.empty()) cv.notify_all();
+    // nothing ready on the device: a short wait (new admissions and evictions wake it)
+    if (!progress) cv.wait_for(lk, std::chrono::microseconds(100));
   }
 }
 
-// one time slice of a batch on the device: on the first, assembly (when the operator is not
-// resident), buffers and right-hand sides; then solver steps
-bool bmi_server::run_slice(Job& j) {
+// one turn of a batch: on the first, assembly (when the operator is not resident), buffers and
+// right-hand sides, and the start of its GMRES task; then the task's next work when its
+// convergence flags have arrived. CG batches, and every batch with slice_iterations == 0, run
+// to the end in their first turn. A batch waits while another admitted batch uses the same
+// operator (the product workspace belongs to the operator).
+int bmi_server::step(Job& j, const std::vector<std::shared_ptr<Job>>& live) {
   const Request& r0 = *j.batch[0];
   const bool gm = r0.op.kind != BMI_LAPLACE_SL;
   auto fail = [&](bmi_status s, const char* what) {
     if (j.st == BMI_OK) j.st = s, j.err = what ? std::string(what) : std::string(bmi_last_error());
   };
   if (!j.started) {
+    for (const auto& k : live)
+      if (k.get() != &j && k->started && k->op == j.op) return 0;
     j.started = true;
     j.S = gm ? 2 : 1;
     for (auto& q : j.batch) j.rhs += q->nrhs;
@@ -346,39 +368,49 @@ bool bmi_server::run_slice(Job& j) {
       if (s != BMI_OK) fail(s, nullptr);
       col += q->nrhs;
     }
-  }
-  bool done = j.st != BMI_OK;
-  if (!done) {
-    // GMRES slices are whole restart cycles; CG runs to the end
-    int steps = r0.maxit - j.used;
-    if (gm && cfg.slice_iterations > 0) {
-      const int cyc = std::max(1, (cfg.slice_iterations + r0.restart - 1) / r0.restart) * r0.restart;
-      steps = std::min(steps, cyc);
+    if (j.st != BMI_OK) {
+      release(j);
+      return 1;
     }
-    std::vector<int> it(j.rhs, 0);
-    std::vector<double> rs(j.rhs, 0.0);
-    const int zero = j.slices == 0 ? 1 : 0;
-    bmi_status s;
-    if (!gm)
-      s = bmi_solve_cg(j.op->h, j.rhs, j.dB, j.dX, r0.tol, steps, zero, it.data(), rs.data(), BMI_MEM_DEVICE,
-                       j.stream);
-    else
-      s = bmi_solve_gmres(j.op->h, j.rhs, j.dB, j.dX, r0.tol, r0.restart, steps, zero, it.data(), rs.data(),
-                          BMI_MEM_DEVICE, j.stream);
     ++j.slices;
-    j.used += steps;
-    for (int c = 0; c < j.rhs; ++c) j.iters[c] += it[c];
-    j.resid = rs;
-    if (s == BMI_OK) {
-      done = true;
-    } else if (s == BMI_NOT_CONVERGED) {
-      if (j.used >= r0.maxit) done = true, j.st = BMI_NOT_CONVERGED, j.err = "solver did not reach the tolerance";
-    } else {
+    if (!gm) {
+      const bmi_status s = bmi_solve_cg(j.op->h, j.rhs, j.dB, j.dX, r0.tol, r0.maxit, 1, j.iters.data(),
+                                        j.resid.data(), BMI_MEM_DEVICE, j.stream);
+      if (s != BMI_OK) fail(s, s == BMI_NOT_CONVERGED ? "solver did not reach the tolerance" : nullptr);
+      release(j);
+      return 1;
+    }
+    const bmi_status s = bmi_gmres_start(j.op->h, j.rhs, j.dB, j.dX, r0.tol, r0.restart, r0.maxit, 1, j.stream,
+                                         &j.task);
+    if (s != BMI_OK) {
       fail(s, nullptr);
-      done = true;
+      release(j);
+      return 1;
+    }
+    if (cfg.slice_iterations > 0) return 2;
+  }
+  int d = 1;
+  if (cfg.slice_iterations > 0) {
+    const bmi_status s = bmi_gmres_poll(j.task, &d);
+    if (s != BMI_OK) fail(s, nullptr), d = 1;
+    if (d == 0) return 0;
+    if (d == 2) {
+      ++j.slices;
+      return 2;
     }
   }
-  if (!done) return false;
+  const bmi_status s = bmi_gmres_finish(j.task, j.iters.data(), j.resid.data());
+  j.task = nullptr;
+  if (s != BMI_OK) fail(s, s == BMI_NOT_CONVERGED ? "solver did not reach the tolerance" : nullptr);
+  release(j);
+  return 1;
+}
+
+void bmi_server::release(Job& j) {
+  if (j.task) {
+    (void)bmi_gmres_finish(j.task, nullptr, nullptr);
+    j.task = nullptr;
+  }
   if (j.dX && (j.st == BMI_OK || j.st == BMI_NOT_CONVERGED)) {
     j.hx.resize(static_cast<size_t>(j.S) * j.op->ndof * j.rhs);
     if (hipMemcpyAsync(j.hx.data(), j.dX, j.hx.size() * sizeof(double), hipMemcpyDeviceToHost, j.stream) !=
@@ -395,7 +427,6 @@ bool bmi_server::run_slice(Job& j) {
   if (j.stream) (void)hipStreamDestroy(j.stream);
   j.dB = j.dX = nullptr;
   j.stream = nullptr;
-  return true;
 }
 
 void bmi_server::finish(const std::shared_ptr<Job>& jp) {
@@ -425,7 +456,7 @@ void bmi_server::finish(const std::shared_ptr<Job>& jp) {
   }
   j.hx.clear();
   used_bytes -= j.ws;
-  j.op->busy = false;
+  --j.op->users;
   j.op->last_used = t;
   if (!j.op->h) {  // assembly failed: forget the operator
     used_bytes -= j.op->bytes;

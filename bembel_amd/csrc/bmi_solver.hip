@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -367,109 +368,233 @@ __global__ void k_gm_backsolve(GmresState s, int m, int nv, int nrhs, double* y)
   }
 }
 
-// batched restarted GMRES (classical Gram-Schmidt applied twice, FP64 Arnoldi); returns true
-// if every column converged.  The host waits for the device once per cycle and every
-// `check` Arnoldi steps; a column that converges inside such a window takes no further
-// steps (its flag is cleared on the device), the others continue.
+// Batched restarted GMRES (classical Gram-Schmidt applied twice, FP64 Arnoldi) as a resumable
+// task: poll() enqueues the next piece of work on the task's stream once the convergence flags
+// of the previous piece have arrived (an asynchronous copy into pinned host memory, then an
+// event), so the calling thread never waits on the device and tasks on different streams
+// overlap there. Pieces: a cycle start (residual R = B - A X -- with res64 the FP64 operator,
+// each cycle then being one refinement step whose correction comes from an Arnoldi basis of the
+// reduced operator -- its norm and V0 = R / beta), then Arnoldi steps up to the next check
+// (every 4 steps), then the cycle's end (back substitution, X += V y). A column that converges
+// inside a window takes no further steps (its flag is cleared on the device), the others go on.
+// gmres() drives one task to the end; the device work is the same either way.
+class GmresRun {
+ public:
+  GmresRun(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int restart, int maxit, hipStream_t st,
+           bool reduced, bool res64)
+      : h_(h), nrhs_(nrhs), B_(B), X_(X), tol_(tol), maxit_(maxit), st_(st), reduced_(reduced), res64_(res64),
+        C_(h2_complex(h)), n_(h2_rows(h)), vs_{C_, n_, nrhs, st}, S_(vs_.S()), m_(std::max(1, std::min(restart, maxit))) {
+    V_.alloc(static_cast<size_t>(vs_.block()) * (m_ + 1));
+    W_.alloc(vs_.block());
+    R_.alloc(vs_.block());
+    d_h1_.alloc(static_cast<size_t>(S_) * (m_ + 1) * nrhs);
+    d_h2_.alloc(static_cast<size_t>(S_) * (m_ + 1) * nrhs);
+    d_nrm_.alloc(static_cast<size_t>(S_) * nrhs);
+    d_H_.alloc(static_cast<size_t>(S_) * (m_ + 1) * m_ * nrhs);
+    d_g_.alloc(static_cast<size_t>(S_) * (m_ + 1) * nrhs);
+    d_cs_.alloc(static_cast<size_t>(S_) * m_ * nrhs);
+    d_sn_.alloc(static_cast<size_t>(S_) * m_ * nrhs);
+    d_bn_.alloc(nrhs);
+    d_resid_.alloc(nrhs);
+    d_coef_.alloc(static_cast<size_t>(2) * S_ * nrhs);
+    d_y_.alloc(static_cast<size_t>(S_) * m_ * nrhs);
+    d_kc_.alloc(nrhs);
+    d_iters_.alloc(nrhs);
+    d_active_.alloc(nrhs);
+    gs_ = GmresState{d_H_.ptr, d_g_.ptr, d_cs_.ptr, d_sn_.ptr, d_bn_.ptr, d_resid_.ptr, d_coef_.ptr,
+                     d_kc_.ptr, d_iters_.ptr, d_active_.ptr};
+    sb_ = static_cast<unsigned>((nrhs + 63) / 64);
+    BMI_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&act_), nrhs, hipHostMallocDefault));
+    const hipError_t e = hipEventCreateWithFlags(&ev_, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      (void)hipHostFree(act_);
+      act_ = nullptr;
+      BMI_CHECK_HIP(e);
+    }
+    try {
+      // ||B|| per column
+      vs_.dots_async(B_, 1, B_, d_nrm_.ptr);
+      cycle_kernel(1);
+    } catch (...) {
+      release();
+      throw;
+    }
+  }
+  ~GmresRun() { release(); }
+  void release() {
+    if (ev_) {
+      (void)hipEventSynchronize(ev_);
+      (void)hipEventDestroy(ev_);
+    }
+    ev_ = nullptr;
+    (void)hipStreamSynchronize(st_);  // the buffers may still be in use
+    if (act_) (void)hipHostFree(act_);
+    act_ = nullptr;
+  }
+  GmresRun(const GmresRun&) = delete;
+  GmresRun& operator=(const GmresRun&) = delete;
+
+  // enqueue the next piece if the flags it depends on have arrived: 0 flags not there yet (nothing
+  // enqueued), 1 finished, 2 work enqueued
+  int poll() {
+    while (true) {
+      if (phase_ == DONE) return 1;
+      if (pending_) {
+        const hipError_t q = hipEventQuery(ev_);
+        if (q == hipErrorNotReady) return 0;
+        BMI_CHECK_HIP(q);
+        pending_ = false;
+      }
+      switch (phase_) {
+        case CYCLE:
+          start_cycle();
+          return 2;
+        case AFTER_CYCLE:
+          if (!any_active()) {
+            converged_ = true;
+            phase_ = DONE;
+            break;
+          }
+          if (total_ >= maxit_) {
+            phase_ = DONE;
+            break;
+          }
+          vs_.axpby(d_coef_.ptr, R_.ptr, d_coef_.ptr + S_ * nrhs_, V_.ptr);
+          k_ = 0;
+          steps();
+          if (phase_ == CYCLE) continue;  // the cycle ended without a check: start the next one
+          return 2;
+        case AFTER_STEPS:
+          if (!any_active()) {
+            end_cycle();
+            continue;
+          }
+          steps();
+          if (phase_ == CYCLE) continue;
+          return 2;
+        default:
+          return 1;
+      }
+    }
+  }
+  // block until the flags of the pending piece have arrived
+  void wait() {
+    if (pending_) BMI_CHECK_HIP(hipEventSynchronize(ev_));
+  }
+  bool converged() const { return converged_; }
+  void results(std::vector<int>& iters, std::vector<double>& resid) {
+    iters.assign(nrhs_, 0);
+    resid.assign(nrhs_, 0.0);
+    BMI_CHECK_HIP(hipMemcpyAsync(iters.data(), d_iters_.ptr, nrhs_ * sizeof(int), hipMemcpyDeviceToHost, st_));
+    BMI_CHECK_HIP(hipMemcpyAsync(resid.data(), d_resid_.ptr, nrhs_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+    BMI_CHECK_HIP(hipStreamSynchronize(st_));
+  }
+
+ private:
+  enum Phase { CYCLE, AFTER_CYCLE, AFTER_STEPS, DONE };
+  static constexpr int CHECK = 4;  // Arnoldi steps between convergence checks
+
+  void cycle_kernel(int first) {
+    if (C_)
+      hipLaunchKernelGGL(k_gm_cycle<true>, dim3(sb_), dim3(64), 0, st_, d_nrm_.ptr, gs_, m_, nrhs_, tol_, first);
+    else
+      hipLaunchKernelGGL(k_gm_cycle<false>, dim3(sb_), dim3(64), 0, st_, d_nrm_.ptr, gs_, m_, nrhs_, tol_, first);
+    BMI_CHECK_HIP(hipGetLastError());
+  }
+  void request_flags() {
+    BMI_CHECK_HIP(hipMemcpyAsync(act_, d_active_.ptr, nrhs_, hipMemcpyDeviceToHost, st_));
+    BMI_CHECK_HIP(hipEventRecord(ev_, st_));
+    pending_ = true;
+  }
+  bool any_active() const {
+    for (int c = 0; c < nrhs_; ++c)
+      if (act_[c]) return true;
+    return false;
+  }
+  void start_cycle() {
+    const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
+    BMI_CHECK_HIP(hipMemcpyAsync(R_.ptr, B_, vs_.block() * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    h2_apply_device(h_, nrhs_, X_, n_, R_.ptr, n_, mone, one, st_, res64_ ? false : reduced_);
+    vs_.dots_async(R_.ptr, 1, R_.ptr, d_nrm_.ptr);
+    cycle_kernel(0);
+    request_flags();
+    phase_ = AFTER_CYCLE;
+  }
+  // Arnoldi steps from k_ up to the next check; then either wait for the flags or end the cycle
+  void steps() {
+    const double one[2] = {1.0, 0.0};
+    do {
+      double* Vk = V_.ptr + vs_.block() * k_;
+      h2_apply_device(h_, nrhs_, Vk, n_, W_.ptr, n_, one, nullptr, st_, reduced_);
+      vs_.dots_async(V_.ptr, k_ + 1, W_.ptr, d_h1_.ptr);
+      vs_.subtract(V_.ptr, k_ + 1, d_h1_.ptr, W_.ptr);
+      vs_.dots_async(V_.ptr, k_ + 1, W_.ptr, d_h2_.ptr);
+      vs_.subtract(V_.ptr, k_ + 1, d_h2_.ptr, W_.ptr);
+      vs_.dots_async(W_.ptr, 1, W_.ptr, d_nrm_.ptr);
+      if (C_)
+        hipLaunchKernelGGL(k_gm_givens<true>, dim3(sb_), dim3(64), 0, st_, d_h1_.ptr, d_h2_.ptr, d_nrm_.ptr, gs_, k_,
+                           m_, nrhs_, tol_);
+      else
+        hipLaunchKernelGGL(k_gm_givens<false>, dim3(sb_), dim3(64), 0, st_, d_h1_.ptr, d_h2_.ptr, d_nrm_.ptr, gs_, k_,
+                           m_, nrhs_, tol_);
+      BMI_CHECK_HIP(hipGetLastError());
+      vs_.axpby(d_coef_.ptr, W_.ptr, d_coef_.ptr + S_ * nrhs_, V_.ptr + vs_.block() * (k_ + 1));
+      ++k_;
+      ++total_;
+    } while (k_ < m_ && total_ < maxit_ && k_ % CHECK != 0);
+    if (k_ < m_ && total_ < maxit_) {
+      request_flags();
+      phase_ = AFTER_STEPS;
+    } else {
+      end_cycle();
+    }
+  }
+  // X += V y, then the next cycle
+  void end_cycle() {
+    if (k_ == 0) {
+      phase_ = DONE;
+      return;
+    }
+    if (C_)
+      hipLaunchKernelGGL(k_gm_backsolve<true>, dim3(sb_), dim3(64), 0, st_, gs_, m_, k_, nrhs_, d_y_.ptr);
+    else
+      hipLaunchKernelGGL(k_gm_backsolve<false>, dim3(sb_), dim3(64), 0, st_, gs_, m_, k_, nrhs_, d_y_.ptr);
+    BMI_CHECK_HIP(hipGetLastError());
+    vs_.combine(V_.ptr, k_, d_y_.ptr, X_);
+    phase_ = CYCLE;
+  }
+
+  bmi_h2* h_;
+  int nrhs_;
+  const double* B_;
+  double* X_;
+  double tol_;
+  int maxit_;
+  hipStream_t st_;
+  bool reduced_, res64_, C_;
+  int64_t n_;
+  Vecs vs_;
+  int S_, m_;
+  DevBuf<double> V_, W_, R_, d_h1_, d_h2_, d_nrm_, d_H_, d_g_, d_cs_, d_sn_, d_bn_, d_resid_, d_coef_, d_y_;
+  DevBuf<int> d_kc_, d_iters_;
+  DevBuf<char> d_active_;
+  GmresState gs_{};
+  unsigned sb_ = 1;
+  char* act_ = nullptr;
+  hipEvent_t ev_ = nullptr;
+  Phase phase_ = CYCLE;
+  bool pending_ = false, converged_ = false;
+  int k_ = 0, total_ = 0;
+};
+
+// returns true when every column converged
 static bool gmres(bmi_h2* h, int nrhs, const double* B, double* X, double tol, int restart, int maxit,
                   std::vector<int>& iters, std::vector<double>& resid, hipStream_t st, bool reduced,
                   bool res64 = false) {
-  const bool C = h2_complex(h);
-  const int64_t n = h2_rows(h);
-  Vecs vs{C, n, nrhs, st};
-  const int S = vs.S();
-  const int m = std::max(1, std::min(restart, maxit));
-  const int check = 4;
-  DevBuf<double> V, W, R, d_h1, d_h2, d_nrm, d_H, d_g, d_cs, d_sn, d_bn, d_resid, d_coef, d_y;
-  DevBuf<int> d_kc, d_iters;
-  DevBuf<char> d_active;
-  V.alloc(static_cast<size_t>(vs.block()) * (m + 1));
-  W.alloc(vs.block());
-  R.alloc(vs.block());
-  d_h1.alloc(static_cast<size_t>(S) * (m + 1) * nrhs);
-  d_h2.alloc(static_cast<size_t>(S) * (m + 1) * nrhs);
-  d_nrm.alloc(static_cast<size_t>(S) * nrhs);
-  d_H.alloc(static_cast<size_t>(S) * (m + 1) * m * nrhs);
-  d_g.alloc(static_cast<size_t>(S) * (m + 1) * nrhs);
-  d_cs.alloc(static_cast<size_t>(S) * m * nrhs);
-  d_sn.alloc(static_cast<size_t>(S) * m * nrhs);
-  d_bn.alloc(nrhs);
-  d_resid.alloc(nrhs);
-  d_coef.alloc(static_cast<size_t>(2) * S * nrhs);
-  d_y.alloc(static_cast<size_t>(S) * m * nrhs);
-  d_kc.alloc(nrhs);
-  d_iters.alloc(nrhs);
-  d_active.alloc(nrhs);
-  const GmresState gs{d_H.ptr, d_g.ptr, d_cs.ptr, d_sn.ptr, d_bn.ptr, d_resid.ptr, d_coef.ptr,
-                      d_kc.ptr, d_iters.ptr, d_active.ptr};
-  const unsigned sb = static_cast<unsigned>((nrhs + 63) / 64);
-  const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
-  std::vector<char> act(nrhs);
-  auto any_active = [&]() {
-    BMI_CHECK_HIP(hipMemcpyAsync(act.data(), d_active.ptr, nrhs, hipMemcpyDeviceToHost, st));
-    BMI_CHECK_HIP(hipStreamSynchronize(st));
-    return std::any_of(act.begin(), act.end(), [](char a) { return a != 0; });
-  };
-  // ||B|| per column
-  vs.dots_async(B, 1, B, d_nrm.ptr);
-  if (C)
-    hipLaunchKernelGGL(k_gm_cycle<true>, dim3(sb), dim3(64), 0, st, d_nrm.ptr, gs, m, nrhs, tol, 1);
-  else
-    hipLaunchKernelGGL(k_gm_cycle<false>, dim3(sb), dim3(64), 0, st, d_nrm.ptr, gs, m, nrhs, tol, 1);
-  BMI_CHECK_HIP(hipGetLastError());
-  int total = 0;
-  bool converged = false;
-  while (true) {
-    // R = B - A X (with res64, the FP64 operator: each cycle is then one refinement step whose
-    // correction comes from an Arnoldi basis of the reduced operator), beta, V0 = R / beta
-    BMI_CHECK_HIP(hipMemcpyAsync(R.ptr, B, vs.block() * sizeof(double), hipMemcpyDeviceToDevice, st));
-    h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st, res64 ? false : reduced);
-    vs.dots_async(R.ptr, 1, R.ptr, d_nrm.ptr);
-    if (C)
-      hipLaunchKernelGGL(k_gm_cycle<true>, dim3(sb), dim3(64), 0, st, d_nrm.ptr, gs, m, nrhs, tol, 0);
-    else
-      hipLaunchKernelGGL(k_gm_cycle<false>, dim3(sb), dim3(64), 0, st, d_nrm.ptr, gs, m, nrhs, tol, 0);
-    BMI_CHECK_HIP(hipGetLastError());
-    if (!any_active()) {
-      converged = true;
-      break;
-    }
-    if (total >= maxit) break;
-    vs.axpby(d_coef.ptr, R.ptr, d_coef.ptr + S * nrhs, V.ptr);
-    int k = 0;
-    for (; k < m && total < maxit; ++k, ++total) {
-      if (k > 0 && k % check == 0 && !any_active()) break;
-      double* Vk = V.ptr + vs.block() * k;
-      h2_apply_device(h, nrhs, Vk, n, W.ptr, n, one, nullptr, st, reduced);
-      vs.dots_async(V.ptr, k + 1, W.ptr, d_h1.ptr);
-      vs.subtract(V.ptr, k + 1, d_h1.ptr, W.ptr);
-      vs.dots_async(V.ptr, k + 1, W.ptr, d_h2.ptr);
-      vs.subtract(V.ptr, k + 1, d_h2.ptr, W.ptr);
-      vs.dots_async(W.ptr, 1, W.ptr, d_nrm.ptr);
-      if (C)
-        hipLaunchKernelGGL(k_gm_givens<true>, dim3(sb), dim3(64), 0, st, d_h1.ptr, d_h2.ptr, d_nrm.ptr, gs, k, m,
-                           nrhs, tol);
-      else
-        hipLaunchKernelGGL(k_gm_givens<false>, dim3(sb), dim3(64), 0, st, d_h1.ptr, d_h2.ptr, d_nrm.ptr, gs, k, m,
-                           nrhs, tol);
-      BMI_CHECK_HIP(hipGetLastError());
-      vs.axpby(d_coef.ptr, W.ptr, d_coef.ptr + S * nrhs, V.ptr + vs.block() * (k + 1));
-    }
-    // X += V y
-    if (k == 0) break;
-    if (C)
-      hipLaunchKernelGGL(k_gm_backsolve<true>, dim3(sb), dim3(64), 0, st, gs, m, k, nrhs, d_y.ptr);
-    else
-      hipLaunchKernelGGL(k_gm_backsolve<false>, dim3(sb), dim3(64), 0, st, gs, m, k, nrhs, d_y.ptr);
-    BMI_CHECK_HIP(hipGetLastError());
-    vs.combine(V.ptr, k, d_y.ptr, X);
-  }
-  iters.assign(nrhs, 0);
-  resid.assign(nrhs, 0.0);
-  BMI_CHECK_HIP(hipMemcpyAsync(iters.data(), d_iters.ptr, nrhs * sizeof(int), hipMemcpyDeviceToHost, st));
-  BMI_CHECK_HIP(hipMemcpyAsync(resid.data(), d_resid.ptr, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
-  BMI_CHECK_HIP(hipStreamSynchronize(st));
-  return converged;
+  GmresRun run(h, nrhs, B, X, tol, restart, maxit, st, reduced, res64);
+  while (run.poll() != 1) run.wait();
+  run.results(iters, resid);
+  return run.converged();
 }
 
 // CG scalars on the device: no host round trip per iteration
@@ -846,6 +971,109 @@ static bmi_status solve_common(const bmi_h2* hc, int nrhs, const double* B, doub
     set_last_error(e.what());
     return BMI_DEVICE_ERROR;
   }
+}
+
+struct bmi_gmres_task {
+  std::unique_ptr<bmi::GmresRun> run;
+  hipStream_t st = nullptr;
+};
+
+bmi_status bmi_gmres_start(const bmi_h2* hc, int nrhs, const double* B, double* X, double tol, int restart,
+                           int maxit, int zero_guess, void* stream, bmi_gmres_task** out) {
+  using namespace bmi;
+  try {
+    BMI_REQUIRE(hc && B && X && out && nrhs >= 1 && tol > 0 && maxit >= 1 && restart >= 1, BMI_INVALID_ARG,
+                "bad argument");
+    *out = nullptr;
+    bmi_h2* h = const_cast<bmi_h2*>(hc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int S = h2_complex(h) ? 2 : 1;
+    if (zero_guess) device_fill(X, S * h2_rows(h) * nrhs, 0.0, st);
+    // mixed handles: cycles start from the FP64 residual (as bmi_solve_gmres)
+    const bool mixed = h2_is_mixed(h);
+    auto t = std::make_unique<bmi_gmres_task>();
+    t->st = st;
+    t->run = std::make_unique<GmresRun>(h, nrhs, B, X, tol, restart, maxit, st, mixed, mixed);
+    *out = t.release();
+    return BMI_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return BMI_DEVICE_ERROR;
+  }
+}
+
+bmi_status bmi_gmres_poll(bmi_gmres_task* t, int* done) {
+  using namespace bmi;
+  try {
+    BMI_REQUIRE(t && t->run && done, BMI_INVALID_ARG, "bad argument");
+    *done = t->run->poll();
+    return BMI_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return BMI_DEVICE_ERROR;
+  }
+}
+
+bmi_status bmi_gmres_finish(bmi_gmres_task* t, int* iters, double* resid) {
+  using namespace bmi;
+  if (!t) {
+    set_last_error("null task");
+    return BMI_INVALID_ARG;
+  }
+  bmi_status status = BMI_OK;
+  try {
+    BMI_REQUIRE(t->run, BMI_INVALID_ARG, "task already finished");
+    while (t->run->poll() != 1) t->run->wait();
+    std::vector<int> it;
+    std::vector<double> rs;
+    t->run->results(it, rs);
+    if (iters) std::copy(it.begin(), it.end(), iters);
+    if (resid) std::copy(rs.begin(), rs.end(), resid);
+    status = t->run->converged() ? BMI_OK : BMI_NOT_CONVERGED;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    status = e.status;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    status = BMI_DEVICE_ERROR;
+  }
+  try {
+    t->run.reset();
+  } catch (...) {
+  }
+  delete t;
+  return status;
+}
+
+bmi_status bmi_stream_create(void** out) {
+  if (!out) {
+    bmi::set_last_error("null argument");
+    return BMI_INVALID_ARG;
+  }
+  hipStream_t s = nullptr;
+  const hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    bmi::set_last_error(hipGetErrorString(e));
+    return BMI_DEVICE_ERROR;
+  }
+  *out = s;
+  return BMI_OK;
+}
+
+bmi_status bmi_stream_destroy(void* stream) {
+  if (!stream) return BMI_OK;
+  const hipError_t e = hipStreamDestroy(static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) {
+    bmi::set_last_error(hipGetErrorString(e));
+    return BMI_DEVICE_ERROR;
+  }
+  return BMI_OK;
 }
 
 bmi_status bmi_solve_gmres(const bmi_h2* h, int nrhs, const double* B, double* X, double tol, int restart, int maxit,

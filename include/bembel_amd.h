@@ -49,9 +49,10 @@ typedef enum {
                                    (one RHS: an FP32 copy of the near-field blocks with the coupling
                                    blocks read from their FP32 copy or, for the real Laplace kernel,
                                    recomputed in FP32 from the Chebyshev surface points, whichever the
-                                   first product measures faster;
-                                   several RHS: FP32 coupling blocks with the FP64 near field), sums in
-                                   FP64; the FP64 operator is kept and solvers refine with it */
+                                   first product measures faster; from 8 RHS on: the FP32 near-field and
+                                   coupling blocks widened to FP64 on the matrix cores; 2-7 RHS: FP32
+                                   coupling blocks with the FP64 near field), sums in FP64; the FP64
+                                   operator is kept and solvers refine with it */
   BMI_PRECISION_MIXED_BF16 = 2  /* as MIXED, and the coupling blocks of the two coarsest levels in BF16 */
 } bmi_precision;
 
@@ -213,6 +214,25 @@ bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, d
                         int maxit, int zero_guess, int* iters, double* resid, int mem,
                         void* stream);
 
+/* The GMRES of bmi_solve_gmres as a task that one thread can interleave with others (no
+   counterpart in the reference).  start enqueues the first work on `stream` (device B and X);
+   poll enqueues the next piece once the convergence flags of the previous one have arrived and
+   returns without waiting on the device (*done: 0 flags not there yet, nothing enqueued;
+   1 the task is finished; 2 work enqueued); finish waits,
+   writes iters/resid (may be NULL) and frees the task, returning BMI_OK or BMI_NOT_CONVERGED.
+   Tasks on different streams overlap on the device.  A handle's product workspace is shared:
+   run one task per H2 handle at a time.  The device work, and so the result, equals
+   bmi_solve_gmres with the same arguments. */
+typedef struct bmi_gmres_task bmi_gmres_task;
+bmi_status bmi_gmres_start(const bmi_h2* h, int nrhs, const double* B, double* X, double tol,
+                           int restart, int maxit, int zero_guess, void* stream,
+                           bmi_gmres_task** out);
+bmi_status bmi_gmres_poll(bmi_gmres_task* task, int* done);
+bmi_status bmi_gmres_finish(bmi_gmres_task* task, int* iters, double* resid);
+/* a non-blocking stream of this library's HIP runtime (default priority); NULL destroys nothing */
+bmi_status bmi_stream_create(void** out);
+bmi_status bmi_stream_destroy(void* stream);
+
 /* ------------------------------------------------------------------ serving */
 /* A request loop on one GPU (no counterpart in the reference, which solves one problem per
    program).  Submitted solves are queued; a scheduler admits them (smallest first, anything
@@ -220,16 +240,18 @@ bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, d
    device-memory budget holds the batch's workspace plus, for an operator that is not resident
    yet, its estimated H2 state (idle resident operators are evicted least recently used first).
    Queued requests with equal operator and solver settings are merged into one batched solve of
-   up to max_batch_rhs right-hand sides.  Admitted batches take turns on the device in time
-   slices of slice_iterations GMRES steps (whole restart cycles; CG batches run to the end), all
-   driven by one device thread, so a short solve does not wait for a long one to finish.
-   Assembled operators stay resident between requests. */
+   up to max_batch_rhs right-hand sides.  One device thread drives every admitted batch: a
+   GMRES batch runs as a task on its own stream (bmi_gmres_start/poll), enqueueing its next work
+   whenever its convergence flags arrive, so the kernels of several batches are in flight on the
+   device together and a short solve does not wait for a long one to finish; CG batches run to
+   the end in their first turn.  Assembled operators stay resident between requests. */
 typedef struct bmi_server bmi_server;
 typedef struct {
   double hbm_budget_bytes; /* device memory the server may hold */
-  int max_active;          /* batches admitted at once (they take turns on the device) */
+  int max_active;          /* batches admitted at once (their work overlaps on the device) */
   int max_batch_rhs;       /* right-hand sides merged into one solve */
-  int slice_iterations;    /* GMRES steps of a batch before the next batch's turn (0: to the end) */
+  int slice_iterations;    /* > 0: GMRES batches run as overlapping tasks; 0: one batch at a time,
+                              each to the end */
 } bmi_server_config;
 typedef struct {
   const char* geometry_path; /* .dat file (GeometryIO.hpp:25-122) */
@@ -252,7 +274,7 @@ typedef struct {
   double residual;           /* largest relative residual over the request's columns */
   int batch_requests, batch_rhs; /* size of the batch the request was solved in */
   int cache_hit;             /* the operator was resident when the batch started */
-  int slices;                /* device turns the batch took */
+  int slices;                /* device turns in which the batch enqueued work */
 } bmi_result_info;
 bmi_status bmi_server_config_default(bmi_server_config* cfg);
 bmi_status bmi_server_create(const bmi_server_config* cfg, bmi_server** out);

@@ -17,7 +17,7 @@ int main(int argc, char** argv) {
     cfg.max_active = 1 + round % 4;
     cfg.max_batch_rhs = round % 3 == 0 ? 1 : 64;
     cfg.hbm_budget_bytes = round % 5 == 0 ? 3e7 : 1e12;  // small budgets force eviction
-    cfg.slice_iterations = round % 2 ? 30 : 0;              // GMRES batches in two turns, or one
+    cfg.slice_iterations = round % 2 ? 30 : 0;              // GMRES batches overlapping, or one at a time
     bmi_server* srv;
     if (bmi_server_create(&cfg, &srv) != BMI_OK) return 1;
     std::vector<std::thread> clients;

@@ -90,6 +90,46 @@ bmi_status bmi_solve_gmres(const bmi_h2* h, int nrhs, const double* B, double* X
                            int* it, double* rs, int, void*) { return solve(h, nrhs, B, X, maxit, zero, it, rs); }
 bmi_status bmi_solve_cg(const bmi_h2* h, int nrhs, const double* B, double* X, double, int maxit, int zero, int* it,
                         double* rs, int, void*) { return solve(h, nrhs, B, X, maxit, zero, it, rs); }
+// a GMRES task that finishes after a random number of polls, some of which find its flags not
+// there yet (0) and some enqueue work (2); finish writes X = B / 2
+struct bmi_gmres_task {
+  const bmi_h2* h;
+  int nrhs;
+  const double* B;
+  double* X;
+  int polls;
+};
+bmi_status bmi_gmres_start(const bmi_h2* h, int nrhs, const double* B, double* X, double, int, int, int, void*,
+                           bmi_gmres_task** out) {
+  device_call();
+  nap(500);
+  thread_local std::mt19937 g(std::hash<std::thread::id>()(std::this_thread::get_id()) + 7);
+  *out = new bmi_gmres_task{h, nrhs, B, X, 2 + static_cast<int>(g() % 6)};
+  return BMI_OK;
+}
+bmi_status bmi_gmres_poll(bmi_gmres_task* t, int* done) {
+  device_call();
+  nap(100);
+  if (t->polls > 0) {
+    --t->polls;
+    *done = (t->polls % 2) ? 0 : 2;
+  } else {
+    *done = 1;
+  }
+  return BMI_OK;
+}
+bmi_status bmi_gmres_finish(bmi_gmres_task* t, int* it, double* rs) {
+  device_call();
+  nap(500);
+  const int S = t->h->cplx ? 2 : 1;
+  for (int64_t i = 0; i < S * t->h->ndof * t->nrhs; ++i) t->X[i] = 0.5 * t->B[i];
+  for (int c = 0; c < t->nrhs; ++c) {
+    if (it) it[c] = 10 + c;
+    if (rs) rs[c] = 1e-11;
+  }
+  delete t;
+  return BMI_OK;
+}
 }
 extern "C" bmi_status bmi_op_desc_default(int kind, bmi_op_desc* d) {
   d->kind = kind; d->kappa_re = 0; d->kappa_im = 0; d->n_cheb = 9; d->eta = 1.6; d->min_cluster_level = 1;

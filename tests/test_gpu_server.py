@@ -1,6 +1,6 @@
-"""Serving runtime on the GPU: results in a mix (batched, taking turns on the device in time
-slices) equal the results of the same requests solved alone, resident operators are reused, and
-the solutions satisfy the FP64 operator."""
+"""Serving runtime on the GPU: results in a mix (batched, several batches' GMRES tasks in flight
+on their own streams at once) equal the results of the same requests solved alone, resident
+operators are reused, and the solutions satisfy the FP64 operator."""
 import numpy as np
 import pytest
 
@@ -46,8 +46,9 @@ def solve_all(srv, reqs):
 @pytest.mark.skip(reason="several batches on the device at once, one host worker thread per batch, ended once in "
                          "a host segmentation fault inside a worker thread; the scheduler is clean under "
                          "ThreadSanitizer and AddressSanitizer with stubbed device calls and the cause below it is "
-                         "not found, so concurrent batches stay off the GPU (the server now drives all batches "
-                         "from one device thread in time slices: test_gpu_server_time_sliced_mix_equals_alone)")
+                         "not found, so that design stays off the GPU (the server now drives all batches from one "
+                         "device thread as GMRES tasks on their own streams: "
+                         "test_gpu_server_overlapping_mix_equals_alone)")
 def test_gpu_server_mix_equals_alone(gpu):
     reqs = requests()
     alone = []
@@ -72,10 +73,12 @@ def test_gpu_server_mix_equals_alone(gpu):
     print("batches with more than one request:", batched)
 
 
-def test_gpu_server_time_sliced_mix_equals_alone(gpu):
-    """Several batches admitted at once take turns on the device in slices of one GMRES restart
-    cycle; a GMRES resumed from its iterate continues as it would have, so every result equals the
-    same request solved alone in one go."""
+def test_gpu_server_overlapping_mix_equals_alone(gpu):
+    """Several batches admitted at once run as GMRES tasks on their own streams, each enqueueing
+    its next work whenever its convergence flags arrive, so their kernels are in flight on the
+    device together; a task's device work is that of a solve alone, so every result equals the
+    same request solved alone (one batch at a time, one request per batch) to the batching
+    tolerance."""
     reqs = requests()
     srv1 = B.Server(max_active=1, max_batch_rhs=1, slice_iterations=0)
     alone = [solve_all(srv1, [r])[0] for r in reqs]
@@ -89,7 +92,7 @@ def test_gpu_server_time_sliced_mix_equals_alone(gpu):
         assert ia["status"] == B.OK and im["status"] == B.OK, (ia, im)
         assert rel(xm, xa) <= 1e-8, (r["M"], r["P"], rel(xm, xa))
     assert mixed[-1][1]["cache_hit"] or mixed[-1][1]["batch_requests"] > 1
-    # some batch needed more than one turn, and batches were admitted while others were running
+    # batches took many turns, and batches were admitted while others were running
     assert max(i["slices"] for _, i in mixed) > 1
     spans = [(i["start_s"], i["end_s"]) for _, i in mixed]
     assert any(a[0] < b[1] and b[0] < a[1] and a != b for a in spans for b in spans)
