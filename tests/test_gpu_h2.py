@@ -271,3 +271,76 @@ def test_gpu_h2_torch_work_is_ordered_before_the_product(gpu):
     z = y * 2.0  # torch reads the product right away
     ref = H @ np.ones(n)
     assert rel(z.cpu().numpy() / 2.0, ref) <= 1e-14
+
+
+def test_gpu_gmres_tasks_interleaved_equal_blocking(gpu):
+    """Two GMRES tasks (bmi_gmres_start/poll/finish) on two handles and two streams, interleaved by
+    polling from one thread, give the same bits and iteration counts as bmi_solve_gmres alone."""
+    import ctypes as C
+    import time
+    import torch
+    lib = B.library()
+    ps = B.AnsatzSpace(B.Geometry(), 3, 2)
+    cases = []
+    for kappa, prec in ((2.0, B.FP64), (3.0, B.MIXED)):
+        op = B.HelmholtzSingleLayerOperator()
+        op.set_wavenumber(kappa)
+        H = B.H2Matrix(ps, op, n_cheb=5, precision=prec)
+        b = B.DiscreteLinearForm(B.PlaneWaves([[0.0, 0.6, 0.8], [1.0, 0.0, 0.0]], kappa), ps).compute(
+            device=True).get_discrete_linear_form()
+        cases.append((H, b.t().contiguous()))  # column-major: (columns, rows) contiguous
+    torch.cuda.synchronize()
+    tol, restart, maxit = 1e-8, 30, 3000
+    n = ps.number_of_dofs
+    ref = []
+    for H, bt in cases:
+        X = torch.zeros_like(bt)
+        it = np.zeros(2, dtype=np.int32)
+        rs = np.zeros(2)
+        dp = C.POINTER(C.c_double)
+        st = lib.bmi_solve_gmres(H._h, 2, C.cast(C.c_void_p(bt.data_ptr()), dp), C.cast(C.c_void_p(X.data_ptr()), dp),
+                                 tol, restart, maxit, 1, it.ctypes.data_as(C.POINTER(C.c_int)),
+                                 rs.ctypes.data_as(C.POINTER(C.c_double)), B.MEM_DEVICE, None)
+        assert st == B.OK, (st, B.library().bmi_last_error())
+        ref.append((X.clone(), it.copy()))
+    streams, tasks, outs = [], [], []
+    for H, bt in cases:
+        s = C.c_void_p()
+        assert lib.bmi_stream_create(C.byref(s)) == B.OK
+        X = torch.zeros_like(bt)
+        torch.cuda.synchronize()
+        t = C.c_void_p()
+        assert lib.bmi_gmres_start(H._h, 2, C.c_void_p(bt.data_ptr()), C.c_void_p(X.data_ptr()), tol, restart,
+                                   maxit, 1, s, C.byref(t)) == B.OK, lib.bmi_last_error()
+        streams.append(s)
+        tasks.append(t)
+        outs.append(X)
+    done = [0, 0]
+    turns = [0, 0]
+    for _ in range(2_000_000):
+        progress = False
+        for k in range(2):
+            if done[k] == 1:
+                continue
+            d = C.c_int()
+            assert lib.bmi_gmres_poll(tasks[k], C.byref(d)) == B.OK, lib.bmi_last_error()
+            done[k] = d.value
+            turns[k] += d.value == 2
+            progress = progress or d.value != 0
+        if done == [1, 1]:
+            break
+        if not progress:
+            time.sleep(50e-6)
+    assert done == [1, 1]
+    for k in range(2):
+        it = np.zeros(2, dtype=np.int32)
+        rs = np.zeros(2)
+        st = lib.bmi_gmres_finish(tasks[k], it.ctypes.data_as(C.POINTER(C.c_int)),
+                                  rs.ctypes.data_as(C.POINTER(C.c_double)))
+        assert st == B.OK
+        assert lib.bmi_stream_destroy(streams[k]) == B.OK
+        torch.cuda.synchronize()
+        assert torch.equal(outs[k], ref[k][0]), k
+        assert list(it) == list(ref[k][1]), (it, ref[k][1])
+        assert turns[k] > 1
+    print("turns with work enqueued:", turns, "iterations:", [list(r[1]) for r in ref])
