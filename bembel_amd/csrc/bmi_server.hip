@@ -322,8 +322,9 @@ void bmi_server::run_device() {
       live.erase(std::find(live.begin(), live.end(), j));
     }
     if (!done.empty()) cv.notify_all();
-    // nothing ready on the device: a short wait (new admissions and evictions wake it)
-    if (!progress) cv.wait_for(lk, std::chrono::microseconds(100));
+    // nothing ready on the device: a short wait (new admissions and evictions wake it); short,
+    // since a task waiting for its flags leaves the device idle until its next turn
+    if (!progress) cv.wait_for(lk, std::chrono::microseconds(20));
   }
 }
 
