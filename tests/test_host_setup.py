@@ -47,6 +47,27 @@ def test_error_reporting_without_gpu():
     assert e.value.status == B.INVALID_ARG
 
 
+def test_invalid_arguments_return_status_without_gpu():
+    """Null handles and out-of-range arguments come back as BMI_INVALID_ARG before any device
+    call (no abort, no crash), for the product, sweep control and task entry points."""
+    lib = B.library()
+    dp = ctypes.POINTER(ctypes.c_double)
+    one = (ctypes.c_double * 2)(1.0, 0.0)
+    x = (ctypes.c_double * 4)()
+    assert lib.bmi_h2_matvec(None, 1, x, 1, x, 1, one, None, B.MEM_HOST, None) == B.INVALID_ARG
+    assert lib.bmi_h2_set_sweep(None, 0) == B.INVALID_ARG
+    assert lib.bmi_h2_use_graphs(None, 1) == B.INVALID_ARG
+    t = ctypes.c_void_p()
+    assert lib.bmi_gmres_start(None, 1, x, x, 1e-8, 30, 10, 1, None, ctypes.byref(t)) == B.INVALID_ARG
+    assert lib.bmi_gmres_poll(None, None) == B.INVALID_ARG
+    assert lib.bmi_gmres_finish(None, None, None) == B.INVALID_ARG
+    assert lib.bmi_stream_destroy(None) == B.OK
+    assert lib.bmi_stream_create(None) == B.INVALID_ARG
+    assert lib.bmi_solve_gmres(None, 1, ctypes.cast(x, dp), ctypes.cast(x, dp), 1e-8, 30, 10, 1, None, None,
+                               B.MEM_HOST, None) == B.INVALID_ARG
+    assert lib.bmi_last_error()
+
+
 CASES = [(None, 0, 1, 0), (None, 2, 2, 0), (None, 4, 2, 0), ((1.0, 0.7, 0.5), 3, 2, 0), ((1.0, 0.7, 0.5), 3, 2, 1),
          (None, 3, 1, 1), ("screen", 2, 0, 0), (None, 6, 2, 0)]
 
