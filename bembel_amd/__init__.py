@@ -216,13 +216,16 @@ class Geometry:
         h = C.c_void_p()
         sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float64)
         _check(library().bmi_geometry_load(filename.encode(), None if sc is None else _dp(sc), C.byref(h)))
+        self._destroy = library().bmi_geometry_destroy
         self._h = h
         self.filename = filename
         self.scale = None if scale is None else tuple(scale)
 
     def __del__(self):
+        # the destroy function is bound when the handle is made: at interpreter exit the module's
+        # globals may already be gone
         if getattr(self, "_h", None):
-            library().bmi_geometry_destroy(self._h)
+            self._destroy(self._h)
             self._h = None
 
     @property
@@ -308,6 +311,7 @@ class AnsatzSpace:
         form = op.form
         h = C.c_void_p()
         _check(library().bmi_space_create(geometry._h, refinement_level, polynomial_degree, form, C.byref(h)))
+        self._destroy = library().bmi_space_destroy
         self._h = h
         self.geometry = geometry  # keeps the geometry alive
         info = np.zeros(8, dtype=np.int64)
@@ -318,8 +322,10 @@ class AnsatzSpace:
         self.form = form
 
     def __del__(self):
+        # the destroy function is bound when the handle is made: at interpreter exit the module's
+        # globals may already be gone
         if getattr(self, "_h", None):
-            library().bmi_space_destroy(self._h)
+            self._destroy(self._h)
             self._h = None
 
     def get_number_of_dofs(self):
@@ -429,6 +435,7 @@ class H2Matrix:
         d = operator.desc(n_cheb=n_cheb, eta=eta, min_cluster_level=min_cluster_level, precision=precision)
         h = C.c_void_p()
         _check(library().bmi_h2_assemble(space._h, C.byref(d), _stream_ptr(stream), C.byref(h)))
+        self._destroy = library().bmi_h2_destroy
         self._h = h
         info = np.zeros(16, dtype=np.int64)
         _check(library().bmi_h2_info(self._h, info.ctypes.data_as(C.POINTER(C.c_int64))))
@@ -438,8 +445,10 @@ class H2Matrix:
                          pair_classes=[int(v) for v in info[11:16]])
 
     def __del__(self):
+        # the destroy function is bound when the handle is made: at interpreter exit the module's
+        # globals may already be gone
         if getattr(self, "_h", None):
-            library().bmi_h2_destroy(self._h)
+            self._destroy(self._h)
             self._h = None
 
     def rows(self):

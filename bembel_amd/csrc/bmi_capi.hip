@@ -470,7 +470,9 @@ static void apply_super_t(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
 
 // ys = A_super xs on the workspace, through a captured graph when possible.  The graph holds
 // only kernel nodes (buffers are cleared by a fill kernel, not a memset node); in-library
-// timing needs host synchronisation per kernel and therefore runs eagerly.
+// timing needs host synchronisation per kernel and therefore runs eagerly.  A single-RHS product
+// (four launches, the longest a few hundred microseconds) also runs eagerly: measured, the next
+// kernel on the stream starts about 8 us after a graph ends, more than the launches save.
 static void apply_super_workspace(bmi_h2* h, int nrhs, bool reduced, bool tl, hipStream_t st) {
   auto run = [&](hipStream_t rs) {
     if (tl)
@@ -479,7 +481,7 @@ static void apply_super_workspace(bmi_h2* h, int nrhs, bool reduced, bool tl, hi
       apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, reduced, rs);
   };
   if (!tl) autotune(h, h->d_xs.ptr, reduced, st);
-  if (!h->use_graphs || timing_enabled()) {
+  if (!h->use_graphs || timing_enabled() || (nrhs == 1 && !tl)) {
     run(st);
     return;
   }

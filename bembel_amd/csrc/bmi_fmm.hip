@@ -321,40 +321,25 @@ __global__ void k_m2m(int ncl, int nrhs, int rows, int np2, const double* __rest
 
 // ------------------------------------------------------------------ fused tree transformations
 // One workgroup owns one cluster at level `top` and walks its subtree down to level `bot`
-// (bot - top <= 3) in LDS; blockIdx.y is the right-hand side.  This replaces one launch per level
-// (each a few microseconds of latency for very little work) by one launch per three levels.
+// (bot - top <= 3) in LDS; blockIdx.y is the right-hand side.  Each transformation is one launch
+// (instead of one per level, each a few microseconds of latency for very little work).
 // Level buffers in LDS: level top + d holds 4^d clusters x rows values.
 __device__ __forceinline__ int tree_lds_off(int d, int rows, int S) {
   return S * rows * (((1 << (2 * d)) - 1) / 3);  // sum_{e<d} 4^e clusters before level top + d
 }
 
-// forward transformation (H2Multipole.hpp:380-422): level bot from the moments of the super
-// vector (from_moment) or from fw; then parents by the transfer matrices up to level top
+// one subtree of the forward transformation (H2Multipole.hpp:380-422): level bot of cluster
+// ctop's subtree from the moments of the super vector (from_moment) or from fw, then parents by
+// the transfer matrices up to level top; every level is written to fw (and bw cleared there)
 template <bool C>
-__global__ __launch_bounds__(256) void k_forward_tree(int top, int bot, int from_moment, int rows, int np2, int dd,
-                                                      int nrhs, const double* __restrict__ mom,
-                                                      const double* __restrict__ T, const double* __restrict__ xs,
-                                                      int64_t ldxs, int64_t xseg, double* __restrict__ fw,
-                                                      const int64_t* __restrict__ lvl_off, double* __restrict__ bwz,
-                                                      int stage) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void forward_subtree(double* lds, int top, int bot, bool from_moment, int ctop, int k,
+                                                int rows, int np2, int dd, int nrhs, const double* mom,
+                                                const double* T, const double* __restrict__ xs, int64_t ldxs,
+                                                int64_t xseg, double* fw, const int64_t* __restrict__ lvl_off,
+                                                double* __restrict__ bwz) {
   constexpr int S = C ? 2 : 1;
-  const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   const int D = bot - top;
-  // the transfer matrix (and the moment matrix for the bottom level) are read many times per
-  // workgroup: stage them in LDS after the level buffers when they fit (stage != 0)
-  if (stage) {
-    double* Ts = lds + tree_lds_off(D + 1, rows, S);
-    for (int q = tid; q < 4 * np2 * np2; q += blockDim.x) Ts[q] = T[q];
-    T = Ts;
-    if (from_moment) {
-      double* Ms = Ts + 4 * np2 * np2;
-      for (int q = tid; q < dd * rows; q += blockDim.x) Ms[q] = mom[q];
-      mom = Ms;
-    }
-    __syncthreads();
-  }
-  // bottom level
   {
     const int n = (1 << (2 * D)) * rows;
     double* L = lds + tree_lds_off(D, rows, S);
@@ -366,6 +351,7 @@ __global__ __launch_bounds__(256) void k_forward_tree(int top, int bot, int from
       if (from_moment) {
         const double* x = xs + S * (static_cast<int64_t>(k) * ldxs + xseg + c * dd);
         double ar = 0.0, ai = 0.0;
+#pragma unroll
         for (int j = 0; j < dd; ++j) {
           const double m = mom[static_cast<int64_t>(j) * rows + row];
           ar += m * x[S * j];
@@ -394,8 +380,10 @@ __global__ __launch_bounds__(256) void k_forward_tree(int top, int bot, int from
       const int cl = idx / rows, row = idx - (idx / rows) * rows;
       const int comp = row / np2, i = row - comp * np2;
       double ar = 0.0, ai = 0.0;
+#pragma unroll 1
       for (int q = 0; q < 4; ++q) {
         const double* ch = Ch + S * ((4 * cl + q) * rows + comp * np2);
+#pragma unroll
         for (int j = 0; j < np2; ++j) {
           const double t = T[static_cast<int64_t>(q * np2 + j) * np2 + i];
           ar += t * ch[S * j];
@@ -416,36 +404,98 @@ __global__ __launch_bounds__(256) void k_forward_tree(int top, int bot, int from
   }
 }
 
-// backward transformation (H2Multipole.hpp:428-469): bw of level top read from global; children
-// get the coupling sums already in bw plus T^T of their parent, down to level bot; level bot is
-// written back, or, with to_leaf, turned into the leaf rows yf = Mom^T bw_bot
-template <bool C>
-__global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_leaf, int rows, int np2, int dd,
-                                                       int nrhs, const double* __restrict__ mom,
-                                                       const double* __restrict__ T, double* __restrict__ bw,
-                                                       const int64_t* __restrict__ lvl_off, double* __restrict__ yf,
-                                                       int64_t ldy, int64_t yseg, int stage) {
+// forward transformation (H2Multipole.hpp:380-422), one segment of levels per launch: workgroup
+// (ctop, rhs) computes the subtree of levels bot (from the moments of the super vector, or from
+// fw) .. top below cluster ctop.  The segments are separate launches: handing a finished level to
+// another workgroup inside one launch needs device-scope fences, which write back and invalidate
+// the L2 of every XCD and measured slower than the launch they save.
+// NPC / DDC != 0 fix np2 (= rows, one FMM component) and dd at compile time: the dot products
+// then unroll and their loads are issued together instead of one latency per term.
+template <bool C, int NPC = 0, int DDC = 0>
+__global__ __launch_bounds__(256) void k_forward_tree(int top, int bot, int from_moment, int rows_, int np2_, int dd_,
+                                                      int nrhs, const double* __restrict__ mom,
+                                                      const double* __restrict__ T, const double* __restrict__ xs,
+                                                      int64_t ldxs, int64_t xseg, double* __restrict__ fw,
+                                                      const int64_t* __restrict__ lvl_off, double* __restrict__ bwz,
+                                                      int lvl_doubles, int stage) {
   extern __shared__ double lds[];
-  constexpr int S = C ? 2 : 1;
-  const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
-  const int D = bot - top;
-  if (stage) {  // as in k_forward_tree
-    double* Ts = lds + tree_lds_off(D + 1, rows, S);
+  const int rows = NPC ? NPC : rows_, np2 = NPC ? NPC : np2_, dd = DDC ? DDC : dd_;
+  const int k = blockIdx.y, tid = threadIdx.x;
+  // the transfer matrix (and the moment matrix for the bottom level) are read many times per
+  // workgroup: stage them in LDS after the level buffers when they fit (stage != 0)
+  if (stage) {
+    double* Ts = lds + lvl_doubles;
     for (int q = tid; q < 4 * np2 * np2; q += blockDim.x) Ts[q] = T[q];
     T = Ts;
-    if (to_leaf) {
+    if (from_moment) {
       double* Ms = Ts + 4 * np2 * np2;
       for (int q = tid; q < dd * rows; q += blockDim.x) Ms[q] = mom[q];
       mom = Ms;
+      __syncthreads();  // the moments use Ms; otherwise the barrier after the bottom level covers T
+    }
+  }
+  forward_subtree<C>(lds, top, bot, from_moment != 0, blockIdx.x, k, rows, np2, dd, nrhs, mom, T, xs, ldxs, xseg,
+                     fw, lvl_off, bwz);
+}
+
+// backward transformation (H2Multipole.hpp:428-469) in one launch: workgroup (ctop, rhs) owns
+// cluster ctop at level top.  It first forms the local expansion of its ancestors from level 0
+// down to ctop (one cluster per level: coupling sum in bw plus T^T of the parent; every
+// workgroup repeats this short chain instead of waiting for a launch above it); then children get
+// their coupling sums plus T^T of their parent, down to level bot; level bot is written back,
+// or, with to_leaf, turned into the leaf rows yf = Mom^T bw_bot.
+template <bool C, int NPC = 0, int DDC = 0>
+__global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_leaf, int rows_, int np2_, int dd_,
+                                                       int nrhs, const double* __restrict__ mom,
+                                                       const double* __restrict__ T, double* __restrict__ bw,
+                                                       const int64_t* __restrict__ lvl_off, double* __restrict__ yf,
+                                                       int64_t ldy, int64_t yseg, int lvl_doubles, int stage) {
+  extern __shared__ double lds[];
+  constexpr int S = C ? 2 : 1;
+  const int rows = NPC ? NPC : rows_, np2 = NPC ? NPC : np2_, dd = DDC ? DDC : dd_;  // as in k_forward_tree
+  const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int D = bot - top;
+  double* A = lds + lvl_doubles;  // ancestors' expansions: (top + 1) x rows, after the staged matrices
+  if (stage) {  // as in k_forward_tree
+    double* Ts = lds + lvl_doubles;
+    for (int q = tid; q < 4 * np2 * np2; q += blockDim.x) Ts[q] = T[q];
+    T = Ts;
+    A = Ts + 4 * np2 * np2;
+    if (to_leaf) {
+      double* Ms = A;
+      for (int q = tid; q < dd * rows; q += blockDim.x) Ms[q] = mom[q];
+      mom = Ms;
+      A = Ms + dd * rows;
+    }
+  }
+  for (int idx = tid; idx < (top + 1) * rows; idx += blockDim.x) {
+    const int l = idx / rows, row = idx - l * rows;
+    const int64_t a = static_cast<int64_t>(ctop) >> (2 * (top - l));
+    const int64_t g = S * (lvl_off[l] * nrhs + (a * nrhs + k) * rows + row);
+    A[S * idx] = bw[g];
+    if constexpr (C) A[S * idx + 1] = bw[g + 1];
+  }
+  __syncthreads();
+  for (int l = 1; l <= top; ++l) {
+    const int q = (ctop >> (2 * (top - l))) & 3;
+    for (int row = tid; row < rows; row += blockDim.x) {
+      const int comp = row / np2, j = row - comp * np2;
+      const double* pa = A + S * ((l - 1) * rows + comp * np2);
+      double ar = A[S * (l * rows + row)], ai = C ? A[S * (l * rows + row) + 1] : 0.0;
+#pragma unroll
+      for (int i = 0; i < np2; ++i) {
+        const double t = T[static_cast<int64_t>(q * np2 + j) * np2 + i];
+        ar += t * pa[S * i];
+        if constexpr (C) ai += t * pa[S * i + 1];
+      }
+      A[S * (l * rows + row)] = ar;
+      if constexpr (C) A[S * (l * rows + row) + 1] = ai;
     }
     __syncthreads();
   }
-  {
-    const int64_t g = S * (lvl_off[top] * nrhs + (static_cast<int64_t>(ctop) * nrhs + k) * rows);
-    for (int idx = tid; idx < rows; idx += blockDim.x) {
-      lds[S * idx] = bw[g + S * idx];
-      if constexpr (C) lds[S * idx + 1] = bw[g + S * idx + 1];
-    }
+  for (int idx = tid; idx < rows; idx += blockDim.x) {
+    lds[S * idx] = A[S * (top * rows + idx)];
+    if constexpr (C) lds[S * idx + 1] = A[S * (top * rows + idx) + 1];
   }
   __syncthreads();
   for (int d = 1; d <= D; ++d) {
@@ -461,6 +511,7 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
       const int64_t g = S * (base + (c * nrhs + k) * rows + row);
       double ar = bw[g], ai = C ? bw[g + 1] : 0.0;
       const double* pa = Pa + S * (pc * rows + comp * np2);
+#pragma unroll
       for (int i = 0; i < np2; ++i) {
         const double t = T[static_cast<int64_t>(q * np2 + j) * np2 + i];
         ar += t * pa[S * i];
@@ -483,6 +534,7 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
       const double* m = mom + static_cast<int64_t>(i) * rows;
       const double* b = Lv + S * cl * rows;
       double ar = 0.0, ai = 0.0;
+#pragma unroll
       for (int q = 0; q < rows; ++q) {
         ar += m[q] * b[S * q];
         if constexpr (C) ai += m[q] * b[S * q + 1];
@@ -1416,15 +1468,15 @@ __global__ __launch_bounds__(256) void k_sym_gather(int ng, const int2* __restri
   double* y = bw + S * (lvl_off[T.x] + static_cast<int64_t>(T.y) * CD) + e;
   const int q0 = grp[w], q1 = grp[w + 1];
   const double* pp = part + static_cast<int64_t>(q0) * E + e;
+  // eight loads in flight, then added in slot order (a target receives 16 partials on average)
   double s = *y;
   int q = 0;
-  for (; q + 4 <= q1 - q0; q += 4) {
-    const double v0 = pp[static_cast<int64_t>(q) * E], v1 = pp[static_cast<int64_t>(q + 1) * E];
-    const double v2 = pp[static_cast<int64_t>(q + 2) * E], v3 = pp[static_cast<int64_t>(q + 3) * E];
-    s += v0;
-    s += v1;
-    s += v2;
-    s += v3;
+  for (; q + 8 <= q1 - q0; q += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = pp[static_cast<int64_t>(q + u) * E];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
   }
   for (; q < q1 - q0; ++q) s += pp[static_cast<int64_t>(q) * E];
   *y = s;
@@ -2397,7 +2449,8 @@ void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int
     const int depth = first ? std::min(2, maxd) : maxd;
     const int top = bot - depth > 0 ? bot - depth : 0;
     const int D = bot - top;
-    size_t lds = sizeof(double) * S * rows * (((1 << (2 * (D + 1))) - 1) / 3);
+    const size_t lvl_doubles = static_cast<size_t>(S) * rows * (((1 << (2 * (D + 1))) - 1) / 3);
+    size_t lds = sizeof(double) * lvl_doubles;
     BMI_REQUIRE(lds <= 64 * 1024, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
     // staging pays off when a workgroup's tree is the whole work (one RHS); with many RHS the
     // matrices stay in the caches across the workgroups of the same subtree
@@ -2405,12 +2458,17 @@ void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int
     const int stage = (nrhs == 1 && staged <= 64 * 1024) ? 1 : 0;
     if (stage) lds = staged;
     const dim3 grid(static_cast<unsigned>(npatch << (2 * top)), static_cast<unsigned>(nrhs));
-    if (cplx)
-      hipLaunchKernelGGL(k_forward_tree<true>, grid, dim3(256), lds, st, top, bot, first ? 1 : 0, rows, np2, dd, nrhs,
-                         mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero, stage);
-    else
-      hipLaunchKernelGGL(k_forward_tree<false>, grid, dim3(256), lds, st, top, bot, first ? 1 : 0, rows, np2, dd, nrhs,
-                         mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero, stage);
+    const int fm = first ? 1 : 0, ld = static_cast<int>(lvl_doubles);
+    bool launched = false;
+#define BMI_FWD(CC, NPC, DDC)                                                                                     \
+  if (!launched && cplx == CC && (NPC == 0 || (rows == NPC && np2 == NPC && dd == DDC))) {                          \
+    hipLaunchKernelGGL((k_forward_tree<CC, NPC, DDC>), grid, dim3(256), lds, st, top, bot, fm, rows, np2, dd, nrhs,  \
+                       mom, T, xs, ldxs, xseg, fw, lvl_off, bw_zero, ld, stage);                                   \
+    launched = true;                                                                                               \
+  }
+    BMI_FWD(false, 16, 36) BMI_FWD(false, 16, 16) BMI_FWD(true, 25, 36) BMI_FWD(true, 25, 16)
+    BMI_FWD(true, 16, 36) BMI_FWD(false, 25, 36) BMI_FWD(false, 0, 0) BMI_FWD(true, 0, 0)
+#undef BMI_FWD
     BMI_CHECK_HIP(hipGetLastError());
     first = false;
     if (top == 0) break;
@@ -2418,42 +2476,37 @@ void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int
   }
 }
 
-// whole backward transformation from level 0 to the leaf rows yf (segments of at most three
-// levels, the last one ending in the leaf rows)
+// whole backward transformation from level 0 to the leaf rows yf: one launch over the clusters at
+// the top of the deepest segment (at most two levels above L), each workgroup recomputing its
+// ancestors' expansions from level 0
 void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
                      const double* T, double* bw, const int64_t* lvl_off, double* yf, int64_t ldy, int64_t yseg,
                      hipStream_t st) {
   const int S = cplx ? 2 : 1;
-  // segment boundaries from the bottom: L, L-3, ... down to 0, walked top-down
   const int maxd = tree_max_depth(rows * S);
   BMI_REQUIRE(maxd >= 1, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
-  std::vector<int> cuts;
-  for (int b = L; b > 0;) {
-    cuts.push_back(b);
-    const int depth = b == L ? std::min(2, maxd) : maxd;
-    b = b - depth > 0 ? b - depth : 0;
+  const int bot = L, top = L - std::min(2, maxd) > 0 ? L - std::min(2, maxd) : 0;
+  const int D = bot - top;
+  const size_t lvl_doubles = static_cast<size_t>(S) * rows * (((1 << (2 * (D + 1))) - 1) / 3);
+  const size_t anc = sizeof(double) * S * rows * (top + 1);
+  size_t lds = sizeof(double) * lvl_doubles + anc;
+  BMI_REQUIRE(lds <= 64 * 1024, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
+  const size_t staged = lds + sizeof(double) * (4 * np2 * np2 + dd * rows);
+  const int stage = (nrhs == 1 && staged <= 64 * 1024) ? 1 : 0;
+  if (stage) lds = staged;
+  const dim3 grid(static_cast<unsigned>(npatch << (2 * top)), static_cast<unsigned>(nrhs));
+  const int ld = static_cast<int>(lvl_doubles);
+  bool launched = false;
+#define BMI_BWD(CC, NPC, DDC)                                                                                       \
+  if (!launched && cplx == CC && (NPC == 0 || (rows == NPC && np2 == NPC && dd == DDC))) {                            \
+    hipLaunchKernelGGL((k_backward_tree<CC, NPC, DDC>), grid, dim3(256), lds, st, top, bot, 1, rows, np2, dd, nrhs,    \
+                       mom, T, bw, lvl_off, yf, ldy, yseg, ld, stage);                                              \
+    launched = true;                                                                                                 \
   }
-  cuts.push_back(0);
-  std::reverse(cuts.begin(), cuts.end());
-  if (cuts.size() == 1) cuts.push_back(0);  // L = 0: one segment of depth 0 ending in the leaf rows
-  for (size_t k = 0; k + 1 < cuts.size(); ++k) {
-    const int top = cuts[k], bot = cuts[k + 1];
-    const int D = bot - top;
-    const int to_leaf = k + 2 == cuts.size() ? 1 : 0;
-    size_t lds = sizeof(double) * S * rows * (((1 << (2 * (D + 1))) - 1) / 3);
-    BMI_REQUIRE(lds <= 64 * 1024, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
-    const size_t staged = lds + sizeof(double) * (4 * np2 * np2 + (to_leaf ? dd * rows : 0));
-    const int stage = (nrhs == 1 && staged <= 64 * 1024) ? 1 : 0;
-    if (stage) lds = staged;
-    const dim3 grid(static_cast<unsigned>(npatch << (2 * top)), static_cast<unsigned>(nrhs));
-    if (cplx)
-      hipLaunchKernelGGL(k_backward_tree<true>, grid, dim3(256), lds, st, top, bot, to_leaf, rows, np2, dd, nrhs, mom,
-                         T, bw, lvl_off, yf, ldy, yseg, stage);
-    else
-      hipLaunchKernelGGL(k_backward_tree<false>, grid, dim3(256), lds, st, top, bot, to_leaf, rows, np2, dd, nrhs, mom,
-                         T, bw, lvl_off, yf, ldy, yseg, stage);
-    BMI_CHECK_HIP(hipGetLastError());
-  }
+  BMI_BWD(false, 16, 36) BMI_BWD(false, 16, 16) BMI_BWD(true, 25, 36) BMI_BWD(true, 25, 16)
+  BMI_BWD(true, 16, 36) BMI_BWD(false, 25, 36) BMI_BWD(false, 0, 0) BMI_BWD(true, 0, 0)
+#undef BMI_BWD
+  BMI_CHECK_HIP(hipGetLastError());
 }
 
 void launch_l2l(bool cplx, int nchild, int nrhs, int rows, int np2, const double* T, const double* parent,
@@ -2526,7 +2579,7 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
   const dim3 grid(static_cast<unsigned>(nbn + nbc));
   bool launched = false;
   {
-    TimedScope ts("matvec_sweep", st);
+    TimedScope ts("matvec_sweep_upper", st);
 #define BMI_FUSED_S(CC, DDD, FTT, CDD, UN, UC, NFTT)                                                                 \
   if (!launched && cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&               \
       near32 == std::is_same<NFTT, float>::value && leaf_variant == UN - 1 && cpl_variant == UC - 1) {              \
