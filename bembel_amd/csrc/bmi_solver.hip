@@ -30,26 +30,28 @@ void convert_to_fp32(const double* src, float* dst, size_t n, hipStream_t st) {
   BMI_CHECK_HIP(hipGetLastError());
 }
 
-// Inner products of many vectors with one in two passes: block (chunk, column c) sums its rows
-// of conj(V_j) W for every j into part[((j * nrhs + c) * nchunk + chunk)], four vectors per pass
-// over the rows (independent loads in flight, one barrier per four), then one thread per (j, c)
-// adds the chunk sums in order (deterministic: the chunk count depends on the sizes only).
-// W's rows are read once per four vectors.
+// Inner products of many vectors with one in two passes: block (chunk, column c, group g of four
+// vectors) sums its rows of conj(V_j) W for j = 4g .. 4g + 3 into
+// part[((j * nrhs + c) * nchunk + chunk)] (the groups run side by side, so a Gram-Schmidt pass
+// over k vectors costs one round of loads, not k / 4), then one wave per (j, c) adds the chunk
+// sums in a fixed order (deterministic: the chunk count depends on the sizes only).
 constexpr int DOT_CHUNKS = 64;
+constexpr int DOT_JB = 4;
 static inline int dot_chunks(int nrhs) { return nrhs <= 2 ? 2 * DOT_CHUNKS : DOT_CHUNKS; }
 template <bool C>
 __global__ __launch_bounds__(256) void k_dots_part(const double* __restrict__ V, int64_t vstride, int nv,
                                                     const double* __restrict__ W, int64_t n, int nrhs, int nchunk,
                                                     double* __restrict__ part) {
   constexpr int S = C ? 2 : 1;
-  constexpr int JB = 4;
+  constexpr int JB = DOT_JB;
   __shared__ double red[2][JB][4];
   const int chunk = blockIdx.x, c = blockIdx.y;
   const int64_t per = (n + nchunk - 1) / nchunk;
   const int64_t r0 = chunk * per, r1 = r0 + per < n ? r0 + per : n;
   const double* w = W + static_cast<int64_t>(S) * n * c;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j0 = 0; j0 < nv; j0 += JB) {
+  {
+    const int j0 = static_cast<int>(blockIdx.z) * JB;
     const int nj = nv - j0 < JB ? nv - j0 : JB;
     double ar[JB], ai[JB];
 #pragma unroll
@@ -86,7 +88,6 @@ __global__ __launch_bounds__(256) void k_dots_part(const double* __restrict__ V,
       part[S * o] = (red[0][q][0] + red[0][q][1]) + (red[0][q][2] + red[0][q][3]);
       if (C) part[S * o + 1] = (red[1][q][0] + red[1][q][1]) + (red[1][q][2] + red[1][q][3]);
     }
-    __syncthreads();
   }
 }
 // one wave per inner product: lane l adds chunks l, l + 64, ... in order, then a fixed shuffle tree
@@ -122,6 +123,7 @@ __global__ void k_subtract(const double* __restrict__ V, int64_t vstride, int nv
   const int c = static_cast<int>(gid / n);
   const int S = C ? 2 : 1;
   double wr = W[S * gid], wi = C ? W[S * gid + 1] : 0.0;
+#pragma unroll 4
   for (int j = 0; j < nv; ++j) {
     const double* v = V + j * vstride + S * gid;
     const double hr = h[S * (j * nrhs + c)];
@@ -201,7 +203,8 @@ struct Vecs {
       BMI_CHECK_HIP(hipStreamSynchronize(st));  // earlier launches may still use the old buffer
       part.alloc(std::max(need, std::max(2 * part.n, static_cast<size_t>(S()) * 32 * nrhs * nchunk)));
     }
-    const dim3 grid(static_cast<unsigned>(nchunk), static_cast<unsigned>(nrhs));
+    const dim3 grid(static_cast<unsigned>(nchunk), static_cast<unsigned>(nrhs),
+                    static_cast<unsigned>((nv + DOT_JB - 1) / DOT_JB));
     const int count = nv * nrhs;
     if (C) {
       hipLaunchKernelGGL(k_dots_part<true>, grid, dim3(256), 0, st, V, block(), nv, W, n, nrhs, nchunk, part.ptr);
