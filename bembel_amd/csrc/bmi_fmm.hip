@@ -444,7 +444,7 @@ __global__ __launch_bounds__(256) void k_forward_tree(int top, int bot, int from
 // workgroup repeats this short chain instead of waiting for a launch above it); then children get
 // their coupling sums plus T^T of their parent, down to level bot; level bot is written back,
 // or, with to_leaf, turned into the leaf rows yf = Mom^T bw_bot.
-template <bool C, int NPC = 0, int DDC = 0>
+template <bool C, int NPC = 0, int DDC = 0, bool ST = false>
 __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_leaf, int rows_, int np2_, int dd_,
                                                        int nrhs, const double* __restrict__ mom,
                                                        const double* __restrict__ T, double* __restrict__ bw,
@@ -455,19 +455,35 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
   const int rows = NPC ? NPC : rows_, np2 = NPC ? NPC : np2_, dd = DDC ? DDC : dd_;  // as in k_forward_tree
   const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
   const int D = bot - top;
+  (void)stage;
   double* A = lds + lvl_doubles;  // ancestors' expansions: (top + 1) x rows, after the staged matrices
-  if (stage) {  // as in k_forward_tree
+  // ST: T and Mom staged in LDS transposed, so that lanes (consecutive child rows, leaf entries)
+  // read consecutive words; in the global layout their reads are np2 or rows words apart, which
+  // puts a wave's lanes on a few LDS banks
+  if constexpr (ST) {
     double* Ts = lds + lvl_doubles;
-    for (int q = tid; q < 4 * np2 * np2; q += blockDim.x) Ts[q] = T[q];
+    const int nt = np2 * np2;
+    for (int e = tid; e < 4 * nt; e += blockDim.x) {
+      const int q = e / nt, r = e - q * nt, i = r / np2, j = r - i * np2;
+      Ts[e] = T[static_cast<int64_t>(q * np2 + j) * np2 + i];
+    }
     T = Ts;
-    A = Ts + 4 * np2 * np2;
+    A = Ts + 4 * nt;
     if (to_leaf) {
       double* Ms = A;
-      for (int q = tid; q < dd * rows; q += blockDim.x) Ms[q] = mom[q];
+      for (int e = tid; e < dd * rows; e += blockDim.x) {
+        const int q = e / dd, i = e - q * dd;
+        Ms[e] = mom[static_cast<int64_t>(i) * rows + q];
+      }
       mom = Ms;
       A = Ms + dd * rows;
     }
   }
+  // entry (j, i) of transfer block q (child row j, parent row i) and moment entry (i, q)
+  auto tval = [&](int q, int j, int i) {
+    return ST ? T[(q * np2 + i) * np2 + j] : T[static_cast<int64_t>(q * np2 + j) * np2 + i];
+  };
+  auto mval = [&](int i, int q) { return ST ? mom[q * dd + i] : mom[static_cast<int64_t>(i) * rows + q]; };
   for (int idx = tid; idx < (top + 1) * rows; idx += blockDim.x) {
     const int l = idx / rows, row = idx - l * rows;
     const int64_t a = static_cast<int64_t>(ctop) >> (2 * (top - l));
@@ -484,7 +500,7 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
       double ar = A[S * (l * rows + row)], ai = C ? A[S * (l * rows + row) + 1] : 0.0;
 #pragma unroll
       for (int i = 0; i < np2; ++i) {
-        const double t = T[static_cast<int64_t>(q * np2 + j) * np2 + i];
+        const double t = tval(q, j, i);
         ar += t * pa[S * i];
         if constexpr (C) ai += t * pa[S * i + 1];
       }
@@ -513,7 +529,7 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
       const double* pa = Pa + S * (pc * rows + comp * np2);
 #pragma unroll
       for (int i = 0; i < np2; ++i) {
-        const double t = T[static_cast<int64_t>(q * np2 + j) * np2 + i];
+        const double t = tval(q, j, i);
         ar += t * pa[S * i];
         if constexpr (C) ai += t * pa[S * i + 1];
       }
@@ -531,13 +547,13 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
     const int n = (1 << (2 * D)) * dd;
     for (int idx = tid; idx < n; idx += blockDim.x) {
       const int cl = idx / dd, i = idx - (idx / dd) * dd;
-      const double* m = mom + static_cast<int64_t>(i) * rows;
       const double* b = Lv + S * cl * rows;
       double ar = 0.0, ai = 0.0;
 #pragma unroll
       for (int q = 0; q < rows; ++q) {
-        ar += m[q] * b[S * q];
-        if constexpr (C) ai += m[q] * b[S * q + 1];
+        const double m = mval(i, q);
+        ar += m * b[S * q];
+        if constexpr (C) ai += m * b[S * q + 1];
       }
       const int64_t c = (static_cast<int64_t>(ctop) << (2 * D)) + cl;
       double* y = yf + S * (static_cast<int64_t>(k) * ldy + yseg + c * dd + i);
@@ -2499,8 +2515,12 @@ void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, in
   bool launched = false;
 #define BMI_BWD(CC, NPC, DDC)                                                                                       \
   if (!launched && cplx == CC && (NPC == 0 || (rows == NPC && np2 == NPC && dd == DDC))) {                            \
-    hipLaunchKernelGGL((k_backward_tree<CC, NPC, DDC>), grid, dim3(256), lds, st, top, bot, 1, rows, np2, dd, nrhs,    \
-                       mom, T, bw, lvl_off, yf, ldy, yseg, ld, stage);                                              \
+    if (stage)                                                                                                       \
+      hipLaunchKernelGGL((k_backward_tree<CC, NPC, DDC, true>), grid, dim3(256), lds, st, top, bot, 1, rows, np2, dd,  \
+                         nrhs, mom, T, bw, lvl_off, yf, ldy, yseg, ld, stage);                                      \
+    else                                                                                                             \
+      hipLaunchKernelGGL((k_backward_tree<CC, NPC, DDC, false>), grid, dim3(256), lds, st, top, bot, 1, rows, np2, dd, \
+                         nrhs, mom, T, bw, lvl_off, yf, ldy, yseg, ld, stage);                                      \
     launched = true;                                                                                                 \
   }
   BMI_BWD(false, 16, 36) BMI_BWD(false, 16, 16) BMI_BWD(true, 25, 36) BMI_BWD(true, 25, 16)
