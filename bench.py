@@ -70,6 +70,23 @@ def measured_traffic(M, P, npts, upper):
     return d["fetch_bytes"] + d["write_bytes"]
 
 
+def streaming_read_gbs(torch, mib=2048, reps=10):
+    """HBM read rate of a plain streaming reduction (torch.sum over a resident FP64 buffer, HIP
+    events): a measured reference for what a read-only sweep can reach, beside the 8 TB/s peak"""
+    x = torch.ones(mib * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    x.sum()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        x.sum()
+    ev1.record()
+    ev1.synchronize()
+    gbs = reps * x.numel() * 8 / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
+    del x
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def run_batched_profile(B, torch):
     """Helmholtz kappa 4, 6,144 elements, P 2, 5x5 Chebyshev nodes, 64 plane-wave right-hand sides:
     per precision an assembly, 8 batched products and 60 batched GMRES(30) steps (the workload of the
@@ -520,6 +537,9 @@ def main():
     sw_bytes = ((info["dense"] * dd * dd + ncpl * rows * rows) * 8 + 2 * nsuper * 8 + 2 * lvl_total * 8 +
                 (ncpl * rows * 8 if sym_sweep else 0))
     achieved = sw_bytes / (sw_avg * 1e-3) / 1e9 if sw_avg > 0 else 0.0
+    read_gbs = streaming_read_gbs(torch)
+    log("streaming read (torch.sum over 2 GiB) %.0f GB/s; sweep at %.0f %% of it" % (read_gbs,
+                                                                                      100 * achieved / read_gbs))
     log("sweep (near field + coupling%s) %.3f ms/launch, %.1f MB -> %.0f GB/s; forward %.3f ms, backward %.3f ms" % (
         ", upper coupling blocks" if sym_sweep else "", sw_avg, sw_bytes / 1e6, achieved, fw_ms / max(fw_n, 1),
         bw_ms / max(bw_n, 1)))
@@ -626,6 +646,7 @@ def main():
                      "kernel": "k_sweep_fused (near-field leaves + coupling blocks of the matvec)",
                      "bytes_per_launch": sw_bytes, "avg_launch_ms": sw_avg, "symmetric_coupling": sym_sweep,
                      "sym_gather_ms": sg_ms / max(sg_n, 1) if sym_sweep else 0.0,
+                     "streaming_read_gbs": read_gbs,
                      "forward_ms": fw_ms / max(fw_n, 1), "backward_ms": bw_ms / max(bw_n, 1)},
         "roofline_assembly": {"bound": "fp64", "achieved": duffy_tflops, "peak": FP64_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": duffy_tflops / FP64_PEAK_TFLOPS,
