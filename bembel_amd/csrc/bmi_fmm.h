@@ -22,10 +22,6 @@ void launch_coupling_assemble(int kind, const int4* d_leaves, int nleaves, int n
 void launch_spmv(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, const double* v, const double* x,
                  int64_t ldx, double* y, int64_t ldy, const double* alpha, const double* beta, hipStream_t st,
                  const double* x2 = nullptr, bool long_rows = false);
-void launch_moment(bool cplx, int ncl, int nrhs, int rows, int mc, const double* mom, const double* xs,
-                   int64_t ldxs, int64_t seg, double* fw, hipStream_t st);
-void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T, const double* child,
-                double* parent, hipStream_t st);
 // variant: 0 = one block per lane group per iteration, 1 = two (chosen by the autotuner)
 // Fc/ncoarse: BF16 copies of the first ncoarse blocks (fp32 only); ld: column stride of F (0: cd)
 void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs, int cd, const int2* tgt,
@@ -67,8 +63,6 @@ void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int
 void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,
                      const double* T, double* bw, const int64_t* lvl_off, double* yf, int64_t ldy, int64_t yseg,
                      hipStream_t st);
-void launch_l2l(bool cplx, int nchild, int nrhs, int rows, int np2, const double* T, const double* parent,
-                double* child, hipStream_t st);
 void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* rp, const int* src, const double* F,
                  int64_t fstride, int64_t fcomp, const double* xs, int64_t ldxs, int64_t xseg, double* ys, int64_t ldys,
                  int64_t yseg, int accumulate, hipStream_t st);

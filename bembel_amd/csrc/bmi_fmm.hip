@@ -284,41 +284,6 @@ __global__ void k_spmv_sub(int nrows, int nrhs, const int* __restrict__ rp, cons
   }
 }
 
-// ------------------------------------------------------------------ forward transformation
-// leaf moments: fw[c][k][comp rows] = Mom (rows x mc) * xs[comp_seg + c*mc .. , k]
-template <bool C>
-__global__ void k_moment(int ncl, int nrhs, int rows, int mc, const double* __restrict__ mom,
-                         const double* __restrict__ xs, int64_t ldxs, int64_t seg, double* fw) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= static_cast<int64_t>(ncl) * nrhs * rows) return;
-  const int i = static_cast<int>(gid % rows);
-  const int64_t ck = gid / rows;
-  const int k = static_cast<int>(ck % nrhs), c = static_cast<int>(ck / nrhs);
-  const double* x = xs + (C ? 2 : 1) * (static_cast<int64_t>(k) * ldxs + seg + static_cast<int64_t>(c) * mc);
-  Val<C> acc;
-  for (int j = 0; j < mc; ++j) acc.fma_real(mom[static_cast<int64_t>(j) * rows + i], Val<C>::load(x, j));
-  acc.store(fw, gid);
-}
-
-// parent moments: fw_l[c][k][comp*np2+i] = sum_q sum_j T(i, q*np2+j) fw_{l+1}[4c+q][k][comp*np2+j]
-template <bool C>
-__global__ void k_m2m(int ncl, int nrhs, int rows, int np2, const double* __restrict__ T,
-                      const double* __restrict__ child, double* parent) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= static_cast<int64_t>(ncl) * nrhs * rows) return;
-  const int row = static_cast<int>(gid % rows);
-  const int64_t ck = gid / rows;
-  const int k = static_cast<int>(ck % nrhs), c = static_cast<int>(ck / nrhs);
-  const int comp = row / np2, i = row % np2;
-  Val<C> acc;
-  for (int q = 0; q < 4; ++q) {
-    const int64_t cbase = ((static_cast<int64_t>(4 * c + q) * nrhs + k) * rows + comp * np2);
-    for (int j = 0; j < np2; ++j)
-      acc.fma_real(T[static_cast<int64_t>(q * np2 + j) * np2 + i], Val<C>::load(child, cbase + j));
-  }
-  acc.store(parent, gid);
-}
-
 // ------------------------------------------------------------------ fused tree transformations
 // One workgroup owns one cluster at level `top` and walks its subtree down to level `bot`
 // (bot - top <= 3) in LDS; blockIdx.y is the right-hand side.  Each transformation is one launch
@@ -603,24 +568,6 @@ __global__ void k_coupling_apply(int ntgt, int nrhs, int cd, int ld, const int2*
 }
 
 // ------------------------------------------------------------------ backward transformation
-// children += T^T parent: bw_{l+1}[4c+q][k][comp*np2+j] += sum_i T(i, q*np2+j) bw_l[c][k][comp*np2+i]
-template <bool C>
-__global__ void k_l2l(int nchild, int nrhs, int rows, int np2, const double* __restrict__ T,
-                      const double* __restrict__ parent, double* child) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= static_cast<int64_t>(nchild) * nrhs * rows) return;
-  const int row = static_cast<int>(gid % rows);
-  const int64_t ck = gid / rows;
-  const int k = static_cast<int>(ck % nrhs), ch = static_cast<int>(ck / nrhs);
-  const int comp = row / np2, j = row % np2;
-  const int c = ch / 4, q = ch % 4;
-  const int64_t pbase = (static_cast<int64_t>(c) * nrhs + k) * rows + comp * np2;
-  Val<C> acc = Val<C>::load(child, gid);
-  for (int i = 0; i < np2; ++i)
-    acc.fma_real(T[static_cast<int64_t>(q * np2 + j) * np2 + i], Val<C>::load(parent, pbase + i));
-  acc.store(child, gid);
-}
-
 // leaf rows: ys[seg + c*dd + i, k] = sum_dense F (dd x dd) xs[cseg + c2*dd .., k]
 //                                    + sum_r Mom(r, i) bw_L[c][k][r]
 template <bool C>
@@ -2350,26 +2297,6 @@ void launch_spmv(bool cplx, int nrows, int nrhs, const int* rp, const int* ci, c
   BMI_CHECK_HIP(hipGetLastError());
 }
 
-void launch_moment(bool cplx, int ncl, int nrhs, int rows, int mc, const double* mom, const double* xs,
-                   int64_t ldxs, int64_t seg, double* fw, hipStream_t st) {
-  const int64_t n = static_cast<int64_t>(ncl) * nrhs * rows;
-  if (cplx)
-    hipLaunchKernelGGL(k_moment<true>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, rows, mc, mom, xs, ldxs, seg, fw);
-  else
-    hipLaunchKernelGGL(k_moment<false>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, rows, mc, mom, xs, ldxs, seg, fw);
-  BMI_CHECK_HIP(hipGetLastError());
-}
-
-void launch_m2m(bool cplx, int ncl, int nrhs, int rows, int np2, const double* T, const double* child,
-                double* parent, hipStream_t st) {
-  const int64_t n = static_cast<int64_t>(ncl) * nrhs * rows;
-  if (cplx)
-    hipLaunchKernelGGL(k_m2m<true>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, rows, np2, T, child, parent);
-  else
-    hipLaunchKernelGGL(k_m2m<false>, dim3(nblk(n)), dim3(256), 0, st, ncl, nrhs, rows, np2, T, child, parent);
-  BMI_CHECK_HIP(hipGetLastError());
-}
-
 void launch_coupling_apply(bool cplx, bool fp32, int variant, int ntgt, int nrhs, int cd, const int2* tgt,
                            const int* rp, const int* src, const void* F, int64_t fstride, int64_t fcomp,
                            const int64_t* lvl_off, const double* fw, double* bw, int accumulate, hipStream_t st,
@@ -2501,11 +2428,17 @@ void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, in
   const int S = cplx ? 2 : 1;
   const int maxd = tree_max_depth(rows * S);
   BMI_REQUIRE(maxd >= 1, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
-  const int bot = L, top = L - std::min(2, maxd) > 0 ? L - std::min(2, maxd) : 0;
-  const int D = bot - top;
-  const size_t lvl_doubles = static_cast<size_t>(S) * rows * (((1 << (2 * (D + 1))) - 1) / 3);
-  const size_t anc = sizeof(double) * S * rows * (top + 1);
-  size_t lds = sizeof(double) * lvl_doubles + anc;
+  // subtree depth: two levels, fewer when the level buffers and the ancestors' expansions (one
+  // per level above) do not fit in 64 KB of LDS
+  const int bot = L;
+  int top = 0;
+  size_t lvl_doubles = 0, lds = 0;
+  for (int depth = std::min(2, maxd); depth >= 0; --depth) {
+    top = L - depth > 0 ? L - depth : 0;
+    lvl_doubles = static_cast<size_t>(S) * rows * (((1 << (2 * (bot - top + 1))) - 1) / 3);
+    lds = sizeof(double) * (lvl_doubles + static_cast<size_t>(S) * rows * (top + 1));
+    if (lds <= 64 * 1024) break;
+  }
   BMI_REQUIRE(lds <= 64 * 1024, BMI_UNSUPPORTED, "cluster rows too large for the tree kernels");
   const size_t staged = lds + sizeof(double) * (4 * np2 * np2 + dd * rows);
   const int stage = (nrhs == 1 && staged <= 64 * 1024) ? 1 : 0;
@@ -2526,16 +2459,6 @@ void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, in
   BMI_BWD(false, 16, 36) BMI_BWD(false, 16, 16) BMI_BWD(true, 25, 36) BMI_BWD(true, 25, 16)
   BMI_BWD(true, 16, 36) BMI_BWD(false, 25, 36) BMI_BWD(false, 0, 0) BMI_BWD(true, 0, 0)
 #undef BMI_BWD
-  BMI_CHECK_HIP(hipGetLastError());
-}
-
-void launch_l2l(bool cplx, int nchild, int nrhs, int rows, int np2, const double* T, const double* parent,
-                double* child, hipStream_t st) {
-  const int64_t n = static_cast<int64_t>(nchild) * nrhs * rows;
-  if (cplx)
-    hipLaunchKernelGGL(k_l2l<true>, dim3(nblk(n)), dim3(256), 0, st, nchild, nrhs, rows, np2, T, parent, child);
-  else
-    hipLaunchKernelGGL(k_l2l<false>, dim3(nblk(n)), dim3(256), 0, st, nchild, nrhs, rows, np2, T, parent, child);
   BMI_CHECK_HIP(hipGetLastError());
 }
 

@@ -294,21 +294,24 @@ __global__ void k_gm_givens(const double* __restrict__ h1, const double* __restr
   s.coef[S * (nrhs + c)] = 0.0;
   if (C) s.coef[S * c + 1] = 0.0, s.coef[S * (nrhs + c) + 1] = 0.0;
   if (!s.active[c]) return;
-  double* col = s.H + (static_cast<int64_t>(c) * (m + 1) * m + static_cast<int64_t>(k) * (m + 1)) * S;
-  double* cs = s.cs + static_cast<int64_t>(c) * m * S;
-  double* sn = s.sn + static_cast<int64_t>(c) * m * S;
-  double* g = s.g + static_cast<int64_t>(c) * (m + 1) * S;
-  for (int j = 0; j <= k; ++j) {
+  double* __restrict__ col = s.H + (static_cast<int64_t>(c) * (m + 1) * m + static_cast<int64_t>(k) * (m + 1)) * S;
+  const double* __restrict__ cs0 = s.cs + static_cast<int64_t>(c) * m * S;
+  const double* __restrict__ sn0 = s.sn + static_cast<int64_t>(c) * m * S;
+  // new Hessenberg column h = h1 + h2 (both Gram-Schmidt passes) with the earlier rotations
+  // applied in order; entry j + 1 is carried in registers from rotation j to rotation j + 1 and
+  // the loads of four rotations are in flight together
+  auto hcol = [&](int j, double& r, double& i) {
     const int64_t q = S * (static_cast<int64_t>(j) * nrhs + c);
-    col[S * j] = h1[q] + h2[q];
-    if (C) col[S * j + 1] = h1[q + 1] + h2[q + 1];
-  }
-  col[S * (k + 1)] = hn;
-  if (C) col[S * (k + 1) + 1] = 0.0;
+    r = h1[q] + h2[q];
+    i = C ? h1[q + 1] + h2[q + 1] : 0.0;
+  };
+  double t1r, t1i;
+  hcol(0, t1r, t1i);
+#pragma unroll 4
   for (int j = 0; j < k; ++j) {
-    const double t1r = col[S * j], t1i = C ? col[S * j + 1] : 0.0;
-    const double t2r = col[S * (j + 1)], t2i = C ? col[S * (j + 1) + 1] : 0.0;
-    const double cr = cs[S * j], ci = C ? cs[S * j + 1] : 0.0, sr = sn[S * j], si = C ? sn[S * j + 1] : 0.0;
+    double t2r, t2i;
+    hcol(j + 1, t2r, t2i);
+    const double cr = cs0[S * j], ci = C ? cs0[S * j + 1] : 0.0, sr = sn0[S * j], si = C ? sn0[S * j + 1] : 0.0;
     double ar, ai, br, bi;
     cmul(cr, -ci, t1r, t1i, ar, ai);  // conj(cs) t1 + conj(sn) t2
     cmul(sr, -si, t2r, t2i, br, bi);
@@ -316,11 +319,14 @@ __global__ void k_gm_givens(const double* __restrict__ h1, const double* __restr
     if (C) col[S * j + 1] = ai + bi;
     cmul(-sr, -si, t1r, t1i, ar, ai);  // -sn t1 + cs t2
     cmul(cr, ci, t2r, t2i, br, bi);
-    col[S * (j + 1)] = ar + br;
-    if (C) col[S * (j + 1) + 1] = ai + bi;
+    t1r = ar + br;
+    t1i = ai + bi;
   }
-  const double x1r = col[S * k], x1i = C ? col[S * k + 1] : 0.0;
-  const double x2r = col[S * (k + 1)], x2i = C ? col[S * (k + 1) + 1] : 0.0;
+  double* cs = s.cs + static_cast<int64_t>(c) * m * S;
+  double* sn = s.sn + static_cast<int64_t>(c) * m * S;
+  double* g = s.g + static_cast<int64_t>(c) * (m + 1) * S;
+  const double x1r = t1r, x1i = C ? t1i : 0.0;
+  const double x2r = hn, x2i = 0.0;
   const double den = sqrt((x1r * x1r + x1i * x1i) + (x2r * x2r + x2i * x2i));
   const double cr = den > 0 ? x1r / den : 1.0, ci = den > 0 ? x1i / den : 0.0;
   const double sr = den > 0 ? x2r / den : 0.0, si = den > 0 ? x2i / den : 0.0;
