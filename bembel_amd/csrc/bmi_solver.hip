@@ -37,7 +37,14 @@ void convert_to_fp32(const double* src, float* dst, size_t n, hipStream_t st) {
 // sums in a fixed order (deterministic: the chunk count depends on the sizes only).
 constexpr int DOT_CHUNKS = 64;
 constexpr int DOT_JB = 4;
-static inline int dot_chunks(int nrhs) { return nrhs <= 2 ? 2 * DOT_CHUNKS : DOT_CHUNKS; }
+// chunks per column: 128 for one or two columns; with more columns about 512 workgroups per group
+// of four vectors, each chunk at least 256 rows (a workgroup's reduction then serves several rows
+// per thread instead of one)
+static inline int dot_chunks(int64_t n, int nrhs) {
+  if (nrhs <= 2) return 2 * DOT_CHUNKS;
+  const int64_t want = (512 + nrhs - 1) / nrhs, cap = std::max<int64_t>(1, (n + 255) / 256);
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(want, cap), 2 * DOT_CHUNKS)));
+}
 template <bool C>
 __global__ __launch_bounds__(256) void k_dots_part(const double* __restrict__ V, int64_t vstride, int nv,
                                                     const double* __restrict__ W, int64_t n, int nrhs, int nchunk,
@@ -197,7 +204,7 @@ struct Vecs {
     BMI_CHECK_HIP(hipStreamSynchronize(st));
   }
   void dots_async(const double* V, int nv, const double* W, double* d_out) {
-    const int nchunk = dot_chunks(nrhs);
+    const int nchunk = dot_chunks(n, nrhs);
     const size_t need = static_cast<size_t>(S()) * nv * nrhs * nchunk;
     if (part.n < need) {
       BMI_CHECK_HIP(hipStreamSynchronize(st));  // earlier launches may still use the old buffer
