@@ -120,6 +120,12 @@ struct bmi_h2 {
   // coupling CSR over (level, cluster) targets
   int ntgt = 0;
   DevBuf<int2> d_tgt;
+  // every cluster of levels 0..L in (level, cluster) order with its coupling blocks (empty rows for
+  // clusters without far-field blocks): the many-RHS coupling sweep writes every cluster's sum,
+  // so the local expansions need no clearing pass
+  int ntgt_all = 0;
+  DevBuf<int2> d_tgt_all;
+  DevBuf<int> d_crp_all;
   DevBuf<int> d_crp, d_csrc;
   std::vector<int64_t> lvl_off;  // per level start in per-RHS units (clusters * rows)
   DevBuf<int64_t> d_lvl_off;
@@ -166,7 +172,7 @@ struct bmi_h2 {
   }
   size_t bytes() const {
     return d_dense.bytes() + d_coup.bytes() + d_coup32.bytes() + d_coup16.bytes() + d_dense32.bytes() +
-           d_fly_pts.bytes() + d_fly_org.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() +
+           d_fly_pts.bytes() + d_fly_org.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() + d_tgt_all.bytes() + d_crp_all.bytes() +
            d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_Tt.bytes() + d_momt[0].bytes() + d_momt[1].bytes() + d_stgt.bytes() + d_sgtgt.bytes() + d_srp.bytes() +
            d_ssrc.bytes() + d_sblk.bytes() + d_sgrp.bytes() + d_sgslot.bytes() + d_part.bytes() + d_xs.bytes() +
            d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes() + d_yf.bytes();
@@ -443,11 +449,13 @@ static void apply_super_t(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
                        cc * vcs, h->d_fw[cc].ptr, h->lvl_off.data(), fs);
     }
     for (int rc = 0; rc < vd; ++rc) {
-      device_fill(h->d_bw.ptr, S * h->lvl_total * nrhs, 0.0, fs);
+      // the first component's pass writes every cluster (zero where there are no blocks)
       for (int cc = 0; cc < vd; ++cc)
-        BMI_REQUIRE(launch_multi_cpl_t(C, co.fp32, h->ntgt, nrhs, h->cd, co.ld, h->d_tgt.ptr, h->d_crp.ptr,
-                                       h->d_csrc.ptr, co.F, vd * vd * co.blk, (rc * vd + cc) * co.blk,
-                                       h->d_lvl_off.ptr, h->d_fw[cc].ptr, h->d_bw.ptr, cc > 0 ? 1 : 0, fs),
+        BMI_REQUIRE(launch_multi_cpl_t(C, co.fp32, cc > 0 ? h->ntgt : h->ntgt_all, nrhs, h->cd, co.ld,
+                                       cc > 0 ? h->d_tgt.ptr : h->d_tgt_all.ptr,
+                                       cc > 0 ? h->d_crp.ptr : h->d_crp_all.ptr, h->d_csrc.ptr, co.F,
+                                       vd * vd * co.blk, (rc * vd + cc) * co.blk, h->d_lvl_off.ptr,
+                                       h->d_fw[cc].ptr, h->d_bw.ptr, cc > 0 ? 1 : 0, fs),
                     BMI_UNSUPPORTED, "coupling block size without an RHS-fastest sweep");
       TimedScope ts("matvec_backward", fs);
       launch_backward_t(C, s->npatch, h->L, h->cd, h->np2, h->dd, nrhs, h->d_momt[rc].ptr, h->d_Tt.ptr, h->d_bw.ptr,
@@ -729,6 +737,25 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   h->d_tgt.upload(tgt, st);
   h->d_crp.upload(crp, st);
   h->d_csrc.upload(csrc, st);
+  {
+    std::vector<int2> all;
+    std::vector<int> rp_all(1, 0);
+    size_t t = 0;
+    for (int l = 0; l <= h->L; ++l)
+      for (int c = 0; c < (s->npatch << (2 * l)); ++c) {
+        all.push_back(make_int2(l, c));
+        if (t < tgt.size() && tgt[t].x == l && tgt[t].y == c) {
+          rp_all.push_back(crp[t + 1]);
+          ++t;
+        } else {
+          rp_all.push_back(rp_all.back());
+        }
+      }
+    BMI_REQUIRE(t == tgt.size(), BMI_DEVICE_ERROR, "coupling targets out of level order");
+    h->ntgt_all = static_cast<int>(all.size());
+    h->d_tgt_all.upload(all, st);
+    h->d_crp_all.upload(rp_all, st);
+  }
   if (vd == 1 && (op.kind == BMI_LAPLACE_SL || op.kind == BMI_HELMHOLTZ_SL) && sym_sweep_supported(h->cplx, h->dd, h->cd))
     build_symmetric_lists(h, tgt, crp, csrc, st);
   // FP32 coupling copy: MIXED pads the columns to whole 16-byte row vectors (the sweeps read rows
@@ -1155,6 +1182,7 @@ bmi_status bmi_h2_estimate_bytes(const bmi_space* s, const bmi_op_desc* op, int 
     if (S == 1) b += lvl_total * 16 + lvl_total / cd * 32;            // node tables
   }
   b += (int64_t(dense.size()) + int64_t(lowrank.size())) * 8;                           // CSR indices
+  if (cd > 0) b += lvl_total / cd * 12;                                                  // all-cluster list
   b += std::max(nrhs, 1) * S * (3 * int64_t(s->nsuper) + (vd + 1) * lvl_total) * 8;   // workspace
   b += int64_t(s->nel) * 25 * 16 * 8 + int64_t(s->val.size()) * 24;                    // tables, T
   *bytes = b;
