@@ -531,7 +531,8 @@ class H2Matrix:
         return Y.reshape(-1) if squeeze else Y
 
     def use_graphs(self, enable=True):
-        """Replay the product from captured HIP graphs (default) or launch kernel by kernel."""
+        """Replay products of two or more columns from captured HIP graphs (default) or launch
+        kernel by kernel (single-column products always launch kernel by kernel)."""
         _check(library().bmi_h2_use_graphs(self._h, 1 if enable else 0))
 
     def set_sweep(self, mode):

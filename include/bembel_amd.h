@@ -173,9 +173,11 @@ bmi_status bmi_h2_export_leaves(const bmi_h2* h, double* data);
 bmi_status bmi_h2_matvec(const bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* Y,
                          int64_t ldy, const double* alpha, const double* beta, int mem,
                          void* stream);
-/* Replay the super-space part of the product from a captured HIP graph (one per RHS count,
-   default on) or launch it kernel by kernel.  No counterpart in the reference (its product is
-   CPU code); exposed so callers can compare both modes. */
+/* Replay the super-space part of products with two or more columns from a captured HIP graph
+   (one per RHS count, default on) or launch it kernel by kernel; single-column products always
+   launch kernel by kernel (a graph's completion delays the next kernel on the stream by more
+   than its few launches save).  No counterpart in the reference (its product is CPU code);
+   exposed so callers can compare both modes. */
 bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable);
 /* Coupling sweep of single-RHS products: -1 (default) lets the first product time both forms
    and keep the faster, 0 reads every coupling block, 1 reads only the upper blocks of a
