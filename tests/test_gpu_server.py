@@ -126,3 +126,39 @@ def test_gpu_server_stream_equals_alone(gpu):
         assert ia["status"] == B.OK and im["status"] == B.OK, (ia, im)
         assert rel(xm, xa) <= 1e-8, (r["M"], r["P"], rel(xm, xa))
     assert mixed[-1][1]["cache_hit"] or mixed[-1][1]["batch_requests"] > 1
+
+
+@pytest.mark.parametrize("kind,M,P,prec,nrhs", [("laplace", 4, 2, "fp64", 1), ("laplace", 4, 1, "mixed", 3),
+                                                 ("helmholtz", 4, 2, "mixed", 1), ("helmholtz", 3, 2, "fp64", 12),
+                                                 ("maxwell", 3, 2, "fp64", 2)])
+def test_gpu_admission_estimate_matches_device_bytes(kind, M, P, prec, nrhs):
+    """The admission estimate (block tree on the host, nothing allocated) against the bytes the
+    handle holds after an nrhs-column product, workspace included: the server admits by it, so it
+    must not undershoot, nor overshoot by much (measured: 0.4 to 7.5 % above)."""
+    import torch
+    geo = B.Geometry(scale=(1.0, 0.7, 0.5)) if kind == "maxwell" else B.Geometry()
+    if kind == "laplace":
+        op = B.LaplaceSingleLayerOperator()
+    elif kind == "helmholtz":
+        op = B.HelmholtzSingleLayerOperator()
+        op.set_wavenumber(2.0)
+    else:
+        op = B.MaxwellSingleLayerOperator()
+        op.set_wavenumber(2.0)
+    sp = B.AnsatzSpace(geo, M, P, op)
+    precision = B.MIXED if prec == "mixed" else B.FP64
+    d = op.desc(n_cheb=4, eta=1.6, min_cluster_level=1, precision=precision)
+    est = np.zeros(1, dtype=np.int64)
+    B._check(B.library().bmi_h2_estimate_bytes(sp._h, B.C.byref(d), nrhs,
+                                                est.ctypes.data_as(B.C.POINTER(B.C.c_int64))))
+    H = B.H2Matrix(sp, op, n_cheb=4, precision=precision)
+    n = sp.number_of_dofs
+    dt = torch.complex128 if H.scalar == np.complex128 else torch.float64
+    x = torch.ones((n, nrhs), dtype=dt, device="cuda") if nrhs > 1 else torch.ones(n, dtype=dt, device="cuda")
+    H.matvec(x)
+    info = np.zeros(16, dtype=np.int64)
+    B._check(B.library().bmi_h2_info(H._h, info.ctypes.data_as(B.C.POINTER(B.C.c_int64))))
+    ratio = est[0] / info[8]
+    print("%s M%d P%d %s nrhs %d: estimate %.1f MB, held %.1f MB, ratio %.3f" % (
+        kind, M, P, prec, nrhs, est[0] / 1e6, info[8] / 1e6, ratio))
+    assert 1.0 <= ratio <= 1.2
