@@ -30,7 +30,7 @@ def make(kind, kappa, M, P, np_, precision=B.FP64):
 
 
 CASES = [(O.LAPLACE, 0j, 3, 2, 4), (O.HELMHOLTZ, 4 + 0j, 3, 2, 5), (O.LAPLACE, 0j, 4, 1, 3),
-         (O.HELMHOLTZ, 0j, 2, 1, 9)]
+         (O.HELMHOLTZ, 0j, 2, 1, 9), (O.HELMHOLTZ, 2 + 0.5j, 3, 2, 4)]  # last: damped (complex) wavenumber
 
 
 @pytest.mark.parametrize("kind,kappa,M,P,np_", CASES)
