@@ -414,13 +414,12 @@ __global__ __launch_bounds__(256) void k_backward_tree(int top, int bot, int to_
                                                        int nrhs, const double* __restrict__ mom,
                                                        const double* __restrict__ T, double* __restrict__ bw,
                                                        const int64_t* __restrict__ lvl_off, double* __restrict__ yf,
-                                                       int64_t ldy, int64_t yseg, int lvl_doubles, int stage) {
+                                                       int64_t ldy, int64_t yseg, int lvl_doubles) {
   extern __shared__ double lds[];
   constexpr int S = C ? 2 : 1;
   const int rows = NPC ? NPC : rows_, np2 = NPC ? NPC : np2_, dd = DDC ? DDC : dd_;  // as in k_forward_tree
   const int ctop = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
   const int D = bot - top;
-  (void)stage;
   double* A = lds + lvl_doubles;  // ancestors' expansions: (top + 1) x rows, after the staged matrices
   // ST: T and Mom staged in LDS transposed, so that lanes (consecutive child rows, leaf entries)
   // read consecutive words; in the global layout their reads are np2 or rows words apart, which
@@ -1423,25 +1422,30 @@ __global__ __launch_bounds__(256) void k_sym_gather(int ng, const int2* __restri
                                                     const double* __restrict__ part,
                                                     const int64_t* __restrict__ lvl_off, double* bw) {
   constexpr int S = C ? 2 : 1;
-  constexpr int E = S * CD;
+  constexpr int E = S * CD;  // doubles per target (even: read as 16-byte pairs)
+  static_assert(E % 2 == 0, "pairs of doubles");
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= static_cast<int64_t>(ng) * E) return;
-  const int w = static_cast<int>(gid / E), e = static_cast<int>(gid - static_cast<int64_t>(w) * E);
+  if (gid >= static_cast<int64_t>(ng) * (E / 2)) return;
+  const int w = static_cast<int>(gid / (E / 2)), e = 2 * static_cast<int>(gid - static_cast<int64_t>(w) * (E / 2));
   const int2 T = gtgt[w];
-  double* y = bw + S * (lvl_off[T.x] + static_cast<int64_t>(T.y) * CD) + e;
+  double2* y = reinterpret_cast<double2*>(bw + S * (lvl_off[T.x] + static_cast<int64_t>(T.y) * CD) + e);
   const int q0 = grp[w], q1 = grp[w + 1];
   const double* pp = part + static_cast<int64_t>(q0) * E + e;
-  // eight loads in flight, then added in slot order (a target receives 16 partials on average)
-  double s = *y;
+  // eight 16-byte loads in flight, then added in slot order (a target receives 16 partials on
+  // average)
+  double2 s = *y;
   int q = 0;
   for (; q + 8 <= q1 - q0; q += 8) {
-    double v[8];
+    double2 v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = pp[static_cast<int64_t>(q + u) * E];
+    for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double2*>(pp + static_cast<int64_t>(q + u) * E);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s += v[u];
+    for (int u = 0; u < 8; ++u) s.x += v[u].x, s.y += v[u].y;
   }
-  for (; q < q1 - q0; ++q) s += pp[static_cast<int64_t>(q) * E];
+  for (; q < q1 - q0; ++q) {
+    const double2 v = *reinterpret_cast<const double2*>(pp + static_cast<int64_t>(q) * E);
+    s.x += v.x, s.y += v.y;
+  }
   *y = s;
 }
 
@@ -2450,10 +2454,10 @@ void launch_backward(bool cplx, int npatch, int L, int rows, int np2, int dd, in
   if (!launched && cplx == CC && (NPC == 0 || (rows == NPC && np2 == NPC && dd == DDC))) {                            \
     if (stage)                                                                                                       \
       hipLaunchKernelGGL((k_backward_tree<CC, NPC, DDC, true>), grid, dim3(256), lds, st, top, bot, 1, rows, np2, dd,  \
-                         nrhs, mom, T, bw, lvl_off, yf, ldy, yseg, ld, stage);                                      \
+                         nrhs, mom, T, bw, lvl_off, yf, ldy, yseg, ld);                                             \
     else                                                                                                             \
       hipLaunchKernelGGL((k_backward_tree<CC, NPC, DDC, false>), grid, dim3(256), lds, st, top, bot, 1, rows, np2, dd, \
-                         nrhs, mom, T, bw, lvl_off, yf, ldy, yseg, ld, stage);                                      \
+                         nrhs, mom, T, bw, lvl_off, yf, ldy, yseg, ld);                                             \
     launched = true;                                                                                                 \
   }
   BMI_BWD(false, 16, 36) BMI_BWD(false, 16, 16) BMI_BWD(true, 25, 36) BMI_BWD(true, 25, 16)
@@ -2543,7 +2547,7 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
   if (!launched) return false;
   if (sl.ngather > 0) {
     TimedScope ts("matvec_sym_gather", st);
-    const dim3 gg(static_cast<unsigned>(nblk(static_cast<int64_t>(sl.ngather) * (cplx ? 2 : 1) * cd)));
+    const dim3 gg(static_cast<unsigned>(nblk(static_cast<int64_t>(sl.ngather) * (cplx ? 2 : 1) * cd / 2)));
     if (cplx && cd == 25)
       hipLaunchKernelGGL((k_sym_gather<true, 25>), gg, dim3(256), 0, st, sl.ngather, sl.gtgt, sl.grp, sl.part,
                          lvl_off, bw);
