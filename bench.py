@@ -422,8 +422,8 @@ def main():
                     help="only the Helmholtz 24,576-element products and GMRES steps (profiling workload)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs the H2 path on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print("bench.py measures one GPU: the H2 path runs on a single MI355X "
+              "(sharding over several GPUs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import torch
