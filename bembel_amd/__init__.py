@@ -178,9 +178,15 @@ def _torch_sync():
         pass
 
 
+# temporaries (column-major copies of torch inputs) of products enqueued with sync=False; the
+# library may still read them, so they are kept until the next synchronize()
+_pending = []
+
+
 def synchronize(stream=None):
     """Wait for the library's work on `stream` (default stream if None)."""
     _check(library().bmi_synchronize(_stream_ptr(stream)))
+    del _pending[:]
 
 
 def _is_torch_cuda(x):
@@ -407,7 +413,10 @@ def _columns(x, n, scalar):
         if X.dtype != want:
             X = X.to(want)
         Xc = X.t().contiguous()  # columns contiguous
-        return Xc, X.shape[1], n, MEM_DEVICE, squeeze, "torch"
+        # a copy made here runs on torch's current stream: the caller must order it before the
+        # library reads it (_torch_sync) and keep it alive until the library is done with it
+        copied = Xc.data_ptr() != x.data_ptr()
+        return Xc, X.shape[1], n, MEM_DEVICE, squeeze, "torch" if not copied else "torch-copy"
     a = np.asarray(x)
     squeeze = a.ndim == 1
     A = a.reshape(-1, 1) if squeeze else a
@@ -418,7 +427,7 @@ def _columns(x, n, scalar):
 
 
 def _dptr(a, backend):
-    if backend == "torch":
+    if backend.startswith("torch"):
         return C.cast(C.c_void_p(a.data_ptr()), C.POINTER(C.c_double))
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -495,10 +504,12 @@ class H2Matrix:
         enqueued, so back-to-back products overlap their host overhead with the device work."""
         n = self.rows()
         X, nrhs, ld, mem, squeeze, backend = _columns(x, n, self.scalar)
-        if backend == "torch" and sync:
+        if backend == "torch-copy" or (backend == "torch" and sync):
             _torch_sync()
+        if backend == "torch-copy" and not sync:
+            _pending.append(X)  # released by synchronize(), after the library's reads
         # beta = 0: Y is written, never read
-        if backend == "torch":
+        if backend.startswith("torch"):
             import torch
             Y = torch.empty_like(X)
         else:
@@ -507,9 +518,9 @@ class H2Matrix:
         be = np.zeros(2)
         _check(library().bmi_h2_matvec(self._h, nrhs, _dptr(X, backend), ld, _dptr(Y, backend), ld, _dp(al), _dp(be),
                                        mem, _stream_ptr(stream)))
-        if backend == "torch" and sync:
+        if backend.startswith("torch") and sync:
             synchronize(stream)
-        Y = Y.t() if backend == "torch" else Y.T
+        Y = Y.t() if backend.startswith("torch") else Y.T
         return Y.reshape(-1) if squeeze else Y
 
     def __matmul__(self, x):
@@ -700,7 +711,7 @@ class _Solver:
         n = A.rows()
         B, nrhs, ld, mem, squeeze, backend = _columns(b, n, A.scalar)
         if x0 is None:
-            if backend == "torch":
+            if backend.startswith("torch"):
                 import torch
                 X = torch.zeros_like(B)
             else:
@@ -708,17 +719,17 @@ class _Solver:
             zero = 1
         else:
             X = _columns(x0, n, A.scalar)[0]
-            X = X.clone() if backend == "torch" else X.copy()
+            X = X.clone() if backend.startswith("torch") else X.copy()
             zero = 0
         it = np.zeros(nrhs, dtype=np.int32)
         rs = np.zeros(nrhs)
-        if backend == "torch":
+        if backend.startswith("torch"):
             _torch_sync()
         st = fn(A._h, nrhs, _dptr(B, backend), _dptr(X, backend), zero, _ip(it), _dp(rs), mem, _stream_ptr(stream))
         self._info = _check(st, allow=(NOT_CONVERGED,))
         self._iters = it
         self._resid = rs
-        X = X.t() if backend == "torch" else X.T
+        X = X.t() if backend.startswith("torch") else X.T
         return X.reshape(-1) if squeeze else X
 
 

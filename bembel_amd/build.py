@@ -28,6 +28,7 @@ SOURCES = {
     "bmi_solver.hip": [],
     "bmi_capi.hip": [],
     "bmi_server.hip": [],
+    "bmi_space_dev.cpp": ["-x", "hip"],
 }
 
 

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -22,7 +23,7 @@ static void set_error(const std::string& m) { g_last_error = m; }
 void set_last_error(const std::string& m) { g_last_error = m; }
 
 // ------------------------------------------------------------------ timing registry
-static bool g_timing = false;
+static std::atomic<bool> g_timing{false};
 static std::mutex g_tmu;
 static std::map<std::string, std::pair<double, int64_t>> g_times;
 
@@ -48,24 +49,6 @@ TimedScope::~TimedScope() {
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-}
-
-// ------------------------------------------------------------------ device copies of the space
-void ensure_device_space(const bmi_space* s, hipStream_t st) {
-  if (s->dev_ready) return;
-  std::vector<double> cp;
-  for (const auto& p : s->geo->patches) cp.insert(cp.end(), p.cp.begin(), p.cp.end());
-  s->d_cp.upload(cp, st);
-  s->d_elem_patch.upload(s->elem_patch, st);
-  s->d_elem_llc.upload(s->elem_llc, st);
-  s->d_rowptr.upload(s->rowptr, st);
-  s->d_col.upload(s->col, st);
-  s->d_val.upload(s->val, st);
-  s->d_trowptr.upload(s->trowptr, st);
-  s->d_tcol.upload(s->tcol, st);
-  s->d_tval.upload(s->tval, st);
-  BMI_CHECK_HIP(hipStreamSynchronize(st));
-  s->dev_ready = true;
 }
 
 __global__ void k_fill(double* p, int64_t n, double v) {

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -66,6 +68,16 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Far-field quadrature table of a space (surface points of the (deg+1)^2 rule on every
+// element, fs doubles per point: FF_STRIDE, or FF_STRIDE_TAN with the two tangents).  Written
+// once, then only read until the space is destroyed.
+constexpr int FF_STRIDE = 8;
+constexpr int FF_STRIDE_TAN = 16;
+struct FFTable {
+  int deg = -1, fs = 0;
+  DevBuf<double> buf;
+};
+
 }  // namespace bmi
 
 struct bmi_geometry {
@@ -97,16 +109,19 @@ struct bmi_space {
   std::vector<double> val;
   std::vector<int> trowptr, tcol;
   std::vector<double> tval;
-  // device copies (created on first use)
+  // Device copies, made on the first device use under dev_mu and complete (the upload stream
+  // synchronised) before dev_ready is set; from then on only read, so calls on different
+  // handles that share the space may run concurrently.
+  mutable std::mutex dev_mu;
   mutable bool dev_ready = false;
   mutable bmi::DevBuf<double> d_cp;       // all patches, 4*pd*pd each
   mutable bmi::DevBuf<int> d_elem_patch;
   mutable bmi::DevBuf<double> d_elem_llc;
   mutable bmi::DevBuf<int> d_rowptr, d_col, d_trowptr, d_tcol;
   mutable bmi::DevBuf<double> d_val, d_tval;
-  mutable bmi::DevBuf<double> d_ffield;  // far-field table per element
-  mutable int ffield_deg = -1;
-  mutable int ffield_stride = 0;
+  // far-field tables, one per (degree, stride), added under dev_mu and never freed or
+  // reallocated before the space is destroyed
+  mutable std::vector<std::unique_ptr<bmi::FFTable>> ffields;
 };
 
 namespace bmi {
@@ -132,8 +147,11 @@ void moment_matrix(int P, int cl, int cr, int np, int comp, std::vector<double>&
                    int* rows, int* cols);
 
 // ------------------------------------------------------------------ device side
+// Both return only when the device data are complete; safe to call from several threads.
 void ensure_device_space(const bmi_space* s, hipStream_t st);
-void ensure_ffield(const bmi_space* s, int deg, hipStream_t st, bool tangents);
+const FFTable* ensure_ffield(const bmi_space* s, int deg, hipStream_t st, bool tangents);
+// enqueues the kernel that fills a far-field table of n^2 points per element (bmi_nearfield.hip)
+void launch_ffield_table(const bmi_space* s, int n, int fs, double* out, hipStream_t st);
 // Runs the near-field pair kernels; out is a device array of scalars (double or
 // interleaved complex).  Entry (i, j) of a block lands at off + i*rs + j*cs (in scalars);
 // comp_rs/comp_cs are the offsets between vector components (Maxwell).

@@ -41,8 +41,6 @@ struct NFParams {
 };
 
 // far-field table record: x0 x1 x2 kappa w s0 s1 pad [ds0 ds1 ds2 dt0 dt1 dt2 pad pad]
-constexpr int FF_STRIDE = 8;
-constexpr int FF_STRIDE_TAN = 16;
 
 // evaluation point of the integrand: physical point, weight (kappa*w or w), local coords;
 // Maxwell also keeps the two tangent vectors
@@ -619,42 +617,45 @@ __global__ void k_ffield(const double* __restrict__ cp, int cp_stride, const int
   }
 }
 
-// the table always carries the tangents when they are asked for once (Maxwell), so one
-// table serves every operator of the space
-void ensure_ffield(const bmi_space* s, int deg, hipStream_t st, bool tangents) {
-  ensure_device_space(s, st);
-  const int fs = (tangents || s->ffield_stride == FF_STRIDE_TAN) ? FF_STRIDE_TAN : FF_STRIDE;
-  if (s->ffield_deg == deg && s->ffield_stride == fs) return;
-  const int n = deg + 1;
-  s->d_ffield.alloc(static_cast<size_t>(s->nel) * n * n * fs);
+void launch_ffield_table(const bmi_space* s, int n, int fs, double* out, hipStream_t st) {
   const int64_t total = static_cast<int64_t>(s->nel) * n * n;
   const int blocks = static_cast<int>((total + 255) / 256);
   const double h = 1.0 / s->n;
   const int stride = 4 * s->pd * s->pd;
   if (s->pd == 2)
     hipLaunchKernelGGL(k_ffield<2>, dim3(blocks), dim3(256), 0, st, s->d_cp.ptr, stride,
-                       s->d_elem_patch.ptr, s->d_elem_llc.ptr, s->nel, n, h, fs, s->d_ffield.ptr);
+                       s->d_elem_patch.ptr, s->d_elem_llc.ptr, s->nel, n, h, fs, out);
   else if (s->pd == 5)
     hipLaunchKernelGGL(k_ffield<5>, dim3(blocks), dim3(256), 0, st, s->d_cp.ptr, stride,
-                       s->d_elem_patch.ptr, s->d_elem_llc.ptr, s->nel, n, h, fs, s->d_ffield.ptr);
+                       s->d_elem_patch.ptr, s->d_elem_llc.ptr, s->nel, n, h, fs, out);
   else
     throw Error(BMI_UNSUPPORTED, "patch degree not supported on the device");
   BMI_CHECK_HIP(hipGetLastError());
-  s->ffield_deg = deg;
-  s->ffield_stride = fs;
 }
 
-template <int P, int KIND, int PD, int CLS>
-static void launch_class(const NFPair* d_pairs, int count, int deg, NFParams prm, hipStream_t st) {
-  if (count == 0) return;
-  constexpr int NT = (CLS == 0) ? 64 : 256;
-  const int nq = (deg + 1) * (deg + 1);
+// dynamic LDS of one workgroup of class CLS at quadrature degree deg (each workgroup sizes its
+// point tables from its own pair's degree)
+template <int P, int KIND, int CLS, int NT>
+static size_t class_lds_bytes(int deg) {
+  const int n = deg + 1, nq = n * n;
   size_t lds = sizeof(double) * ((NT / 64) + 1) * BlockAcc<P, KIND>::TOT;
-  const int n = deg + 1;
   if (CLS == 1) lds = std::max(lds, 2 * nq * sizeof(Pt<KIND>));
   if (CLS == 3) lds = std::max(lds, static_cast<size_t>(8 * n * edge_chunk<Pt<KIND>>(n, NT)) * sizeof(Pt<KIND>));
   if (CLS == 4)
     lds = std::max(lds, static_cast<size_t>(4 * nq + 2 * nq * vertex_chunk<Pt<KIND>>(n, NT)) * sizeof(Pt<KIND>));
+  return lds;
+}
+
+// one launch for the pairs [d_pairs, d_pairs + count) of class CLS; degs lists the distinct
+// quadrature degrees among them, and the launch gets the largest LDS any of them needs (the
+// table size is not monotone in the degree: the chunk sizes change with it)
+template <int P, int KIND, int PD, int CLS>
+static void launch_class(const NFPair* d_pairs, int count, const std::vector<int>& degs, NFParams prm,
+                         hipStream_t st) {
+  if (count == 0) return;
+  constexpr int NT = (CLS == 0) ? 64 : 256;
+  size_t lds = 0;
+  for (int d : degs) lds = std::max(lds, class_lds_bytes<P, KIND, CLS, NT>(d));
   BMI_REQUIRE(lds <= 160 * 1024, BMI_UNSUPPORTED, "quadrature degree too large for LDS staging");
   const char* name = CLS == 2 ? "nf_identical" : CLS == 3 ? "nf_edge" : CLS == 4 ? "nf_vertex" : "nf_regular";
   TimedScope ts(name, st);
@@ -669,18 +670,23 @@ static void launch_all(const std::vector<NFPair>& pairs, const NFPair* d_pairs, 
                        hipStream_t st) {
   // pairs are sorted by class, then by falling degree: one launch per class holds every
   // quadrature degree of the class (each workgroup reads its pair's degree; LDS is sized for the
-  // first, largest one), the most expensive pairs first so the tail of the launch is short
+  // largest need among the degrees present), the most expensive pairs first so the tail of the
+  // launch is short
   size_t a = 0;
   while (a < pairs.size()) {
     size_t b = a;
-    while (b < pairs.size() && pairs[b].cls == pairs[a].cls) ++b;
-    const int cnt = static_cast<int>(b - a), deg = pairs[a].deg;
+    std::vector<int> degs;
+    while (b < pairs.size() && pairs[b].cls == pairs[a].cls) {
+      if (degs.empty() || degs.back() != pairs[b].deg) degs.push_back(pairs[b].deg);
+      ++b;
+    }
+    const int cnt = static_cast<int>(b - a);
     switch (pairs[a].cls) {
-      case 0: launch_class<P, KIND, PD, 0>(d_pairs + a, cnt, deg, prm, st); break;
-      case 1: launch_class<P, KIND, PD, 1>(d_pairs + a, cnt, deg, prm, st); break;
-      case 2: launch_class<P, KIND, PD, 2>(d_pairs + a, cnt, deg, prm, st); break;
-      case 3: launch_class<P, KIND, PD, 3>(d_pairs + a, cnt, deg, prm, st); break;
-      default: launch_class<P, KIND, PD, 4>(d_pairs + a, cnt, deg, prm, st); break;
+      case 0: launch_class<P, KIND, PD, 0>(d_pairs + a, cnt, degs, prm, st); break;
+      case 1: launch_class<P, KIND, PD, 1>(d_pairs + a, cnt, degs, prm, st); break;
+      case 2: launch_class<P, KIND, PD, 2>(d_pairs + a, cnt, degs, prm, st); break;
+      case 3: launch_class<P, KIND, PD, 3>(d_pairs + a, cnt, degs, prm, st); break;
+      default: launch_class<P, KIND, PD, 4>(d_pairs + a, cnt, degs, prm, st); break;
     }
     a = b;
   }
@@ -712,7 +718,7 @@ void launch_nearfield(const bmi_space* s, const bmi_op_desc& op, std::vector<NFP
                       int64_t rs, int64_t cs, int64_t comp_rs, int64_t comp_cs, hipStream_t st) {
   ensure_device_space(s, st);
   const int ff = farfield_degree(op.kind, s->P);
-  ensure_ffield(s, ff, st, op.kind == BMI_MAXWELL_SL);
+  const FFTable* tab = ensure_ffield(s, ff, st, op.kind == BMI_MAXWELL_SL);
   std::stable_sort(pairs.begin(), pairs.end(), [](const NFPair& a, const NFPair& b) {
     if (a.cls != b.cls) return a.cls < b.cls;
     return a.deg > b.deg;
@@ -728,9 +734,9 @@ void launch_nearfield(const bmi_space* s, const bmi_op_desc& op, std::vector<NFP
   prm.kre = op.kappa_re;
   prm.kim = op.kappa_im;
   prm.dummy_fn = op.dummy_fn;
-  prm.ffield = s->d_ffield.ptr;
+  prm.ffield = tab->buf.ptr;
   prm.nff = (ff + 1) * (ff + 1);
-  prm.ffs = s->ffield_stride;
+  prm.ffs = tab->fs;
   prm.out = out;
   prm.rs = rs;
   prm.cs = cs;
@@ -753,7 +759,7 @@ void launch_nearfield(const bmi_space* s, const bmi_op_desc& op, std::vector<NFP
 void launch_integrate_jobs(const bmi_space* s, const bmi_op_desc& op, const std::vector<int>& jobs,
                            int qdeg, double* out, hipStream_t st) {
   ensure_device_space(s, st);
-  ensure_ffield(s, qdeg, st, false);
+  const FFTable* tab = ensure_ffield(s, qdeg, st, false);
   std::vector<NFPair> pairs;
   const int nj = static_cast<int>(jobs.size() / 5);
   for (int k = 0; k < nj; ++k) {
@@ -778,9 +784,9 @@ void launch_integrate_jobs(const bmi_space* s, const bmi_op_desc& op, const std:
   prm.kre = op.kappa_re;
   prm.kim = op.kappa_im;
   prm.dummy_fn = op.dummy_fn;
-  prm.ffield = s->d_ffield.ptr;
+  prm.ffield = tab->buf.ptr;
   prm.nff = (qdeg + 1) * (qdeg + 1);
-  prm.ffs = s->ffield_stride;
+  prm.ffs = tab->fs;
   prm.out = out;
   prm.rs = 0;
   prm.cs = 0;

@@ -73,6 +73,9 @@ void or_space_T(const or_space* s, int* rowptr, int* col, double* val);
    LinearOperator/LinearOperatorBase.hpp:56-89, evaluateBilinearForm.hpp:21-40).
    out[4] = class, rot1, rot2, degree (max with far-field degree) */
 void or_pair_info(const or_space* s, int e1, int e2, int out[4], double* dist);
+/* getNearfieldQuadratureDegree of a single-layer operator (LinearOperatorBase.hpp:69-89),
+   before the max with the far-field degree */
+int or_nfield_degree(int P, double dist, int level);
 
 /* evaluateBilinearForm for one element pair.  block: (vd*I2) x (vd*I2) row-major,
    rows = local functions of e1; complex kinds write interleaved re/im. */

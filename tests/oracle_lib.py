@@ -41,6 +41,8 @@ def lib():
         L.or_space_clusters.argtypes = [vp, C.c_int, dp, dp]
         L.or_space_T.argtypes = [vp, ip, ip, dp]
         L.or_pair_info.argtypes = [vp, C.c_int, C.c_int, ip, dp]
+        L.or_nfield_degree.restype = C.c_int
+        L.or_nfield_degree.argtypes = [C.c_int, C.c_double, C.c_int]
         L.or_pair_block.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp]
         L.or_integrate.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_int, C.c_int, C.c_int, dp]
