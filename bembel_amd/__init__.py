@@ -109,6 +109,7 @@ def library():
         "bmi_h2_leaf_matrix": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
         "bmi_h2_export_leaves": (C.c_int, [vp, dp]),
         "bmi_h2_matvec": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, C.c_int, vp]),
+        "bmi_h2_matvec_noalloc": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, vp]),
         "bmi_h2_matvec_super": (C.c_int, [vp, C.c_int, dp, dp, vp]),
         "bmi_h2_use_graphs": (C.c_int, [vp, C.c_int]),
         "bmi_h2_set_sweep": (C.c_int, [vp, C.c_int]),

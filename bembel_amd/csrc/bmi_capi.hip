@@ -27,10 +27,11 @@ static std::atomic<bool> g_timing{false};
 static std::mutex g_tmu;
 static std::map<std::string, std::pair<double, int64_t>> g_times;
 
-bool timing_enabled() { return g_timing; }
+static thread_local bool tl_timing_off = false;
+bool timing_enabled() { return g_timing && !tl_timing_off; }
 
 TimedScope::TimedScope(const char* n, hipStream_t s) : name(n), st(s) {
-  if (!g_timing) return;
+  if (!timing_enabled()) return;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);
   (void)hipEventRecord(a, st);
@@ -198,8 +199,25 @@ static void ensure_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
 
 static bool has_far(const bmi_h2* h) { return h->L >= 0 && h->lvl_total > 0; }
 
+// Workspace of one product: the handle's own buffers (bmi_h2_matvec, the solvers), or a region of
+// the per-device scratch array (bmi_h2_matvec_noalloc).  Only products on the handle's own
+// buffers may create side streams, events and graphs; the others run every kernel on the
+// caller's stream.
+struct Work {
+  double* xs;     // super vector of the input
+  double* ys;     // near-field rows
+  double* yf;     // far-field rows
+  double* fw[2];  // multipole coefficients per FMM component
+  double* bw;     // local coefficients
+  bool own;
+};
+static Work handle_work(bmi_h2* h) {
+  return Work{h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, {h->d_fw[0].ptr, h->d_fw[1].ptr}, h->d_bw.ptr, true};
+}
+
 // far-field chain: forward transformation, coupling, backward transformation, leaf rows -> yf
-static void apply_far(bmi_h2* h, int nrhs, const double* xs, double* yf, bool reduced, hipStream_t st) {
+static void apply_far(bmi_h2* h, const Work& w, int nrhs, const double* xs, double* yf, bool reduced,
+                      hipStream_t st) {
   const bmi_space* s = h->s;
   const bool C = h->cplx;
   const int S = C ? 2 : 1;
@@ -211,16 +229,16 @@ static void apply_far(bmi_h2* h, int nrhs, const double* xs, double* yf, bool re
   for (int cc = 0; cc < h->vd; ++cc) {
     TimedScope ts("matvec_forward", st);
     launch_forward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[cc].ptr, h->d_T.ptr, xs, N, cc * vcs,
-                   h->d_fw[cc].ptr, h->d_lvl_off.ptr, st);
+                   w.fw[cc], h->d_lvl_off.ptr, st);
   }
   for (int rc = 0; rc < h->vd; ++rc) {
-    device_fill(h->d_bw.ptr, S * h->lvl_total * nrhs, 0.0, st);
+    device_fill(w.bw, S * h->lvl_total * nrhs, 0.0, st);
     for (int cc = 0; cc < h->vd; ++cc)
       launch_coupling_apply(C, co.fp32, std::max(h->cpl_variant, 0), h->ntgt, nrhs, rows, h->d_tgt.ptr,
                             h->d_crp.ptr, h->d_csrc.ptr, co.F, h->vd * h->vd * cblk, (rc * h->vd + cc) * cblk,
-                            h->d_lvl_off.ptr, h->d_fw[cc].ptr, h->d_bw.ptr, 1, st, co.Fc, co.ncoarse, co.ld);
+                            h->d_lvl_off.ptr, w.fw[cc], w.bw, 1, st, co.Fc, co.ncoarse, co.ld);
     TimedScope ts("matvec_backward", st);
-    launch_backward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[rc].ptr, h->d_T.ptr, h->d_bw.ptr,
+    launch_backward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[rc].ptr, h->d_T.ptr, w.bw,
                     h->d_lvl_off.ptr, yf, N, rc * vcs, st);
   }
 }
@@ -340,8 +358,8 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
 
 // super-space apply: ys = A_near xs on `st`, yf = A_far xs on the side stream (joined back into
 // `st` before returning); the caller adds the two
-static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, double* yf, bool reduced,
-                        hipStream_t st) {
+static void apply_super(bmi_h2* h, const Work& w, int nrhs, const double* xs, double* ys, double* yf,
+                        bool reduced, hipStream_t st) {
   const bool far = has_far(h);
   if (far && reduced && h->red1 && nrhs == 1) {
     // reduced operator, one RHS: FP32 near-field blocks and the coupling evaluated on the fly
@@ -350,21 +368,23 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
     {
       TimedScope ts("matvec_forward", st);
       launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
-                     h->d_fw[0].ptr, h->d_lvl_off.ptr, st, h->d_bw.ptr);
+                     w.fw[0], h->d_lvl_off.ptr, st, w.bw);
     }
-    const int v = std::max(h->fly_variant, 0);
+    // the variant the first product timed; before that (an allocation-free product on a handle
+    // that has not run bmi_h2_matvec yet) the on-the-fly sweep for real kernels, else the FP32 copy
+    const int v = h->fly_variant >= 0 ? h->fly_variant : (h->fly ? 0 : 2);
     const bool ok =
         v < 2 ? launch_sweep_fly(h->cplx, v, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys,
                                  h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr, h->d_csrc.ptr, h->d_lvl_off.ptr,
-                                 h->d_fly_pts.ptr, h->d_fly_org.ptr, h->op.kappa_re, h->op.kappa_im, h->d_fw[0].ptr,
-                                 h->d_bw.ptr, st)
+                                 h->d_fly_pts.ptr, h->d_fly_org.ptr, h->op.kappa_re, h->op.kappa_im, w.fw[0],
+                                 w.bw, st)
               : launch_sweep_fused(h->cplx, true, ((v - 2) & 3) >> 1, (v - 2) & 1, h->ncl_leaf, h->dd,
                                    h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense32.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr,
-                                   h->d_crp.ptr, h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, h->d_fw[0].ptr,
-                                   h->d_bw.ptr, st, nullptr, 0, true, h->ld32, v >= 6 ? &h->sl : nullptr);
+                                   h->d_crp.ptr, h->d_csrc.ptr, h->d_coup32.ptr, h->d_lvl_off.ptr, w.fw[0],
+                                   w.bw, st, nullptr, 0, true, h->ld32, v >= 6 ? &h->sl : nullptr);
     if (ok) {
       TimedScope ts("matvec_backward", st);
-      launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
+      launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, w.bw,
                       h->d_lvl_off.ptr, yf, N, 0, st);
       return;
     }
@@ -377,18 +397,24 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
     {
       TimedScope ts("matvec_forward", st);
       launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
-                     h->d_fw[0].ptr, h->d_lvl_off.ptr, st, h->d_bw.ptr);
+                     w.fw[0], h->d_lvl_off.ptr, st, w.bw);
     }
     const bool use_sym = h->sym && h->sym_fused && !co.fp32;
     if (launch_sweep_fused(h->cplx, co.fp32, h->fused_leaf_variant, h->fused_cpl_variant, h->ncl_leaf, h->dd,
                            h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr,
-                           h->d_crp.ptr, h->d_csrc.ptr, co.F, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
+                           h->d_crp.ptr, h->d_csrc.ptr, co.F, h->d_lvl_off.ptr, w.fw[0], w.bw, st, co.Fc,
                            co.ncoarse, false, co.ld, use_sym ? &h->sl : nullptr)) {
       TimedScope ts("matvec_backward", st);
-      launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, h->d_bw.ptr,
+      launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, w.bw,
                       h->d_lvl_off.ptr, yf, N, 0, st);
       return;
     }
+  }
+  if (far && !w.own) {
+    // one stream: the two chains write disjoint vectors, so the order gives the same bits
+    apply_far(h, w, nrhs, xs, yf, reduced, st);
+    apply_near(h, nrhs, xs, ys, st);
+    return;
   }
   if (far) {
     if (!h->side_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking));
@@ -396,7 +422,7 @@ static void apply_super(bmi_h2* h, int nrhs, const double* xs, double* ys, doubl
     if (!h->ev_join) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
     BMI_CHECK_HIP(hipEventRecord(h->ev_fork, st));
     BMI_CHECK_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
-    apply_far(h, nrhs, xs, yf, reduced, h->side_stream);
+    apply_far(h, w, nrhs, xs, yf, reduced, h->side_stream);
     BMI_CHECK_HIP(hipEventRecord(h->ev_join, h->side_stream));
   }
   apply_near(h, nrhs, xs, ys, st);
@@ -411,7 +437,7 @@ static bool use_rhs_fastest(const bmi_h2* h, int nrhs, bool reduced) {
   return !(reduced && h->op.precision != BMI_PRECISION_FP64 && h->ncoarse > 0);  // no BF16 levels here
 }
 
-static void apply_super_t(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
+static void apply_super_t(bmi_h2* h, const Work& w, int nrhs, bool reduced, hipStream_t st) {
   const bmi_space* s = h->s;
   const bool C = h->cplx;
   const int S = C ? 2 : 1;
@@ -419,17 +445,22 @@ static void apply_super_t(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
   const int64_t vcs = static_cast<int64_t>(s->nel) * s->I2;
   const bool far = has_far(h);
   if (far) {
-    if (!h->side_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking));
-    if (!h->ev_fork) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
-    if (!h->ev_join) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
-    BMI_CHECK_HIP(hipEventRecord(h->ev_fork, st));
-    BMI_CHECK_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
-    hipStream_t fs = h->side_stream;
+    // the far chain runs on a side stream beside the near-field sweep (own workspace), or after
+    // it on the caller's stream
+    hipStream_t fs = st;
+    if (w.own) {
+      if (!h->side_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking));
+      if (!h->ev_fork) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+      if (!h->ev_join) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+      BMI_CHECK_HIP(hipEventRecord(h->ev_fork, st));
+      BMI_CHECK_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
+      fs = h->side_stream;
+    }
     const CouplingOperand co = coupling_operand(h, reduced);
     for (int cc = 0; cc < vd; ++cc) {
       TimedScope ts("matvec_forward", fs);
-      launch_forward_t(C, s->npatch, h->L, h->cd, h->np2, h->dd, nrhs, h->d_mom[cc].ptr, h->d_T.ptr, h->d_xs.ptr,
-                       cc * vcs, h->d_fw[cc].ptr, h->lvl_off.data(), fs);
+      launch_forward_t(C, s->npatch, h->L, h->cd, h->np2, h->dd, nrhs, h->d_mom[cc].ptr, h->d_T.ptr, w.xs,
+                       cc * vcs, w.fw[cc], h->lvl_off.data(), fs);
     }
     for (int rc = 0; rc < vd; ++rc) {
       // the first component's pass writes every cluster (zero where there are no blocks)
@@ -438,13 +469,13 @@ static void apply_super_t(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
                                        cc > 0 ? h->d_tgt.ptr : h->d_tgt_all.ptr,
                                        cc > 0 ? h->d_crp.ptr : h->d_crp_all.ptr, h->d_csrc.ptr, co.F,
                                        vd * vd * co.blk, (rc * vd + cc) * co.blk, h->d_lvl_off.ptr,
-                                       h->d_fw[cc].ptr, h->d_bw.ptr, cc > 0 ? 1 : 0, fs),
+                                       w.fw[cc], w.bw, cc > 0 ? 1 : 0, fs),
                     BMI_UNSUPPORTED, "coupling block size without an RHS-fastest sweep");
       TimedScope ts("matvec_backward", fs);
-      launch_backward_t(C, s->npatch, h->L, h->cd, h->np2, h->dd, nrhs, h->d_momt[rc].ptr, h->d_Tt.ptr, h->d_bw.ptr,
-                        h->lvl_off.data(), h->d_yf.ptr, rc * vcs, fs);
+      launch_backward_t(C, s->npatch, h->L, h->cd, h->np2, h->dd, nrhs, h->d_momt[rc].ptr, h->d_Tt.ptr, w.bw,
+                        h->lvl_off.data(), w.yf, rc * vcs, fs);
     }
-    BMI_CHECK_HIP(hipEventRecord(h->ev_join, fs));
+    if (w.own) BMI_CHECK_HIP(hipEventRecord(h->ev_join, fs));
   }
   // FP32 near-field blocks with the reduced operator of a mixed handle
   const bool n32 = reduced && h->red1;
@@ -453,10 +484,10 @@ static void apply_super_t(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
   for (int rc = 0; rc < vd; ++rc)
     for (int cc = 0; cc < vd; ++cc)
       BMI_REQUIRE(launch_multi_near_t(C, n32, h->ncl_leaf, nrhs, h->dd, h->d_drp.ptr, h->d_dsrc.ptr, F,
-                                      vd * vd * dblk, (rc * vd + cc) * dblk, h->d_xs.ptr, cc * vcs, h->d_ys.ptr,
+                                      vd * vd * dblk, (rc * vd + cc) * dblk, w.xs, cc * vcs, w.ys,
                                       rc * vcs, cc > 0 ? 1 : 0, st),
                   BMI_UNSUPPORTED, "near-field block size without an RHS-fastest sweep");
-  if (far) BMI_CHECK_HIP(hipStreamWaitEvent(st, h->ev_join, 0));
+  if (far && w.own) BMI_CHECK_HIP(hipStreamWaitEvent(st, h->ev_join, 0));
 }
 
 // ys = A_super xs on the workspace, through a captured graph when possible.  The graph holds
@@ -465,11 +496,12 @@ static void apply_super_t(bmi_h2* h, int nrhs, bool reduced, hipStream_t st) {
 // (four launches, the longest a few hundred microseconds) also runs eagerly: measured, the next
 // kernel on the stream starts about 8 us after a graph ends, more than the launches save.
 static void apply_super_workspace(bmi_h2* h, int nrhs, bool reduced, bool tl, hipStream_t st) {
+  const Work w = handle_work(h);
   auto run = [&](hipStream_t rs) {
     if (tl)
-      apply_super_t(h, nrhs, reduced, rs);
+      apply_super_t(h, w, nrhs, reduced, rs);
     else
-      apply_super(h, nrhs, h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, reduced, rs);
+      apply_super(h, w, nrhs, w.xs, w.ys, w.yf, reduced, rs);
   };
   if (!tl) autotune(h, h->d_xs.ptr, reduced, st);
   if (!h->use_graphs || timing_enabled() || (nrhs == 1 && !tl)) {
@@ -499,24 +531,93 @@ static void apply_super_workspace(bmi_h2* h, int nrhs, bool reduced, bool tl, hi
   BMI_CHECK_HIP(hipGraphLaunch(it->second, st));
 }
 
+// Y = alpha T^T (ys + yf) + beta Y with ys + yf = A_super T X, on the workspace w
+static void apply_dof(bmi_h2* h, const Work& w, int nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                      const double* alpha, const double* beta, hipStream_t st, bool reduced) {
+  const bmi_space* s = h->s;
+  const int64_t N = s->nsuper;
+  const bool tl = use_rhs_fastest(h, nrhs, reduced);
+  if (tl)
+    launch_spmv_tin(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, w.xs, st);
+  else
+    launch_spmv(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, w.xs, N, nullptr,
+                nullptr, st);
+  if (w.own)
+    apply_super_workspace(h, nrhs, reduced, tl, st);
+  else if (tl)
+    apply_super_t(h, w, nrhs, reduced, st);
+  else
+    apply_super(h, w, nrhs, w.xs, w.ys, w.yf, reduced, st);
+  if (tl)
+    launch_spmv_tout(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, w.ys,
+                     has_far(h) ? w.yf : nullptr, Y, ldy, alpha, beta, st);
+  else
+    launch_spmv(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, w.ys, N, Y, ldy, alpha,
+                beta, st, has_far(h) ? w.yf : nullptr, true);
+}
+
 void h2_apply_device(bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy, const double* alpha,
                      const double* beta, hipStream_t st, bool reduced) {
-  const bmi_space* s = h->s;
   ensure_workspace(h, nrhs, st);
-  const int64_t N = s->nsuper;
-  if (use_rhs_fastest(h, nrhs, reduced)) {
-    launch_spmv_tin(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, h->d_xs.ptr, st);
-    apply_super_workspace(h, nrhs, reduced, true, st);
-    launch_spmv_tout(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, h->d_ys.ptr,
-                     has_far(h) ? h->d_yf.ptr : nullptr, Y, ldy, alpha, beta, st);
-    return;
-  }
-  launch_spmv(h->cplx, s->nsuper, nrhs, s->d_rowptr.ptr, s->d_col.ptr, s->d_val.ptr, X, ldx, h->d_xs.ptr, N, nullptr,
-              nullptr, st);
-  apply_super_workspace(h, nrhs, reduced, false, st);
-  launch_spmv(h->cplx, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, h->d_ys.ptr, N, Y, ldy, alpha,
-              beta, st, has_far(h) ? h->d_yf.ptr : nullptr, true);
+  apply_dof(h, handle_work(h), nrhs, X, ldx, Y, ldy, alpha, beta, st, reduced);
 }
+
+// ------------------------------------------------------------------ allocation-free product
+// The workspace of bmi_h2_matvec_noalloc lives in this array: one copy per GPU, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, never the device memory: the
+// device address of each GPU's copy comes from hipGetSymbolAddress (scratch_ptr below).
+static __device__ __align__(256) double scratch[BMI_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64] = {};
+
+// device address of the current GPU's scratch array, looked up once per device
+static double* scratch_ptr() {
+  int dev = 0;
+  BMI_CHECK_HIP(hipGetDevice(&dev));
+  BMI_REQUIRE(dev >= 0 && dev < 64, BMI_UNSUPPORTED, "device ordinal above 63");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    BMI_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BMI_REQUIRE(p != nullptr, BMI_DEVICE_ERROR, "no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+// doubles of a region rounded up to 256 bytes
+static int64_t scratch_round(int64_t n) { return (n + 31) / 32 * 32; }
+
+// workspace regions of an nrhs-column product of h in the scratch array; 0 if it does not fit
+static int64_t noalloc_doubles(const bmi_h2* h, int nrhs) {
+  const int S = h->cplx ? 2 : 1;
+  const int64_t vec = scratch_round(static_cast<int64_t>(S) * h->s->nsuper * nrhs);
+  const int64_t mp = scratch_round(static_cast<int64_t>(S) * h->lvl_total * nrhs);
+  return 3 * vec + (h->vd + 1) * mp;
+}
+
+static Work noalloc_work(const bmi_h2* h, int nrhs, double* base) {
+  const int S = h->cplx ? 2 : 1;
+  const int64_t vec = scratch_round(static_cast<int64_t>(S) * h->s->nsuper * nrhs);
+  const int64_t mp = scratch_round(static_cast<int64_t>(S) * h->lvl_total * nrhs);
+  Work w;
+  w.xs = base;
+  w.ys = base + vec;
+  w.yf = base + 2 * vec;
+  w.fw[0] = base + 3 * vec;
+  w.fw[1] = h->vd > 1 ? base + 3 * vec + mp : nullptr;
+  w.bw = base + 3 * vec + h->vd * mp;
+  w.own = false;
+  return w;
+}
+
+// in-library timing creates events; it is off for the calling thread while an allocation-free
+// product is enqueued
+struct NoTiming {
+  NoTiming() { tl_timing_off = true; }
+  ~NoTiming() { tl_timing_off = false; }
+};
 
 int64_t h2_rows(const bmi_h2* h) { return h->s->ndof; }
 bool h2_complex(const bmi_h2* h) { return h->cplx; }
@@ -1265,6 +1366,28 @@ bmi_status bmi_h2_matvec(const bmi_h2* hc, int nrhs, const double* X, int64_t ld
   BMI_API_END
 }
 
+bmi_status bmi_h2_matvec_noalloc(const bmi_h2* hc, int nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                                 const double* alpha, const double* beta, void* stream) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(hc && X && Y && alpha, BMI_INVALID_ARG, "null argument");
+  BMI_REQUIRE(nrhs >= 0, BMI_INVALID_ARG, "negative nrhs");
+  bmi_h2* h = const_cast<bmi_h2*>(hc);
+  const int64_t D = h->s->ndof;
+  BMI_REQUIRE(ldx >= D && ldy >= D, BMI_INVALID_ARG, "leading dimension smaller than the number of DOFs");
+  if (nrhs == 0) return BMI_OK;
+  const int64_t need = noalloc_doubles(h, nrhs) * static_cast<int64_t>(sizeof(double));
+  if (need > static_cast<int64_t>(BMI_NOALLOC_SCRATCH_BYTES))
+    throw Error(BMI_INVALID_ARG, "bmi_h2_matvec_noalloc: the product of " + std::to_string(nrhs) +
+                                     " columns needs " + std::to_string(need) + " bytes of workspace, above the " +
+                                     std::to_string(static_cast<int64_t>(BMI_NOALLOC_SCRATCH_BYTES)) +
+                                     "-byte scratch array (use fewer columns or bmi_h2_matvec)");
+  NoTiming nt;
+  const Work w = noalloc_work(h, nrhs, scratch_ptr());
+  apply_dof(h, w, nrhs, X, ldx, Y, ldy, alpha, beta, static_cast<hipStream_t>(stream), h2_is_mixed(h));
+  BMI_CHECK_HIP(hipGetLastError());
+  BMI_API_END
+}
+
 bmi_status bmi_h2_matvec_super(const bmi_h2* hc, int nrhs, const double* Xs, double* Ys, void* stream) {
   BMI_API_BEGIN
   BMI_REQUIRE(hc && Xs && Ys && nrhs >= 0, BMI_INVALID_ARG, "bad argument");
@@ -1272,7 +1395,7 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* hc, int nrhs, const double* Xs, dou
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (nrhs == 0) return BMI_OK;
   ensure_workspace(h, nrhs, st);
-  apply_super(h, nrhs, Xs, Ys, h->d_yf.ptr, h2_is_mixed(h), st);
+  apply_super(h, handle_work(h), nrhs, Xs, Ys, h->d_yf.ptr, h2_is_mixed(h), st);
   const int S = h->cplx ? 2 : 1;
   if (has_far(h)) device_add(Ys, h->d_yf.ptr, static_cast<int64_t>(S) * h->s->nsuper * nrhs, st);
   BMI_CHECK_HIP(hipStreamSynchronize(st));

@@ -187,6 +187,19 @@ bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable);
 bmi_status bmi_h2_set_sweep(bmi_h2* h, int mode);
 /* wait for the library's work on `stream` (NULL: the default stream) */
 bmi_status bmi_synchronize(void* stream);
+/* Allocation-free form of bmi_h2_matvec for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops); no counterpart in the reference.  Device X and Y
+   only; the product is enqueued on `stream` and the call returns without waiting.  The product's
+   workspace (super vectors and multipole coefficients: 8 bytes x (3 x super DOFs + (vd + 1) x
+   multipole rows) x nrhs, doubled for complex operators) lives in one fixed scratch array of
+   BMI_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; larger products are refused
+   with BMI_INVALID_ARG.  Every call on a GPU shares that GPU's array: calls that overlap on one
+   GPU (several streams or threads) race and must be serialised by the caller.  Every kernel runs
+   on `stream` (no side stream, no graph, no in-library timing); results equal bmi_h2_matvec's to
+   rounding (the same kernels, on the variants bmi_h2_matvec timed if it has run). */
+#define BMI_NOALLOC_SCRATCH_BYTES (64ull << 20)
+bmi_status bmi_h2_matvec_noalloc(const bmi_h2* h, int nrhs, const double* X, int64_t ldx, double* Y,
+                                 int64_t ldy, const double* alpha, const double* beta, void* stream);
 /* same operator applied in the super space (no T, no T^T); device pointers only */
 bmi_status bmi_h2_matvec_super(const bmi_h2* h, int nrhs, const double* Xs, double* Ys,
                                void* stream);
