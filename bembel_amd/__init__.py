@@ -20,7 +20,9 @@ import numpy as np
 __all__ = [
     "BembelError", "Geometry", "AnsatzSpace", "LaplaceSingleLayerOperator", "HelmholtzSingleLayerOperator",
     "MaxwellSingleLayerOperator", "DummyOperator", "DiscreteOperator", "H2Matrix", "DenseMatrix",
-    "DiscreteLinearForm", "DirichletTrace", "PlaneWaves", "RotatedTangentialTrace", "GMRES", "ConjugateGradient", "library", "timing",
+    "DiscreteLinearForm", "DirichletTrace", "PlaneWaves", "RotatedTangentialTrace", "HelmholtzFundamentalSolution",
+    "HarmonicFunction", "DiscretePotential", "LaplaceSingleLayerPotential", "HelmholtzSingleLayerPotential",
+    "MaxwellSingleLayerPotential", "GMRES", "ConjugateGradient", "library", "timing",
     "SPHERE_DAT", "FP64", "MIXED", "MIXED_BF16", "Server", "synchronize",
 ]
 
@@ -121,6 +123,7 @@ def library():
         "bmi_stream_destroy": (C.c_int, [vp]),
         "bmi_synchronize": (C.c_int, [vp]),
         "bmi_rhs_assemble": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp, C.c_int, vp]),
+        "bmi_potential_evaluate": (C.c_int, [vp, C.POINTER(OpDesc), dp, C.c_int, dp, dp, C.c_int, vp]),
         "bmi_solve_gmres": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, ip, dp,
                                       C.c_int, vp]),
         "bmi_solve_cg": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, ip, dp, C.c_int, vp]),
@@ -626,6 +629,28 @@ class PlaneWaves:
         self.kappa = float(kappa)
 
 
+class HelmholtzFundamentalSolution:
+    """g_k(x) = exp(-i kappa |x - c_k|) / |x - c_k| (Data::HelmholtzFundamentalSolution,
+    examples/Data.hpp:37-44; one column per center)."""
+
+    def __init__(self, centers, kappa):
+        c = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+        self.fn = 3
+        self.params = np.ascontiguousarray(c)
+        self.nrhs = c.shape[0]
+        self.kappa = float(kappa)
+
+
+class HarmonicFunction:
+    """g(x) = 4 x^2 - 3 y^2 - z^2 (Data::HarmonicFunction, examples/Data.hpp:14-16)."""
+
+    def __init__(self):
+        self.fn = 4
+        self.params = np.zeros((1, 3))
+        self.nrhs = 1
+        self.kappa = 0.0
+
+
 class RotatedTangentialTrace:
     """Rotated tangential trace of Hertz dipole fields for the EFIE right-hand side
     (LinearForm/RotatedTangentialTrace.hpp with the examples/Data.hpp:76-90 Dipole); one column per
@@ -653,7 +678,7 @@ class DiscreteLinearForm:
 
     def compute(self, stream=None, device=False):
         n = self.space.number_of_dofs
-        cplx = self.data.fn != 0
+        cplx = self.data.fn not in (0, 4)
         S = 2 if cplx else 1
         nrhs = self.data.nrhs
         if device:
@@ -676,6 +701,75 @@ class DiscreteLinearForm:
 
     def get_discrete_linear_form(self):
         return self._value
+
+
+# ---------------------------------------------------------------------------- potentials
+class _Potential:
+    kind = None
+
+    def __init__(self):
+        self._kappa = 0j
+
+    def set_wavenumber(self, k):
+        self._kappa = complex(k)
+
+    def get_wavenumber(self):
+        return self._kappa
+
+
+class LaplaceSingleLayerPotential(_Potential):
+    """Laplace/SingleLayerPotential.hpp: 1/(4 pi |x - y|)."""
+    kind = LAPLACE
+
+
+class HelmholtzSingleLayerPotential(_Potential):
+    """Helmholtz/SingleLayerPotential.hpp: exp(-i kappa |x - y|)/(4 pi |x - y|)."""
+    kind = HELMHOLTZ
+
+
+class MaxwellSingleLayerPotential(_Potential):
+    """Maxwell/SingleLayerPotential.hpp: kernel times the vector density plus the kernel gradient
+    times its surface divergence over kappa^2."""
+    kind = MAXWELL
+
+
+class DiscretePotential:
+    """DiscretePotential<Potential, Op>(ansatz_space) (Potential/DiscretePotential.hpp:20-120):
+    set_cauchy_data(rho), then evaluate(points) -> (n,) for Laplace/Helmholtz, (n, 3) for Maxwell."""
+
+    def __init__(self, potential, space):
+        self._pot = potential
+        self.space = space
+        self._rho = None
+
+    def get_potential(self):
+        return self._pot
+
+    def set_cauchy_data(self, rho):
+        cplx = self._pot.kind != LAPLACE
+        r = rho.detach().cpu().numpy() if _is_torch_cuda(rho) else np.asarray(rho)
+        self._rho = np.ascontiguousarray(r.reshape(-1), dtype=np.complex128 if cplx else np.float64)
+        if len(self._rho) != self.space.number_of_dofs:
+            raise BembelError(INVALID_ARG, "Cauchy data must have one entry per DOF")
+
+    def evaluate(self, points, stream=None):
+        if self._rho is None:
+            raise BembelError(INVALID_ARG, "set_cauchy_data() has not been called")
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        n = len(pts)
+        kind = self._pot.kind
+        cplx = kind != LAPLACE
+        dim = 3 if kind == MAXWELL else 1
+        d = OpDesc()
+        _check(library().bmi_op_desc_default(kind, C.byref(d)))
+        k = self._pot.get_wavenumber()
+        d.kappa_re, d.kappa_im = k.real, k.imag
+        out = np.zeros(n * dim * (2 if cplx else 1))
+        rin = self._rho.view(np.float64) if cplx else self._rho
+        _check(library().bmi_potential_evaluate(self.space._h, C.byref(d), _dp(rin), n, _dp(pts), _dp(out), MEM_HOST,
+                                                _stream_ptr(stream)))
+        v = (out.view(np.complex128) if cplx else out).reshape(dim, n).T
+        return v[:, 0].copy() if dim == 1 else v.copy()
 
 
 # ---------------------------------------------------------------------------- solvers

@@ -29,6 +29,7 @@ SOURCES = {
     "bmi_capi.hip": [],
     "bmi_server.hip": [],
     "bmi_space_dev.cpp": ["-x", "hip"],
+    "bmi_potential.hip": [],
 }
 
 

@@ -818,9 +818,9 @@ __device__ __forceinline__ void dipole_field(const double* X, double kappa, cons
 }
 
 // DiscreteLinearForm::compute (LinearForm/DiscreteLinearForm.hpp:44-83) with DirichletTrace
-// (fn 0: point source, fn 1: plane waves) or RotatedTangentialTrace of a dipole field
-// (fn 2, RotatedTangentialTrace.hpp:39-82): super-space load vector, one thread per
-// (element, column), (P+2)^2-point rule
+// (fn 0: point source, fn 1: plane waves, fn 3: Helmholtz fundamental solution, fn 4: harmonic
+// polynomial) or RotatedTangentialTrace of a dipole field (fn 2, RotatedTangentialTrace.hpp:39-82):
+// super-space load vector, one thread per (element, column), (P+2)^2-point rule
 template <int PD, int P>
 __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* __restrict__ elem_patch,
                       const double* __restrict__ elem_llc, int nel, double h, int fn, const double* __restrict__ prm,
@@ -850,6 +850,17 @@ __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* _
       const double* d = prm + 3 * k;
       sincos(kappa * (d[0] * X[0] + d[1] * X[1] + d[2] * X[2]), &fi, &fr);
       fr *= nn * w, fi *= nn * w;
+    } else if (fn == 3) {
+      // Helmholtz fundamental solution exp(-i kappa r) / r (examples/Data.hpp:37-44)
+      const double* x0 = prm + 3 * k;
+      const double d0 = X[0] - x0[0], d1 = X[1] - x0[1], d2 = X[2] - x0[2];
+      const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      double sn, cs;
+      sincos(kappa * r, &sn, &cs);
+      fr = cs * nn * w / r, fi = -sn * nn * w / r;
+    } else if (fn == 4) {
+      // harmonic polynomial 4x^2 - 3y^2 - z^2 (examples/Data.hpp:14-16)
+      fr = (4 * X[0] * X[0] - 3 * X[1] * X[1] - X[2] * X[2]) * nn * w;
     } else {
       const double* d = prm + 6 * k;
       double Er[3], Ei[3];
@@ -882,7 +893,7 @@ __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* _
   const int64_t vcs = static_cast<int64_t>(nel) * I2;  // second vector component (fn 2)
   for (int m = 0; m < I2; ++m) {
     const int64_t o = static_cast<int64_t>(k) * nsuper + static_cast<int64_t>(e) * I2 + m;
-    if (fn == 0) {
+    if (fn == 0 || fn == 4) {
       out[o] = ar[m];
     } else {
       out[2 * o] = ar[m], out[2 * o + 1] = ai[m];
@@ -901,12 +912,13 @@ bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, in
                             int mem, void* stream) {
   try {
     BMI_REQUIRE(s && params && out && nrhs >= 1, BMI_INVALID_ARG, "bad argument");
-    BMI_REQUIRE(fn >= 0 && fn <= 2, BMI_INVALID_ARG, "unknown data function");
+    BMI_REQUIRE(fn >= 0 && fn <= 4, BMI_INVALID_ARG, "unknown data function");
     BMI_REQUIRE((fn == 2) == (s->form == BMI_FORM_DIVCONFORMING), BMI_INVALID_ARG,
                 "Dirichlet traces need a scalar space, the rotated tangential trace a DivConforming one");
     hipStream_t st = static_cast<hipStream_t>(stream);
     ensure_device_space(s, st);
-    const int S = fn == 0 ? 1 : 2;
+    const bool real = fn == 0 || fn == 4;
+    const int S = real ? 1 : 2;
     DevBuf<double> prm, sup, dout;
     const int per = fn == 2 ? 6 : 3;
     prm.upload(std::vector<double>(params, params + per * nrhs), st);
@@ -929,7 +941,7 @@ bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, in
       dout.alloc(static_cast<size_t>(S) * s->ndof * nrhs);
       y = dout.ptr;
     }
-    launch_spmv(fn != 0, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, sup.ptr, s->nsuper, y, s->ndof,
+    launch_spmv(!real, s->ndof, nrhs, s->d_trowptr.ptr, s->d_tcol.ptr, s->d_tval.ptr, sup.ptr, s->nsuper, y, s->ndof,
                 nullptr, nullptr, st);
     if (mem != BMI_MEM_DEVICE)
       BMI_CHECK_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, st));

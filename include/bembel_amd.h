@@ -212,9 +212,25 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* h, int nrhs, const double* Xs, doub
    fn 2: rotated tangential trace of a Hertz dipole field for the EFIE
          (LinearForm/RotatedTangentialTrace.hpp:39-82 with examples/Data.hpp:76-90 Dipole):
          params[6k..6k+5] = position, moment; DivConforming spaces; complex
+   fn 3: g(x) = exp(-i kappa |x - p_k|) / |x - p_k| (Data::HelmholtzFundamentalSolution,
+         examples/Data.hpp:37-44; p_k = params[3k..3k+2]; complex)
+   fn 4: g(x) = 4 x^2 - 3 y^2 - z^2 (Data::HarmonicFunction, examples/Data.hpp:14-16; real;
+         params: 3 ignored doubles per column)
    out: ndof x nrhs, host or device (mem) */
 bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, int nrhs,
                             double kappa, double* out, int mem, void* stream);
+
+/* ------------------------------------------------------------------ potentials */
+/* DiscretePotential<{Laplace,Helmholtz,Maxwell}SingleLayerPotential, Op>::evaluate
+   (Potential/DiscretePotential.hpp:49-103; Laplace/SingleLayerPotential.hpp,
+   Helmholtz/SingleLayerPotential.hpp, Maxwell/SingleLayerPotential.hpp): the single-layer
+   potential of the Cauchy data rho (ndof entries, complex interleaved for Helmholtz and Maxwell)
+   at npoints points (3 doubles each), with the (P+2)^2-point rule per element.  op gives the
+   kind and the wavenumber.  out: npoints x dim column-major (dim 1, or 3 for Maxwell), complex
+   interleaved for Helmholtz and Maxwell.  rho, points and out are host or device pointers (mem). */
+bmi_status bmi_potential_evaluate(const bmi_space* s, const bmi_op_desc* op, const double* rho,
+                                  int npoints, const double* points, double* out, int mem,
+                                  void* stream);
 
 /* ------------------------------------------------------------------ solvers */
 /* restarted GMRES (consumer: Eigen::GMRES in examples/HelmholtzSingleLayerH2.cpp:82-85)

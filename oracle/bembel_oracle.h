@@ -124,6 +124,14 @@ void or_h2_matvec_super(const or_h2* h, int nrhs, const double* x, double* y);
    DivConforming spaces, LinearForm/RotatedTangentialTrace.hpp, examples/Data.hpp:76-90),
    out: ndof entries (real) or 2*ndof (complex) */
 void or_rhs(const or_space* s, int fn, const double* p, double kre, double* out);
+/* (continued) fn 3: Helmholtz fundamental solution exp(-i kappa |x-p|)/|x-p| (complex),
+   fn 4: the harmonic polynomial 4x^2 - 3y^2 - z^2 (real; p unused) (examples/Data.hpp) */
+
+/* DiscretePotential<*SingleLayerPotential, Op>::evaluate (Potential/DiscretePotential.hpp:49-103)
+   of the Cauchy data rho (ndof, complex interleaved for Helmholtz/Maxwell) at npts points;
+   out: npts x dim column-major (dim 3 for Maxwell), complex interleaved for Helmholtz/Maxwell */
+void or_potential(const or_space* s, int kind, double kre, double kim, const double* rho, int npts,
+                  const double* pts, double* out);
 
 /* solvers (reference consumers are Eigen CG/GMRES; restated textbook algorithms) */
 int or_cg(int n, const double* A, const double* b, double* x, double tol, int maxit);

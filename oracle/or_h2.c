@@ -604,7 +604,7 @@ void or_rhs(const or_space* s, int fn, const double* p, double kre, double* out)
     return;
   }
   int P = s->P, I = P + 1, I2 = s->I2, deg = P + 1, nq1 = deg + 1, nq = nq1 * nq1;
-  int cplx = fn == 1;
+  int cplx = fn == 1 || fn == 3;
   int c = cplx ? 2 : 1;
   const double* qx = or_gl_nodes(nq1);
   const double* qw = or_gl_weights(nq1);
@@ -624,9 +624,17 @@ void or_rhs(const or_space* s, int fn, const double* p, double kre, double* out)
       if (fn == 0) {
         double d0 = q[3] - p[0], d1 = q[4] - p[1], d2 = q[5] - p[2];
         f = 1. / sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-      } else {
+      } else if (fn == 1) {
         double ph = kre * ((p[0] * q[3] + p[1] * q[4]) + p[2] * q[5]);
         f = cos(ph) + OR_I * sin(ph);
+      } else if (fn == 3) {
+        /* Data::HelmholtzFundamentalSolution (examples/Data.hpp:37-44): exp(-i k r)/r */
+        double d0 = q[3] - p[0], d1 = q[4] - p[1], d2 = q[5] - p[2];
+        double r = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+        f = cexp(-OR_I * kre * r) / r;
+      } else {
+        /* Data::HarmonicFunction (examples/Data.hpp:14-16) */
+        f = 4 * q[3] * q[3] - 3 * q[4] * q[4] - q[5] * q[5];
       }
       double complex integrand = f * kap * q[2];
       double X[32], Y[32];
@@ -648,6 +656,89 @@ void or_rhs(const or_space* s, int fn, const double* p, double kre, double* out)
     for (int k = s->rowptr[r]; k < s->rowptr[r + 1]; ++k)
       for (int q = 0; q < c; ++q) out[c * s->colidx[k] + q] += s->vals[k] * sup[c * r + q];
   free(sup);
+}
+
+/* ------------------------------------------------------------ potentials */
+/* DiscretePotential<*SingleLayerPotential, Op>::evaluate (Potential/DiscretePotential.hpp:49-103):
+   rule GS[P+1] on every element, weight h^2 w (map2surface), Cauchy data through
+   FunctionEvaluator (AnsatzSpace/FunctionEvaluatorEval.hpp: value c^T phi / h, vector value
+   (dsx c_s + dtx c_t) / h, divergence (phi_dx c_s + phi_dy c_t) / h^2).  Integrands:
+   Laplace/Helmholtz K(x,y) rho(y) |dsx x dtx| w (Laplace/SingleLayerPotential.hpp:44-78,
+   Helmholtz/SingleLayerPotential.hpp:44-78); Maxwell (K j(y) + grad_x K div(y) / kappa^2) w
+   without the surface measure (Maxwell/SingleLayerPotential.hpp:44-104).
+   out: npts x dim column-major (dim 1 or 3), complex interleaved for Helmholtz/Maxwell. */
+void or_potential(const or_space* s, int kind, double kre, double kim, const double* rho, int npts,
+                  const double* pts, double* out) {
+  int P = s->P, I = P + 1, I2 = s->I2, nq1 = P + 2, nq = nq1 * nq1;
+  int cplx = kind != OR_LAPLACE, vd = s->vd, dim = kind == OR_MAXWELL ? 3 : 1;
+  double complex kappa = kre + OR_I * kim, kappa2 = kappa * kappa;
+  const double* qx = or_gl_nodes(nq1);
+  const double* qw = or_gl_weights(nq1);
+  size_t N = s->nsuper;
+  /* super coefficients T rho */
+  double complex* sup = (double complex*)calloc(N, sizeof(double complex));
+  for (size_t r = 0; r < N; ++r)
+    for (int k = s->rowptr[r]; k < s->rowptr[r + 1]; ++k) {
+      int j = s->colidx[k];
+      sup[r] += s->vals[k] * (cplx ? rho[2 * j] + OR_I * rho[2 * j + 1] : rho[j]);
+    }
+  size_t vcs = (size_t)s->nel * I2;
+  double complex* acc = (double complex*)calloc((size_t)npts * dim, sizeof(double complex));
+  for (int e = 0; e < s->nel; ++e) {
+    int node = s->elem[e];
+    double h = 1. / (1 << s->nodes[node].level);
+    int id = s->nodes[node].id;
+    for (int i = 0; i < nq; ++i) {
+      double xi[2] = {qx[i / nq1], qx[i % nq1]}, q[12];
+      or_map2surface(s, node, xi, h * h * (qw[i / nq1] * qw[i % nq1]), q);
+      double X[32], Y[32], DX[32], DY[32];
+      or_bernstein(P, xi[0], X);
+      or_bernstein(P, xi[1], Y);
+      or_bernstein_der(P, xi[0], DX);
+      or_bernstein_der(P, xi[1], DY);
+      double complex v0 = 0, v1 = 0, dv = 0;
+      for (int iy = 0; iy < I; ++iy)
+        for (int ix = 0; ix < I; ++ix) {
+          int m = iy * I + ix;
+          double complex c0 = sup[(size_t)id * I2 + m];
+          double complex c1 = vd > 1 ? sup[vcs + (size_t)id * I2 + m] : 0;
+          v0 += c0 * (X[ix] * Y[iy]);
+          v1 += c1 * (X[ix] * Y[iy]);
+          dv += c0 * (DX[ix] * Y[iy]) + c1 * (X[ix] * DY[iy]);
+        }
+      double ws = q[2];
+      for (int t = 0; t < npts; ++t) {
+        const double* x = pts + 3 * t;
+        double d0 = x[0] - q[3], d1 = x[1] - q[4], d2 = x[2] - q[5];
+        double r = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+        if (kind == OR_LAPLACE) {
+          double c0 = q[7] * q[11] - q[8] * q[10], c1 = q[8] * q[9] - q[6] * q[11], c2 = q[6] * q[10] - q[7] * q[9];
+          double kap = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+          acc[t] += 1. / 4. / OR_PI / r * (v0 / h) * kap * ws;
+        } else if (kind == OR_HELMHOLTZ) {
+          double c0 = q[7] * q[11] - q[8] * q[10], c1 = q[8] * q[9] - q[6] * q[11], c2 = q[6] * q[10] - q[7] * q[9];
+          double kap = sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+          acc[t] += cexp(-OR_I * kappa * r) / 4. / OR_PI / r * (v0 / h) * kap * ws;
+        } else {
+          double complex K = cexp(-OR_I * kappa * r) / 4. / OR_PI / r;
+          double complex G = cexp(-OR_I * kappa * r) * (-1. - OR_I * kappa * r) / 4. / OR_PI / (r * r * r);
+          double complex div = dv / h / h;
+          double dd[3] = {d0, d1, d2};
+          for (int k = 0; k < 3; ++k) {
+            double complex val = (q[6 + k] * v0 + q[9 + k] * v1) / h;
+            acc[(size_t)k * npts + t] += (K * val + 1. / kappa2 * (G * dd[k]) * div) * ws;
+          }
+        }
+      }
+    }
+  }
+  for (size_t k = 0; k < (size_t)npts * dim; ++k) {
+    if (cplx)
+      out[2 * k] = creal(acc[k]), out[2 * k + 1] = cimag(acc[k]);
+    else
+      out[k] = creal(acc[k]);
+  }
+  free(acc), free(sup);
 }
 
 /* ------------------------------------------------------------ solvers */

@@ -67,6 +67,7 @@ def lib():
         L.or_h2_matvec.argtypes = [vp, C.c_int, dp, dp]
         L.or_h2_matvec_super.argtypes = [vp, C.c_int, dp, dp]
         L.or_rhs.argtypes = [vp, C.c_int, dp, C.c_double, dp]
+        L.or_potential.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp]
         L.or_cg.restype = C.c_int
         L.or_cg.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_int]
         L.or_gmres_dense.restype = C.c_int
@@ -213,8 +214,22 @@ class Space:
             return A.view(np.complex128).reshape(D, D).T.copy()
         return A.reshape(D, D).T.copy()
 
+    def potential(self, kind, rho, points, kappa=0j):
+        """DiscretePotential of the Cauchy data rho at points (n, 3): (n,) or (n, 3) for Maxwell"""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        n = len(pts)
+        dim = 3 if kind == MAXWELL else 1
+        cplx = kind != LAPLACE
+        r = np.ascontiguousarray(rho, dtype=np.complex128 if cplx else np.float64)
+        rin = r.view(np.float64) if cplx else r
+        out = np.zeros(n * dim * (2 if cplx else 1))
+        lib().or_potential(self.h, kind, kappa.real, kappa.imag, _dp(rin), n, _dp(pts), _dp(out))
+        v = out.view(np.complex128) if cplx else out
+        v = v.reshape(dim, n).T
+        return v[:, 0].copy() if dim == 1 else v.copy()
+
     def rhs(self, fn, p, kappa=0.0):
-        cplx = fn in (1, 2)
+        cplx = fn in (1, 2, 3)
         out = np.zeros(self.ndof * (2 if cplx else 1))
         pa = np.asarray(p, dtype=np.float64)
         lib().or_rhs(self.h, fn, _dp(pa), kappa, _dp(out))
