@@ -21,7 +21,7 @@ __all__ = [
     "BembelError", "Geometry", "AnsatzSpace", "LaplaceSingleLayerOperator", "HelmholtzSingleLayerOperator",
     "MaxwellSingleLayerOperator", "DummyOperator", "DiscreteOperator", "H2Matrix", "DenseMatrix",
     "DiscreteLinearForm", "DirichletTrace", "PlaneWaves", "RotatedTangentialTrace", "HelmholtzFundamentalSolution",
-    "HarmonicFunction", "DiscretePotential", "LaplaceSingleLayerPotential", "HelmholtzSingleLayerPotential",
+    "HarmonicFunction", "FunctionTrace", "DiscretePotential", "LaplaceSingleLayerPotential", "HelmholtzSingleLayerPotential",
     "MaxwellSingleLayerPotential", "GMRES", "ConjugateGradient", "library", "timing",
     "SPHERE_DAT", "FP64", "MIXED", "MIXED_BF16", "Server", "synchronize",
 ]
@@ -124,6 +124,8 @@ def library():
         "bmi_synchronize": (C.c_int, [vp]),
         "bmi_rhs_assemble": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp, C.c_int, vp]),
         "bmi_potential_evaluate": (C.c_int, [vp, C.POINTER(OpDesc), dp, C.c_int, dp, dp, C.c_int, vp]),
+        "bmi_linear_form_points": (C.c_int, [vp, i64p, dp]),
+        "bmi_rhs_from_values": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, C.c_int, vp]),
         "bmi_solve_gmres": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, ip, dp,
                                       C.c_int, vp]),
         "bmi_solve_cg": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, ip, dp, C.c_int, vp]),
@@ -651,6 +653,22 @@ class HarmonicFunction:
         self.kappa = 0.0
 
 
+class FunctionTrace:
+    """Data given as a Python function of the surface points, as DirichletTrace<Scalar>::
+    set_function / RotatedTangentialTrace<Scalar>::set_function take an arbitrary function
+    (LinearForm/DirichletTrace.hpp:31-70, LinearForm/RotatedTangentialTrace.hpp:30-87).
+    f maps an (n, 3) array of points to (n,) scalars (real or complex) or, with vector=True, to
+    (n, 3) complex field values whose rotated tangential trace is taken (DivConforming spaces).
+    One column; the function is called once with every point of the (P+2)^2-point rule."""
+
+    def __init__(self, f, scalar=np.float64, vector=False):
+        self.f = f
+        self.vector = vector
+        self.fn = 7 if vector else (5 if scalar == np.float64 else 6)
+        self.nrhs = 1
+        self.kappa = 0.0
+
+
 class RotatedTangentialTrace:
     """Rotated tangential trace of Hertz dipole fields for the EFIE right-hand side
     (LinearForm/RotatedTangentialTrace.hpp with the examples/Data.hpp:76-90 Dipole); one column per
@@ -676,11 +694,31 @@ class DiscreteLinearForm:
         self.space = space
         self._value = None
 
+    def _values(self):
+        """data of a FunctionTrace at the linear-form points, flattened for bmi_rhs_from_values"""
+        npts = C.c_int64()
+        _check(library().bmi_linear_form_points(self.space._h, C.byref(npts), None))
+        pts = np.zeros((npts.value, 3))
+        _check(library().bmi_linear_form_points(self.space._h, C.byref(npts), _dp(pts)))
+        v = np.asarray(self.data.f(pts))
+        if self.data.fn == 5:
+            return np.ascontiguousarray(v.reshape(-1), dtype=np.float64)
+        if self.data.fn == 6:
+            return np.ascontiguousarray(v.reshape(-1), dtype=np.complex128).view(np.float64)
+        return np.ascontiguousarray(v.reshape(-1, 3), dtype=np.complex128).reshape(-1).view(np.float64)
+
     def compute(self, stream=None, device=False):
         n = self.space.number_of_dofs
-        cplx = self.data.fn not in (0, 4)
+        cplx = self.data.fn not in (0, 4, 5)
         S = 2 if cplx else 1
         nrhs = self.data.nrhs
+        if self.data.fn >= 5:
+            vals = self._values()
+            out = np.zeros(S * n)
+            _check(library().bmi_rhs_from_values(self.space._h, self.data.fn - 5, _dp(vals), 1, _dp(out), MEM_HOST,
+                                                 _stream_ptr(stream)))
+            self._value = out.view(np.complex128) if cplx else out
+            return self
         if device:
             import torch
             out = torch.zeros(S * n * nrhs, dtype=torch.float64, device="cuda")

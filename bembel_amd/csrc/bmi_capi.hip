@@ -994,6 +994,14 @@ bmi_status bmi_geometry_surface_point(const bmi_geometry* g, int patch, double s
   BMI_API_END
 }
 
+bmi_status bmi_linear_form_points(const bmi_space* s, int64_t* npoints, double* out) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(s && npoints, BMI_INVALID_ARG, "null argument");
+  *npoints = static_cast<int64_t>(s->nel) * (s->P + 2) * (s->P + 2);
+  if (out) linear_form_points(s, out);
+  BMI_API_END
+}
+
 bmi_status bmi_space_create(const bmi_geometry* g, int M, int P, int form, bmi_space** out) {
   BMI_API_BEGIN
   BMI_REQUIRE(g && out, BMI_INVALID_ARG, "null argument");

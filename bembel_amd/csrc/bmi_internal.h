@@ -132,6 +132,8 @@ void build_transformation(bmi_space* s);
 int farfield_degree(int kind, int P);
 int nearfield_degree(int kind, int P, double dist, int level);
 void compare_elements(const bmi_space* s, int e1, int e2, int out[3], double* dist);
+// points of the (P+2)^2-point rule of the linear forms, 3 doubles per point (host)
+void linear_form_points(const bmi_space* s, double* out);
 
 struct Leaf {
   int cc, level, c1, c2;

@@ -430,6 +430,23 @@ static const double bmi_gl_weights_host[BMI_GL_TABLE_SIZE] = BMI_GL_WEIGHTS_INIT
 
 namespace bmi {
 
+// points of the (P+2)^2-point rule of the linear forms on every element (Morton order), point
+// k = element * (P+2)^2 + a * (P+2) + b at local coordinates (x_a, x_b)
+void linear_form_points(const bmi_space* s, double* out) {
+  const int n = s->P + 2;
+  const double* qx = bmi_gl_nodes_host + bmi_gl_offset_host[n - 1];
+  const double h = 1.0 / s->n;
+  for (int e = 0; e < s->nel; ++e) {
+    const Patch& pt = s->geo->patches[s->elem_patch[e]];
+    for (int q = 0; q < n * n; ++q) {
+      double sp[12];
+      patch_surface_point(pt, s->elem_llc[2 * e] + h * qx[q / n], s->elem_llc[2 * e + 1] + h * qx[q % n], sp);
+      double* o = out + 3 * (static_cast<int64_t>(e) * n * n + q);
+      o[0] = sp[3], o[1] = sp[4], o[2] = sp[5];
+    }
+  }
+}
+
 static void moment1d(int P, int cl, int cr, int np, bool deriv, std::vector<double>& out) {
   const int n = 1 << cr, N = 1 << cl, pp1 = P + 1;
   const double h = 1. / n, H = 1. / N;

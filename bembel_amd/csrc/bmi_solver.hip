@@ -824,7 +824,7 @@ __device__ __forceinline__ void dipole_field(const double* X, double kappa, cons
 template <int PD, int P>
 __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* __restrict__ elem_patch,
                       const double* __restrict__ elem_llc, int nel, double h, int fn, const double* __restrict__ prm,
-                      int nrhs, double kappa, int64_t nsuper, double* out) {
+                      int nrhs, double kappa, int64_t nsuper, double* out, const double* __restrict__ vals) {
   constexpr int I = P + 1, I2 = I * I, n = P + 2;
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= static_cast<int64_t>(nel) * nrhs) return;
@@ -861,10 +861,21 @@ __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* _
     } else if (fn == 4) {
       // harmonic polynomial 4x^2 - 3y^2 - z^2 (examples/Data.hpp:14-16)
       fr = (4 * X[0] * X[0] - 3 * X[1] * X[1] - X[2] * X[2]) * nn * w;
+    } else if (fn == 5 || fn == 6) {
+      // caller-evaluated data at the rule's points: real or complex scalars
+      const int64_t vi = (static_cast<int64_t>(k) * nel + e) * (n * n) + q;
+      fr = (fn == 5 ? vals[vi] : vals[2 * vi]) * nn * w;
+      fi = fn == 5 ? 0.0 : vals[2 * vi + 1] * nn * w;
     } else {
-      const double* d = prm + 6 * k;
       double Er[3], Ei[3];
-      dipole_field(X, kappa, d, d + 3, Er, Ei);
+      if (fn == 7) {
+        // caller-evaluated complex field (3 components, interleaved) at the rule's points
+        const double* v = vals + 6 * ((static_cast<int64_t>(k) * nel + e) * (n * n) + q);
+        Er[0] = v[0], Ei[0] = v[1], Er[1] = v[2], Ei[1] = v[3], Er[2] = v[4], Ei[2] = v[5];
+      } else {
+        const double* d = prm + 6 * k;
+        dipole_field(X, kappa, d, d + 3, Er, Ei);
+      }
       const double nv[3] = {c0 / nn, c1 / nn, c2 / nn};
       const double er = Er[0] * nv[0] + Er[1] * nv[1] + Er[2] * nv[2];
       const double ei = Ei[0] * nv[0] + Ei[1] * nv[1] + Ei[2] * nv[2];
@@ -893,11 +904,11 @@ __global__ void k_rhs(const double* __restrict__ cp, int cp_stride, const int* _
   const int64_t vcs = static_cast<int64_t>(nel) * I2;  // second vector component (fn 2)
   for (int m = 0; m < I2; ++m) {
     const int64_t o = static_cast<int64_t>(k) * nsuper + static_cast<int64_t>(e) * I2 + m;
-    if (fn == 0 || fn == 4) {
+    if (fn == 0 || fn == 4 || fn == 5) {
       out[o] = ar[m];
     } else {
       out[2 * o] = ar[m], out[2 * o + 1] = ai[m];
-      if (fn == 2) out[2 * (o + vcs)] = br[m], out[2 * (o + vcs) + 1] = bi[m];
+      if (fn == 2 || fn == 7) out[2 * (o + vcs)] = br[m], out[2 * (o + vcs) + 1] = bi[m];
     }
   }
 }
@@ -908,20 +919,31 @@ using namespace bmi;
 
 extern "C" {
 
-bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, int nrhs, double kappa, double* out,
-                            int mem, void* stream) {
+// fn 0-4: built-in data with params; fn 5-7: values at the linear-form points (vals, host or
+// device per mem)
+static bmi_status rhs_common(const bmi_space* s, int fn, const double* params, const double* values, int nrhs,
+                             double kappa, double* out, int mem, void* stream) {
   try {
-    BMI_REQUIRE(s && params && out && nrhs >= 1, BMI_INVALID_ARG, "bad argument");
-    BMI_REQUIRE(fn >= 0 && fn <= 4, BMI_INVALID_ARG, "unknown data function");
-    BMI_REQUIRE((fn == 2) == (s->form == BMI_FORM_DIVCONFORMING), BMI_INVALID_ARG,
+    BMI_REQUIRE(s && out && nrhs >= 1, BMI_INVALID_ARG, "bad argument");
+    BMI_REQUIRE(fn >= 0 && fn <= 7, BMI_INVALID_ARG, "unknown data function");
+    BMI_REQUIRE(fn >= 5 ? values != nullptr : params != nullptr, BMI_INVALID_ARG, "null data");
+    const bool vec = fn == 2 || fn == 7;
+    BMI_REQUIRE(vec == (s->form == BMI_FORM_DIVCONFORMING), BMI_INVALID_ARG,
                 "Dirichlet traces need a scalar space, the rotated tangential trace a DivConforming one");
     hipStream_t st = static_cast<hipStream_t>(stream);
     ensure_device_space(s, st);
-    const bool real = fn == 0 || fn == 4;
+    const bool real = fn == 0 || fn == 4 || fn == 5;
     const int S = real ? 1 : 2;
-    DevBuf<double> prm, sup, dout;
+    DevBuf<double> prm, sup, dout, dval;
     const int per = fn == 2 ? 6 : 3;
-    prm.upload(std::vector<double>(params, params + per * nrhs), st);
+    if (fn < 5) prm.upload(std::vector<double>(params, params + per * nrhs), st);
+    const double* vals = values;
+    if (fn >= 5 && mem != BMI_MEM_DEVICE) {
+      const int64_t npts = static_cast<int64_t>(s->nel) * (s->P + 2) * (s->P + 2);
+      const int per_v = fn == 5 ? 1 : fn == 6 ? 2 : 6;
+      dval.upload(std::vector<double>(values, values + per_v * npts * nrhs), st);
+      vals = dval.ptr;
+    }
     sup.alloc(static_cast<size_t>(S) * s->nsuper * nrhs);
     const int64_t tot = static_cast<int64_t>(s->nel) * nrhs;
     const unsigned blocks = static_cast<unsigned>((tot + 127) / 128);
@@ -931,7 +953,7 @@ bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, in
   if (s->pd == PD && s->P == PP) {                                                                            \
     hipLaunchKernelGGL((k_rhs<PD, PP>), dim3(blocks), dim3(128), 0, st, s->d_cp.ptr, stride, s->d_elem_patch.ptr, \
                        s->d_elem_llc.ptr, s->nel, h, fn, prm.ptr, nrhs, kappa, static_cast<int64_t>(s->nsuper),   \
-                       sup.ptr);                                                                              \
+                       sup.ptr, vals);                                                                        \
   } else
     BMI_RHS(5, 1) BMI_RHS(5, 2) BMI_RHS(2, 1) BMI_RHS(2, 2) { throw Error(BMI_UNSUPPORTED, "rhs kernel not instantiated"); }
 #undef BMI_RHS
@@ -954,6 +976,24 @@ bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, in
     set_last_error(e.what());
     return BMI_DEVICE_ERROR;
   }
+}
+
+bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, int nrhs, double kappa, double* out,
+                            int mem, void* stream) {
+  if (fn < 0 || fn > 4) {
+    set_last_error("unknown data function");
+    return BMI_INVALID_ARG;
+  }
+  return rhs_common(s, fn, params, nullptr, nrhs, kappa, out, mem, stream);
+}
+
+bmi_status bmi_rhs_from_values(const bmi_space* s, int kind, const double* values, int nrhs, double* out, int mem,
+                               void* stream) {
+  if (kind < 0 || kind > 2) {
+    set_last_error("unknown value kind");
+    return BMI_INVALID_ARG;
+  }
+  return rhs_common(s, 5 + kind, nullptr, values, nrhs, 0.0, out, mem, stream);
 }
 
 static bmi_status solve_common(const bmi_h2* hc, int nrhs, const double* B, double* X, double tol, int restart, int maxit,

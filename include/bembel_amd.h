@@ -220,6 +220,18 @@ bmi_status bmi_h2_matvec_super(const bmi_h2* h, int nrhs, const double* Xs, doub
 bmi_status bmi_rhs_assemble(const bmi_space* s, int fn, const double* params, int nrhs,
                             double kappa, double* out, int mem, void* stream);
 
+/* The same linear forms for data the caller evaluates (a DirichletTrace or RotatedTangentialTrace
+   with an arbitrary function, LinearForm/DirichletTrace.hpp:31-70,
+   LinearForm/RotatedTangentialTrace.hpp:30-87).  bmi_linear_form_points gives the points of the
+   (P+2)^2-point rule on every element (point k = element * (P+2)^2 + q, 3 doubles each; host
+   code, no device use; out may be NULL to query *npoints).  bmi_rhs_from_values takes the data
+   at those points, column by column: kind 0 real scalars, 1 complex scalars (interleaved),
+   2 complex 3-vectors (6 doubles per point, DivConforming spaces: the rotated tangential trace
+   is formed on the device).  values are host or device pointers (mem), like out. */
+bmi_status bmi_linear_form_points(const bmi_space* s, int64_t* npoints, double* out);
+bmi_status bmi_rhs_from_values(const bmi_space* s, int kind, const double* values, int nrhs,
+                               double* out, int mem, void* stream);
+
 /* ------------------------------------------------------------------ potentials */
 /* DiscretePotential<{Laplace,Helmholtz,Maxwell}SingleLayerPotential, Op>::evaluate
    (Potential/DiscretePotential.hpp:49-103; Laplace/SingleLayerPotential.hpp,

@@ -64,3 +64,26 @@ def test_gpu_rhs_full_size_closed_forms(gpu):
     g = B.DiscreteLinearForm(B.PlaneWaves(_unit(np.random.default_rng(1), 64), k), ps).compute().get_discrete_linear_form()
     ex = 4 * math.pi * math.sin(k) / k
     assert np.abs(c @ g - ex).max() <= 1e-11 * abs(ex)
+
+
+@pytest.mark.parametrize("M,P", [(2, 1), (3, 2)])
+def test_gpu_rhs_from_caller_values(gpu, M, P):
+    """DirichletTrace / RotatedTangentialTrace with an arbitrary function (evaluated by the
+    caller at bmi_linear_form_points) equal the built-in device data functions"""
+    ps = B.AnsatzSpace(B.Geometry(), M, P)
+    x0 = np.array([0.2, -0.1, 0.05])
+    f = B.DiscreteLinearForm(B.FunctionTrace(lambda X: 1.0 / np.linalg.norm(X - x0, axis=1)), ps).compute()
+    g = B.DiscreteLinearForm(B.DirichletTrace(x0), ps).compute()
+    assert rel(f.get_discrete_linear_form(), g.get_discrete_linear_form()) <= TOL
+    d, k = np.array([0.0, 0.6, 0.8]), 3.0
+    f = B.DiscreteLinearForm(B.FunctionTrace(lambda X: np.exp(1j * k * (X @ d)), np.complex128), ps).compute()
+    g = B.DiscreteLinearForm(B.PlaneWaves(d, k), ps).compute()
+    assert rel(f.get_discrete_linear_form(), g.get_discrete_linear_form()) <= TOL
+    op = B.MaxwellSingleLayerOperator()
+    pm = B.AnsatzSpace(B.Geometry(scale=ELLIPSOID), M, P, op)
+    pos, mom = np.array([0.1, -0.05, 0.08]), np.array([0.02, 0.05, -0.07])
+    from test_oracle import _dipole_np
+    f = B.DiscreteLinearForm(B.FunctionTrace(lambda X: np.array([_dipole_np(x, 2.0, pos, mom) for x in X]),
+                                             vector=True), pm).compute()
+    g = B.DiscreteLinearForm(B.RotatedTangentialTrace(pos, mom, 2.0), pm).compute()
+    assert rel(f.get_discrete_linear_form(), g.get_discrete_linear_form()) <= TOL

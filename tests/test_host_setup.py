@@ -114,3 +114,28 @@ def test_pair_classification_matches_oracle(M, P):
     for k, (i, j) in enumerate(pairs):
         want, _ = os_.pair_info(int(i), int(j))
         assert list(got[k]) == list(want), (i, j, got[k], want)
+
+
+def test_linear_form_points_on_the_surface():
+    """bmi_linear_form_points (host code): (P+2)^2 points per element on the unit sphere, in the
+    element order and local order of the oracle's right-hand-side rule"""
+    for M, P in [(1, 1), (2, 2)]:
+        ps = B.AnsatzSpace(B.Geometry(), M, P)
+        n = ctypes.c_int64()
+        assert B.library().bmi_linear_form_points(ps._h, ctypes.byref(n), None) == B.OK
+        assert n.value == ps.number_of_elements * (P + 2) ** 2
+        pts = np.zeros((n.value, 3))
+        B.library().bmi_linear_form_points(ps._h, ctypes.byref(n), B._dp(pts))
+        assert np.abs(np.linalg.norm(pts, axis=1) - 1).max() < 1e-14
+        # first element, first point: the oracle's map2surface at the first Gauss node pair
+        og = O.Geometry()
+        osp = O.Space(og, M, P)
+        el = osp.elements()
+        t, _ = np.polynomial.legendre.leggauss(P + 2)
+        t = (t + 1) / 2
+        h = 2.0 ** -M
+        for e in (0, ps.number_of_elements - 1):
+            for q in (0, (P + 2) ** 2 - 1):
+                st = el["llc"][e] + h * np.array([t[q // (P + 2)], t[q % (P + 2)]])
+                ref = og.surface_point(int(el["patch"][e]), st)[3:6]
+                assert np.allclose(pts[e * (P + 2) ** 2 + q], ref, atol=1e-14)
