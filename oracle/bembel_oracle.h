@@ -112,6 +112,8 @@ void or_h2_leaves(const or_h2* h, int* leaves);
 /* leaf matrix (component block rc for Maxwell), column-major, complex interleaved */
 void or_h2_leaf_matrix(const or_h2* h, int leaf, int comp_r, int comp_c, double* F);
 void or_h2_set_leaf_matrix(or_h2* h, int leaf, int comp_r, int comp_c, const double* F);
+/* assemble only the listed leaves of an H2 built with assemble = 0 (sampled full-size checks) */
+void or_h2_assemble_leaves(or_h2* h, int n, const int* leaves);
 int or_h2_import_leaves(or_h2* h, int n, const int* keys, const double* data);
 /* res = A * x for nrhs columns (H2DenseProduct.hpp:39-164), DOF space; complex interleaved */
 void or_h2_matvec(const or_h2* h, int nrhs, const double* x, double* y);

@@ -356,6 +356,14 @@ void or_h2_leaf_matrix(const or_h2* h, int leaf, int r, int c, double* F) {
     memset(F, 0, sizeof(double) * m * d * d);
 }
 
+void or_h2_assemble_leaves(or_h2* h, int n, const int* leaves) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < n; ++k) {
+    int li = leaves[k];
+    if (li >= 0 && li < h->nleaves && !h->F[(size_t)li * h->vd * h->vd]) assemble_leaf(h, li);
+  }
+}
+
 void or_h2_set_leaf_matrix(or_h2* h, int leaf, int r, int c, const double* F) {
   int d = h->dim[leaf], m = h->cplx ? 2 : 1;
   double** dst = &h->F[leaf * h->vd * h->vd + r * h->vd + c];

@@ -62,6 +62,7 @@ def lib():
         L.or_h2_leaves.argtypes = [vp, ip]
         L.or_h2_leaf_matrix.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.or_h2_set_leaf_matrix.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+        L.or_h2_assemble_leaves.argtypes = [vp, C.c_int, ip]
         L.or_h2_import_leaves.restype = C.c_int
         L.or_h2_import_leaves.argtypes = [vp, C.c_int, ip, dp]
         L.or_h2_matvec.argtypes = [vp, C.c_int, dp, dp]
@@ -312,6 +313,10 @@ class H2:
         else:
             a = np.ascontiguousarray(F.T.reshape(-1))
         lib().or_h2_set_leaf_matrix(self.h, i, r, c, _dp(a))
+
+    def assemble_leaves(self, idx):
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        lib().or_h2_assemble_leaves(self.h, len(a), _ip(a))
 
     def import_leaves(self, keys, data):
         k = np.ascontiguousarray(keys, dtype=np.int32)
