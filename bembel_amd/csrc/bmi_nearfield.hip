@@ -293,6 +293,139 @@ __host__ __device__ constexpr int vertex_chunk(int n, int nt) {
   return pick_chunk(n * n * n, nt, mx > 0 ? mx : 1);
 }
 
+// Factorised vertex rule (scalar operators): a lane fixes one outer point and sums
+// kernel x basis vector of the varying point over (its share of) the inner index, then adds the
+// outer product with the fixed point's basis vector to its block: (P+1)^2 multiply-adds per
+// evaluation instead of (P+1)^4.  A shape is the chunk of outer indices staged per LDS table
+// and the number of lanes that split one outer point's inner range (even for complex
+// operators, whose lane pairs hold the real and imaginary parts of the block).  The score
+// trades busy lanes against the outer products, which every lane of an outer point repeats.
+struct FactorShape {
+  int chunk, lanes;
+};
+__host__ __device__ constexpr FactorShape factor_shape(int outer_per_chunk, int max_chunk, int nt, bool cplx,
+                                                       int eval_cost) {
+  FactorShape best{1, cplx ? 2 : 1};
+  long best_score = -1;
+  for (int c = 1; c <= max_chunk; ++c) {
+    const int units = c * outer_per_chunk;
+    int S = nt / units;
+    if (cplx) S -= S & 1;
+    if (S < (cplx ? 2 : 1)) break;
+    const long score = static_cast<long>(units) * S * 1000 / (eval_cost + S);
+    if (score > best_score) best_score = score, best = FactorShape{c, S};
+  }
+  return best;
+}
+// vertex rule: chunks of first outer coordinates (n outer points each), inner range n^2
+__host__ __device__ constexpr FactorShape vertex_shape(int n, int nt, int ptsize, bool cplx) {
+  const int cap = DUFFY_TABLE_BYTES / (2 * n * n * ptsize);
+  return factor_shape(n, cap < n ? (cap > 0 ? cap : 1) : n, nt, cplx, 90);
+}
+// kernel times both point weights, (kr, ki): the scalar integrands of eval() without the basis
+template <int KIND>
+__device__ __forceinline__ void kernel_ww(const Pt<KIND>& a, const Pt<KIND>& b, const NFParams& prm, double& kr,
+                                          double& ki) {
+  const double d0 = a.x0 - b.x0, d1 = a.x1 - b.x1, d2 = a.x2 - b.x2;
+  const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  if constexpr (KIND == BMI_LAPLACE_SL) {
+    kr = (a.W * b.W) * (0.25 * M_1_PI) / r, ki = 0.0;
+  } else if constexpr (KIND == BMI_HELMHOLTZ_SL) {
+    double sn, cs;
+    sincos(prm.kre * r, &sn, &cs);
+    const double damp = prm.kim != 0.0 ? exp(prm.kim * r) : 1.0;
+    const double mag = damp * (a.W * b.W) * (0.25 * M_1_PI) / r;
+    kr = mag * cs, ki = -mag * sn;
+  } else {
+    kr = dummy_value(prm.dummy_fn, a, b) * a.W * b.W, ki = 0.0;
+  }
+}
+
+// lane-private sum of (kernel x basis vector) over inner points
+template <int P, bool CPLX>
+struct VecSum {
+  static constexpr int I = P + 1, I2 = I * I;
+  double r[I2], i[CPLX ? I2 : 1];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int k = 0; k < I2; ++k) {
+      r[k] = 0.0;
+      if constexpr (CPLX) i[k] = 0.0;
+    }
+  }
+  // += (kr + i ki) * basis(s0, s1)
+  __device__ __forceinline__ void add(double kr, double ki, double s0, double s1) {
+    double X[I], Y[I];
+    bernstein_basis<P>(s0, X);
+    bernstein_basis<P>(s1, Y);
+#pragma unroll
+    for (int iy = 0; iy < I; ++iy)
+#pragma unroll
+      for (int ix = 0; ix < I; ++ix) {
+        const double b = X[ix] * Y[iy];
+        r[iy * I + ix] += kr * b;
+        if constexpr (CPLX) i[iy * I + ix] += ki * b;
+      }
+  }
+  // the part of the block this lane holds (complex: role 0 real, role 1 imaginary), summed
+  // with the lane pair's other half of the inner range; all lanes must call this together
+  __device__ __forceinline__ void total(int role, double* w) const {
+#pragma unroll
+    for (int k = 0; k < I2; ++k) {
+      if constexpr (CPLX) {
+        const double orr = __shfl_xor(r[k], 1, 64), oi = __shfl_xor(i[k], 1, 64);
+        w[k] = role ? i[k] + oi : r[k] + orr;
+      } else {
+        w[k] = r[k];
+      }
+    }
+  }
+};
+
+// One factorised sub-region: the fixed point F against the inner points inner(j) for j = sub,
+// sub + S, ... < count.  COLS: F supplies the rows (element 1) and the inner points the columns;
+// otherwise the inner points supply the rows.  Adds the outer product of F's basis with the
+// summed vector to this lane's part of the block.  All lanes of the wave must call this
+// together (the lane pairs of complex operators exchange their sums).
+template <int P, int KIND, bool COLS, class Acc, class PtK, class Inner>
+__device__ __forceinline__ void factor_pass(Acc& acc, const PtK& F, int role, bool act, int sub, int S, int count,
+                                            const NFParams& prm, Inner inner) {
+  constexpr int I = P + 1, I2 = I * I;
+  VecSum<P, Acc::CPLX> u;
+  u.zero();
+  if (act) {
+#pragma unroll 1
+    for (int j = sub; j < count; j += S) {
+      const PtK b = inner(j);
+      double kr, ki;
+      if constexpr (COLS)
+        kernel_ww<KIND>(F, b, prm, kr, ki);
+      else
+        kernel_ww<KIND>(b, F, prm, kr, ki);
+      u.add(kr, ki, b.s0, b.s1);
+    }
+  }
+  double w[I2];
+  u.total(role, w);
+  if (!act) return;
+  double X[I], Y[I];
+  bernstein_basis<P>(F.s0, X);
+  bernstein_basis<P>(F.s1, Y);
+#pragma unroll
+  for (int iy = 0; iy < I; ++iy)
+#pragma unroll
+    for (int ix = 0; ix < I; ++ix) {
+      const double f = X[ix] * Y[iy];
+#pragma unroll
+      for (int k = 0; k < I2; ++k) {
+        if constexpr (COLS)
+          acc.acc[(iy * I + ix) * I2 + k] += f * w[k];
+        else
+          acc.acc[k * I2 + iy * I + ix] += w[k] * f;
+      }
+    }
+}
+
 // DuffyTrick::tau (DuffyTrick/tau.hpp:26-42)
 __device__ __forceinline__ void tau(double x, double y, int c, double& u, double& v) {
   switch (c) {
@@ -429,6 +562,67 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
         acc.add(b, a, prm, rc, cc, valid, lane);
       }
     }
+  } else if constexpr (CLS == 4 && !Acc::MAXW) {
+    // integrate4, factorised: the four outer points per i as below; per chunk of first outer
+    // coordinates the inner points p5 (element 2) and p6 (element 1) are tabulated, and a lane
+    // fixed to one outer index i sums over its share of j: p5 gives columns for the rows of
+    // q0, q1, p6 gives rows for the columns of q2, q3.
+    const int r1 = pr.rot1, r2 = pr.rot2;
+    constexpr bool CPLX = Acc::CPLX;
+    PtK* q = reinterpret_cast<PtK*>(lds);  // 4 * nq
+    PtK* tin = q + 4 * nq;
+    for (int i = tid; i < nq; i += NT) {
+      const double xi0 = gx[i / n], wi = gw[i / n] * gw[i % n];
+      const double xi1 = gx[i % n] * xi0;
+      const double w = h * h * wi * (xi0 * xi0 * xi0);
+      double u, v;
+      tau(xi0, xi1, r1, u, v);
+      q[4 * i + 0] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, w);
+      tau(xi1, xi0, r1, u, v);
+      q[4 * i + 1] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, w);
+      tau(xi0, xi1, r2, u, v);
+      q[4 * i + 2] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, w);
+      tau(xi1, xi0, r2, u, v);
+      q[4 * i + 3] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, w);
+    }
+    const FactorShape shp = vertex_shape(n, NT, static_cast<int>(sizeof(PtK)), CPLX);
+    const int C0 = shp.chunk, S = shp.lanes;
+    const int role = lane & (Acc::G - 1);
+    for (int a0 = 0; a0 < n; a0 += C0) {
+      const int cn = n - a0 < C0 ? n - a0 : C0;
+      __syncthreads();
+      for (int e = tid; e < 2 * cn * nq; e += NT) {
+        const int which = e / (cn * nq), rem = e - which * (cn * nq);
+        const int c0 = rem / nq, j = rem - c0 * nq;
+        const double xi0 = gx[a0 + c0];
+        const double e0 = xi0 * gx[j / n], e1 = xi0 * gx[j % n];
+        const double wj = gw[j / n] * gw[j % n];
+        double u, v;
+        if (which == 0) {
+          tau(e0, e1, r2, u, v);
+          tin[e] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, wj);
+        } else {
+          tau(e0, e1, r1, u, v);
+          tin[e] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, wj);
+        }
+      }
+      __syncthreads();
+      const int units = cn * n;
+      const int unit = tid / S, sub = tid - (tid / S) * S;
+      const bool act = unit < units;
+      const int uu = act ? unit : 0;
+      const int c0 = uu / n, i1 = uu - (uu / n) * n;
+      const int i = (a0 + c0) * n + i1;
+      const PtK* p5 = tin + c0 * nq;
+      const PtK* p6 = tin + (cn + c0) * nq;
+#pragma unroll 1
+      for (int k = 0; k < 2; ++k)  // rows q0, q1 (element 1), columns p5 (element 2)
+        factor_pass<P, KIND, true>(acc, q[4 * i + k], role, act, sub, S, nq, prm, [&](int j) { return p5[j]; });
+#pragma unroll 1
+      for (int k = 2; k < 4; ++k)  // rows p6 (element 1), columns q2, q3 (element 2)
+        factor_pass<P, KIND, false>(acc, q[4 * i + k], role, act, sub, S, nq, prm, [&](int j) { return p6[j]; });
+    }
+    __syncthreads();
   } else if constexpr (CLS == 3) {
     // integrate3: common edge, 6 Duffy sub-regions.  With t1 = xi0 (1 - xi1), t2 = (1 - xi0)(1 - xi1),
     // e0 = xi1 xj0, e1 = xi1 xj1, t3 = xi0 (1 - e0), t4 = (1 - xi0)(1 - e0), sub-region k = 2g + odd
@@ -640,9 +834,11 @@ static size_t class_lds_bytes(int deg) {
   const int n = deg + 1, nq = n * n;
   size_t lds = sizeof(double) * ((NT / 64) + 1) * BlockAcc<P, KIND>::TOT;
   if (CLS == 1) lds = std::max(lds, 2 * nq * sizeof(Pt<KIND>));
+  constexpr bool FACT = KIND != BMI_MAXWELL_SL;  // factorised vertex rule
+  constexpr int PS = static_cast<int>(sizeof(Pt<KIND>));
+  const int cv = FACT ? vertex_shape(n, NT, PS, BlockAcc<P, KIND>::CPLX).chunk : vertex_chunk<Pt<KIND>>(n, NT);
   if (CLS == 3) lds = std::max(lds, static_cast<size_t>(8 * n * edge_chunk<Pt<KIND>>(n, NT)) * sizeof(Pt<KIND>));
-  if (CLS == 4)
-    lds = std::max(lds, static_cast<size_t>(4 * nq + 2 * nq * vertex_chunk<Pt<KIND>>(n, NT)) * sizeof(Pt<KIND>));
+  if (CLS == 4) lds = std::max(lds, static_cast<size_t>(4 * nq + 2 * nq * cv) * sizeof(Pt<KIND>));
   return lds;
 }
 

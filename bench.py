@@ -116,6 +116,20 @@ def run_batched_profile(B, torch):
         del H
 
 
+def run_assembly_profile(B, torch):
+    """one H2 assembly each of the Laplace (4x4 nodes) and Helmholtz (kappa 4, 5x5 nodes) single
+    layers on 24,576 elements, P 2 (the workload of the near-field quadrature counters)"""
+    for name, op, npts in (("laplace", B.LaplaceSingleLayerOperator(), 4), ("helmholtz", B.HelmholtzSingleLayerOperator(), 5)):
+        if name == "helmholtz":
+            op.set_wavenumber(4.0)
+        space = B.AnsatzSpace(B.Geometry(), 6, 2, op)
+        t0 = time.perf_counter()
+        H = B.H2Matrix(space, op, n_cheb=npts)
+        torch.cuda.synchronize()
+        log(name, "assembly %.3f s, pair classes %s" % (time.perf_counter() - t0, H.info["pair_classes"]))
+        del H
+
+
 def run_helmholtz_profile(B, torch):
     """Helmholtz kappa 4, 24,576 elements, P 2, 5x5 Chebyshev nodes: per precision an assembly, 23
     products and 300 GMRES(30) steps (the workload of the Helmholtz kernel statistics)"""
@@ -420,6 +434,9 @@ def main():
                     help="only the Helmholtz 6,144-element 64-RHS products and GMRES steps (profiling workload)")
     ap.add_argument("--helmholtz-profile", action="store_true",
                     help="only the Helmholtz 24,576-element products and GMRES steps (profiling workload)")
+    ap.add_argument("--assembly-profile", action="store_true",
+                    help="only one H2 assembly each of the Laplace (np 4) and Helmholtz (kappa 4, np 5) operators "
+                         "on 24,576 elements, P 2 (profiling workload of the near-field quadrature)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the H2 path runs on a single MI355X "
@@ -435,6 +452,9 @@ def main():
         return
     if a.batched_profile:
         run_batched_profile(B, torch)
+        return
+    if a.assembly_profile:
+        run_assembly_profile(B, torch)
         return
     M, P, npts = a.level, 2, 4
     geo = B.Geometry()
