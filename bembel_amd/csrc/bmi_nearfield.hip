@@ -516,6 +516,29 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       }
       acc.add(pa, pb, prm, rc, cc, valid, lane);
     }
+  } else if constexpr (CLS == 1 && !Acc::MAXW) {
+    // integrate1, factorised: surface points of both elements staged in LDS; a lane fixes a
+    // point of element 1 and sums over (its share of) the points of element 2
+    PtK* q1 = reinterpret_cast<PtK*>(lds);
+    PtK* q2 = q1 + nq;
+    for (int k = tid; k < nq; k += NT) {
+      const double u = gx[k / n], v = gx[k % n], w = h * (gw[k / n] * gw[k % n]);
+      q1[k] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, w);
+      q2[k] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, w);
+    }
+    __syncthreads();
+    constexpr bool CPLX = Acc::CPLX;
+    int S = NT / nq;
+    if (CPLX) S -= S & 1;
+    S = S < (CPLX ? 2 : 1) ? (CPLX ? 2 : 1) : S;
+    const int per_round = NT / S;
+    const int role = lane & (Acc::G - 1);
+    for (int u0 = 0; u0 < nq; u0 += per_round) {
+      const int unit = u0 + tid / S, sub = tid - (tid / S) * S;
+      const bool act = tid / S < per_round && unit < nq;
+      factor_pass<P, KIND, true>(acc, q1[act ? unit : 0], role, act, sub, S, nq, prm, [&](int j) { return q2[j]; });
+    }
+    __syncthreads();
   } else if constexpr (CLS == 1) {
     // integrate1: tensor Gauss rule; surface points of both elements staged in LDS
     PtK* q1 = reinterpret_cast<PtK*>(lds);
