@@ -293,6 +293,72 @@ __host__ __device__ constexpr int vertex_chunk(int n, int nt) {
   return pick_chunk(n * n * n, nt, mx > 0 ? mx : 1);
 }
 
+// Edge rule, scalar operators: the computed points of sub-regions g = 1, 2 are tau(t4 or 1 - t4,
+// e1; r) with t4 fixed by (i, j0), so for one outer point they lie on a coordinate line of the
+// patch.  The patch is contracted once per line in the fixed direction (a degree-(PD-1) curve of
+// homogeneous points and cross derivatives, 2 PD x 4 values held in registers) and each point on
+// the line costs a one-dimensional evaluation.
+template <int PD>
+__device__ __forceinline__ void line_contract(const double* __restrict__ cp, bool fix_s, double f, double* L) {
+  double B[PD], E[PD];
+  bernstein_basis_der<PD - 1>(f, B, E);
+#pragma unroll
+  for (int a = 0; a < PD; ++a) {
+    double r[4] = {0, 0, 0, 0}, rs[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < PD; ++b) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double c = fix_s ? cp[4 * (b * PD + a) + k] : cp[4 * (a * PD + b) + k];
+        r[k] += B[b] * c;
+        rs[k] += E[b] * c;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L[4 * a + k] = r[k], L[4 * PD + 4 * a + k] = rs[k];
+  }
+}
+
+// a point of a contracted line at local coordinates (u, v) with weight w, as make_point
+template <int PD, int KIND>
+__device__ __forceinline__ Pt<KIND> line_point(const double* L, bool fix_s, double lx, double ly, double h, double u,
+                                               double v, double w) {
+  double B[PD], E[PD];
+  bernstein_basis_der<PD - 1>(fix_s ? ly + h * v : lx + h * u, B, E);
+  double p[4] = {0, 0, 0, 0}, d1[4] = {0, 0, 0, 0}, d2[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < PD; ++a)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      p[k] += B[a] * L[4 * a + k];
+      d1[k] += B[a] * L[4 * PD + 4 * a + k];  // across the line (the fixed direction)
+      d2[k] += E[a] * L[4 * a + k];           // along the line
+    }
+  const double bot = 1.0 / p[3];
+  const double b2 = bot * bot;
+  double X[3], Ds[3], Dt[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    X[k] = p[k] * bot;
+    const double ds = fix_s ? d1[k] : d2[k], dt = fix_s ? d2[k] : d1[k];
+    const double dsw = fix_s ? d1[3] : d2[3], dtw = fix_s ? d2[3] : d1[3];
+    Ds[k] = (ds * p[3] - p[k] * dsw) * b2;
+    Dt[k] = (dt * p[3] - p[k] * dtw) * b2;
+  }
+  Pt<KIND> q;
+  q.x0 = X[0], q.x1 = X[1], q.x2 = X[2];
+  q.s0 = u, q.s1 = v;
+  if constexpr (KIND == BMI_DUMMY) {
+    q.W = w;
+  } else {
+    const double c0 = Ds[1] * Dt[2] - Ds[2] * Dt[1];
+    const double c1 = Ds[2] * Dt[0] - Ds[0] * Dt[2];
+    const double c2 = Ds[0] * Dt[1] - Ds[1] * Dt[0];
+    q.W = sqrt(c0 * c0 + c1 * c1 + c2 * c2) * w;
+  }
+  return q;
+}
+
 // Factorised vertex rule (scalar operators): a lane fixes one outer point and sums
 // kernel x basis vector of the varying point over (its share of) the inner index, then adds the
 // outer product with the fixed point's basis vector to its block: (P+1)^2 multiply-adds per
@@ -322,6 +388,13 @@ __host__ __device__ constexpr FactorShape vertex_shape(int n, int nt, int ptsize
   const int cap = DUFFY_TABLE_BYTES / (2 * n * n * ptsize);
   return factor_shape(n, cap < n ? (cap > 0 ? cap : 1) : n, nt, cplx, 90);
 }
+// edge rule, factorised: chunks of outer indices i (n outer points (i, j0) each), inner range n
+__host__ __device__ constexpr FactorShape edge_shape(int n, int nt, int ptsize, bool cplx) {
+  const int cap = DUFFY_TABLE_BYTES / (8 * n * ptsize);
+  const int mx = cap < n * n ? cap : n * n;
+  return factor_shape(n, mx > 0 ? mx : 1, nt, cplx, 48);
+}
+
 // kernel times both point weights, (kr, ki): the scalar integrands of eval() without the basis
 template <int KIND>
 __device__ __forceinline__ void kernel_ww(const Pt<KIND>& a, const Pt<KIND>& b, const NFParams& prm, double& kr,
@@ -585,6 +658,96 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
         acc.add(b, a, prm, rc, cc, valid, lane);
       }
     }
+  } else if constexpr (CLS == 3 && !Acc::MAXW) {
+    // integrate3, factorised: the sub-regions and points of the unfactorised form below.  Outer
+    // point (i, j0), inner index j1.  g = 0: rows (t1, e0) and columns (t2, e1) from the tables;
+    // g = 1: rows (t3, xi1) from the tables, columns on element 2 at tau(t4 or 1 - t4, e1; r2);
+    // g = 2: rows on element 1 at tau(t4 or 1 - t4, e1; r1), columns (t3, xi1) from the tables.
+    // The computed points come from the contracted line of their (i, j0).  The weight products
+    // are split into the outer factor (w and the j0 part of the inner weights) and gw[j1].
+    const int r1 = pr.rot1, r2 = pr.rot2;
+    constexpr bool CPLX = Acc::CPLX;
+    const FactorShape shp = edge_shape(n, NT, static_cast<int>(sizeof(PtK)), CPLX);
+    const int CI = shp.chunk, S = shp.lanes;
+    const int role = lane & (Acc::G - 1);
+    const bool fs1 = (r1 & 1) == 0, fs2 = (r2 & 1) == 0;  // tau(x, y; r) fixes u for r = 0, 2
+    PtK* tab = reinterpret_cast<PtK*>(lds);
+    for (int i0 = 0; i0 < nq; i0 += CI) {
+      const int cn = nq - i0 < CI ? nq - i0 : CI;
+      const int per = cn * n;
+      __syncthreads();
+      for (int e = tid; e < 8 * per; e += NT) {
+        const int kind = e / per, rem = e - kind * per;
+        const int ci = rem / n, jx = rem - ci * n;
+        const int i = i0 + ci;
+        const double xi0 = gx[i / n], xi1 = gx[i % n], xj = gx[jx];
+        const bool odd = kind & 1;
+        double u, v;
+        if (kind < 2) {
+          const double t1 = xi0 * (1 - xi1);
+          tau(odd ? 1 - t1 : t1, xi1 * xj, r1, u, v);
+          tab[e] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, 1.0);
+        } else if (kind < 4) {
+          const double t3 = xi0 * (1 - xi1 * xj);
+          tau(odd ? 1 - t3 : t3, xi1, r1, u, v);
+          tab[e] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, 1.0);
+        } else if (kind < 6) {
+          const double t3 = xi0 * (1 - xi1 * xj);
+          tau(odd ? 1 - t3 : t3, xi1, r2, u, v);
+          tab[e] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, 1.0);
+        } else {
+          const double t2 = (1 - xi0) * (1 - xi1);
+          tau(odd ? 1 - t2 : t2, xi1 * xj, r2, u, v);
+          tab[e] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, 1.0);
+        }
+      }
+      __syncthreads();
+      const int units = cn * n;
+      const int unit = tid / S, sub = tid - (tid / S) * S;
+      const bool act = unit < units;
+      const int uu = act ? unit : 0;
+      const int ci = uu / n, j0 = uu - (uu / n) * n;
+      const int i = i0 + ci;
+      const double xi0 = gx[i / n], xi1 = gx[i % n], wi = gw[i / n] * gw[i % n];
+      const double w = h * h * wi * xi1 * xi1;
+      const double e0 = xi1 * gx[j0];
+      const double t4 = (1 - xi0) * (1 - e0);
+      const double fa = w * (gw[j0] * (1 - xi1)), fb = w * (gw[j0] * (1 - e0));
+      const int ta = ci * n + j0;
+      const PtK* colt = tab + ci * n;
+#pragma unroll 1
+      for (int odd = 0; odd < 2; ++odd) {
+        const double x = odd ? 1 - t4 : t4;
+        // g = 0: table rows and columns
+        PtK F = tab[odd * per + ta];
+        F.W *= fa;
+        factor_pass<P, KIND, true>(acc, F, role, act, sub, S, n, prm, [&](int j1) {
+          PtK b = colt[(6 + odd) * per + j1];
+          b.W *= gw[j1];
+          return b;
+        });
+        // g = 1: table rows, columns on the line of element 2
+        double L[8 * PD];
+        line_contract<PD>(cp2, fs2, fs2 ? l2x + h * (r2 < 2 ? x : 1 - x) : l2y + h * (r2 < 2 ? x : 1 - x), L);
+        F = tab[(2 + odd) * per + ta];
+        F.W *= fb;
+        factor_pass<P, KIND, true>(acc, F, role, act, sub, S, n, prm, [&](int j1) {
+          double u, v;
+          tau(x, xi1 * gx[j1], r2, u, v);
+          return line_point<PD, KIND>(L, fs2, l2x, l2y, h, u, v, gw[j1]);
+        });
+        // g = 2: rows on the line of element 1, table columns
+        line_contract<PD>(cp1, fs1, fs1 ? l1x + h * (r1 < 2 ? x : 1 - x) : l1y + h * (r1 < 2 ? x : 1 - x), L);
+        F = tab[(4 + odd) * per + ta];
+        F.W *= fb;
+        factor_pass<P, KIND, false>(acc, F, role, act, sub, S, n, prm, [&](int j1) {
+          double u, v;
+          tau(x, xi1 * gx[j1], r1, u, v);
+          return line_point<PD, KIND>(L, fs1, l1x, l1y, h, u, v, gw[j1]);
+        });
+      }
+    }
+    __syncthreads();
   } else if constexpr (CLS == 4 && !Acc::MAXW) {
     // integrate4, factorised: the four outer points per i as below; per chunk of first outer
     // coordinates the inner points p5 (element 2) and p6 (element 1) are tabulated, and a lane
@@ -852,7 +1015,7 @@ void launch_ffield_table(const bmi_space* s, int n, int fs, double* out, hipStre
 
 // dynamic LDS of one workgroup of class CLS at quadrature degree deg (each workgroup sizes its
 // point tables from its own pair's degree)
-template <int P, int KIND, int CLS, int NT>
+template <int P, int KIND, int PD, int CLS, int NT>
 static size_t class_lds_bytes(int deg) {
   const int n = deg + 1, nq = n * n;
   size_t lds = sizeof(double) * ((NT / 64) + 1) * BlockAcc<P, KIND>::TOT;
@@ -860,7 +1023,8 @@ static size_t class_lds_bytes(int deg) {
   constexpr bool FACT = KIND != BMI_MAXWELL_SL;  // factorised vertex rule
   constexpr int PS = static_cast<int>(sizeof(Pt<KIND>));
   const int cv = FACT ? vertex_shape(n, NT, PS, BlockAcc<P, KIND>::CPLX).chunk : vertex_chunk<Pt<KIND>>(n, NT);
-  if (CLS == 3) lds = std::max(lds, static_cast<size_t>(8 * n * edge_chunk<Pt<KIND>>(n, NT)) * sizeof(Pt<KIND>));
+  const int ce = FACT ? edge_shape(n, NT, PS, BlockAcc<P, KIND>::CPLX).chunk : edge_chunk<Pt<KIND>>(n, NT);
+  if (CLS == 3) lds = std::max(lds, static_cast<size_t>(8 * n * ce) * sizeof(Pt<KIND>));
   if (CLS == 4) lds = std::max(lds, static_cast<size_t>(4 * nq + 2 * nq * cv) * sizeof(Pt<KIND>));
   return lds;
 }
@@ -874,7 +1038,7 @@ static void launch_class(const NFPair* d_pairs, int count, const std::vector<int
   if (count == 0) return;
   constexpr int NT = (CLS == 0) ? 64 : 256;
   size_t lds = 0;
-  for (int d : degs) lds = std::max(lds, class_lds_bytes<P, KIND, CLS, NT>(d));
+  for (int d : degs) lds = std::max(lds, class_lds_bytes<P, KIND, PD, CLS, NT>(d));
   BMI_REQUIRE(lds <= 160 * 1024, BMI_UNSUPPORTED, "quadrature degree too large for LDS staging");
   const char* name = CLS == 2 ? "nf_identical" : CLS == 3 ? "nf_edge" : CLS == 4 ? "nf_vertex" : "nf_regular";
   TimedScope ts(name, st);
