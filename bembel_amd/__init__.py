@@ -491,6 +491,22 @@ class H2Matrix:
             return out.view(np.complex128).reshape(d, d).T.copy()
         return out.reshape(d, d).T.copy()
 
+    def export_leaf_data(self, indices):
+        """(keys, data) of the leaves `indices` only, in the layout of export_leaves"""
+        idx = np.asarray(indices, dtype=np.int64)
+        keys = self.leaves()[idx]
+        vd = self.space.vector_dimension
+        S = 2 if self.scalar == np.complex128 else 1
+        dims = np.where(keys[:, 0] == 2, self.info["dense_dim"], self.info["coupling_dim"])
+        data = np.empty(int(np.sum(dims.astype(np.int64) ** 2)) * vd * vd * S)
+        o = 0
+        for k, d in zip(idx.tolist(), dims.tolist()):
+            for r in range(vd):
+                for c in range(vd):
+                    _check(library().bmi_h2_leaf_matrix(self._h, int(k), r, c, _dp(data[o:])))
+                    o += d * d * S
+        return keys, data
+
     def export_leaves(self):
         """(keys, data): keys as in leaves(), data = all leaf matrices back to back (float64,
         complex entries interleaved)."""

@@ -138,6 +138,14 @@ struct bmi_h2 {
   hipStream_t side_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   DevBuf<double> d_yf;
+  // Maxwell far field (mxf): d_coup holds the kernel values K (np2 x np2) of each coupling leaf;
+  // the tangents of every cluster entry's nodes (6 doubles per node, device and host copies) and
+  // the four-column workspace of the sweep
+  bool mxf = false;
+  double mx_qre = 0.0, mx_qim = 0.0;  // -1 / kappa^2
+  DevBuf<double> d_mx_tan, d_mxz;
+  DevBuf<int> d_mx_order;  // cluster entries by decreasing number of coupling leaves
+  std::vector<double> h_mx_tan;
   // sweep kernel variants picked by timing on first use (-1: not yet); the fused single-RHS
   // sweep has its own pair
   int leaf_variant = -1, cpl_variant = -1;
@@ -159,7 +167,8 @@ struct bmi_h2 {
            d_fly_pts.bytes() + d_fly_org.bytes() + d_drp.bytes() + d_dsrc.bytes() + d_tgt.bytes() + d_tgt_all.bytes() + d_crp_all.bytes() +
            d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_Tt.bytes() + d_momt[0].bytes() + d_momt[1].bytes() + d_stgt.bytes() + d_sgtgt.bytes() + d_srp.bytes() +
            d_ssrc.bytes() + d_sblk.bytes() + d_sgrp.bytes() + d_sgslot.bytes() + d_part.bytes() + d_xs.bytes() +
-           d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes() + d_yf.bytes();
+           d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes() + d_yf.bytes() + d_mx_tan.bytes() +
+           d_mxz.bytes() + d_mx_order.bytes();
   }
 };
 
@@ -177,6 +186,11 @@ struct CouplingOperand {
   int64_t blk;  // entries per component block
 };
 static CouplingOperand coupling_operand(const bmi_h2* h, bool reduced) {
+  if (h->mxf) {
+    const int64_t blk = static_cast<int64_t>(h->np2) * h->np2;
+    if (reduced && h->op.precision != BMI_PRECISION_FP64) return {true, h->d_coup32.ptr, nullptr, 0, h->np2, blk};
+    return {false, h->d_coup.ptr, nullptr, 0, h->np2, blk};
+  }
   if (reduced && h->op.precision != BMI_PRECISION_FP64)
     return {true, h->d_coup32.ptr, h->ncoarse > 0 ? h->d_coup16.ptr : nullptr, h->ncoarse, h->ld32,
             static_cast<int64_t>(h->ld32) * h->cd};
@@ -194,6 +208,7 @@ static void ensure_workspace(bmi_h2* h, int nrhs, hipStream_t st) {
   for (int c = 0; c < h->vd; ++c) h->d_fw[c].alloc(S * h->lvl_total * nrhs);
   h->d_bw.alloc(S * h->lvl_total * nrhs);
   h->d_yf.alloc(S * N * nrhs);
+  if (h->mxf) h->d_mxz.alloc(2 * S * h->lvl_total * nrhs);  // 4 np2 columns per entry = 2 cd
   h->ws_rhs = nrhs;
 }
 
@@ -210,9 +225,11 @@ struct Work {
   double* fw[2];  // multipole coefficients per FMM component
   double* bw;     // local coefficients
   bool own;
+  double* z;      // Maxwell far field: four columns per cluster entry
 };
 static Work handle_work(bmi_h2* h) {
-  return Work{h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, {h->d_fw[0].ptr, h->d_fw[1].ptr}, h->d_bw.ptr, true};
+  return Work{h->d_xs.ptr, h->d_ys.ptr, h->d_yf.ptr, {h->d_fw[0].ptr, h->d_fw[1].ptr}, h->d_bw.ptr, true,
+              h->d_mxz.ptr};
 }
 
 // far-field chain: forward transformation, coupling, backward transformation, leaf rows -> yf
@@ -230,6 +247,21 @@ static void apply_far(bmi_h2* h, const Work& w, int nrhs, const double* xs, doub
     TimedScope ts("matvec_forward", st);
     launch_forward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[cc].ptr, h->d_T.ptr, xs, N, cc * vcs,
                    w.fw[cc], h->d_lvl_off.ptr, st);
+  }
+  if (h->mxf) {
+    // four columns per cluster entry from both components' moments, one sweep over the kernel
+    // values writing both components' local expansions (into the moment buffers, consumed), then
+    // the backward transformation per component
+    launch_mx_prep(h->lvl_total / rows, nrhs, h->np2, rows, w.fw[0], w.fw[1], h->d_mx_tan.ptr, w.z, st);
+    launch_mx_coupling(co.fp32, h->ntgt_all, nrhs, h->np2, rows, h->d_mx_order.ptr, h->d_tgt_all.ptr,
+                       h->d_crp_all.ptr, h->d_csrc.ptr,
+                       co.F, h->d_lvl_off.ptr, h->d_mx_tan.ptr, w.z, w.fw[0], w.fw[1], h->mx_qre, h->mx_qim, st);
+    for (int rc = 0; rc < h->vd; ++rc) {
+      TimedScope ts("matvec_backward", st);
+      launch_backward(C, s->npatch, h->L, rows, h->np2, h->dd, nrhs, h->d_mom[rc].ptr, h->d_T.ptr, w.fw[rc],
+                      h->d_lvl_off.ptr, yf, N, rc * vcs, st);
+    }
+    return;
   }
   for (int rc = 0; rc < h->vd; ++rc) {
     device_fill(w.bw, S * h->lvl_total * nrhs, 0.0, st);
@@ -281,7 +313,7 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
     if (v == 0 || t < best) best = t, pick = v;
   }
   h->leaf_variant = pick;
-  if (has_far(h)) {
+  if (has_far(h) && !h->mxf) {
     const int S = h->cplx ? 2 : 1;
     const CouplingOperand co = coupling_operand(h, reduced);
     const bool fp32 = co.fp32;
@@ -433,7 +465,7 @@ static void apply_super(bmi_h2* h, const Work& w, int nrhs, const double* xs, do
 // vectors: [level offset + cluster * rows + row][rhs]): each wave's lanes are 64 consecutive
 // right-hand sides, so every vector access of the block sweeps is one contiguous row segment
 static bool use_rhs_fastest(const bmi_h2* h, int nrhs, bool reduced) {
-  if (nrhs < MULTI_MIN_RHS || !multi_t_supported(h->cplx, h->dd, h->cd)) return false;
+  if (h->mxf || nrhs < MULTI_MIN_RHS || !multi_t_supported(h->cplx, h->dd, h->cd)) return false;
   return !(reduced && h->op.precision != BMI_PRECISION_FP64 && h->ncoarse > 0);  // no BF16 levels here
 }
 
@@ -594,7 +626,7 @@ static int64_t noalloc_doubles(const bmi_h2* h, int nrhs) {
   const int S = h->cplx ? 2 : 1;
   const int64_t vec = scratch_round(static_cast<int64_t>(S) * h->s->nsuper * nrhs);
   const int64_t mp = scratch_round(static_cast<int64_t>(S) * h->lvl_total * nrhs);
-  return 3 * vec + (h->vd + 1) * mp;
+  return 3 * vec + (h->vd + 1) * mp + (h->mxf ? 2 * mp : 0);
 }
 
 static Work noalloc_work(const bmi_h2* h, int nrhs, double* base) {
@@ -609,6 +641,7 @@ static Work noalloc_work(const bmi_h2* h, int nrhs, double* base) {
   w.fw[1] = h->vd > 1 ? base + 3 * vec + mp : nullptr;
   w.bw = base + 3 * vec + h->vd * mp;
   w.own = false;
+  w.z = h->mxf ? base + 3 * vec + (h->vd + 1) * mp : nullptr;
   return w;
 }
 
@@ -771,11 +804,20 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   for (int l = 0; l <= h->L; ++l) h->lvl_off[l + 1] = h->lvl_off[l] + static_cast<int64_t>(ncl_at(s, l)) * h->cd;
   h->lvl_total = h->lvl_off[h->L + 1];
   h->d_lvl_off.upload(h->lvl_off, st);
-  // ---- coupling blocks
-  const int64_t cblk = static_cast<int64_t>(h->cd) * h->cd;
+  // ---- coupling blocks (Maxwell: the kernel values of each leaf, see k_mx_coupling)
+  h->mxf = op.kind == BMI_MAXWELL_SL;
+  if (h->mxf) {
+    const double k2re = op.kappa_re * op.kappa_re - op.kappa_im * op.kappa_im, k2im = 2 * op.kappa_re * op.kappa_im;
+    const double den = k2re * k2re + k2im * k2im;
+    BMI_REQUIRE(den > 0, BMI_INVALID_ARG, "Maxwell operator with wavenumber zero");
+    h->mx_qre = -k2re / den, h->mx_qim = k2im / den;
+    BMI_REQUIRE(h->np2 <= 256, BMI_UNSUPPORTED, "Maxwell far field with more than 16x16 Chebyshev nodes");
+  }
+  const int64_t cblk = h->mxf ? static_cast<int64_t>(h->np2) * h->np2 : static_cast<int64_t>(h->cd) * h->cd;
+  const int bpl = h->mxf ? 1 : vd * vd;  // stored blocks per leaf
   const int nl = static_cast<int>(h->lowrank.size());
-  h->d_coup.alloc(static_cast<size_t>(S) * vd * vd * cblk * nl);
-  if (nl > 0) {
+  h->d_coup.alloc(static_cast<size_t>(S) * bpl * cblk * nl);
+  if (nl > 0 || h->mxf) {
     std::vector<double> cheb(h->np);
     cheb_points(h->np, cheb.data());
     DevBuf<double> d_cheb;
@@ -793,7 +835,25 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
     for (int k = 0; k < nl; ++k) lv[k] = make_int4(h->lowrank[k].level, h->lowrank[k].c1, h->lowrank[k].c2, 0);
     DevBuf<int4> d_lv;
     d_lv.upload(lv, st);
-    launch_coupling_assemble(op.kind, d_lv.ptr, nl, h->np2, d_ptrs.ptr, op.kappa_re, op.kappa_im, h->d_coup.ptr, st);
+    if (nl > 0)
+      launch_coupling_assemble(op.kind, d_lv.ptr, nl, h->np2, d_ptrs.ptr, op.kappa_re, op.kappa_im, h->d_coup.ptr,
+                               st);
+    if (h->mxf) {
+      // tangents (d/ds, d/dt) of every cluster entry's nodes, in the entry order of the multipole
+      // arrays (level offset / rows + cluster)
+      const int64_t ne = h->lvl_total / h->cd;
+      h->h_mx_tan.assign(static_cast<size_t>(ne) * h->np2 * 6, 0.0);
+      for (int l = 0; l <= h->L; ++l) {
+        const int ncl = ncl_at(s, l);
+        std::vector<double> hp(static_cast<size_t>(ncl) * h->np2 * CP_STRIDE);
+        BMI_CHECK_HIP(hipMemcpyAsync(hp.data(), pts[l].ptr, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        BMI_CHECK_HIP(hipStreamSynchronize(st));
+        const int64_t e0 = h->lvl_off[l] / h->cd;
+        for (int64_t q = 0; q < static_cast<int64_t>(ncl) * h->np2; ++q)
+          for (int x = 0; x < 6; ++x) h->h_mx_tan[(e0 * h->np2 + q) * 6 + x] = hp[q * CP_STRIDE + 4 + x];
+      }
+      h->d_mx_tan.upload(h->h_mx_tan, st);
+    }
     if (op.precision == BMI_PRECISION_MIXED && vd == 1 && !h->cplx) {
       // node tables of every level at the cluster offsets of the multipole arrays
       h->d_fly_pts.alloc(static_cast<size_t>(h->lvl_total));
@@ -839,24 +899,32 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
     h->ntgt_all = static_cast<int>(all.size());
     h->d_tgt_all.upload(all, st);
     h->d_crp_all.upload(rp_all, st);
+    if (h->mxf) {
+      std::vector<int> order(all.size());
+      for (size_t q = 0; q < order.size(); ++q) order[q] = static_cast<int>(q);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return rp_all[a + 1] - rp_all[a] > rp_all[b + 1] - rp_all[b]; });
+      h->d_mx_order.upload(order, st);
+    }
   }
   if (vd == 1 && (op.kind == BMI_LAPLACE_SL || op.kind == BMI_HELMHOLTZ_SL) && sym_sweep_supported(h->cplx, h->dd, h->cd))
     build_symmetric_lists(h, tgt, crp, csrc, st);
   // FP32 coupling copy: MIXED pads the columns to whole 16-byte row vectors (the sweeps read rows
   // as vectors); MIXED_BF16 keeps the FP64 layout, shared with its BF16 blocks
-  h->ld32 = op.precision == BMI_PRECISION_MIXED ? fp32_coupling_ld(h->cplx, h->cd) : h->cd;
+  h->ld32 = h->mxf ? h->np2 : op.precision == BMI_PRECISION_MIXED ? fp32_coupling_ld(h->cplx, h->cd) : h->cd;
+  const int cdim = h->mxf ? h->np2 : h->cd;  // rows of a stored block
   if (op.precision != BMI_PRECISION_FP64)
-    h->d_coup32.alloc(static_cast<size_t>(S) * vd * vd * h->ld32 * h->cd * nl);
+    h->d_coup32.alloc(static_cast<size_t>(S) * bpl * h->ld32 * cdim * nl);
   BMI_CHECK_HIP(hipStreamSynchronize(st));
   if (op.precision != BMI_PRECISION_FP64 && h->d_coup.n)
-    convert_blocks_to_fp32(h->d_coup.ptr, h->d_coup32.ptr, static_cast<int64_t>(nl) * vd * vd, h->cd, h->ld32,
-                           h->cplx, st);
+    convert_blocks_to_fp32(h->d_coup.ptr, h->d_coup32.ptr, static_cast<int64_t>(nl) * bpl, cdim, h->ld32, h->cplx,
+                           st);
   h->red1 = op.precision == BMI_PRECISION_MIXED && vd == 1;
   if (h->red1) {
     h->d_dense32.alloc(h->d_dense.n);
     if (h->d_dense.n) convert_to_fp32(h->d_dense.ptr, h->d_dense32.ptr, h->d_dense.n, st);
   }
-  if (op.precision == BMI_PRECISION_MIXED_BF16) {
+  if (op.precision == BMI_PRECISION_MIXED_BF16 && !h->mxf) {  // Maxwell: FP32 at every level
     // blocks are sorted by level: the coarse ones (levels 0 and 1) come first
     h->ncoarse = 0;
     while (h->ncoarse < nl && h->lowrank[h->ncoarse].level <= 1) ++h->ncoarse;
@@ -1266,9 +1334,15 @@ bmi_status bmi_h2_estimate_bytes(const bmi_space* s, const bmi_op_desc* op, int 
   const int64_t dd = (int64_t(1) << (2 * mcl)) * s->I2, cd = nc * op->n_cheb * op->n_cheb;
   const int L = s->M - mcl;
   const int64_t lvl_total = int64_t(s->npatch) * (((int64_t(1) << (2 * (L + 1))) - 1) / 3) * cd;
-  int64_t b = S * vd * vd * (int64_t(dense.size()) * dd * dd + int64_t(lowrank.size()) * cd * cd) * 8;
+  // Maxwell stores the kernel values of a coupling leaf once (np2 x np2), plus node tangents
+  const bool mxf = op->kind == BMI_MAXWELL_SL;
+  const int64_t np2 = int64_t(op->n_cheb) * op->n_cheb;
+  const int64_t cpl = mxf ? np2 * np2 : vd * vd * cd * cd;  // stored entries per coupling leaf
+  int64_t b = S * (vd * vd * int64_t(dense.size()) * dd * dd + int64_t(lowrank.size()) * cpl) * 8;
   const int64_t ld32 = op->precision == BMI_PRECISION_MIXED ? fp32_coupling_ld(S == 2, static_cast<int>(cd)) : cd;
-  if (op->precision != BMI_PRECISION_FP64) b += S * vd * vd * int64_t(lowrank.size()) * ld32 * cd * 4;
+  if (op->precision != BMI_PRECISION_FP64)
+    b += S * int64_t(lowrank.size()) * (mxf ? np2 * np2 : vd * vd * ld32 * cd) * 4;
+  if (mxf) b += lvl_total / cd * np2 * 6 * 8 + std::max(nrhs, 1) * S * 2 * lvl_total * 8;
   if (op->precision == BMI_PRECISION_MIXED && vd == 1) {
     b += S * int64_t(dense.size()) * dd * dd * 4;                     // FP32 near field
     if (S == 1) b += lvl_total * 16 + lvl_total / cd * 32;            // node tables
@@ -1314,6 +1388,28 @@ bmi_status bmi_h2_leaves(const bmi_h2* h, int* leaves) {
   BMI_API_END
 }
 
+namespace bmi {
+// component block (r, c) of the Maxwell coupling leaf k in the reference layout (cd x cd,
+// column-major, complex): [[K o (t_r(x_i) . t_c(y_j)), 0], [0, -K / kappa^2]] from the leaf's
+// kernel values Kh (np2 x np2 complex) and the tangents of both clusters' nodes
+static void mx_expand_leaf(const bmi_h2* h, int k, int r, int c, const double* Kh, double* F) {
+  const int n2 = h->np2, cd = h->cd;
+  const Leaf& lf = h->lowrank[k];
+  const int64_t e1 = h->lvl_off[lf.level] / cd + lf.c1, e2 = h->lvl_off[lf.level] / cd + lf.c2;
+  std::fill(F, F + 2 * static_cast<int64_t>(cd) * cd, 0.0);
+  for (int j = 0; j < n2; ++j)
+    for (int i = 0; i < n2; ++i) {
+      const double* a = &h->h_mx_tan[(e1 * n2 + i) * 6 + 3 * r];
+      const double* b = &h->h_mx_tan[(e2 * n2 + j) * 6 + 3 * c];
+      const double d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+      const double kr = Kh[2 * (j * n2 + i)], ki = Kh[2 * (j * n2 + i) + 1];
+      const int64_t o = static_cast<int64_t>(j) * cd + i, o1 = static_cast<int64_t>(j + n2) * cd + i + n2;
+      F[2 * o] = kr * d, F[2 * o + 1] = ki * d;
+      F[2 * o1] = h->mx_qre * kr - h->mx_qim * ki, F[2 * o1 + 1] = h->mx_qre * ki + h->mx_qim * kr;
+    }
+}
+}  // namespace bmi
+
 bmi_status bmi_h2_leaf_matrix(const bmi_h2* h, int leaf, int r, int c, double* F) {
   BMI_API_BEGIN
   BMI_REQUIRE(h && F, BMI_INVALID_ARG, "null argument");
@@ -1325,6 +1421,12 @@ bmi_status bmi_h2_leaf_matrix(const bmi_h2* h, int leaf, int r, int c, double* F
     const int64_t blk = static_cast<int64_t>(h->dd) * h->dd;
     const double* src = h->d_dense.ptr + S * ((static_cast<int64_t>(leaf) * h->vd * h->vd + r * h->vd + c) * blk);
     BMI_CHECK_HIP(hipMemcpy(F, src, S * blk * sizeof(double), hipMemcpyDeviceToHost));
+  } else if (h->mxf) {
+    const int64_t kb = static_cast<int64_t>(h->np2) * h->np2;
+    std::vector<double> Kh(2 * kb);
+    BMI_CHECK_HIP(hipMemcpy(Kh.data(), h->d_coup.ptr + 2 * (leaf - nd) * kb, 2 * kb * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    mx_expand_leaf(h, leaf - nd, r, c, Kh.data(), F);
   } else {
     const int64_t blk = static_cast<int64_t>(h->cd) * h->cd;
     const double* src =
@@ -1338,7 +1440,19 @@ bmi_status bmi_h2_export_leaves(const bmi_h2* h, double* data) {
   BMI_API_BEGIN
   BMI_REQUIRE(h && data, BMI_INVALID_ARG, "null argument");
   if (h->d_dense.n) BMI_CHECK_HIP(hipMemcpy(data, h->d_dense.ptr, h->d_dense.bytes(), hipMemcpyDeviceToHost));
-  if (h->d_coup.n) BMI_CHECK_HIP(hipMemcpy(data + h->d_dense.n, h->d_coup.ptr, h->d_coup.bytes(), hipMemcpyDeviceToHost));
+  if (h->mxf && h->d_coup.n) {
+    // Maxwell: every leaf's component blocks expanded to the reference layout
+    std::vector<double> Kh(h->d_coup.n);
+    BMI_CHECK_HIP(hipMemcpy(Kh.data(), h->d_coup.ptr, h->d_coup.bytes(), hipMemcpyDeviceToHost));
+    const int64_t kb = static_cast<int64_t>(h->np2) * h->np2, blk = static_cast<int64_t>(h->cd) * h->cd;
+    double* out = data + h->d_dense.n;
+    for (int k = 0; k < static_cast<int>(h->lowrank.size()); ++k)
+      for (int r = 0; r < h->vd; ++r)
+        for (int c = 0; c < h->vd; ++c)
+          mx_expand_leaf(h, k, r, c, Kh.data() + 2 * k * kb, out + 2 * ((static_cast<int64_t>(k) * h->vd + r) * h->vd + c) * blk);
+  } else if (h->d_coup.n) {
+    BMI_CHECK_HIP(hipMemcpy(data + h->d_dense.n, h->d_coup.ptr, h->d_coup.bytes(), hipMemcpyDeviceToHost));
+  }
   BMI_API_END
 }
 

@@ -54,6 +54,14 @@ bool launch_multi_cpl_t(bool cplx, bool fp32, int ntgt, int nrhs, int cd, int ld
 // column stride of the FP32 coupling copy: cd rounded up to whole 16-byte row vectors
 int fp32_coupling_ld(bool cplx, int cd);
 void convert_blocks_to_fp32(const double* src, float* dst, int64_t nb, int cd, int ld, bool cplx, hipStream_t st);
+// Maxwell far field from the kernel values of each coupling leaf: the four columns (tangent-
+// weighted value moments, summed derivative moments) of every cluster entry, then one sweep
+// that writes both components' local expansions of every cluster entry
+void launch_mx_prep(int64_t ne, int nrhs, int np2, int cd, const double* fw0, const double* fw1, const double* tan,
+                    double* Z, hipStream_t st);
+void launch_mx_coupling(bool fp32, int ntgt_all, int nrhs, int np2, int cd, const int* order, const int2* tgt_all,
+                        const int* rp_all, const int* src, const void* K, const int64_t* lvl_off, const double* tan, const double* Z,
+                        double* bw0, double* bw1, double qre, double qim, hipStream_t st);
 void convert_to_bf16(const double* src, unsigned short* dst, int64_t n, hipStream_t st);
 // forward transformation of all levels (moments at level L, transfers up to level 0)
 void launch_forward(bool cplx, int npatch, int L, int rows, int np2, int dd, int nrhs, const double* mom,

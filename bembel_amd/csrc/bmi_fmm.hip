@@ -116,27 +116,14 @@ __global__ void k_coupling_assemble(const int4* __restrict__ leaves, int nleaves
       out[2 * idx] = kr * f;
       out[2 * idx + 1] = ki * f;
     } else {
-      // 4x4 interpolant [[K t.t', 0], [0, -K/kappa^2]] per vector-component pair
-      const double dd[2][2] = {{a[4] * b[4] + a[5] * b[5] + a[6] * b[6], a[4] * b[7] + a[5] * b[8] + a[6] * b[9]},
-                               {a[7] * b[4] + a[8] * b[5] + a[9] * b[6], a[7] * b[7] + a[8] * b[8] + a[9] * b[9]}};
-      const double den = k2re * k2re + k2im * k2im;
-      const double qr = -(kr * k2re + ki * k2im) / den, qi = -(ki * k2re - kr * k2im) / den;
-#pragma unroll
-      for (int ii = 0; ii < 2; ++ii)
-#pragma unroll
-        for (int jj = 0; jj < 2; ++jj) {
-          const int64_t cb = base + (ii * VD + jj) * blk;
-          const int64_t i0 = cb + static_cast<int64_t>(j) * cd + i;                  // (a=0,b=0)
-          const int64_t i1 = cb + static_cast<int64_t>(j + np2) * cd + i + np2;      // (a=1,b=1)
-          const int64_t i01 = cb + static_cast<int64_t>(j + np2) * cd + i;           // (0,1)
-          const int64_t i10 = cb + static_cast<int64_t>(j) * cd + i + np2;           // (1,0)
-          out[2 * i0] = kr * dd[ii][jj];
-          out[2 * i0 + 1] = ki * dd[ii][jj];
-          out[2 * i1] = qr;
-          out[2 * i1 + 1] = qi;
-          out[2 * i01] = 0.0, out[2 * i01 + 1] = 0.0;
-          out[2 * i10] = 0.0, out[2 * i10 + 1] = 0.0;
-        }
+      // Maxwell: the kernel values alone, K(x_i, y_j) (np2 x np2 per leaf).  The reference's
+      // interpolant per vector-component pair (r, c) is [[K t_r(x).t_c(y), 0], [0, -K/kappa^2]]
+      // (t_0, t_1 the tangents d/ds, d/dt); the product applies the tangents and -1/kappa^2 per
+      // cluster around one sweep over K (k_mx_coupling), and bmi_h2_leaf_matrix expands a leaf
+      const int64_t idx = static_cast<int64_t>(leaf) * per + static_cast<int64_t>(j) * np2 + i;
+      out[2 * idx] = kr;
+      out[2 * idx + 1] = ki;
+      (void)k2re, (void)k2im;
     }
   }
 }
@@ -564,6 +551,117 @@ __global__ void k_coupling_apply(int ntgt, int nrhs, int cd, int ld, const int2*
     acc.fma_real(1.0, prev);
   }
   acc.store(bw, o);
+}
+
+// ------------------------------------------------------------------ Maxwell far field
+// The Maxwell coupling leaf (level, t, s) is K_ts (np2 x np2 kernel values at the Chebyshev nodes
+// of clusters t and s), and its component block (r, c) in the reference is
+//   [[K_ts o (t_r(x_i) . t_c(y_j)), 0], [0, -K_ts / kappa^2]]
+// over (value moments, derivative moments) (Maxwell/SingleLayerOperator.hpp:90-120,
+// H2Matrix.hpp:213-232).  Summed over c, the local expansions of component r are
+//   value rows:      t_r(x_i) . sum_s K_ts u_s,   u_s(j) = sum_c t_c(y_j) fw_c[s](j)      (3-vectors)
+//   derivative rows: -1/kappa^2 sum_s K_ts v_s,   v_s(j) = sum_c fw_c[s](np2 + j)
+// so one sweep reads each K_ts once and multiplies it with four columns (u's three components
+// and v), instead of four component blocks of (2 np2)^2 entries.
+//
+// tan: per cluster entry e (level offset + cluster) and node j, 6 doubles (t_0, t_1); Z: per entry
+// and right-hand side, 4 columns of np2 complex values (u_x, u_y, u_z, v).
+__global__ void k_mx_prep(int64_t ne, int nrhs, int np2, int cd, const double* __restrict__ fw0,
+                          const double* __restrict__ fw1, const double* __restrict__ tan, double* __restrict__ Z) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= ne * nrhs * np2) return;
+  const int j = static_cast<int>(gid % np2);
+  const int64_t ek = gid / np2;  // e * nrhs + k
+  const int64_t e = ek / nrhs;
+  const int64_t f = ek * cd;
+  const double a0r = fw0[2 * (f + j)], a0i = fw0[2 * (f + j) + 1];
+  const double a1r = fw1[2 * (f + j)], a1i = fw1[2 * (f + j) + 1];
+  const double* t = tan + (e * np2 + j) * 6;
+  double* z = Z + 2 * (ek * 4 * np2 + j);
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    z[2 * x * np2] = t[x] * a0r + t[3 + x] * a1r;
+    z[2 * x * np2 + 1] = t[x] * a0i + t[3 + x] * a1i;
+  }
+  z[6 * np2] = fw0[2 * (f + np2 + j)] + fw1[2 * (f + np2 + j)];
+  z[6 * np2 + 1] = fw0[2 * (f + np2 + j) + 1] + fw1[2 * (f + np2 + j) + 1];
+}
+
+// One workgroup per (cluster entry of levels 0..L, right-hand side), NG groups of GS threads: group
+// g takes the target's leaves g, g + NG, ... and thread i < np2 of a group owns node i of the
+// target.  Per leaf the group stages the source's four columns in LDS (read by every lane at the
+// same address) and its lanes stream their rows of K (consecutive lanes, consecutive addresses);
+// the groups' sums are added in group order through LDS, and the epilogue writes both
+// components' local expansions of the target (zero for a target without leaves, so the
+// expansions need no clearing pass).  Several groups per target keep enough loads in flight:
+// there are only as many targets as clusters; `order` lists the targets by decreasing leaf count,
+// so the longest workgroups start first.
+template <typename FT, int GS, int NG>
+__global__ __launch_bounds__(GS* NG) void k_mx_coupling(int nrhs, int np2, int cd, const int* __restrict__ order,
+                                                         const int2* __restrict__ tgt,
+                                                         const int* __restrict__ rp, const int* __restrict__ src,
+                                                         const FT* __restrict__ K, const int64_t* __restrict__ lvl_off,
+                                                         const double* __restrict__ tan, const double* __restrict__ Z,
+                                                         double* __restrict__ bw0, double* __restrict__ bw1, double qre,
+                                                         double qim) {
+  __shared__ double zs[NG * 8 * GS];
+  const int t = order[blockIdx.x], k = blockIdx.y;
+  const int g = threadIdx.x / GS, i = threadIdx.x % GS;
+  double* zg = zs + g * 8 * GS;
+  const int2 T = tgt[t];
+  const int64_t e0 = lvl_off[T.x] / cd;  // first entry of the level
+  const int64_t per = static_cast<int64_t>(np2) * np2;
+  const int pe = rp[t + 1];
+  double wr[4] = {0, 0, 0, 0}, wi[4] = {0, 0, 0, 0};
+  for (int p0 = rp[t]; p0 < pe; p0 += NG) {
+    const int p = p0 + g;
+    const bool on = p < pe;
+    __syncthreads();
+    if (on) {
+      const double* zsrc = Z + 2 * ((e0 + src[p]) * nrhs + k) * 4 * np2;
+      for (int q = i; q < 8 * np2; q += GS) zg[q] = zsrc[q];
+    }
+    __syncthreads();
+    if (on && i < np2) {
+      const FT* kp = K + 2 * (static_cast<int64_t>(p) * per + i);
+#pragma unroll 9
+      for (int j = 0; j < np2; ++j) {
+        const double gr = static_cast<double>(kp[2 * static_cast<int64_t>(j) * np2]);
+        const double gi = static_cast<double>(kp[2 * static_cast<int64_t>(j) * np2 + 1]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const double zr = zg[2 * (c * np2 + j)], zi = zg[2 * (c * np2 + j) + 1];
+          wr[c] += gr * zr - gi * zi;
+          wi[c] += gr * zi + gi * zr;
+        }
+      }
+    }
+  }
+  // group sums in group order
+  __syncthreads();
+  if (g > 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) zs[((g - 1) * 8 + 2 * c) * GS + i] = wr[c], zs[((g - 1) * 8 + 2 * c + 1) * GS + i] = wi[c];
+  }
+  __syncthreads();
+  if (g > 0 || i >= np2) return;
+  for (int h = 1; h < NG; ++h)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) wr[c] += zs[((h - 1) * 8 + 2 * c) * GS + i], wi[c] += zs[((h - 1) * 8 + 2 * c + 1) * GS + i];
+  const int64_t e = e0 + T.y;
+  const double* tt = tan + (e * np2 + i) * 6;
+  const int64_t o = (e * nrhs + k) * cd;
+  double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    r0 += tt[x] * wr[x], i0 += tt[x] * wi[x];
+    r1 += tt[3 + x] * wr[x], i1 += tt[3 + x] * wi[x];
+  }
+  const double dr = qre * wr[3] - qim * wi[3], di = qre * wi[3] + qim * wr[3];
+  bw0[2 * (o + i)] = r0, bw0[2 * (o + i) + 1] = i0;
+  bw1[2 * (o + i)] = r1, bw1[2 * (o + i) + 1] = i1;
+  bw0[2 * (o + np2 + i)] = dr, bw0[2 * (o + np2 + i) + 1] = di;
+  bw1[2 * (o + np2 + i)] = dr, bw1[2 * (o + np2 + i) + 1] = di;
 }
 
 // ------------------------------------------------------------------ backward transformation
@@ -2893,6 +2991,35 @@ bool launch_multi_cpl_t(bool cplx, bool fp32, int ntgt, int nrhs, int cd, int ld
 int fp32_coupling_ld(bool cplx, int cd) {
   const int rpl = cplx ? 2 : 4;
   return (cd + rpl - 1) / rpl * rpl;
+}
+
+void launch_mx_prep(int64_t ne, int nrhs, int np2, int cd, const double* fw0, const double* fw1, const double* tan,
+                    double* Z, hipStream_t st) {
+  const int64_t n = ne * nrhs * np2;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mx_prep, dim3(nblk(n)), dim3(256), 0, st, ne, nrhs, np2, cd, fw0, fw1, tan, Z);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+void launch_mx_coupling(bool fp32, int ntgt_all, int nrhs, int np2, int cd, const int* order, const int2* tgt_all,
+                        const int* rp_all, const int* src, const void* K, const int64_t* lvl_off, const double* tan, const double* Z,
+                        double* bw0, double* bw1, double qre, double qim, hipStream_t st) {
+  if (ntgt_all <= 0 || nrhs <= 0) return;
+  if (np2 > 256) throw Error(BMI_UNSUPPORTED, "Maxwell far field with more than 16x16 Chebyshev nodes");
+  const dim3 grid(static_cast<unsigned>(ntgt_all), static_cast<unsigned>(nrhs));
+  TimedScope ts("coupling_apply", st);
+#define BMI_MXC(FT, GS, NG)                                                                                      \
+  hipLaunchKernelGGL((k_mx_coupling<FT, GS, NG>), grid, dim3(GS * NG), 0, st, nrhs, np2, cd, order, tgt_all,       \
+                     rp_all, src, static_cast<const FT*>(K), lvl_off, tan, Z, bw0, bw1, qre, qim)
+  if (np2 <= 64) {
+    if (fp32) BMI_MXC(float, 64, 16); else BMI_MXC(double, 64, 16);
+  } else if (np2 <= 128) {
+    if (fp32) BMI_MXC(float, 128, 8); else BMI_MXC(double, 128, 8);
+  } else {
+    if (fp32) BMI_MXC(float, 256, 4); else BMI_MXC(double, 256, 4);
+  }
+#undef BMI_MXC
+  BMI_CHECK_HIP(hipGetLastError());
 }
 
 void convert_blocks_to_fp32(const double* src, float* dst, int64_t nb, int cd, int ld, bool cplx, hipStream_t st) {

@@ -147,6 +147,8 @@ int or_gmres_h2(const or_h2* h, const double* b, double* x, double tol, int rest
 double or_time_assembly_sample(const or_h2* h, int npairs, const int* pairs, int nblocks,
                                const int* blocks, int nthreads);
 double or_time_matvec(const or_h2* h, int nrhs, int reps, int nthreads);
+/* OpenMP threads of the oracle's later parallel regions (or_h2_build, matvecs): CPU baseline */
+void or_set_num_threads(int nthreads);
 
 #ifdef __cplusplus
 }

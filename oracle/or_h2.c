@@ -905,6 +905,10 @@ double or_time_assembly_sample(const or_h2* h, int npairs, const int* pairs, int
   return t1 - t0;
 }
 
+void or_set_num_threads(int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+}
+
 double or_time_matvec(const or_h2* h, int nrhs, int reps, int nthreads) {
   int c = h->cplx ? 2 : 1;
   size_t D = h->s->ndof;

@@ -77,6 +77,8 @@ def lib():
         L.or_gmres_h2.argtypes = [vp, dp, dp, C.c_double, C.c_int, C.c_int]
         L.or_time_assembly_sample.restype = C.c_double
         L.or_time_assembly_sample.argtypes = [vp, C.c_int, ip, C.c_int, ip, C.c_int]
+        L.or_set_num_threads.restype = None
+        L.or_set_num_threads.argtypes = [C.c_int]
         L.or_time_matvec.restype = C.c_double
         L.or_time_matvec.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         _lib = L
