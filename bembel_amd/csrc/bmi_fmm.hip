@@ -1190,6 +1190,37 @@ __device__ __forceinline__ void coupling_wave_r2(int t, const CplArgs& a) {
 // supplies its partial value v(j) of every column j < W*R; on return out[r] holds the group's sum
 // of column R*li + r. Recursive halving: under mask m each lane keeps the half of its columns
 // selected by (li & m) and adds its partner's copy of that half.
+// the partner value of a recursive-halving round with mask m inside rows of 16 lanes, through a
+// DPP lane move (no LDS): m = 8 pairs lane i with 15 - i (row mirror), m = 4 with 7 - i (half-row
+// mirror), m = 2 and 1 with i ^ 2 and i ^ 1 (quad permutes).  Each partner differs from the lane
+// in bit m and, with the rounds taken from the largest mask down, agrees in every bit an earlier
+// round split on, so both hold the same columns.  Larger masks go through the LDS permute.
+template <int M>
+__device__ __forceinline__ double halving_partner(double v) {
+  constexpr int CTRL = M == 8 ? 0x140 : M == 4 ? 0x141 : M == 2 ? 0x4E : 0xB1;
+  if constexpr (M >= 16) {
+    return __shfl_xor(v, M, 64);
+  } else {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+  }
+}
+
+// rounds m, m / 2, ..., 1 of the recursive halving on h (m * 2 * R live values)
+template <int M, int R, int NH>
+__device__ __forceinline__ void halving_rounds(int li, double (&h)[NH]) {
+  if constexpr (M >= 1) {
+    const bool up = (li & M) != 0;
+#pragma unroll
+    for (int k = 0; k < M * R; ++k) {
+      const double lo = h[k], hi = h[k + M * R];
+      h[k] = (up ? hi : lo) + halving_partner<M>(up ? lo : hi);
+    }
+    halving_rounds<M / 2, R>(li, h);
+  }
+}
+
 template <int W, int R, class V>
 __device__ __forceinline__ void group_column_sums(int li, V v, double (&out)[R]) {
   constexpr int N = W * R;
@@ -1203,17 +1234,9 @@ __device__ __forceinline__ void group_column_sums(int li, V v, double (&out)[R])
 #pragma unroll
     for (int k = 0; k < N / 2; ++k) {
       const double lo = v(k), hi = v(k + N / 2);
-      h[k] = (up0 ? hi : lo) + __shfl_xor(up0 ? lo : hi, M0, 64);
+      h[k] = (up0 ? hi : lo) + halving_partner<M0>(up0 ? lo : hi);
     }
-#pragma unroll
-    for (int m = M0 / 2; m >= 1; m >>= 1) {
-      const bool up = (li & m) != 0;
-#pragma unroll
-      for (int k = 0; k < m * R; ++k) {
-        const double lo = h[k], hi = h[k + m * R];
-        h[k] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, m, 64);
-      }
-    }
+    halving_rounds<M0 / 2, R>(li, h);
 #pragma unroll
     for (int r = 0; r < R; ++r) out[r] = h[r];
   }
