@@ -8,6 +8,8 @@
 #include "bembel_amd.h"
 extern std::atomic<bool> g_teardown;
 extern std::atomic<int> g_wrong_thread;
+extern std::atomic<bool> g_destroyed;
+extern std::atomic<int> g_after_destroy;
 void reset_device_owner();
 int main(int argc, char** argv) {
   const int rounds = argc > 1 ? atoi(argv[1]) : 20;
@@ -59,6 +61,12 @@ int main(int argc, char** argv) {
     g_teardown = true;
     bmi_server_destroy(srv);
     g_teardown = false;
+    // nothing of the server may still run once destroy returned (a thread that outlives it can be
+    // inside a device call while the process exits and the runtime is torn down)
+    g_destroyed = true;
+    std::this_thread::sleep_for(std::chrono::milliseconds(20));
+    g_destroyed = false;
+    if (g_after_destroy.load()) { printf("round %d: device calls after bmi_server_destroy\n", round); return 4; }
     reset_device_owner();
     if (bad) { printf("round %d: %d bad results\n", round, bad); return 2; }
     if (g_wrong_thread.load()) { printf("round %d: device calls from more than one thread\n", round); return 3; }

@@ -23,7 +23,12 @@ void reset_device_owner() {
   std::lock_guard<std::mutex> lk(g_owner_mu);
   g_owner = std::thread::id();
 }
+// device calls made after the driver stopped waiting on the server (bmi_server_destroy returned):
+// none may happen, the device thread is joined before the server is freed
+std::atomic<bool> g_destroyed{false};
+std::atomic<int> g_after_destroy{0};
 static void device_call() {
+  if (g_destroyed.load()) g_after_destroy.fetch_add(1);
   if (g_teardown.load()) return;
   std::lock_guard<std::mutex> lk(g_owner_mu);
   if (g_owner == std::thread::id()) g_owner = std::this_thread::get_id();

@@ -43,12 +43,11 @@ def solve_all(srv, reqs):
     return [srv.result(i, timeout=240) for i in ids]
 
 
-@pytest.mark.skip(reason="several batches on the device at once, one host worker thread per batch, ended once in "
-                         "a host segmentation fault inside a worker thread; the scheduler is clean under "
-                         "ThreadSanitizer and AddressSanitizer with stubbed device calls and the cause below it is "
-                         "not found, so that design stays off the GPU (the server now drives all batches from one "
-                         "device thread as GMRES tasks on their own streams: "
-                         "test_gpu_server_overlapping_mix_equals_alone)")
+@pytest.mark.skip(reason="the thread-per-batch design it tested is gone: its detached workers could outlive the "
+                         "server (a worker's last operator reference freed device memory after bmi_server_destroy "
+                         "returned; nothing joined the workers at exit), which ended once in a host segmentation "
+                         "fault inside a worker thread; the server now drives all batches from one joined device "
+                         "thread (test_gpu_server_overlapping_mix_equals_alone)")
 def test_gpu_server_mix_equals_alone(gpu):
     reqs = requests()
     alone = []
