@@ -262,7 +262,7 @@ def run_batched(B, torch, rng, steps):
         out[name + "_matvec_dof_rhs_per_s"] = n * nrhs / t
         r1 = rhs[:, :1].contiguous()
         HH.matvec(r1)  # first single-RHS product of the handle (graph capture)
-        t1, _ = timed(torch, lambda: HH.matvec(r1, sync=False), max(3, steps // 4))
+        t1, _ = timed(torch, lambda: HH.matvec(r1, sync=False), max(20, steps))
         out[name + "_matvec_1rhs_ms"] = t1 * 1e3
     log("batched 64 RHS: matvec fp64 %.2f ms, mixed %.2f ms" % (out["fp64_matvec_ms"], out["mixed_matvec_ms"]))
     # matrix-core roofline of the FP64 64-RHS block sweeps: algorithmic flops (8 per complex

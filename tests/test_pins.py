@@ -206,6 +206,11 @@ def test_rhs_plants_are_caught():
     bw = S.rhs(1, np.array([0.0, 0.0, 1.0]), 4.0)
     assert abs((c @ bw).imag) < 1e-10  # symmetric sphere: the sine part integrates to zero
     assert abs((c @ bw).real - 4 * math.pi * math.sin(4.0) / 4.0) < 1e-9
+    # a wrong wavenumber inside the plane wave (4.1 for 4) or a moved source point is seen
+    bk = S.rhs(1, np.array([0.0, 0.0, 1.0]), 4.1)
+    assert abs((c @ bk).real - 4 * math.pi * math.sin(4.0) / 4.0) > 1e-2
+    bo = S.rhs(0, np.array([2.0, 0.5, -0.3]))
+    assert abs(c @ bo - 4 * math.pi / np.linalg.norm([2.1, 0.5, -0.3])) > 1e-2
 
 
 # ------------------------------------------------------------------ oracle GMRES
