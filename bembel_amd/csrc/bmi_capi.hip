@@ -2,7 +2,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -11,6 +13,7 @@
 #include <memory>
 #include <mutex>
 #include <sstream>
+#include <thread>
 
 #include "bmi_fmm.h"
 #include "bmi_internal.h"
@@ -50,6 +53,21 @@ TimedScope::~TimedScope() {
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+}
+
+static double wall_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+HostTimedScope::HostTimedScope(const char* n) : name(n) {
+  if (timing_enabled()) t0 = wall_ms();
+}
+HostTimedScope::~HostTimedScope() {
+  if (t0 < 0) return;
+  const double ms = wall_ms() - t0;
+  std::lock_guard<std::mutex> g(g_tmu);
+  auto& e = g_times[name];
+  e.first += ms;
+  e.second += 1;
 }
 
 __global__ void k_fill(double* p, int64_t n, double v) {
@@ -732,38 +750,64 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   BMI_REQUIRE(h->L >= 0, BMI_UNSUPPORTED, "refinement level below min_cluster_level");
   h->dd = (1 << (2 * h->mcl)) * s->I2;
   h->cd = h->nc * h->np2;
-  build_block_tree(s, op.eta, h->mcl, h->dense, h->lowrank);
+  {
+    HostTimedScope hs("assembly_host_block_tree");
+    build_block_tree(s, op.eta, h->mcl, h->dense, h->lowrank);
+  }
   const int S = h->cplx ? 2 : 1;
   const int vd = h->vd;
   // ---- near field
   const int ne = 1 << (2 * h->mcl);
   const int64_t dblk = static_cast<int64_t>(h->dd) * h->dd;
   h->d_dense.alloc(static_cast<size_t>(S) * vd * vd * dblk * h->dense.size());
-  std::vector<NFPair> pairs;
-  pairs.reserve(h->dense.size() * ne * ne);
+  // every element pair of every dense leaf: class, Duffy rotations and quadrature degree, in
+  // parallel over ranges of leaves (each pair has a fixed slot; the host decisions are pure
+  // functions of the space)
+  std::vector<NFPair> pairs(h->dense.size() * ne * ne);
   const int ff = farfield_degree(op.kind, s->P);
-  for (size_t k = 0; k < h->dense.size(); ++k) {
-    const Leaf& lf = h->dense[k];
-    BMI_REQUIRE(lf.level == h->L, BMI_UNSUPPORTED, "dense leaf above the leaf level");
-    for (int b = 0; b < ne; ++b)
-      for (int a = 0; a < ne; ++a) {
-        NFPair p;
-        p.e1 = lf.c1 * ne + a;
-        p.e2 = lf.c2 * ne + b;
-        int cp[3];
-        double dist;
-        compare_elements(s, p.e1, p.e2, cp, &dist);
-        int nf = nearfield_degree(op.kind, s->P, dist, s->M);
-        nf = nf >= ff ? nf : ff;
-        BMI_REQUIRE(nf < 50, BMI_UNSUPPORTED, "near-field quadrature degree too large");
-        p.cls = (cp[2] == 0) ? (nf == ff ? 0 : 1) : cp[2];
-        p.rot1 = cp[0];
-        p.rot2 = cp[1];
-        p.deg = nf;
-        p.off = static_cast<int64_t>(k) * vd * vd * dblk + static_cast<int64_t>(b * s->I2) * h->dd + a * s->I2;
-        h->pair_class_count[p.cls]++;
-        pairs.push_back(p);
+  {
+    HostTimedScope hs("assembly_host_pairs");
+    const int64_t nd = static_cast<int64_t>(h->dense.size());
+    const int nth = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(
+        std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())), nd / 256)));
+    std::vector<std::array<int, 5>> counts(nth, std::array<int, 5>{0, 0, 0, 0, 0});
+    std::vector<std::string> errors(nth);
+    auto work = [&](int t) {
+      try {
+        for (int64_t k = nd * t / nth; k < nd * (t + 1) / nth; ++k) {
+          const Leaf& lf = h->dense[k];
+          BMI_REQUIRE(lf.level == h->L, BMI_UNSUPPORTED, "dense leaf above the leaf level");
+          for (int b = 0; b < ne; ++b)
+            for (int a = 0; a < ne; ++a) {
+              NFPair& p = pairs[(k * ne + b) * ne + a];
+              p.e1 = lf.c1 * ne + a;
+              p.e2 = lf.c2 * ne + b;
+              int cp[3];
+              double dist;
+              compare_elements(s, p.e1, p.e2, cp, &dist);
+              int nf = nearfield_degree(op.kind, s->P, dist, s->M);
+              nf = nf >= ff ? nf : ff;
+              BMI_REQUIRE(nf < 50, BMI_UNSUPPORTED, "near-field quadrature degree too large");
+              p.cls = (cp[2] == 0) ? (nf == ff ? 0 : 1) : cp[2];
+              p.rot1 = cp[0];
+              p.rot2 = cp[1];
+              p.deg = nf;
+              p.off = k * vd * vd * dblk + static_cast<int64_t>(b * s->I2) * h->dd + a * s->I2;
+              counts[t][p.cls]++;
+            }
+        }
+      } catch (const std::exception& e) {
+        errors[t] = e.what();
       }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    for (int t = 0; t < nth; ++t) {
+      BMI_REQUIRE(errors[t].empty(), BMI_UNSUPPORTED, errors[t]);
+      for (int c = 0; c < 5; ++c) h->pair_class_count[c] += counts[t][c];
+    }
   }
   h->npairs = static_cast<int64_t>(pairs.size());
   // component block (rc, cc) of leaf k starts at (k*vd*vd + rc*vd + cc) * dblk

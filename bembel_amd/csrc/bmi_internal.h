@@ -172,6 +172,13 @@ struct TimedScope {
   TimedScope(const char* n, hipStream_t s);
   ~TimedScope();
 };
+// host phases under the same registry (wall clock, when timing is enabled)
+struct HostTimedScope {
+  const char* name;
+  double t0 = -1.0;
+  explicit HostTimedScope(const char* n);
+  ~HostTimedScope();
+};
 bool timing_enabled();
 void set_last_error(const std::string& m);
 

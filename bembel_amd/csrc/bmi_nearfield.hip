@@ -1102,12 +1102,24 @@ void launch_nearfield(const bmi_space* s, const bmi_op_desc& op, std::vector<NFP
   ensure_device_space(s, st);
   const int ff = farfield_degree(op.kind, s->P);
   const FFTable* tab = ensure_ffield(s, ff, st, op.kind == BMI_MAXWELL_SL);
-  std::stable_sort(pairs.begin(), pairs.end(), [](const NFPair& a, const NFPair& b) {
-    if (a.cls != b.cls) return a.cls < b.cls;
-    return a.deg > b.deg;
-  });
   DevBuf<NFPair> d_pairs;
-  d_pairs.upload(pairs, st);
+  {
+    // stable counting sort by class, then by decreasing degree (degrees < 50): the launches take
+    // one class each, its pairs with the largest rules first
+    HostTimedScope hs("assembly_host_pair_sort");
+    constexpr int ND = 50;
+    std::vector<int64_t> start(5 * ND + 1, 0);
+    for (const NFPair& p : pairs) {
+      BMI_REQUIRE(p.cls >= 0 && p.cls < 5 && p.deg >= 0 && p.deg < ND, BMI_UNSUPPORTED,
+                  "near-field pair class or quadrature degree out of range");
+      start[p.cls * ND + (ND - 1 - p.deg) + 1]++;
+    }
+    for (int b = 0; b < 5 * ND; ++b) start[b + 1] += start[b];
+    std::vector<NFPair> sorted(pairs.size());
+    for (const NFPair& p : pairs) sorted[start[p.cls * ND + (ND - 1 - p.deg)]++] = p;
+    pairs.swap(sorted);
+    d_pairs.upload(pairs, st);
+  }
   NFParams prm;
   prm.cp = s->d_cp.ptr;
   prm.cp_stride = 4 * s->pd * s->pd;

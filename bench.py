@@ -744,6 +744,12 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    # warm-up: one assembly and product on a 96-element sphere loads the library's code objects
+    # and sizes the runtime's pools (a one-time cost of the process, about 0.1 s), untimed
+    wsp = B.AnsatzSpace(B.Geometry(), 2, P, op)
+    wh = B.H2Matrix(wsp, op, n_cheb=npts)
+    wh.matvec(np.ones(wsp.number_of_dofs))
+    del wh, wsp
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     H = B.H2Matrix(space, op, n_cheb=npts)
@@ -756,7 +762,7 @@ def main():
     # per-kernel breakdown of the assembly (second assembly, events around every launch)
     duffy_ms = reg_ms = cpl_ms = 0.0
     duffy_n = reg_n = cpl_n = 0
-    cls_ms = {}
+    cls_ms, host_ms = {}, {}
     if not a.no_breakdown:
         B.timing.reset()
         B.timing.enable(True)
@@ -767,9 +773,11 @@ def main():
         duffy_n = sum(B.timing.get("nf_" + k)[1] for k in ("identical", "edge", "vertex"))
         reg_ms, reg_n = B.timing.get("nf_regular")
         cpl_ms, cpl_n = B.timing.get("coupling_assemble")
+        host_ms = {k: B.timing.get("assembly_host_" + k)[0] for k in ("block_tree", "pairs", "pair_sort")}
         del H2b
-    log("assembly kernels: nf_duffy %.1f ms (%d launches: %s), nf_regular %.1f ms (%d), coupling %.1f ms" % (
-        duffy_ms, duffy_n, ", ".join("%s %.1f" % kv for kv in cls_ms.items()), reg_ms, reg_n, cpl_ms))
+    log("assembly kernels: nf_duffy %.1f ms (%d launches: %s), nf_regular %.1f ms (%d), coupling %.1f ms; host: %s" % (
+        duffy_ms, duffy_n, ", ".join("%s %.1f" % kv for kv in cls_ms.items()), reg_ms, reg_n, cpl_ms,
+        ", ".join("%s %.1f ms" % kv for kv in host_ms.items())))
 
     # ---------------------------------------------------------------- matvec (the step)
     g = torch.Generator(device="cuda")
@@ -928,7 +936,7 @@ def main():
         "h2_assembly_s": t_asm,
         "space_setup_s": t_space,
         "assembly_kernels_ms": {"nf_duffy": duffy_ms, "nf_duffy_by_class": cls_ms, "nf_regular": reg_ms,
-                                "coupling_assemble": cpl_ms},
+                                "coupling_assemble": cpl_ms, "host_ms": host_ms},
         "h2": {k: info[k] for k in ("dense", "lowrank", "pairs", "pair_classes", "device_bytes")},
         "solve": {"solver": "CG", "tol": 1e-10, "fp64_iterations": it64, "fp64_p50_s": pct(lat64, 50),
                   "fp64_p99_s": pct(lat64, 99), "mixed_p50_s": pct(latm, 50), "mixed_p99_s": pct(latm, 99),
