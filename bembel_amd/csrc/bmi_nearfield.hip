@@ -564,6 +564,7 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
   const int cc = KIND == BMI_MAXWELL_SL ? static_cast<int>(blockIdx.y) & 1 : 0;
   Acc acc;
   acc.zero();
+  double aval = 0.0;  // the block entry a thread owns (factorised identical rule)
   // every lane runs the same number of iterations (lane groups swap values); lanes past the end
   // evaluate a valid point with zero weight
 
@@ -630,7 +631,137 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       acc.add(q1[i], q2[j], prm, rc, cc, valid, lane);
     }
     __syncthreads();
+  } else if constexpr (CLS == 2 && (KIND == BMI_LAPLACE_SL || KIND == BMI_HELMHOLTZ_SL)) {
+    // integrate2, factorised (symmetric kernels).  The sub-simplex pairs of the unfactorised form
+    // below are, with t1 = xj0 (1 - xi0), t3 = t1 + xi0 (outer unit (i, j0)) and t2 = xj1 (1 - xi0 xi1),
+    // t4 = t2 + xi0 xi1 (inner index j1): (t1,t2)-(t3,t4), (t1,t4)-(t3,t2), (t2,t1)-(t4,t3),
+    // (t2,t3)-(t4,t1).  For one unit the eight points lie on four coordinate lines of the patch
+    // (s = t1, s = t3, t = t1, t = t3), contracted once per round into LDS (one line per thread),
+    // and each point is a one-dimensional evaluation.  The tensor basis splits the block: with
+    // B(x) the Bernstein values, A((ix,iy),(jx,jy)) += B_ix(t1) B_jx(t3) U01(iy,jy) + B_iy(t1)
+    // B_jy(t3) U23(ix,jx), U01 = sum over j1 of K0 B(t2) B(t4)^T + K1 B(t4) B(t2)^T and U23 likewise
+    // with K2, K3 (the fourth pair taken in the other order, which the symmetric kernel allows).
+    // The rule adds every pair in both orders, so the block is A + A^T, formed when it is written.
+    // S = 4 lanes share a unit (j1 = sub, sub + 4, ...) and add their sums; the unit's sums, basis
+    // values and outer weight go to LDS, and after each round every thread adds the units'
+    // contributions to the one entry of A it owns (no per-lane block in registers).
+    constexpr bool CPLX = Acc::CPLX;
+    constexpr int I = P + 1;
+    constexpr int S = 4;
+    constexpr int UPR = NT / S;                // units per round
+    constexpr int LS = 8 * PD;                 // doubles of one contracted line
+    constexpr int US = 4 * LS + 2;             // doubles per unit: the 16-byte pad puts the 16 units
+                                               // a wave reads at once on different LDS banks
+    constexpr int UR = 7 + (CPLX ? 4 : 2) * I2;  // record of a unit: sc, B(t1), B(t3), U01, U23
+    constexpr int NE = Acc::TOT;               // entries of A (real parts, then imaginary parts)
+    static_assert(UPR * 4 == NT && NE <= NT, "one line and at most one block entry per thread");
+    double* ua = lds + UPR * US;
+    const int units = nq * n;
+    const int myu = tid / S, sub = tid - (tid / S) * S;
+    // the block entry this thread owns
+    const int epart = tid / (I2 * I2), erem = tid - epart * (I2 * I2);
+    const int erow = erem / I2, ecol = erem - erow * I2;
+    const int eiy = erow / I, eix = erow - eiy * I, ejy = ecol / I, ejx = ecol - ejy * I;
+    for (int u0 = 0; u0 < units; u0 += UPR) {
+      __syncthreads();
+      {
+        const int uu = u0 + (tid >> 2), q = tid & 3;
+        if (uu < units) {
+          const int i = uu / n, j0 = uu - (uu / n) * n;
+          const double xi0 = gx[i / n];
+          const double t1 = gx[j0] * (1 - xi0), tf = (q & 1) ? t1 + xi0 : t1;
+          const bool fs = q < 2;
+          line_contract<PD>(cp1, fs, fs ? l1x + h * tf : l1y + h * tf, lds + (tid >> 2) * US + q * LS);
+        }
+      }
+      __syncthreads();
+      const int uu = u0 + myu;
+      const bool act = uu < units;
+      const int uc = act ? uu : 0;
+      const int i = uc / n, j0 = uc - (uc / n) * n;
+      const double xi0 = gx[i / n], xi1 = gx[i % n], wi = gw[i / n] * gw[i % n];
+      const double t1 = gx[j0] * (1 - xi0), t3 = t1 + xi0;
+      // lanes sub = 0, 2 sum U01 over the s-lines, sub = 1, 3 sum U23 over the t-lines, each pair
+      // taking every second j1
+      const bool tl = (sub & 1) != 0;
+      double ur[I2], ui[CPLX ? I2 : 1];
+#pragma unroll
+      for (int k = 0; k < I2; ++k) {
+        ur[k] = 0.0;
+        if constexpr (CPLX) ui[k] = 0.0;
+      }
+      if (act) {
+#pragma unroll 1
+        for (int j1 = sub >> 1; j1 < n; j1 += 2) {
+          const double t2 = gx[j1] * (1 - xi0 * xi1), t4 = t2 + xi0 * xi1;
+          const double wj = gw[j1];
+          // the lines are read again in every iteration, one at a time (an opaque offset keeps
+          // the compiler from holding them in registers across the loop)
+          int off = myu * US + (tl ? 2 * LS : 0);
+          asm volatile("" : "+v"(off));
+          const double* Lq = lds + off;
+          // s-lines: (t1,t2)-(t3,t4) and (t1,t4)-(t3,t2); t-lines: (t2,t1)-(t4,t3) and
+          // (t4,t1)-(t2,t3) (the second pair in the other order); either way the first line
+          // holds the points at (t2, t4) of the first member, the second line those of the second
+          const PtK a0 = line_point<PD, KIND>(Lq, !tl, l1x, l1y, h, tl ? t2 : t1, tl ? t1 : t2, 1.0);
+          const PtK a1 = line_point<PD, KIND>(Lq, !tl, l1x, l1y, h, tl ? t4 : t1, tl ? t1 : t4, tl ? wj : 1.0);
+          __builtin_amdgcn_sched_barrier(0);
+          const PtK b0 = line_point<PD, KIND>(Lq + LS, !tl, l1x, l1y, h, tl ? t4 : t3, tl ? t3 : t4, wj);
+          const PtK b1 = line_point<PD, KIND>(Lq + LS, !tl, l1x, l1y, h, tl ? t2 : t3, tl ? t3 : t2, tl ? 1.0 : wj);
+          double kar, kai, kbr, kbi;
+          kernel_ww<KIND>(a0, b0, prm, kar, kai);
+          kernel_ww<KIND>(a1, b1, prm, kbr, kbi);
+          double B2[I], B4[I];
+          bernstein_basis<P>(t2, B2);
+          bernstein_basis<P>(t4, B4);
+#pragma unroll
+          for (int p = 0; p < I; ++p)
+#pragma unroll
+            for (int q = 0; q < I; ++q) {
+              const double f = B2[p] * B4[q], g = B4[p] * B2[q];
+              ur[p * I + q] += kar * f + kbr * g;
+              if constexpr (CPLX) ui[p * I + q] += kai * f + kbi * g;
+            }
+        }
+      }
+      // the two lanes of each half add their sums; lane sub = 0 writes the unit's record with
+      // U01, lane sub = 1 adds U23
+#pragma unroll
+      for (int k = 0; k < I2; ++k) {
+        ur[k] += __shfl_xor(ur[k], 2, 64);
+        if constexpr (CPLX) ui[k] += __shfl_xor(ui[k], 2, 64);
+      }
+      if (sub < 2) {
+        double* r = ua + myu * UR;
+        if (sub == 0) {
+          double B1[I], B3[I];
+          bernstein_basis<P>(t1, B1);
+          bernstein_basis<P>(t3, B3);
+          r[0] = act ? h * h * wi * xi0 * (1 - xi0) * (1 - xi0 * xi1) * gw[j0] : 0.0;
+#pragma unroll
+          for (int k = 0; k < I; ++k) r[1 + k] = B1[k], r[1 + I + k] = B3[k];
+        }
+        const int o = tl ? I2 : 0;
+#pragma unroll
+        for (int k = 0; k < I2; ++k) {
+          r[7 + o + k] = ur[k];
+          if constexpr (CPLX) r[7 + 2 * I2 + o + k] = ui[k];
+        }
+      }
+      __syncthreads();
+      if (tid < NE) {
+        const int o01 = 7 + (epart ? 2 * I2 : 0), o23 = o01 + I2;
+#pragma unroll 4
+        for (int m = 0; m < UPR; ++m) {
+          const double* r = ua + m * UR;
+          aval += r[0] * (r[1 + eix] * r[1 + I + ejx] * r[o01 + eiy * I + ejy] +
+                          r[1 + eiy] * r[1 + I + ejy] * r[o23 + eix * I + ejx]);
+        }
+      }
+    }
+    __syncthreads();
   } else if constexpr (CLS == 2) {
+    // integrate2: identical elements, 8 Duffy sub-simplices, symmetrised  } else if constexpr (CLS == 2) {
     // integrate2: identical elements, 8 Duffy sub-simplices, symmetrised
     const int total = nq * nq;
     for (int base = 0; base < total; base += NT) {
@@ -953,18 +1084,26 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
   // reduce the per-lane partial blocks and write the result
   double* red = lds;
   double* sum = lds + (NT / 64) * Acc::TOT;
-  reduce_block<NT>(acc, red, tid, sum);
+  if constexpr (CLS == 2 && (KIND == BMI_LAPLACE_SL || KIND == BMI_HELMHOLTZ_SL)) {
+    if (tid < Acc::TOT) sum[tid] = aval;  // each entry summed by its owner
+    __syncthreads();
+  } else {
+    reduce_block<NT>(acc, red, tid, sum);
+  }
   double* o = prm.out;
   constexpr bool CPLX = Acc::CPLX;
   const int64_t base = pr.off + rc * prm.comp_rs + cc * prm.comp_cs;
+  // the factorised identical rule accumulated A; its block is A + A^T
+  constexpr bool SYMW = CLS == 2 && (KIND == BMI_LAPLACE_SL || KIND == BMI_HELMHOLTZ_SL);
   for (int k = tid; k < I2 * I2; k += NT) {
     const int i = k / I2, j = k - (k / I2) * I2;
     const int64_t idx = base + i * prm.rs + j * prm.cs;
+    const int kt = j * I2 + i;
     if (CPLX) {
-      o[2 * idx] = sum[k];
-      o[2 * idx + 1] = sum[I2 * I2 + k];
+      o[2 * idx] = SYMW ? sum[k] + sum[kt] : sum[k];
+      o[2 * idx + 1] = SYMW ? sum[I2 * I2 + k] + sum[I2 * I2 + kt] : sum[I2 * I2 + k];
     } else {
-      o[idx] = sum[k];
+      o[idx] = SYMW ? sum[k] + sum[kt] : sum[k];
     }
   }
 }
@@ -1025,6 +1164,9 @@ static size_t class_lds_bytes(int deg) {
   const int cv = FACT ? vertex_shape(n, NT, PS, BlockAcc<P, KIND>::CPLX).chunk : vertex_chunk<Pt<KIND>>(n, NT);
   const int ce = FACT ? edge_shape(n, NT, PS, BlockAcc<P, KIND>::CPLX).chunk : edge_chunk<Pt<KIND>>(n, NT);
   if (CLS == 3) lds = std::max(lds, static_cast<size_t>(8 * n * ce) * sizeof(Pt<KIND>));
+  if (CLS == 2 && (KIND == BMI_LAPLACE_SL || KIND == BMI_HELMHOLTZ_SL))
+    lds = std::max(lds, static_cast<size_t>(NT / 4) * (32 * PD + 2 + 7 + (BlockAcc<P, KIND>::CPLX ? 4 : 2) *
+                                                     (P + 1) * (P + 1)) * sizeof(double));  // lines, unit records
   if (CLS == 4) lds = std::max(lds, static_cast<size_t>(4 * nq + 2 * nq * cv) * sizeof(Pt<KIND>));
   return lds;
 }
