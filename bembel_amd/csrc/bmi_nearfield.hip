@@ -591,28 +591,73 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       acc.add(pa, pb, prm, rc, cc, valid, lane);
     }
   } else if constexpr (CLS == 1 && !Acc::MAXW) {
-    // integrate1, factorised: surface points of both elements staged in LDS; a lane fixes a
-    // point of element 1 and sums over (its share of) the points of element 2
+    // integrate1, factorised: surface points of both elements staged in LDS; S lanes share a
+    // point of element 1 and each sums kernel x basis vector over its share of the points of
+    // element 2.  The lanes' vectors and the outer points' basis values go to LDS, and every
+    // thread adds the outer products into the one block entry it owns (no per-lane block, no
+    // workgroup reduction).
+    constexpr bool CPLX = Acc::CPLX;
+    constexpr int I = P + 1;
+    constexpr int VR = CPLX ? 2 * I2 : I2;  // doubles of one lane's vector
+    constexpr int NE = Acc::TOT;
+    static_assert(NE <= NT, "at most one block entry per thread");
     PtK* q1 = reinterpret_cast<PtK*>(lds);
     PtK* q2 = q1 + nq;
+    double* rec = reinterpret_cast<double*>(q2 + nq);  // NT vectors
+    double* bas = rec + NT * VR;                       // basis products of the round's outer points
     for (int k = tid; k < nq; k += NT) {
       const double u = gx[k / n], v = gx[k % n], w = h * (gw[k / n] * gw[k % n]);
       q1[k] = make_point<PD, KIND>(cp1, l1x, l1y, h, u, v, w);
       q2[k] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, w);
     }
     __syncthreads();
-    constexpr bool CPLX = Acc::CPLX;
-    int S = NT / nq;
-    if (CPLX) S -= S & 1;
-    S = S < (CPLX ? 2 : 1) ? (CPLX ? 2 : 1) : S;
+    const int S = NT / nq > 1 ? NT / nq : 1;
     const int per_round = NT / S;
-    const int role = lane & (Acc::G - 1);
+    const int epart = tid / (I2 * I2), erem = tid - epart * (I2 * I2);
+    const int erow = erem / I2, ecol = erem - erow * I2;
     for (int u0 = 0; u0 < nq; u0 += per_round) {
       const int unit = u0 + tid / S, sub = tid - (tid / S) * S;
       const bool act = tid / S < per_round && unit < nq;
-      factor_pass<P, KIND, true>(acc, q1[act ? unit : 0], role, act, sub, S, nq, prm, [&](int j) { return q2[j]; });
+      VecSum<P, CPLX> vs;
+      vs.zero();
+      if (act) {
+        const PtK F = q1[unit];
+#pragma unroll 1
+        for (int j = sub; j < nq; j += S) {
+          const PtK b = q2[j];
+          double kr, ki;
+          kernel_ww<KIND>(F, b, prm, kr, ki);
+          vs.add(kr, ki, b.s0, b.s1);
+        }
+        if (sub == 0) {
+          double X[I], Y[I];
+          bernstein_basis<P>(F.s0, X);
+          bernstein_basis<P>(F.s1, Y);
+#pragma unroll
+          for (int iy = 0; iy < I; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < I; ++ix) bas[(tid / S) * I2 + iy * I + ix] = X[ix] * Y[iy];
+        }
+      }
+      double* r = rec + tid * VR;
+#pragma unroll
+      for (int k = 0; k < I2; ++k) {
+        r[k] = vs.r[k];
+        if constexpr (CPLX) r[I2 + k] = vs.i[k];
+      }
+      __syncthreads();
+      if (tid < NE) {
+        const int nu = nq - u0 < per_round ? nq - u0 : per_round;
+        const int o = epart * I2 + ecol;
+#pragma unroll 1
+        for (int m = 0; m < nu; ++m) {
+          double v = 0.0;
+          for (int s2 = 0; s2 < S; ++s2) v += rec[(m * S + s2) * VR + o];
+          aval += bas[m * I2 + erow] * v;
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   } else if constexpr (CLS == 1) {
     // integrate1: tensor Gauss rule; surface points of both elements staged in LDS
     PtK* q1 = reinterpret_cast<PtK*>(lds);
@@ -1084,7 +1129,7 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
   // reduce the per-lane partial blocks and write the result
   double* red = lds;
   double* sum = lds + (NT / 64) * Acc::TOT;
-  if constexpr (CLS == 2 && (KIND == BMI_LAPLACE_SL || KIND == BMI_HELMHOLTZ_SL)) {
+  if constexpr ((CLS == 2 && (KIND == BMI_LAPLACE_SL || KIND == BMI_HELMHOLTZ_SL)) || (CLS == 1 && !Acc::MAXW)) {
     if (tid < Acc::TOT) sum[tid] = aval;  // each entry summed by its owner
     __syncthreads();
   } else {
@@ -1159,6 +1204,10 @@ static size_t class_lds_bytes(int deg) {
   const int n = deg + 1, nq = n * n;
   size_t lds = sizeof(double) * ((NT / 64) + 1) * BlockAcc<P, KIND>::TOT;
   if (CLS == 1) lds = std::max(lds, 2 * nq * sizeof(Pt<KIND>));
+  if (CLS == 1 && KIND != BMI_MAXWELL_SL)  // points, the lanes' vectors, the outer basis values
+    lds = std::max(lds, 2 * nq * sizeof(Pt<KIND>) + static_cast<size_t>(NT) *
+                                                        ((BlockAcc<P, KIND>::CPLX ? 3 : 2) * (P + 1) * (P + 1)) *
+                                                        sizeof(double));
   constexpr bool FACT = KIND != BMI_MAXWELL_SL;  // factorised vertex rule
   constexpr int PS = static_cast<int>(sizeof(Pt<KIND>));
   const int cv = FACT ? vertex_shape(n, NT, PS, BlockAcc<P, KIND>::CPLX).chunk : vertex_chunk<Pt<KIND>>(n, NT);
