@@ -70,6 +70,31 @@ HostTimedScope::~HostTimedScope() {
   e.second += 1;
 }
 
+// element-pair sub-blocks from their mirrors: component (r, c) of the I2 x I2 sub-block at
+// map[2m] = (component (c, r) of the sub-block at map[2m + 1])^T; sub-blocks are column-major with
+// leading dimension dd inside their leaf, components dblk apart, S doubles per entry
+__global__ void k_mirror_blocks(int64_t nm, const int64_t* __restrict__ map, int I2, int dd, int64_t dblk, int vd,
+                                int S, double* dense) {
+  const int64_t per = static_cast<int64_t>(vd) * vd * I2 * I2;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= nm * per) return;
+  const int64_t m = gid / per;
+  const int q = static_cast<int>(gid - m * per);
+  const int comp = q / (I2 * I2), e = q - comp * (I2 * I2);
+  const int r = comp / vd, c = comp - r * vd, j = e / I2, i = e - j * I2;  // entry (i, j) of component (r, c)
+  const int64_t dst = map[2 * m] + comp * dblk + static_cast<int64_t>(j) * dd + i;
+  const int64_t src = map[2 * m + 1] + (c * vd + r) * dblk + static_cast<int64_t>(i) * dd + j;
+  for (int t = 0; t < S; ++t) dense[S * dst + t] = dense[S * src + t];
+}
+void launch_mirror_blocks(int64_t nm, const int64_t* map, int I2, int dd, int64_t dblk, int vd, int S, double* dense,
+                          hipStream_t st) {
+  const int64_t n = nm * vd * vd * I2 * I2;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mirror_blocks, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, nm, map, I2, dd,
+                     dblk, vd, S, dense);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
 __global__ void k_fill(double* p, int64_t n, double v) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -760,26 +785,67 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   const int ne = 1 << (2 * h->mcl);
   const int64_t dblk = static_cast<int64_t>(h->dd) * h->dd;
   h->d_dense.alloc(static_cast<size_t>(S) * vd * vd * dblk * h->dense.size());
-  // every element pair of every dense leaf: class, Duffy rotations and quadrature degree, in
-  // parallel over ranges of leaves (each pair has a fixed slot; the host decisions are pure
-  // functions of the space)
-  std::vector<NFPair> pairs(h->dense.size() * ne * ne);
+  // The single-layer bilinear forms are symmetric (Laplace, Helmholtz: a(u, v) = a(v, u) with the
+  // unconjugated product; Maxwell: component block (r, c) of an element pair (e1, e2) is the
+  // transpose of block (c, r) of (e2, e1)), and so is the block tree.  Every element pair (e1, e2)
+  // whose mirror (e2, e1) is integrated -- all pairs of a leaf (c1, c2) with c1 > c2 whose partner
+  // (c2, c1) exists, and the pairs below the diagonal of a leaf (c, c) -- is copied as a transpose
+  // after the quadrature instead of integrated again (equal to integrating it to rounding: the
+  // same quadrature points, summed in another order).
+  const bool sym_kernel = op.kind != BMI_DUMMY;  // the test integrands are not symmetric
+  std::vector<int> quad;       // leaves with integrated pairs
+  std::vector<int> partner(h->dense.size(), -1);
+  for (int k = 0; k < static_cast<int>(h->dense.size()); ++k) {
+    const Leaf& lf = h->dense[k];
+    if (lf.c1 > lf.c2 && sym_kernel) {
+      const auto it = std::lower_bound(h->dense.begin(), h->dense.end(), lf, [&](const Leaf& x, const Leaf&) {
+        return x.c1 != lf.c2 ? x.c1 < lf.c2 : x.c2 < lf.c1;
+      });
+      if (it != h->dense.end() && it->c1 == lf.c2 && it->c2 == lf.c1) partner[k] = static_cast<int>(it - h->dense.begin());
+    }
+    if (partner[k] < 0) quad.push_back(k);
+  }
+  auto diag = [&](int k) { return sym_kernel && h->dense[k].c1 == h->dense[k].c2; };
+  // sub-block base of element pair (a, b) (row element a, column element b) of leaf k
+  auto sub_off = [&](int64_t k, int a, int b) {
+    return k * vd * vd * dblk + static_cast<int64_t>(b * s->I2) * h->dd + a * s->I2;
+  };
+  std::vector<int64_t> pstart(quad.size() + 1, 0);
+  for (size_t q = 0; q < quad.size(); ++q) pstart[q + 1] = pstart[q] + (diag(quad[q]) ? ne * (ne + 1) / 2 : ne * ne);
+  // the transposed copies: (destination, source) sub-block bases
+  std::vector<int64_t> mirror;
+  for (int k = 0; k < static_cast<int>(h->dense.size()); ++k)
+    for (int b = 0; b < ne; ++b)
+      for (int a = 0; a < ne; ++a) {
+        if (partner[k] >= 0)
+          mirror.push_back(sub_off(k, a, b)), mirror.push_back(sub_off(partner[k], b, a));
+        else if (diag(k) && a > b)
+          mirror.push_back(sub_off(k, a, b)), mirror.push_back(sub_off(k, b, a));
+      }
+  // every integrated element pair: class, Duffy rotations and quadrature degree, in parallel
+  // over ranges of leaves (each pair has a fixed slot; the host decisions are pure functions of
+  // the space)
+  std::vector<NFPair> pairs(pstart.back());
   const int ff = farfield_degree(op.kind, s->P);
   {
     HostTimedScope hs("assembly_host_pairs");
-    const int64_t nd = static_cast<int64_t>(h->dense.size());
+    const int64_t nd = static_cast<int64_t>(quad.size());
     const int nth = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(
         std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())), nd / 256)));
     std::vector<std::array<int, 5>> counts(nth, std::array<int, 5>{0, 0, 0, 0, 0});
     std::vector<std::string> errors(nth);
     auto work = [&](int t) {
       try {
-        for (int64_t k = nd * t / nth; k < nd * (t + 1) / nth; ++k) {
+        for (int64_t kq = nd * t / nth; kq < nd * (t + 1) / nth; ++kq) {
+          const int64_t k = quad[kq];
           const Leaf& lf = h->dense[k];
           BMI_REQUIRE(lf.level == h->L, BMI_UNSUPPORTED, "dense leaf above the leaf level");
+          const bool dg = diag(static_cast<int>(k));
+          int64_t slot = pstart[kq];
           for (int b = 0; b < ne; ++b)
             for (int a = 0; a < ne; ++a) {
-              NFPair& p = pairs[(k * ne + b) * ne + a];
+              if (dg && a > b) continue;
+              NFPair& p = pairs[slot++];
               p.e1 = lf.c1 * ne + a;
               p.e2 = lf.c2 * ne + b;
               int cp[3];
@@ -792,7 +858,7 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
               p.rot1 = cp[0];
               p.rot2 = cp[1];
               p.deg = nf;
-              p.off = k * vd * vd * dblk + static_cast<int64_t>(b * s->I2) * h->dd + a * s->I2;
+              p.off = sub_off(k, a, b);
               counts[t][p.cls]++;
             }
         }
@@ -812,6 +878,13 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   h->npairs = static_cast<int64_t>(pairs.size());
   // component block (rc, cc) of leaf k starts at (k*vd*vd + rc*vd + cc) * dblk
   launch_nearfield(s, op, pairs, h->d_dense.ptr, 1, h->dd, vd * dblk, dblk, st);
+  if (!mirror.empty()) {
+    DevBuf<int64_t> d_mirror;
+    d_mirror.upload(mirror, st);
+    launch_mirror_blocks(static_cast<int64_t>(mirror.size() / 2), d_mirror.ptr, s->I2, h->dd, dblk, vd, S,
+                         h->d_dense.ptr, st);
+    BMI_CHECK_HIP(hipStreamSynchronize(st));
+  }
   // dense CSR over leaf clusters (leaves sorted by (c1, c2))
   h->ncl_leaf = ncl_at(s, h->L);
   std::vector<int> drp(h->ncl_leaf + 1, 0), dsrc(h->dense.size());
