@@ -569,10 +569,10 @@ class H2Matrix:
         _check(library().bmi_h2_use_graphs(self._h, 1 if enable else 0))
 
     def set_sweep(self, mode):
-        """Coupling sweep of single-RHS products: None (timed choice, default), "full" (every
-        coupling block) or "upper" (upper blocks of a symmetric kernel with the transposed
-        products formed from them)."""
-        m = {None: -1, "full": 0, "upper": 1}[mode]
+        """Sweep of single-RHS products: None (timed choice, default), "full" (every block),
+        "upper" (upper coupling blocks of a symmetric kernel with the transposed products formed
+        from them) or "upper_near" (the same for the near-field leaves too)."""
+        m = {None: -1, "full": 0, "upper": 1, "upper_near": 2}[mode]
         _check(library().bmi_h2_set_sweep(self._h, m))
 
     def get_dense(self):

@@ -165,6 +165,13 @@ struct bmi_h2 {
   int sym_fused = 0;  // chosen by the autotuner: FP64 single-RHS sweeps read only the upper blocks
   int sweep_mode = -1;  // -1: timed choice, 0: every coupling block, 1: upper blocks
   DevBuf<int2> d_stgt, d_sgtgt;
+  // the same for the near field (upper and diagonal leaves, partials added into the rows)
+  bool nsym = false;
+  bmi::SymList nsl{};
+  DevBuf<int2> d_ntgt, d_ngtgt;
+  DevBuf<int> d_nrp, d_nsrc, d_nblk, d_nslot, d_ngrp;
+  DevBuf<double> d_npart;
+  DevBuf<int64_t> d_zero;
   DevBuf<int> d_srp, d_ssrc, d_sblk, d_sgrp, d_sgslot;
   DevBuf<double> d_part;
   bmi::SymList sl{};
@@ -211,7 +218,8 @@ struct bmi_h2 {
            d_crp.bytes() + d_csrc.bytes() + d_T.bytes() + d_mom[0].bytes() + d_mom[1].bytes() + d_Tt.bytes() + d_momt[0].bytes() + d_momt[1].bytes() + d_stgt.bytes() + d_sgtgt.bytes() + d_srp.bytes() +
            d_ssrc.bytes() + d_sblk.bytes() + d_sgrp.bytes() + d_sgslot.bytes() + d_part.bytes() + d_xs.bytes() +
            d_ys.bytes() + d_fw[0].bytes() + d_fw[1].bytes() + d_bw.bytes() + d_yf.bytes() + d_mx_tan.bytes() +
-           d_mxz.bytes() + d_mx_order.bytes();
+           d_mxz.bytes() + d_mx_order.bytes() + d_ntgt.bytes() + d_ngtgt.bytes() + d_nrp.bytes() + d_nsrc.bytes() +
+           d_nblk.bytes() + d_nslot.bytes() + d_ngrp.bytes() + d_npart.bytes() + d_zero.bytes();
   }
 };
 
@@ -378,8 +386,11 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
       // full block lists, and (symmetric kernels, FP64 blocks) the upper blocks with transposed
       // partials: the upper form reads half the coupling bytes but adds the column sums
       int pl = -1, pc = -1, ps = 0;
-      const int nsym = (h->sym && !fp32) ? 2 : 1;
-      for (int sy = h->sweep_mode == 1 && nsym == 2 ? 1 : 0; sy < (h->sweep_mode == 0 ? 1 : nsym); ++sy)
+      // forms: 0 every block, 1 upper coupling blocks, 2 upper coupling blocks and upper near-field leaves
+      const int nsym = (h->sym && !fp32) ? (h->nsym ? 3 : 2) : 1;
+      const int s0 = h->sweep_mode >= 0 ? std::min(h->sweep_mode, nsym - 1) : 0;
+      const int s1 = h->sweep_mode >= 0 ? s0 : nsym - 1;
+      for (int sy = s0; sy <= s1; ++sy)
         for (int vl = 0; vl < 2; ++vl)
           for (int vc = 0; vc < 2; ++vc) {
             bool ok = true;
@@ -387,7 +398,8 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
               ok = launch_sweep_fused(h->cplx, fp32, vl, vc, h->ncl_leaf, h->dd, h->d_drp.ptr, h->d_dsrc.ptr,
                                       h->d_dense.ptr, xs, h->d_ys.ptr, h->ntgt, h->cd, h->d_tgt.ptr, h->d_crp.ptr,
                                       h->d_csrc.ptr, coupF, h->d_lvl_off.ptr, h->d_fw[0].ptr, h->d_bw.ptr, st, co.Fc,
-                                      co.ncoarse, false, co.ld, sy ? &h->sl : nullptr);
+                                      co.ncoarse, false, co.ld, sy ? &h->sl : nullptr,
+                                      sy == 2 ? &h->nsl : nullptr);
             });
             if (ok && (pl < 0 || t < best)) best = t, pl = vl, pc = vc, ps = sy;
           }
@@ -403,7 +415,7 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
       // whichever is faster (arithmetic vs bytes)
       // (variants 6-9: the FP32 copy's upper blocks with transposed partials)
       int pv = -1;
-      const int v0 = (h->sym && h->sweep_mode == 1) ? 6 : (h->fly ? 0 : 2);
+      const int v0 = (h->sym && h->sweep_mode >= 1) ? 6 : (h->fly ? 0 : 2);
       const int v1 = (h->sym && h->sweep_mode != 0) ? 10 : 6;
       for (int v = v0; v < v1; ++v) {
         bool ok = true;
@@ -478,7 +490,8 @@ static void apply_super(bmi_h2* h, const Work& w, int nrhs, const double* xs, do
     if (launch_sweep_fused(h->cplx, co.fp32, h->fused_leaf_variant, h->fused_cpl_variant, h->ncl_leaf, h->dd,
                            h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr,
                            h->d_crp.ptr, h->d_csrc.ptr, co.F, h->d_lvl_off.ptr, w.fw[0], w.bw, st, co.Fc,
-                           co.ncoarse, false, co.ld, use_sym ? &h->sl : nullptr)) {
+                           co.ncoarse, false, co.ld, use_sym ? &h->sl : nullptr,
+                           use_sym && h->sym_fused == 2 ? &h->nsl : nullptr)) {
       TimedScope ts("matvec_backward", st);
       launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, w.bw,
                       h->d_lvl_off.ptr, yf, N, 0, st);
@@ -761,6 +774,53 @@ static void build_symmetric_lists(bmi_h2* h, const std::vector<int2>& tgt, const
   BMI_CHECK_HIP(hipStreamSynchronize(st));
 }
 
+// Near-field leaves of a symmetric kernel pair up as well: leaf (c2, c1) is the transpose of
+// (c1, c2) (the assembly copies it so).  For single-RHS sweeps list, per row cluster, its
+// diagonal leaf and the leaves (c, c2) with c2 > c; each of the latter also gives the transposed
+// product for the rows of c2, gathered per column cluster in list order.  Leaves the handle
+// unchanged when a leaf has no partner.
+static void build_near_symmetric_lists(bmi_h2* h, const std::vector<int>& drp, const std::vector<int>& dsrc,
+                                       hipStream_t st) {
+  const int ncl = h->ncl_leaf;
+  std::vector<int2> tg(ncl), col;  // col: (column cluster, entry)
+  std::vector<int> rp(1, 0), src, blk, slot;
+  for (int c = 0; c < ncl; ++c) {
+    tg[c] = make_int2(0, c);
+    for (int p = drp[c]; p < drp[c + 1]; ++p) {
+      const int c2 = dsrc[p];
+      if (c2 < c) continue;
+      if (c2 > c && !std::binary_search(dsrc.begin() + drp[c2], dsrc.begin() + drp[c2 + 1], c)) return;
+      if (c2 > c) col.push_back(make_int2(c2, static_cast<int>(src.size())));
+      src.push_back(c2);
+      blk.push_back(p);
+      slot.push_back(-1);
+    }
+    rp.push_back(static_cast<int>(src.size()));
+  }
+  std::stable_sort(col.begin(), col.end(), [](const int2& a, const int2& b) { return a.x < b.x; });
+  std::vector<int2> gtgt;
+  std::vector<int> grp(1, 0);
+  for (size_t q = 0; q < col.size(); ++q) {
+    if (gtgt.empty() || gtgt.back().y != col[q].x) gtgt.push_back(make_int2(0, col[q].x)), grp.push_back(grp.back());
+    grp.back()++;
+    slot[col[q].y] = static_cast<int>(q);
+  }
+  const int S = h->cplx ? 2 : 1;
+  h->d_ntgt.upload(tg, st);
+  h->d_nrp.upload(rp, st);
+  h->d_nsrc.upload(src, st);
+  h->d_nblk.upload(blk, st);
+  h->d_nslot.upload(slot, st);
+  h->d_ngtgt.upload(gtgt, st);
+  h->d_ngrp.upload(grp, st);
+  h->d_zero.upload(std::vector<int64_t>(1, 0), st);
+  h->d_npart.alloc(static_cast<size_t>(S) * std::max<size_t>(col.size(), 1) * h->dd);
+  h->nsl = SymList{ncl, h->d_ntgt.ptr, h->d_nrp.ptr, h->d_nsrc.ptr, h->d_nblk.ptr, h->d_npart.ptr, h->d_nslot.ptr,
+                   static_cast<int>(gtgt.size()), h->d_ngtgt.ptr, h->d_ngrp.ptr, h->d_zero.ptr};
+  h->nsym = true;
+  BMI_CHECK_HIP(hipStreamSynchronize(st));
+}
+
 static void assemble_h2(bmi_h2* h, hipStream_t st) {
   const bmi_space* s = h->s;
   const bmi_op_desc& op = h->op;
@@ -892,6 +952,8 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
   for (int c = 0; c < h->ncl_leaf; ++c) drp[c + 1] += drp[c];
   h->d_drp.upload(drp, st);
   h->d_dsrc.upload(dsrc, st);
+  if (vd == 1 && (op.kind == BMI_LAPLACE_SL || op.kind == BMI_HELMHOLTZ_SL) && h->dd == 36)
+    build_near_symmetric_lists(h, drp, dsrc, st);
   // ---- FMM constants
   std::vector<double> T, mom;
   transfer_matrix(h->np, T);
@@ -1658,8 +1720,9 @@ bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable) {
 bmi_status bmi_h2_set_sweep(bmi_h2* h, int mode) {
   BMI_API_BEGIN
   BMI_REQUIRE(h, BMI_INVALID_ARG, "null handle");
-  BMI_REQUIRE(mode >= -1 && mode <= 1, BMI_INVALID_ARG, "sweep mode must be -1, 0 or 1");
-  BMI_REQUIRE(mode != 1 || h->sym, BMI_UNSUPPORTED, "no upper-block sweep for this operator or block size");
+  BMI_REQUIRE(mode >= -1 && mode <= 2, BMI_INVALID_ARG, "sweep mode must be -1, 0, 1 or 2");
+  BMI_REQUIRE(mode < 1 || h->sym, BMI_UNSUPPORTED, "no upper-block sweep for this operator or block size");
+  BMI_REQUIRE(mode != 2 || h->nsym, BMI_UNSUPPORTED, "no upper near-field sweep for this operator or block size");
   h->sweep_mode = mode;
   // choose again at the next product
   h->leaf_variant = h->cpl_variant = -1;

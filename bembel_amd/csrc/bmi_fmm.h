@@ -91,6 +91,9 @@ struct SymList {
   int ngather;       // targets receiving partials and the ranges of their slots
   const int2* gtgt;
   const int* grp;
+  // near-field lists (row clusters of the leaf level as targets of level 0; the diagonal leaf of a
+  // cluster has slot -1): their level offsets are one zero
+  const int64_t* lvl_off = nullptr;
 };
 
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
@@ -98,7 +101,7 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
                         const double* fw, double* bw, hipStream_t st, const unsigned short* Fc = nullptr,
                         int ncoarse = 0, bool near32 = false, int ld = 0,
-                        const SymList* sym = nullptr);
+                        const SymList* sym = nullptr, const SymList* nsym = nullptr);
 // FP32 node tables (offsets from each cluster's node 0, weights; node 0 in FP64) of one level
 void launch_fly_points(const double* pts, int ncl, int np2, float4* out, double4* org, hipStream_t st);
 // single-RHS sweep of the reduced operator: FP32 near-field blocks, coupling evaluated on the fly

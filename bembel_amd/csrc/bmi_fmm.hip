@@ -1320,8 +1320,9 @@ __device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
               return j < CD ? static_cast<double>(fr[u][jj]) * ti + static_cast<double>(fi[u][jj]) * tr : 0.0;
             },
             ci);
-      if (ok && act) {
-        double* pp = a.part + S * (static_cast<int64_t>(a.slot[pu]) * CD + li);
+      const int sl = a.slot[pu];
+      if (ok && act && sl >= 0) {  // a diagonal near-field leaf (slot -1) has no transposed product
+        double* pp = a.part + S * (static_cast<int64_t>(sl) * CD + li);
         pp[0] = cr[0];
         if constexpr (C) pp[1] = ci[0];
       }
@@ -1573,7 +1574,7 @@ __global__ __launch_bounds__(256) void k_sym_gather(int ng, const int2* __restri
 // the two sweeps of a single-RHS product in one launch: workgroups [0, nb_near) run the near
 // field, the rest the coupling blocks, so both streams of leaves share the memory system
 template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT = double, bool SYM = false>
-__global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, int nb_near) {
+__global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, int nb_near, CplArgs na) {
   const int b = static_cast<int>(blockIdx.x);
   const int w = static_cast<int>(threadIdx.x >> 6);
   // real FP64 blocks: two rows per lane (16-byte loads)
@@ -1587,6 +1588,13 @@ __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, in
   if (b < nb_near) {
     const int c = __builtin_amdgcn_readfirstlane(b * 4 + w);
     if (c < la.ncl) {
+      if constexpr (SYM && std::is_same<NFT, double>::value) {
+        // upper and diagonal near-field leaves with transposed partials (when listed)
+        if (na.rp) {
+          coupling_wave_sym<C, DD, UBN>(c, na);
+          return;
+        }
+      }
       if constexpr (R2N)
         leaf_wave_v<C, NFT, DD, UBN>(c, la);
       else
@@ -2640,9 +2648,17 @@ void launch_leaf(bool cplx, int variant, int ncl, int nrhs, int dd, const int* r
 
 static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd,
                                    const LeafArgs& la, int cd, const void* cF, int ld, const int64_t* lvl_off,
-                                   const double* fw, double* bw, bool near32, const SymList& sl, hipStream_t st) {
+                                   const double* fw, double* bw, bool near32, const SymList& sl, const SymList* nsl,
+                                   hipStream_t st) {
   CplArgs ca{sl.ntgt, 1, sl.tgt, sl.rp, sl.src, cF, static_cast<int64_t>(ld) * cd, 0, lvl_off, fw, bw, 0, nullptr,
              0, ld, sl.blk, sl.part, sl.slot};
+  // near field from its upper and diagonal leaves (FP64 blocks): rows and vectors of the super
+  // space, the row clusters as targets of one level at offset zero
+  const bool nsym = nsl && !near32 && nsl->ntgt == ncl;
+  CplArgs na{};
+  if (nsym)
+    na = CplArgs{nsl->ntgt, 1, nsl->tgt, nsl->rp, nsl->src, la.F, static_cast<int64_t>(dd) * dd, 0, nsl->lvl_off,
+                 la.xs, la.ys, 0, nullptr, 0, dd, nsl->blk, nsl->part, nsl->slot};
   const int nbn = (ncl + 3) / 4, nbc = (sl.ntgt + 3) / 4;
   const dim3 grid(static_cast<unsigned>(nbn + nbc));
   bool launched = false;
@@ -2651,7 +2667,7 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
 #define BMI_FUSED_S(CC, DDD, FTT, CDD, UN, UC, NFTT)                                                                 \
   if (!launched && cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&               \
       near32 == std::is_same<NFTT, float>::value && leaf_variant == UN - 1 && cpl_variant == UC - 1) {              \
-    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC, NFTT, true>), grid, dim3(256), 0, st, la, ca, nbn); \
+    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC, NFTT, true>), grid, dim3(256), 0, st, la, ca, nbn, na); \
     BMI_CHECK_HIP(hipGetLastError());                                                                                \
     launched = true;                                                                                                 \
   }
@@ -2666,6 +2682,19 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
 #undef BMI_FUSED_S
   }
   if (!launched) return false;
+  if (nsym && nsl->ngather > 0) {
+    TimedScope ts("matvec_sym_gather", st);
+    const dim3 gg(static_cast<unsigned>(nblk(static_cast<int64_t>(nsl->ngather) * (cplx ? 2 : 1) * dd / 2)));
+    if (cplx && dd == 36)
+      hipLaunchKernelGGL((k_sym_gather<true, 36>), gg, dim3(256), 0, st, nsl->ngather, nsl->gtgt, nsl->grp, nsl->part,
+                         nsl->lvl_off, la.ys);
+    else if (!cplx && dd == 36)
+      hipLaunchKernelGGL((k_sym_gather<false, 36>), gg, dim3(256), 0, st, nsl->ngather, nsl->gtgt, nsl->grp,
+                         nsl->part, nsl->lvl_off, la.ys);
+    else
+      throw Error(BMI_UNSUPPORTED, "symmetric near-field gather not instantiated");
+    BMI_CHECK_HIP(hipGetLastError());
+  }
   if (sl.ngather > 0) {
     TimedScope ts("matvec_sym_gather", st);
     const dim3 gg(static_cast<unsigned>(nblk(static_cast<int64_t>(sl.ngather) * (cplx ? 2 : 1) * cd / 2)));
@@ -2688,7 +2717,7 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
                         const int* dsrc, const void* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,
                         const double* fw, double* bw, hipStream_t st, const unsigned short* Fc, int ncoarse,
-                        bool near32, int ld, const SymList* sym) {
+                        bool near32, int ld, const SymList* sym, const SymList* nsym) {
   const LeafArgs la{ncl, 1, drp, dsrc, dF, static_cast<int64_t>(dd) * dd, 0, xs, 0, 0, ys, 0, 0, 0};
   if (ld <= 0) ld = cd;
   if (sym) {
@@ -2696,7 +2725,7 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
     const int rpl = fp32 ? (cplx ? 2 : 4) : 1;
     if (Fc || (fp32 && ld != (cd + rpl - 1) / rpl * rpl)) return false;
     return launch_sweep_fused_sym(cplx, fp32, leaf_variant, cpl_variant, ncl, dd, la, cd, cF, ld, lvl_off, fw, bw,
-                                  near32, *sym, st);
+                                  near32, *sym, nsym, st);
   }
   const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(ld) * cd, 0, lvl_off, fw, bw, 0, Fc, ncoarse, ld};
   const int nbn = (ncl + 3) / 4, nbc = (ntgt + 3) / 4;
@@ -2705,7 +2734,8 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
 #define BMI_FUSED(CC, DDD, FTT, CDD, UN, UC, NFTT)                                                              \
   if (cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&                        \
       near32 == std::is_same<NFTT, float>::value && leaf_variant == UN - 1 && cpl_variant == UC - 1) {          \
-    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC, NFTT>), grid, dim3(256), 0, st, la, ca, nbn);  \
+    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC, NFTT>), grid, dim3(256), 0, st, la, ca, nbn,    \
+                       CplArgs{});                                                                                \
     BMI_CHECK_HIP(hipGetLastError());                                                                           \
     return true;                                                                                                \
   }

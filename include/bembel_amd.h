@@ -179,11 +179,11 @@ bmi_status bmi_h2_matvec(const bmi_h2* h, int nrhs, const double* X, int64_t ldx
    than its few launches save).  No counterpart in the reference (its product is CPU code);
    exposed so callers can compare both modes. */
 bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable);
-/* Coupling sweep of single-RHS products: -1 (default) lets the first product time both forms
-   and keep the faster, 0 reads every coupling block, 1 reads only the upper blocks of a
-   symmetric kernel (Laplace, Helmholtz) and forms the transposed products from them
-   (BMI_UNSUPPORTED where that form is not available).  No counterpart in the reference (its
-   product is CPU code); exposed so callers can compare both forms. */
+/* Sweep of single-RHS products: -1 (default) lets the first product time the forms and keep
+   the fastest, 0 reads every block, 1 reads only the upper coupling blocks of a symmetric kernel
+   (Laplace, Helmholtz) and forms the transposed products from them, 2 does the same for the
+   near-field leaves as well (BMI_UNSUPPORTED where a form is not available).  No counterpart in
+   the reference (its product is CPU code); exposed so callers can compare the forms. */
 bmi_status bmi_h2_set_sweep(bmi_h2* h, int mode);
 /* wait for the library's work on `stream` (NULL: the default stream) */
 bmi_status bmi_synchronize(void* stream);

@@ -72,9 +72,10 @@ def test_gpu_h2_matvec_matches_oracle(gpu, kind, kappa, M, P, np_):
 
 @pytest.mark.parametrize("kind,kappa,M,P,np_", [CASES[0], CASES[1]])
 def test_gpu_h2_upper_block_sweep(gpu, kind, kappa, M, P, np_):
-    """Single-RHS products that read only the upper coupling blocks (the transposed products
-    formed from them, partials gathered per target) equal the oracle like the sweep over every
-    block, for the FP64 and the mixed operator; each form is forced in turn."""
+    """Single-RHS products that read only the upper coupling blocks, and also only the upper
+    near-field leaves (the transposed products formed from them, partials gathered per target),
+    equal the oracle like the sweep over every block, for the FP64 and the mixed operator; each
+    form is forced in turn."""
     os_, ps, op, H = make(kind, kappa, M, P, np_)
     oh = O.H2(os_, kind, kappa, np_)
     rng = np.random.default_rng(11)
@@ -84,12 +85,13 @@ def test_gpu_h2_upper_block_sweep(gpu, kind, kappa, M, P, np_):
         x = x + 1j * rng.uniform(-1, 1, (n, 1))
     yo = oh.matvec(x)
     ys = {}
-    for mode in ("full", "upper"):
+    for mode in ("full", "upper", "upper_near"):
         H.set_sweep(mode)
         ys[mode] = H @ x
         print(mode, "rel", rel(ys[mode], yo))
         assert rel(ys[mode], yo) <= TOL
     assert rel(ys["upper"], ys["full"]) <= 1e-14
+    assert rel(ys["upper_near"], ys["full"]) <= 1e-14
     H.set_sweep(None)
     assert rel(H @ x, yo) <= TOL
     Hm = B.H2Matrix(ps, op, n_cheb=np_, precision=B.MIXED)
