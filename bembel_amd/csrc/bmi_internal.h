@@ -3,10 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bembel_amd.h"
@@ -42,6 +44,23 @@ void patch_eval(const Patch& p, double s, double t, double out[3]);
 void patch_surface_point(const Patch& p, double s, double t, double out[12]);
 
 // ------------------------------------------------------------------ device buffer
+// BMI_POISON_ALLOC=1: every floating-point buffer of the library starts filled with 0xFF bytes (a
+// NaN in every double and float), so a kernel that reads values it has not written gives NaN on
+// every run instead of only when the allocator hands back memory that held such bit patterns.
+// Index buffers are left alone (a poisoned index would turn the same mistake into a fault).
+inline bool poison_allocations() {
+  static const bool on = [] {
+    const char* v = std::getenv("BMI_POISON_ALLOC");
+    return v != nullptr && v[0] != '\0' && v[0] != '0';
+  }();
+  return on;
+}
+inline void poison_fill(void* p, size_t bytes) {
+  if (!poison_allocations() || bytes == 0) return;
+  BMI_CHECK_HIP(hipMemsetAsync(p, 0xFF, bytes, nullptr));
+  BMI_CHECK_HIP(hipStreamSynchronize(nullptr));
+}
+
 template <typename T>
 struct DevBuf {
   T* ptr = nullptr;
@@ -60,6 +79,7 @@ struct DevBuf {
     if (count == 0) return;
     BMI_CHECK_HIP(hipMalloc(&ptr, count * sizeof(T)));
     n = count;
+    if constexpr (std::is_floating_point<T>::value) poison_fill(ptr, count * sizeof(T));
   }
   void upload(const std::vector<T>& v, hipStream_t st) {
     alloc(v.size());

@@ -361,6 +361,12 @@ int bmi_server::step(Job& j, const std::vector<std::shared_ptr<Job>>& live) {
     if (j.st == BMI_OK && (hipMalloc(&j.dB, sizeof(double) * j.S * n * j.rhs) != hipSuccess ||
                            hipMalloc(&j.dX, sizeof(double) * j.S * n * j.rhs) != hipSuccess))
       fail(BMI_OUT_OF_MEMORY, "workspace allocation failed");
+    if (j.st == BMI_OK && bmi::poison_allocations()) {
+      const size_t bytes = sizeof(double) * j.S * n * j.rhs;
+      if (hipMemsetAsync(j.dB, 0xFF, bytes, nullptr) != hipSuccess ||
+          hipMemsetAsync(j.dX, 0xFF, bytes, nullptr) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess)
+        fail(BMI_DEVICE_ERROR, "poison fill failed");
+    }
     int col = 0;
     for (auto& q : j.batch) {
       if (j.st != BMI_OK) break;

@@ -153,13 +153,17 @@ __global__ void k_axpby(const double* __restrict__ a, const double* __restrict__
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= n * nrhs) return;
   const int c = static_cast<int>(gid / n);
+  // b = 0 never reads Y (a new Krylov basis vector lands in memory nothing has written yet)
   if (C) {
     const double ar = a[2 * c], ai = a[2 * c + 1], br = b[2 * c], bi = b[2 * c + 1];
-    const double xr = X[2 * gid], xi = X[2 * gid + 1], yr = Y[2 * gid], yi = Y[2 * gid + 1];
+    const double xr = X[2 * gid], xi = X[2 * gid + 1];
+    const bool rd = br != 0.0 || bi != 0.0;
+    const double yr = rd ? Y[2 * gid] : 0.0, yi = rd ? Y[2 * gid + 1] : 0.0;
     Y[2 * gid] = ar * xr - ai * xi + br * yr - bi * yi;
     Y[2 * gid + 1] = ar * xi + ai * xr + br * yi + bi * yr;
   } else {
-    Y[gid] = a[c] * X[gid] + b[c] * Y[gid];
+    const double bc = b[c];
+    Y[gid] = a[c] * X[gid] + (bc != 0.0 ? bc * Y[gid] : 0.0);
   }
 }
 
@@ -504,6 +508,9 @@ class GmresRun {
     BMI_CHECK_HIP(hipMemcpyAsync(iters.data(), d_iters_.ptr, nrhs_ * sizeof(int), hipMemcpyDeviceToHost, st_));
     BMI_CHECK_HIP(hipMemcpyAsync(resid.data(), d_resid_.ptr, nrhs_ * sizeof(double), hipMemcpyDeviceToHost, st_));
     BMI_CHECK_HIP(hipStreamSynchronize(st_));
+    // a NaN or infinite residual (the device flags treat it as inactive) never counts as converged
+    for (const double r : resid)
+      if (!std::isfinite(r)) converged_ = false;
   }
 
  private:
@@ -761,10 +768,15 @@ static bool refine(bmi_h2* h, int nrhs, const double* B, double* X, double tol, 
     h2_apply_device(h, nrhs, X, n, R.ptr, n, mone, one, st, false);
     vs.dots(R.ptr, 1, R.ptr, d_small.ptr, buf);
     all = true;
+    bool finite = true;  // a NaN or infinite residual ends the refinement unconverged
     double need = 1.0, ratio = 0.0;
     for (int c = 0; c < nrhs; ++c) {
       const double rn = std::sqrt(std::max(buf[S * c], 0.0));
       resid[c] = rn / bn[c];
+      if (!std::isfinite(rn)) {
+        all = finite = false;
+        continue;
+      }
       if (outer > 0 && prev[c] > 0) ratio = std::max(ratio, rn / prev[c]);
       prev[c] = rn;
       if (resid[c] > tol) {
@@ -772,7 +784,7 @@ static bool refine(bmi_h2* h, int nrhs, const double* B, double* X, double tol, 
         need = std::min(need, tol * bn[c] / rn);
       }
     }
-    if (all || used >= maxit || outer >= 30) break;
+    if (all || !finite || used >= maxit || outer >= 30) break;
     if (outer > 0) tin = ratio > 3 * last ? std::min(std::max(ratio, 1e-6), 1e-2) : std::max(last * 0.1, 1e-6);
     // inner solve A_reduced D = R to the accuracy this step needs
     const double tstep = std::max(need, tin);

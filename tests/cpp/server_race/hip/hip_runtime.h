@@ -16,4 +16,5 @@ hipError_t hipMalloc(void** p, size_t n);
 template <class T> hipError_t hipMalloc(T** p, size_t n) { return hipMalloc(reinterpret_cast<void**>(p), n); }
 hipError_t hipFree(void* p);
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t);
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t);
 const char* hipGetErrorString(hipError_t);

@@ -26,6 +26,10 @@ hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipSt
   std::memcpy(d, s, n);
   return 0;
 }
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
+  std::memset(d, v, n);
+  return 0;
+}
 const char* hipGetErrorString(hipError_t) { return "stub"; }
 
 namespace bmi {
