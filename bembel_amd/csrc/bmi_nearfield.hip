@@ -806,7 +806,6 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
     }
     __syncthreads();
   } else if constexpr (CLS == 2) {
-    // integrate2: identical elements, 8 Duffy sub-simplices, symmetrised  } else if constexpr (CLS == 2) {
     // integrate2: identical elements, 8 Duffy sub-simplices, symmetrised
     const int total = nq * nq;
     for (int base = 0; base < total; base += NT) {

@@ -17,6 +17,7 @@ after all GPU legs, with OpenMP threads bound one per physical core.
 usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
 import argparse
+import gc
 import json
 import math
 import os
@@ -720,6 +721,9 @@ def main():
     import bembel_amd as B
     torch.cuda.set_device(0)
     torch.cuda.init()
+    # no cyclic garbage collection inside timed regions (as timeit does)
+    gc.collect()
+    gc.disable()
     if a.helmholtz_profile:
         run_helmholtz_profile(B, torch)
         return
