@@ -1880,61 +1880,126 @@ __global__ __launch_bounds__(256) void k_multi_lds(MultiArgs a) {
 
 // ------------------------------------------------------------------ block rows on the matrix cores (many RHS)
 // Same product as k_multi_lds, on FP64 MFMA (v_mfma_f64_16x16x4f64): y_owner (ROUT x 64) = sum over
-// the owner's blocks of F_p (ROUT x RIN) x_src(p) (RIN x 64). The block (widened to FP64) and its
-// x tile are staged in LDS, zero-padded to MT*16 rows and KS*4 columns; wave w computes the
-// 16 right-hand sides [16w, 16w + 16) for all MT row tiles. Operand layout of the instruction:
-// A(i, k) on lane i + 16k, B(k, j) on lane j + 16k, D(i, j) on lane j + 16 (i % 4), element i / 4.
-// Complex products are four real ones (Re += Fr Xr - Fi Xi, Im += Fr Xi + Fi Xr).
+// the owner's blocks of F_p (ROUT x RIN) x_src(p) (RIN x 64); wave w computes the 16 right-hand
+// sides [16w, 16w + 16) for all row tiles. Operand layout of the instruction: A(i, k) on lane
+// i + 16k, B(k, j) on lane j + 16k, D(i, j) on lane j + 16 (i % 4), element i / 4.
+// The block (widened to FP64) is staged in LDS as planes of real and of imaginary parts, zero-padded
+// to whole 16-row tiles and 4-column steps, so that the 16 lanes of a row tile read 16 consecutive
+// doubles. Complex blocks times complex vectors are three real products (Gauss: Fr Xr, Fi Xi and
+// (Fr + Fi)(Xr + Xi)), real blocks times complex vectors two. Complex blocks with 4 rows past the
+// last full tile (36 = 2 x 16 + 4) compute those rows in one more tile, real parts in its rows 0-3
+// and imaginary parts in rows 4-7, as [Fr, -Fi; Fi, Fr] [Xr; Xi] (two products per column step).
+// Two forms:
+// * XL = false (tree steps): the x tile is staged in LDS like the block, one block at a time,
+//   one workgroup per owner;
+// * XL = true (block sweeps): x is not shared between waves (each wave multiplies its own 16
+//   columns), so every lane loads its operands itself; the blocks alternate between two LDS
+//   buffers (one barrier per block), and each workgroup runs over several owners, the next
+//   owner's first block fetched during the last block of the current one.
+// Owners go to XCDs in contiguous ranges (workgroups are handed to the 8 XCDs in turn): neighbouring
+// owners share most of their source clusters, and each XCD has its own L2.
 typedef double bmi_d4 __attribute__((ext_vector_type(4)));
 
-template <bool XC, bool FC, typename FT, int ROUT, int RIN>
-__global__ __launch_bounds__(256) void k_multi_mfma(MultiArgs a) {
+// the owner range [start, start + cnt) of XCD k for nown owners
+__device__ __forceinline__ void xcd_range(int k, int nown, int& start, int& cnt) {
+  const int q = nown >> 3, rem = nown & 7;
+  start = k * q + (k < rem ? k : rem);
+  cnt = q + (k < rem ? 1 : 0);
+}
+
+template <bool XC, bool FC, typename FT, int ROUT, int RIN, bool XL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_multi_mfma(MultiArgs a) {
   static_assert(XC || !FC, "complex blocks need complex vectors");
   constexpr int SX = XC ? 2 : 1, SF = FC ? 2 : 1;
   constexpr int MT = (ROUT + 15) / 16, KS = (RIN + 3) / 4;
-  constexpr int FLD = MT * 16;   // LDS column stride of the block
   constexpr int NP = MT == 1 ? 2 : 1;  // independent accumulator chains per tile
+  constexpr bool TAIL = FC && MT >= 2 && ROUT % 16 == 4;
+  constexpr int MTF = TAIL ? MT - 1 : MT;  // tiles of the three-product form
+  constexpr int FLD = MT * 16;             // LDS column stride of a plane
+  constexpr int PL = KS * 4 * FLD;         // doubles per plane
   constexpr int LDMAX = (ROUT + 3) / 4 * 4;
   constexpr int NFE = (LDMAX * RIN + 255) / 256;  // block elements per thread (bound)
-  constexpr int NXE = (RIN * 64 + 255) / 256;     // x tile elements per thread
-  __shared__ double Fs[SF * KS * 4 * FLD];
-  __shared__ double Xs[SX * KS * 4 * 64];
+  constexpr int NBUF = XL ? 2 : 1;
+  constexpr int XPL = XL ? 0 : KS * 4 * 64;       // doubles per x plane (staged x)
+  constexpr int NXE = (RIN * 64 + 255) / 256;     // staged x elements per thread
+  __shared__ double Fs[NBUF * SF * PL + SX * XPL];
+  double* Xs = Fs + NBUF * SF * PL;
+  // LDS positions: entry (column c, row i) of a block plane and (row j, column k) of an x plane.
+  // Odd columns (rows) swap their halves of 16 doubles where the stride is an even multiple of
+  // 16, so the two lane quarters of a half-wave (consecutive columns) fall on different banks.
+  auto fpos = [](int c, int i) { return c * FLD + (MT % 2 == 0 ? (i ^ ((c & 1) << 4)) : i); };
+  auto xpos = [](int j, int k) { return j * 64 + (k ^ ((j & 1) << 4)); };
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int r = blockIdx.x;
+  const int li = lane & 15, lk = lane >> 4;
+  const int n = w * 16 + li;  // this lane's column of the x tile
   const int k0 = static_cast<int>(blockIdx.y) * 64;
   const int kmax = a.nrhs - 1 - k0;
-  int cl = r;
-  int64_t xo = a.xoff, yo = a.yoff;
-  if (a.tgt) {
-    const int2 T = a.tgt[r];
-    cl = T.y;
-    const int64_t lo = a.lvl_off[T.x] * a.nrhs;
-    xo += lo;
-    yo += lo;
-  }
+  const int kc = n <= kmax ? n : kmax;  // columns past the last right-hand side repeat it
   const int mode = a.mode;
-  int p0, p1;
-  if (mode == 0) {
-    p0 = a.rp[r];
-    p1 = a.rp[r + 1];
-  } else if (mode == 1) {
-    p0 = 4 * r;
-    p1 = p0 + 4;
-  } else {
-    p0 = r;
-    p1 = r + 1;
+  struct Own {
+    int cl, p0, p1;
+    int64_t xo, yo;
+  };
+  auto owner = [&](int r) {
+    Own o;
+    o.cl = r;
+    o.xo = a.xoff;
+    o.yo = a.yoff;
+    if (a.tgt) {
+      const int2 T = a.tgt[r];
+      o.cl = T.y;
+      const int64_t lo = a.lvl_off[T.x] * a.nrhs;
+      o.xo += lo;
+      o.yo += lo;
+    }
+    if (mode == 0) {
+      o.p0 = a.rp[r];
+      o.p1 = a.rp[r + 1];
+    } else if (mode == 1) {
+      o.p0 = 4 * r;
+      o.p1 = o.p0 + 4;
+    } else {
+      o.p0 = r;
+      o.p1 = r + 1;
+    }
+    return o;
+  };
+  auto source = [&](int p) { return mode == 0 ? a.src[p] : (mode == 2 ? (p >> 2) : p); };
+  // zero padding: rows [ROUT, RREAD) of every column and columns [RIN, KS * 4) of the block rows
+  // (and x rows [RIN, KS * 4) when x is staged); the copies below write inside only
+  {
+    constexpr int RREAD = TAIL ? ROUT : MT * 16;
+    constexpr int PR = RREAD > ROUT ? RREAD - ROUT : 1;
+    constexpr int NZR = KS * 4 * (RREAD - ROUT), NZ = NZR + (KS * 4 - RIN) * ROUT;
+    for (int u = tid; u < NBUF * SF * NZ; u += 256) {
+      const int pl = u / NZ, v = u - pl * NZ;
+      int c, i;
+      if (v < NZR) {
+        c = v / PR;
+        i = ROUT + (v - c * PR);
+      } else {
+        c = RIN + (v - NZR) / ROUT;
+        i = (v - NZR) - (c - RIN) * ROUT;
+      }
+      Fs[pl * PL + fpos(c, i)] = 0.0;
+    }
+    if constexpr (!XL) {
+      constexpr int NZX = (KS * 4 - RIN) * 64;
+      for (int u = tid; u < SX * NZX; u += 256) {
+        const int pl = u / NZX, v = u - pl * NZX;
+        Xs[pl * XPL + xpos(RIN + (v >> 6), v & 63)] = 0.0;
+      }
+    }
   }
-  // zero padding (rows >= ROUT, columns >= RIN) stays zero: the copies below write inside only
-  for (int u = tid; u < SF * KS * 4 * FLD; u += 256) Fs[u] = 0.0;
-  for (int u = tid; u < SX * KS * 4 * 64; u += 256) Xs[u] = 0.0;
   const int ldf = a.ldf;
   const int fe = ldf * RIN;
   const FT* __restrict__ F = static_cast<const FT*>(a.F);
   FT pfr[NFE], pfi[FC ? NFE : 1];
-  double pxr[NXE], pxi[XC ? NXE : 1];
-  auto fetch = [&](int p) {
+  double pxr[XL ? 1 : NXE], pxi[XC && !XL ? NXE : 1];  // staged x: the thread's share of the tile
+  double xr[XL ? KS : 1], xi[XC && XL ? KS : 1];      // lane-loaded x: (row 4 ks + lk, column n)
+  // block p into registers (and, staged x, its x tile)
+  auto fetch = [&](const Own& o, int p) {
     const int bi = mode == 0 ? p : (mode == 3 ? 0 : (p & 3));
-    const int sc = mode == 0 ? a.src[p] : (mode == 2 ? (p >> 2) : p);
     const FT* f = F + SF * (static_cast<int64_t>(bi) * a.fstride + a.fcomp);
 #pragma unroll
     for (int q = 0; q < NFE; ++q) {
@@ -1944,72 +2009,101 @@ __global__ __launch_bounds__(256) void k_multi_mfma(MultiArgs a) {
         if constexpr (FC) pfi[q] = f[SF * e + 1];
       }
     }
-    const double* x = a.x + SX * (xo + static_cast<int64_t>(sc) * a.xcs);
+    if constexpr (!XL) {
+      const double* x = a.x + SX * (o.xo + static_cast<int64_t>(source(p)) * a.xcs);
 #pragma unroll
-    for (int q = 0; q < NXE; ++q) {
-      const int e = tid + 256 * q;
-      if ((RIN * 64) % 256 == 0 || e < RIN * 64) {
-        const int j = e >> 6;
-        int kl = e & 63;
-        kl = kl <= kmax ? kl : kmax;
-        const double* xe = x + SX * (static_cast<int64_t>(j) * a.xrs + static_cast<int64_t>(k0 + kl) * a.xks);
-        pxr[q] = xe[0];
-        if constexpr (XC) pxi[q] = xe[1];
+      for (int q = 0; q < NXE; ++q) {
+        const int e = tid + 256 * q;
+        if ((RIN * 64) % 256 == 0 || e < RIN * 64) {
+          const int j = e >> 6;
+          int kl = e & 63;
+          kl = kl <= kmax ? kl : kmax;
+          const double* xe = x + SX * (static_cast<int64_t>(j) * a.xrs + static_cast<int64_t>(k0 + kl) * a.xks);
+          pxr[q] = xe[0];
+          if constexpr (XC) pxi[q] = xe[1];
+        }
       }
     }
   };
-  auto commit = [&]() {
+  // the lane's x operands of block p (lane-loaded x)
+  auto load_x = [&](const Own& o, int p) {
+    const double* x = a.x + SX * (o.xo + static_cast<int64_t>(source(p)) * a.xcs + static_cast<int64_t>(k0 + kc) * a.xks);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int kr = ks * 4 + lk;
+      if (RIN % 4 == 0 || kr < RIN) {
+        const double* xe = x + SX * static_cast<int64_t>(kr) * a.xrs;
+        xr[ks] = xe[0];
+        if constexpr (XC) xi[ks] = xe[1];
+      } else {
+        xr[ks] = 0.0;
+        if constexpr (XC) xi[ks] = 0.0;
+      }
+    }
+  };
+  auto commit = [&](int buf) {
+    double* Fb = Fs + buf * (SF * PL);
 #pragma unroll
     for (int q = 0; q < NFE; ++q) {
       const int e = tid + 256 * q;
       if (e < fe) {
         const int c = e / ldf, i = e - c * ldf;
         if (i < ROUT) {
-          Fs[SF * (c * FLD + i)] = static_cast<double>(pfr[q]);
-          if constexpr (FC) Fs[SF * (c * FLD + i) + 1] = static_cast<double>(pfi[q]);
+          Fb[fpos(c, i)] = static_cast<double>(pfr[q]);
+          if constexpr (FC) Fb[PL + fpos(c, i)] = static_cast<double>(pfi[q]);
         }
       }
     }
+    if constexpr (!XL) {
 #pragma unroll
-    for (int q = 0; q < NXE; ++q) {
-      const int e = tid + 256 * q;
-      if ((RIN * 64) % 256 == 0 || e < RIN * 64) {
-        Xs[SX * e] = pxr[q];
-        if constexpr (XC) Xs[SX * e + 1] = pxi[q];
+      for (int q = 0; q < NXE; ++q) {
+        const int e = tid + 256 * q;
+        if ((RIN * 64) % 256 == 0 || e < RIN * 64) {
+          const int j = e >> 6, kl = e & 63;
+          Xs[xpos(j, kl)] = pxr[q];
+          if constexpr (XC) Xs[XPL + xpos(j, kl)] = pxi[q];
+        }
       }
     }
   };
-  bmi_d4 accr[MT][NP], acci[XC ? MT : 1][NP];
+  // complex blocks: P1 = Fr Xr (accr), P2 = Fi Xi (accq), P3 = (Fr + Fi)(Xr + Xi) (acci);
+  // Re = P1 - P2, Im = P3 - P1 - P2; the tail tile accumulates in accr[MT - 1]
+  bmi_d4 accr[MT][NP], acci[XC ? MT : 1][NP], accq[FC ? MT : 1][NP];
+  auto reset = [&]() {
 #pragma unroll
-  for (int m = 0; m < MT; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int h = 0; h < NP; ++h) {
-      accr[m][h] = bmi_d4{0.0, 0.0, 0.0, 0.0};
-      if constexpr (XC) acci[m][h] = bmi_d4{0.0, 0.0, 0.0, 0.0};
-    }
-  const int li = lane & 15, lk = lane >> 4;
-  const int n = w * 16 + li;  // this lane's column of the x tile
-  if (p0 < p1) fetch(p0);
-  for (int p = p0; p < p1; ++p) {
-    __syncthreads();  // previous block no longer read (first pass: zero fill done)
-    commit();
-    __syncthreads();
-    if (p + 1 < p1) fetch(p + 1);
+      for (int h = 0; h < NP; ++h) {
+        accr[m][h] = bmi_d4{0.0, 0.0, 0.0, 0.0};
+        if constexpr (XC) acci[m][h] = bmi_d4{0.0, 0.0, 0.0, 0.0};
+        if constexpr (FC) accq[m][h] = bmi_d4{0.0, 0.0, 0.0, 0.0};
+      }
+  };
+  // the products of one staged block (planes at Fb)
+  auto multiply = [&](const double* Fb) {
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       const int kr = ks * 4 + lk;
       const int h = NP == 2 ? (ks & 1) : 0;
-      const double br = Xs[SX * (kr * 64 + n)];
-      const double bi = XC ? Xs[SX * (kr * 64 + n) + 1] : 0.0;
+      const double br = XL ? xr[XL ? ks : 0] : Xs[xpos(kr, n)];
+      const double bi = XC ? (XL ? xi[XL ? ks : 0] : Xs[XPL + xpos(kr, n)]) : 0.0;
+      const double bs = br + bi;
+      if constexpr (TAIL) {
+        const int tr = MTF * 16 + (li & 3);
+        const double tfr = Fb[fpos(kr, tr)], tfi = Fb[PL + fpos(kr, tr)];
+        const double a1 = li < 4 ? tfr : (li < 8 ? tfi : 0.0);
+        const double a2 = li < 4 ? -tfi : (li < 8 ? tfr : 0.0);
+        accr[MT - 1][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, br, accr[MT - 1][h], 0, 0, 0);
+        accr[MT - 1][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bi, accr[MT - 1][h], 0, 0, 0);
+      }
 #pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        const double fr = Fs[SF * (kr * FLD + m * 16 + li)];
+      for (int m = 0; m < MTF; ++m) {
+        const double fr = Fb[fpos(kr, m * 16 + li)];
         if constexpr (FC) {
-          const double fi = Fs[SF * (kr * FLD + m * 16 + li) + 1];
+          const double fi = Fb[PL + fpos(kr, m * 16 + li)];
           accr[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, br, accr[m][h], 0, 0, 0);
-          acci[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bi, acci[m][h], 0, 0, 0);
-          accr[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fi, bi, accr[m][h], 0, 0, 0);
-          acci[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi, br, acci[m][h], 0, 0, 0);
+          accq[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi, bi, accq[m][h], 0, 0, 0);
+          acci[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr + fi, bs, acci[m][h], 0, 0, 0);
         } else if constexpr (XC) {
           accr[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, br, accr[m][h], 0, 0, 0);
           acci[m][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bi, acci[m][h], 0, 0, 0);
@@ -2018,29 +2112,106 @@ __global__ __launch_bounds__(256) void k_multi_mfma(MultiArgs a) {
         }
       }
     }
-  }
-  if (n > kmax) return;
-  double* y = a.y + SX * (yo + static_cast<int64_t>(cl) * a.ycs + static_cast<int64_t>(k0 + n) * a.yks);
-#pragma unroll
-  for (int m = 0; m < MT; ++m)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = m * 16 + 4 * v + lk;
-      if (ROUT % 16 != 0 && row >= ROUT) continue;
-      double vr = accr[m][0][v], vi = 0.0;
-      if constexpr (NP == 2) vr += accr[m][1][v];
-      if constexpr (XC) {
-        vi = acci[m][0][v];
-        if constexpr (NP == 2) vi += acci[m][1][v];
-      }
+  };
+  // the owner's rows from the accumulators (zero: rows of an owner without blocks)
+  auto write_rows = [&](const Own& o, bool zero) {
+    if (n > kmax) return;
+    if (zero && a.accumulate) return;
+    double* y = a.y + SX * (o.yo + static_cast<int64_t>(o.cl) * a.ycs + static_cast<int64_t>(k0 + n) * a.yks);
+    auto put = [&](int row, double vr, double vi) {
       double* yr = y + SX * static_cast<int64_t>(row) * a.yrs;
+      if (zero) vr = vi = 0.0;
       if (a.accumulate) {
         vr += yr[0];
         if constexpr (XC) vi += yr[1];
       }
       yr[0] = vr;
       if constexpr (XC) yr[1] = vi;
+    };
+    if constexpr (TAIL) put(MTF * 16 + lk, accr[MT - 1][0][0], accr[MT - 1][0][1]);
+#pragma unroll
+    for (int m = 0; m < MTF; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m * 16 + 4 * v + lk;
+        if (ROUT % 16 != 0 && row >= ROUT) continue;
+        double vr = accr[m][0][v], vi = 0.0;
+        if constexpr (NP == 2) vr += accr[m][1][v];
+        if constexpr (XC) {
+          vi = acci[m][0][v];
+          if constexpr (NP == 2) vi += acci[m][1][v];
+        }
+        if constexpr (FC) {
+          double q = accq[m][0][v];
+          if constexpr (NP == 2) q += accq[m][1][v];
+          vi -= vr + q;  // P3 - (P1 + P2)
+          vr -= q;       // P1 - P2
+        }
+        put(row, vr, vi);
+      }
+  };
+  reset();
+  if constexpr (!XL) {
+    const Own o = owner(static_cast<int>(blockIdx.x));
+    if (o.p0 < o.p1) fetch(o, o.p0);
+    for (int p = o.p0; p < o.p1; ++p) {
+      __syncthreads();  // previous block no longer read (first pass: zero fill done)
+      commit(0);
+      __syncthreads();
+      if (p + 1 < o.p1) fetch(o, p + 1);
+      multiply(Fs);
     }
+    write_rows(o, false);
+  } else {
+    // this workgroup's owners: start + slot, start + slot + S, ... of its XCD's range
+    int start, cnt;
+    xcd_range(static_cast<int>(blockIdx.x & 7), a.nown, start, cnt);
+    const int slot = static_cast<int>(blockIdx.x >> 3), S = static_cast<int>(gridDim.x >> 3);
+    // the first owner with blocks from index t on; owners without blocks get their rows here
+    auto advance = [&](int t, Own& o) {
+      for (; t < cnt; t += S) {
+        o = owner(start + t);
+        if (o.p0 < o.p1) break;
+        write_rows(o, true);
+      }
+      return t;
+    };
+    Own cur;
+    int oi = advance(slot, cur);
+    if (oi >= cnt) return;
+    int p = cur.p0;
+    fetch(cur, p);
+    __syncthreads();  // zero fill done
+    commit(0);
+    load_x(cur, p);
+    __syncthreads();
+    int buf = 0;
+    while (true) {
+      // the next block: the owner's next, or the first of this workgroup's next owner
+      Own nx = cur;
+      int pn = p + 1, on = oi;
+      if (pn >= cur.p1) {
+        on = advance(oi + S, nx);
+        pn = nx.p0;
+      }
+      const bool more = on < cnt;
+      if (more) fetch(nx, pn);
+      multiply(Fs + buf * (SF * PL));
+      if (on != oi || !more) {
+        write_rows(cur, false);
+        reset();
+      }
+      if (!more) break;
+      // the other buffer was last read before the previous barrier
+      commit(1 - buf);
+      load_x(nx, pn);
+      __syncthreads();
+      buf ^= 1;
+      cur = nx;
+      p = pn;
+      oi = on;
+    }
+  }
 }
 
 // FP32 blocks of a reduced operator on the FP32 matrix cores (v_mfma_f32_16x16x4f32, twice the
@@ -2857,16 +3028,53 @@ static bool launch_lds(bool xc, bool fc, bool fp32, int rout, int rin, const Mul
   return false;
 }
 
-// block products and tree steps on the matrix cores
+// compute units of the current device
+static int device_cus() {
+  static const int n = [] {
+    int d = 0, v = 0;
+    if (hipGetDevice(&d) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, d) !=
+                                              hipSuccess || v < 8)
+      v = 256;
+    return v;
+  }();
+  return n;
+}
+
+// a block sweep: 8 S workgroups, S per XCD, each over every S-th owner of its XCD's range; S is
+// what an XCD holds at once (measured, 64-RHS Helmholtz M = 5: near-field sweep 0.22 ms against
+// 0.25 ms with one owner per workgroup, coupling sweep 0.44 against 0.47 ms)
+template <bool X, typename FTT, int R>
+static void launch_mfma_sweep(const MultiArgs& ma, unsigned rhs_tiles, hipStream_t st) {
+  const void* fn = reinterpret_cast<const void*>(&k_multi_mfma<X, X, FTT, R, R, true>);
+  static const int per_cu = [fn] {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+  }();
+  const int per_xcd = (ma.nown + 7) / 8;
+  const int S = std::min(per_xcd, std::max(1, device_cus() / 8 * per_cu));
+  if (S <= 0) return;
+  hipLaunchKernelGGL((k_multi_mfma<X, X, FTT, R, R, true>), dim3(static_cast<unsigned>(8 * S), rhs_tiles, 1),
+                     dim3(256), 0, st, ma);
+  BMI_CHECK_HIP(hipGetLastError());
+}
+
+// block products (XL form) and tree steps (staged x) on the matrix cores
 static bool launch_mfma(bool xc, bool fc, bool fp32, int rout, int rin, const MultiArgs& ma, dim3 grid,
                         hipStream_t st) {
-#define BMI_MF(X, FCC, FTT, RO, RI)                                                                 \
-  if (xc == X && fc == FCC && fp32 == std::is_same<FTT, float>::value && rout == RO && rin == RI) { \
-    hipLaunchKernelGGL((k_multi_mfma<X, FCC, FTT, RO, RI>), grid, dim3(256), 0, st, ma);           \
-    BMI_CHECK_HIP(hipGetLastError());                                                               \
-    return true;                                                                                    \
+#define BMI_MF(X, FCC, FTT, RO, RI)                                                                  \
+  if (xc == X && fc == FCC && fp32 == std::is_same<FTT, float>::value && rout == RO && rin == RI) {  \
+    hipLaunchKernelGGL((k_multi_mfma<X, FCC, FTT, RO, RI, false>), grid, dim3(256), 0, st, ma);      \
+    BMI_CHECK_HIP(hipGetLastError());                                                                \
+    return true;                                                                                     \
   }
-#define BMI_MF_BLK(X, R) BMI_MF(X, X, double, R, R) BMI_MF(X, X, float, R, R)
+#define BMI_MF_SWEEP(X, FTT, R)                                                                       \
+  if (ma.mode == 0 && xc == X && fc == X && fp32 == std::is_same<FTT, float>::value && rout == R &&    \
+      rin == R) {                                                                                      \
+    launch_mfma_sweep<X, FTT, R>(ma, grid.y, st);                                                      \
+    return true;                                                                                       \
+  }
+#define BMI_MF_BLK(X, R) BMI_MF_SWEEP(X, double, R) BMI_MF_SWEEP(X, float, R)
 #define BMI_MF_TREE(X, A, B) BMI_MF(X, false, double, A, B) BMI_MF(X, false, double, B, A)
   BMI_MF_BLK(false, 16)
   BMI_MF_BLK(false, 25)
@@ -2877,6 +3085,8 @@ static bool launch_mfma(bool xc, bool fc, bool fp32, int rout, int rin, const Mu
   BMI_MF_BLK(true, 36)
   BMI_MF(true, false, double, 16, 16)
   BMI_MF(true, false, double, 25, 25)
+  BMI_MF(false, false, double, 16, 16)
+  BMI_MF(false, false, double, 25, 25)
   BMI_MF_TREE(false, 16, 36)
   BMI_MF_TREE(false, 25, 16)
   BMI_MF_TREE(false, 25, 36)
@@ -2887,6 +3097,7 @@ static bool launch_mfma(bool xc, bool fc, bool fp32, int rout, int rin, const Mu
   BMI_MF_TREE(true, 32, 36)
 #undef BMI_MF_TREE
 #undef BMI_MF_BLK
+#undef BMI_MF_SWEEP
 #undef BMI_MF
   return false;
 }
