@@ -189,6 +189,24 @@ def _torch_sync():
 _pending = []
 
 
+def build_info():
+    """Provenance of the loaded library: the source digest it was built from (bmi_version), the
+    digest of the sources in this tree, whether they match, and the library file's own sha256."""
+    import hashlib
+    from . import build as _build
+    version = library().bmi_version().decode()
+    built = version.rsplit(" ", 1)[-1] if " sources " in version else "unknown"
+    try:
+        tree = _build.source_digest()
+    except OSError:
+        tree = "unavailable"
+    path = os.environ.get("BEMBEL_AMD_LIBRARY", LIB_PATH)
+    with open(path, "rb") as f:
+        lib_sha = hashlib.sha256(f.read()).hexdigest()[:16]
+    return {"version": version, "library_sources": built, "tree_sources": tree, "match": built == tree,
+            "library_sha256": lib_sha}
+
+
 def synchronize(stream=None):
     """Wait for the library's work on `stream` (default stream if None)."""
     _check(library().bmi_synchronize(_stream_ptr(stream)))

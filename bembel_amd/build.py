@@ -33,9 +33,22 @@ SOURCES = {
 }
 
 
-def _compile(src, extra):
+def source_digest():
+    """sha256 (first 16 hex digits) over the library's sources and its C header, in name order: the
+    library built from them reports the same digest in bmi_version()"""
+    h = hashlib.sha256()
+    names = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp", ".h", ".inc")))
+    paths = [os.path.join(CSRC, f) for f in names] + [os.path.join(ROOT, "include", "bembel_amd.h")]
+    for path in paths:
+        h.update(os.path.basename(path).encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def _compile(src, extra, digest):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + COMMON + extra + ['-DBMI_SOURCE_DIGEST="%s"' % digest, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr[-8000:]))
@@ -44,8 +57,9 @@ def _compile(src, extra):
 
 def build_product(jobs=8):
     os.makedirs(BUILD, exist_ok=True)
+    digest = source_digest()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_compile, s, e) for s, e in SOURCES.items()]
+        futs = [ex.submit(_compile, s, e, digest) for s, e in SOURCES.items()]
         objs = [f.result() for f in futs]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs

@@ -1137,7 +1137,10 @@ static void assemble_h2(bmi_h2* h, hipStream_t st) {
 extern "C" {
 
 const char* bmi_last_error(void) { return bmi::g_last_error.c_str(); }
-const char* bmi_version(void) { return "bembel_amd 0.1 (gfx950)"; }
+#ifndef BMI_SOURCE_DIGEST
+#define BMI_SOURCE_DIGEST "unknown"
+#endif
+const char* bmi_version(void) { return "bembel_amd 0.1 (gfx950) sources " BMI_SOURCE_DIGEST; }
 
 bmi_status bmi_op_desc_default(int kind, bmi_op_desc* out) {
   BMI_API_BEGIN

@@ -964,6 +964,7 @@ def main():
         "helmholtz_assembly_gmres": headline,
         "maxwell_efie": maxwell,
         "serving": serving,
+        "build": B.build_info(),
     }
     print(json.dumps(out), flush=True)
 

@@ -37,6 +37,13 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_library_built_from_these_sources():
+    """the loaded library reports the digest of the sources it was built from; it must be the
+    digest of the sources in this tree (a stale build fails here, not silently on the GPU)"""
+    info = B.build_info()
+    assert info["match"], info
+
+
 def test_error_reporting_without_gpu():
     with pytest.raises(B.BembelError) as e:
         B.Geometry("/nonexistent/file.dat")
