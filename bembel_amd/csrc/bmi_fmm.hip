@@ -1573,8 +1573,9 @@ __global__ __launch_bounds__(256) void k_sym_gather(int ng, const int2* __restri
 
 // the two sweeps of a single-RHS product in one launch: workgroups [0, nb_near) run the near
 // field, the rest the coupling blocks, so both streams of leaves share the memory system
-template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT = double, bool SYM = false>
-__global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, int nb_near, CplArgs na) {
+template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT, bool SYM, bool NSYM>
+__device__ __forceinline__ void sweep_fused_body(const LeafArgs& la, const CplArgs& ca, int nb_near,
+                                                 const CplArgs& na) {
   const int b = static_cast<int>(blockIdx.x);
   const int w = static_cast<int>(threadIdx.x >> 6);
   // real FP64 blocks: two rows per lane (16-byte loads)
@@ -1588,8 +1589,8 @@ __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, in
   if (b < nb_near) {
     const int c = __builtin_amdgcn_readfirstlane(b * 4 + w);
     if (c < la.ncl) {
-      if constexpr (SYM && std::is_same<NFT, double>::value) {
-        // upper and diagonal near-field leaves with transposed partials (when listed)
+      if constexpr (SYM && NSYM && std::is_same<NFT, double>::value) {
+        // upper and diagonal near-field leaves with transposed partials
         if (na.rp) {
           coupling_wave_sym<C, DD, UBN>(c, na);
           return;
@@ -1626,6 +1627,19 @@ __global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, in
       }
     }
   }
+}
+
+// every block of both sweeps (SYM false), or the upper coupling blocks with transposed partials
+// (SYM true; NSYM: the upper near-field leaves as well, a separate instantiation because its
+// registers would otherwise bound the occupancy of the coupling-only form)
+template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT = double>
+__global__ __launch_bounds__(256) void k_sweep_fused(LeafArgs la, CplArgs ca, int nb_near, CplArgs na) {
+  sweep_fused_body<C, DD, FT, CD, UBN, UBC, NFT, false, false>(la, ca, nb_near, na);
+}
+
+template <bool C, int DD, typename FT, int CD, int UBN, int UBC, typename NFT, bool NSYM>
+__global__ __launch_bounds__(256) void k_sweep_fused_sym(LeafArgs la, CplArgs ca, int nb_near, CplArgs na) {
+  sweep_fused_body<C, DD, FT, CD, UBN, UBC, NFT, true, NSYM>(la, ca, nb_near, na);
 }
 
 // ------------------------------------------------------------------ many right-hand sides
@@ -2838,7 +2852,12 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
 #define BMI_FUSED_S(CC, DDD, FTT, CDD, UN, UC, NFTT)                                                                 \
   if (!launched && cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&               \
       near32 == std::is_same<NFTT, float>::value && leaf_variant == UN - 1 && cpl_variant == UC - 1) {              \
-    hipLaunchKernelGGL((k_sweep_fused<CC, DDD, FTT, CDD, UN, UC, NFTT, true>), grid, dim3(256), 0, st, la, ca, nbn, na); \
+    if (nsym && std::is_same<NFTT, double>::value)                                                                   \
+      hipLaunchKernelGGL((k_sweep_fused_sym<CC, DDD, FTT, CDD, UN, UC, NFTT, true>), grid, dim3(256), 0, st, la,   \
+                         ca, nbn, na);                                                                               \
+    else                                                                                                             \
+      hipLaunchKernelGGL((k_sweep_fused_sym<CC, DDD, FTT, CDD, UN, UC, NFTT, false>), grid, dim3(256), 0, st, la,  \
+                         ca, nbn, na);                                                                               \
     BMI_CHECK_HIP(hipGetLastError());                                                                                \
     launched = true;                                                                                                 \
   }
