@@ -105,9 +105,10 @@ struct BlockAcc {
   static constexpr int I2 = I * I;
   static constexpr bool CPLX = KIND == BMI_HELMHOLTZ_SL || KIND == BMI_MAXWELL_SL;
   static constexpr bool MAXW = KIND == BMI_MAXWELL_SL;
-  // lanes per group: real blocks fit one lane (P <= 2), complex ones are split into their real
-  // and imaginary parts (two lanes)
-  static constexpr int G = CPLX ? 2 : 1;
+  // lanes per group: two lanes hold a block, real blocks split into two halves of their rows,
+  // complex ones into their real and imaginary parts (splitting complex blocks into four lanes
+  // as well measured slower: Helmholtz edge rule 182 -> 303 ms, more sums exchanged per pass)
+  static constexpr int G = 2;
   static constexpr int RH = CPLX ? G / 2 : G;         // row halves
   static constexpr int NR = (I2 + RH - 1) / RH;       // rows held per lane
   static constexpr int ID = MAXW ? I : 1;
@@ -369,30 +370,30 @@ __device__ __forceinline__ Pt<KIND> line_point(const double* L, bool fix_s, doub
 struct FactorShape {
   int chunk, lanes;
 };
-__host__ __device__ constexpr FactorShape factor_shape(int outer_per_chunk, int max_chunk, int nt, bool cplx,
+__host__ __device__ constexpr FactorShape factor_shape(int outer_per_chunk, int max_chunk, int nt, int g,
                                                        int eval_cost) {
-  FactorShape best{1, cplx ? 2 : 1};
+  FactorShape best{1, g};
   long best_score = -1;
   for (int c = 1; c <= max_chunk; ++c) {
     const int units = c * outer_per_chunk;
     int S = nt / units;
-    if (cplx) S -= S & 1;
-    if (S < (cplx ? 2 : 1)) break;
+    S -= S % g;  // whole lane groups per unit
+    if (S < g) break;
     const long score = static_cast<long>(units) * S * 1000 / (eval_cost + S);
     if (score > best_score) best_score = score, best = FactorShape{c, S};
   }
   return best;
 }
 // vertex rule: chunks of first outer coordinates (n outer points each), inner range n^2
-__host__ __device__ constexpr FactorShape vertex_shape(int n, int nt, int ptsize, bool cplx) {
+__host__ __device__ constexpr FactorShape vertex_shape(int n, int nt, int ptsize, int g) {
   const int cap = DUFFY_TABLE_BYTES / (2 * n * n * ptsize);
-  return factor_shape(n, cap < n ? (cap > 0 ? cap : 1) : n, nt, cplx, 90);
+  return factor_shape(n, cap < n ? (cap > 0 ? cap : 1) : n, nt, g, 90);
 }
 // edge rule, factorised: chunks of outer indices i (n outer points (i, j0) each), inner range n
-__host__ __device__ constexpr FactorShape edge_shape(int n, int nt, int ptsize, bool cplx) {
+__host__ __device__ constexpr FactorShape edge_shape(int n, int nt, int ptsize, int g) {
   const int cap = DUFFY_TABLE_BYTES / (8 * n * ptsize);
   const int mx = cap < n * n ? cap : n * n;
-  return factor_shape(n, mx > 0 ? mx : 1, nt, cplx, 48);
+  return factor_shape(n, mx > 0 ? mx : 1, nt, g, 48);
 }
 
 // kernel times both point weights, (kr, ki): the scalar integrands of eval() without the basis
@@ -440,17 +441,20 @@ struct VecSum {
         if constexpr (CPLX) i[iy * I + ix] += ki * b;
       }
   }
-  // the part of the block this lane holds (complex: role 0 real, role 1 imaginary), summed
-  // with the lane pair's other half of the inner range; all lanes must call this together
+  // the vector summed over the G lanes of this lane's group (each holds a share of the inner
+  // range): real parts, or for complex vectors the part of this lane's role (role & 1: 0 real,
+  // 1 imaginary); all lanes must call this together
+  template <int G>
   __device__ __forceinline__ void total(int role, double* w) const {
 #pragma unroll
     for (int k = 0; k < I2; ++k) {
-      if constexpr (CPLX) {
-        const double orr = __shfl_xor(r[k], 1, 64), oi = __shfl_xor(i[k], 1, 64);
-        w[k] = role ? i[k] + oi : r[k] + orr;
-      } else {
-        w[k] = r[k];
+      double sr = r[k], si = CPLX ? i[k] : 0.0;
+#pragma unroll
+      for (int m = 1; m < G; m <<= 1) {
+        sr += __shfl_xor(sr, m, 64);
+        if constexpr (CPLX) si += __shfl_xor(si, m, 64);
       }
+      w[k] = (CPLX && (role & 1)) ? si : sr;
     }
   }
 };
@@ -479,24 +483,28 @@ __device__ __forceinline__ void factor_pass(Acc& acc, const PtK& F, int role, bo
     }
   }
   double w[I2];
-  u.total(role, w);
+  u.template total<Acc::G>(role, w);
   if (!act) return;
   double X[I], Y[I];
   bernstein_basis<P>(F.s0, X);
   bernstein_basis<P>(F.s1, Y);
+  double f[I2];
 #pragma unroll
   for (int iy = 0; iy < I; ++iy)
 #pragma unroll
-    for (int ix = 0; ix < I; ++ix) {
-      const double f = X[ix] * Y[iy];
+    for (int ix = 0; ix < I; ++ix) f[iy * I + ix] = X[ix] * Y[iy];
+  // this lane's rows: half (role bit above the real/imaginary bit) x NR .. + NR
+  const int half = Acc::RH == 2 ? (Acc::CPLX ? (role >> 1) : role) : 0;
 #pragma unroll
-      for (int k = 0; k < I2; ++k) {
-        if constexpr (COLS)
-          acc.acc[(iy * I + ix) * I2 + k] += f * w[k];
-        else
-          acc.acc[k * I2 + iy * I + ix] += w[k] * f;
-      }
-    }
+  for (int ii = 0; ii < Acc::NR; ++ii) {
+    const int row = half * Acc::NR + ii;
+    const bool in = Acc::RH == 1 || row < I2;
+    const int rr = in ? row : 0;
+    // COLS: F's basis gives the rows; otherwise the summed vector does
+    const double a = in ? (COLS ? f[rr] : w[rr]) : 0.0;
+#pragma unroll
+    for (int k = 0; k < I2; ++k) acc.acc[ii * I2 + k] += a * (COLS ? w[k] : f[k]);
+  }
 }
 
 // DuffyTrick::tau (DuffyTrick/tau.hpp:26-42)
@@ -842,7 +850,7 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
     // are split into the outer factor (w and the j0 part of the inner weights) and gw[j1].
     const int r1 = pr.rot1, r2 = pr.rot2;
     constexpr bool CPLX = Acc::CPLX;
-    const FactorShape shp = edge_shape(n, NT, static_cast<int>(sizeof(PtK)), CPLX);
+    const FactorShape shp = edge_shape(n, NT, static_cast<int>(sizeof(PtK)), Acc::G);
     const int CI = shp.chunk, S = shp.lanes;
     const int role = lane & (Acc::G - 1);
     const bool fs1 = (r1 & 1) == 0, fs2 = (r2 & 1) == 0;  // tau(x, y; r) fixes u for r = 0, 2
@@ -946,7 +954,7 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       tau(xi1, xi0, r2, u, v);
       q[4 * i + 3] = make_point<PD, KIND>(cp2, l2x, l2y, h, u, v, w);
     }
-    const FactorShape shp = vertex_shape(n, NT, static_cast<int>(sizeof(PtK)), CPLX);
+    const FactorShape shp = vertex_shape(n, NT, static_cast<int>(sizeof(PtK)), Acc::G);
     const int C0 = shp.chunk, S = shp.lanes;
     const int role = lane & (Acc::G - 1);
     for (int a0 = 0; a0 < n; a0 += C0) {
@@ -1209,8 +1217,8 @@ static size_t class_lds_bytes(int deg) {
                                                         sizeof(double));
   constexpr bool FACT = KIND != BMI_MAXWELL_SL;  // factorised vertex rule
   constexpr int PS = static_cast<int>(sizeof(Pt<KIND>));
-  const int cv = FACT ? vertex_shape(n, NT, PS, BlockAcc<P, KIND>::CPLX).chunk : vertex_chunk<Pt<KIND>>(n, NT);
-  const int ce = FACT ? edge_shape(n, NT, PS, BlockAcc<P, KIND>::CPLX).chunk : edge_chunk<Pt<KIND>>(n, NT);
+  const int cv = FACT ? vertex_shape(n, NT, PS, BlockAcc<P, KIND>::G).chunk : vertex_chunk<Pt<KIND>>(n, NT);
+  const int ce = FACT ? edge_shape(n, NT, PS, BlockAcc<P, KIND>::G).chunk : edge_chunk<Pt<KIND>>(n, NT);
   if (CLS == 3) lds = std::max(lds, static_cast<size_t>(8 * n * ce) * sizeof(Pt<KIND>));
   if (CLS == 2 && (KIND == BMI_LAPLACE_SL || KIND == BMI_HELMHOLTZ_SL))
     lds = std::max(lds, static_cast<size_t>(NT / 4) * (32 * PD + 2 + 7 + (BlockAcc<P, KIND>::CPLX ? 4 : 2) *
