@@ -716,8 +716,9 @@ bool h2_is_mixed(const bmi_h2* h) { return h->op.precision != BMI_PRECISION_FP64
 // ------------------------------------------------------------------ assembly
 // Coupling blocks of a symmetric kernel pair up: (level, c2, c1) is the transpose of (level, c1,
 // c2) (same kernel values; the two weights multiply in the other order). For single-RHS sweeps
-// list the upper blocks (c1 < c2) per target, and per column cluster the upper blocks whose
-// transposed products it receives. Leaves the handle unchanged when a block has no partner.
+// list one block per pair under its smaller cluster c1 (the upper block (c1, c2) for real
+// kernels, the lower block (c2, c1) for complex ones), and per cluster c2 the entries whose
+// partials it receives. Leaves the handle unchanged when a block has no partner.
 static void build_symmetric_lists(bmi_h2* h, const std::vector<int2>& tgt, const std::vector<int>& crp,
                                   const std::vector<int>& csrc, hipStream_t st) {
   const int nt = static_cast<int>(tgt.size());
@@ -741,7 +742,11 @@ static void build_symmetric_lists(bmi_h2* h, const std::vector<int2>& tgt, const
       if (!any) utgt.push_back(tgt[t]), urp.push_back(urp.back()), any = true;
       ucol.push_back(make_int2(t2, static_cast<int>(usrc.size())));
       usrc.push_back(c2);
-      ublk.push_back(p);
+      // complex kernels: the lower block (c2, c1), whose row products are the partials and whose
+      // column products sum into the rows of c1
+      ublk.push_back(h->cplx ? static_cast<int>(std::lower_bound(csrc.begin() + crp[t2], csrc.begin() + crp[t2 + 1], c1) -
+                                                csrc.begin())
+                             : p);
       urp.back()++;
     }
   }

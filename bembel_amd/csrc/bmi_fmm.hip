@@ -718,9 +718,16 @@ struct LeafArgs {
   int accumulate;
 };
 
+// wave-uniform double from lane l of v
+__device__ __forceinline__ double lane_double(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
 template <bool C, int DD, int RT, int UB, typename FT = double>
 __device__ __forceinline__ void leaf_wave(int c, const LeafArgs& a) {
   constexpr int S = C ? 2 : 1;
+  static_assert(DD <= 64, "leaf rows exceed a wave");
   const int lane = threadIdx.x & 63;
   const int i = lane < DD ? lane : 0;  // spare lanes shadow row 0 and store nothing
   const int p0 = a.rp[c], p1 = a.rp[c + 1];
@@ -730,45 +737,70 @@ __device__ __forceinline__ void leaf_wave(int c, const LeafArgs& a) {
     double ar[RT], ai[RT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) ar[r] = 0.0, ai[r] = 0.0;
-    for (int p = p0; p < p1; p += UB) {
-      const int nb = p1 - p < UB ? p1 - p : UB;
-      FT fr[UB][DD], fi[UB][C ? DD : 1];
+    // the source clusters of 64 entries at a time, one per lane: the block loop reads them with
+    // lane reads instead of a dependent load per block
+    for (int q0 = p0; q0 < p1; q0 += 64) {
+      const int srcv = q0 + lane < p1 ? a.src[q0 + lane] : 0;
+      const int q1 = p1 - q0 < 64 ? p1 : q0 + 64;
+      for (int p = q0; p < q1; p += UB) {
+        const int nb = q1 - p < UB ? q1 - p : UB;
+        FT fr[UB][DD], fi[UB][C ? DD : 1];
+        // the source vectors: one coalesced load per lane (lane j: entry j), issued before the
+        // blocks and handed to every lane with lane reads.  Read straight from global memory, each
+        // entry would be a load of one address by the whole wave, and these wait on each other
+        double xr0[UB][RT], xi0[UB][RT];
 #pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        if (u >= nb) break;
-        const FT* f = F + S * (static_cast<int64_t>(p + u) * a.fstride + a.fcomp);
+        for (int u = 0; u < UB; ++u) {
+          if (UB > 1 && u >= nb) break;
+          const int s2 = __builtin_amdgcn_readlane(srcv, p + u - q0);
 #pragma unroll
-        for (int j = 0; j < DD; ++j) {
-          if constexpr (C && std::is_same<FT, double>::value) {
-            const double2 v = *reinterpret_cast<const double2*>(f + 2 * (j * DD + i));
-            fr[u][j] = v.x, fi[u][j] = v.y;
-          } else if constexpr (C) {
-            const float2 v = *reinterpret_cast<const float2*>(f + 2 * (j * DD + i));
-            fr[u][j] = v.x, fi[u][j] = v.y;
-          } else {
-            fr[u][j] = f[j * DD + i];
+          for (int r = 0; r < RT; ++r) {
+            const int k = k0 + r;
+            if (RT > 1 && k >= a.nrhs) break;
+            const double* x = xs + S * (static_cast<int64_t>(k) * a.ldxs + a.xseg + static_cast<int64_t>(s2) * DD + i);
+            if constexpr (C) {
+              const double2 v = *reinterpret_cast<const double2*>(x);
+              xr0[u][r] = v.x, xi0[u][r] = v.y;
+            } else {
+              xr0[u][r] = x[0];
+            }
           }
         }
-      }
 #pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        if (u >= nb) break;
-        const int s2 = a.src[p + u];
-#pragma unroll
-        for (int r = 0; r < RT; ++r) {
-          const int k = k0 + r;
-          if (k >= a.nrhs) break;
-          const double* x = xs + S * (static_cast<int64_t>(k) * a.ldxs + a.xseg + static_cast<int64_t>(s2) * DD);
+        for (int u = 0; u < UB; ++u) {
+          if (UB > 1 && u >= nb) break;
+          const FT* f = F + S * (static_cast<int64_t>(p + u) * a.fstride + a.fcomp);
 #pragma unroll
           for (int j = 0; j < DD; ++j) {
-            const double f_r = static_cast<double>(fr[u][j]);
-            if constexpr (C) {
-              const double f_i = static_cast<double>(fi[u][j]);
-              const double xr = x[2 * j], xi = x[2 * j + 1];
-              ar[r] += f_r * xr - f_i * xi;
-              ai[r] += f_r * xi + f_i * xr;
+            if constexpr (C && std::is_same<FT, double>::value) {
+              const double2 v = *reinterpret_cast<const double2*>(f + 2 * (j * DD + i));
+              fr[u][j] = v.x, fi[u][j] = v.y;
+            } else if constexpr (C) {
+              const float2 v = *reinterpret_cast<const float2*>(f + 2 * (j * DD + i));
+              fr[u][j] = v.x, fi[u][j] = v.y;
             } else {
-              ar[r] += f_r * x[j];
+              fr[u][j] = f[j * DD + i];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+          if (UB > 1 && u >= nb) break;
+#pragma unroll
+          for (int r = 0; r < RT; ++r) {
+            if (RT > 1 && k0 + r >= a.nrhs) break;
+#pragma unroll
+            for (int j = 0; j < DD; ++j) {
+              const double f_r = static_cast<double>(fr[u][j]);
+              const double xr = lane_double(xr0[u][r], j);
+              if constexpr (C) {
+                const double f_i = static_cast<double>(fi[u][j]);
+                const double xi = lane_double(xi0[u][r], j);
+                ar[r] += f_r * xr - f_i * xi;
+                ai[r] += f_r * xi + f_i * xr;
+              } else {
+                ar[r] += f_r * xr;
+              }
             }
           }
         }
@@ -989,38 +1021,55 @@ __device__ __forceinline__ void leaf_wave_v(int c, const LeafArgs& a) {
     yr[q] = 0.0;
     if constexpr (C) yi[q] = 0.0;
   }
-  for (int p = p0; p < p1; p += UB) {
-    RowVec<C, FT> fv[UB][JC];
-    double xr[UB][JC], xi[UB][C ? JC : 1];
+  // the source clusters of 64 entries at a time, one per lane, read with lane reads; the source
+  // vector of a block with one coalesced load (lane j: entry j), its entries handed to the
+  // groups' lanes by lane permutes (read straight from global memory, each entry would be a
+  // load of the same few addresses by the whole wave, waiting on the source index)
+  const int le = lane < DD ? lane : 0;
+  for (int q0 = p0; q0 < p1; q0 += 64) {
+    const int srcv = q0 + lane < p1 ? a.src[q0 + lane] : 0;
+    const int q1 = p1 - q0 < 64 ? p1 : q0 + 64;
+    for (int p = q0; p < q1; p += UB) {
+      RowVec<C, FT> fv[UB][JC];
+      double xor_[UB], xoi[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) {
-      const int pu = p + u < p1 ? p + u : p1 - 1;
-      const FT* f = F + S * (static_cast<int64_t>(pu) * a.fstride + a.fcomp);
-      const double* x = xs + S * (a.xseg + static_cast<int64_t>(a.src[pu]) * DD);
+      for (int u = 0; u < UB; ++u) {
+        const int pu = p + u < q1 ? p + u : q1 - 1;
+        const double* x = xs + S * (a.xseg + static_cast<int64_t>(__builtin_amdgcn_readlane(srcv, pu - q0)) * DD + le);
+        if constexpr (C) {
+          const double2 v = *reinterpret_cast<const double2*>(x);
+          xor_[u] = v.x, xoi[u] = v.y;
+        } else {
+          xor_[u] = x[0], xoi[u] = 0.0;
+        }
+        const FT* f = F + S * (static_cast<int64_t>(pu) * a.fstride + a.fcomp);
 #pragma unroll
-      for (int jj = 0; jj < JC; ++jj) {
-        const bool in = g * JC + jj < DD && p + u < p1;
-        const int j = g * JC + jj < DD ? g * JC + jj : DD - 1;
-        fv[u][jj].load(f + S * (j * DD + RPL * l));
-        xr[u][jj] = in ? x[S * j] : 0.0;
-        if constexpr (C) xi[u][jj] = in ? x[S * j + 1] : 0.0;
+        for (int jj = 0; jj < JC; ++jj) {
+          const int j = g * JC + jj < DD ? g * JC + jj : DD - 1;
+          fv[u][jj].load(f + S * (j * DD + RPL * l));
+        }
       }
-    }
 #pragma unroll
-    for (int u = 0; u < UB; ++u)
+      for (int u = 0; u < UB; ++u)
 #pragma unroll
-      for (int jj = 0; jj < JC; ++jj)
+        for (int jj = 0; jj < JC; ++jj) {
+          const bool in = g * JC + jj < DD && p + u < q1;
+          const int j = g * JC + jj < DD ? g * JC + jj : DD - 1;
+          const double xr = in ? __shfl(xor_[u], j, 64) : 0.0;
+          const double xi = C && in ? __shfl(xoi[u], j, 64) : 0.0;
 #pragma unroll
-        for (int q = 0; q < RPL; ++q) {
-          const double fr = static_cast<double>(fv[u][jj].r[q]);
-          if constexpr (C) {
-            const double fi = static_cast<double>(fv[u][jj].i[q]);
-            yr[q] += fr * xr[u][jj] - fi * xi[u][jj];
-            yi[q] += fr * xi[u][jj] + fi * xr[u][jj];
-          } else {
-            yr[q] += fr * xr[u][jj];
+          for (int q = 0; q < RPL; ++q) {
+            const double fr = static_cast<double>(fv[u][jj].r[q]);
+            if constexpr (C) {
+              const double fi = static_cast<double>(fv[u][jj].i[q]);
+              yr[q] += fr * xr - fi * xi;
+              yi[q] += fr * xi + fi * xr;
+            } else {
+              yr[q] += fr * xr;
+            }
           }
         }
+    }
   }
   // group 0 collects the other groups' sums
 #pragma unroll
@@ -1340,6 +1389,115 @@ __device__ __forceinline__ void coupling_wave_sym(int t, const CplArgs& a) {
   }
 }
 
+// Complex blocks from the lower side: for the loop's cluster c (its x_c in LDS, the same for every
+// entry) each entry names a partner s > c and the stored block B = F(s, c) (rows of s, columns of
+// c), the transpose of the upper block F(c, s).  The wave takes one block at a time: lane i of
+// half h holds row i of the block's h-th column half.
+//  * B x_c: a dot product inside the lane over its half, the two halves added across the wave,
+//    is the entry's partial for the rows of s;
+//  * B^T x_s = F(c, s) x_s adds B(i, j) x_s(i) into the lane's column accumulators (x_s(i) is one
+//    coalesced load per lane), summed over the half's lanes once, after the loop, into y_c.
+// The upper-block form needs a column sum over the lanes for every block; this form one per loop.
+template <bool C, int CD, int UB, typename FT = double, int LD = CD>
+__device__ __forceinline__ void coupling_wave_lsym(int t, const CplArgs& a) {
+  static_assert(C, "complex blocks");
+  static_assert(CD <= 32, "block rows exceed half a wave");
+  constexpr int S = 2;
+  constexpr int JH = (CD + 1) / 2;  // columns of the first half (the second has CD - JH)
+  __shared__ double xl[4][S * CD];  // x_c of each wave of the workgroup
+  const int lane = threadIdx.x & 63;
+  const int wv = static_cast<int>(threadIdx.x >> 6);
+  const int h = lane >> 5, li = lane & 31;
+  const bool act = li < CD;
+  const int i = act ? li : 0;
+  const int j0 = h * JH;
+  const int2 T = a.tgt[t];
+  const int64_t off = a.lvl_off[T.x];
+  const int p0 = a.rp[t], p1 = a.rp[t + 1];
+  const FT* __restrict__ F = static_cast<const FT*>(a.F);
+  {
+    const double* xc = a.fw + S * (off + static_cast<int64_t>(T.y) * CD);
+    if (lane < S * CD) xl[wv][lane] = xc[lane];
+  }
+  __builtin_amdgcn_wave_barrier();
+  double acr[JH], aci[JH];
+#pragma unroll
+  for (int jj = 0; jj < JH; ++jj) acr[jj] = 0.0, aci[jj] = 0.0;
+  // block, partner and partial slot of 64 entries at a time, one per lane, read by the block
+  // loop with lane reads (no dependent load before each block)
+  for (int q0 = p0; q0 < p1; q0 += 64) {
+    const bool lq = q0 + lane < p1;
+    const int blkv = lq ? a.blk[q0 + lane] : 0;
+    const int srcv = lq ? a.src[q0 + lane] : 0;
+    const int slotv = lq ? a.slot[q0 + lane] : 0;
+    const int q1 = p1 - q0 < 64 ? p1 : q0 + 64;
+    for (int p = q0; p < q1; p += UB) {
+      FT fr[UB][JH], fi[UB][JH];
+      double sr[UB], si[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        const bool ok = p + u < q1;
+        const int e = (ok ? p + u : p) - q0;
+        const double2 xs = *reinterpret_cast<const double2*>(
+            a.fw + S * (off + static_cast<int64_t>(__builtin_amdgcn_readlane(srcv, e)) * CD + i));
+        sr[u] = ok && act ? xs.x : 0.0;  // idle lanes and spare entries add nothing to the columns
+        si[u] = ok && act ? xs.y : 0.0;
+        const FT* f = F + S * (static_cast<int64_t>(__builtin_amdgcn_readlane(blkv, e)) * a.fstride + a.fcomp);
+#pragma unroll
+        for (int jj = 0; jj < JH; ++jj) {
+          // the second half is one column short when CD is odd: its last slot reads column j0
+          // and counts as zero
+          const bool has = jj < CD - JH || h == 0;
+          const int j = has ? j0 + jj : j0;
+          if constexpr (std::is_same<FT, float>::value) {
+            const float2 v = *reinterpret_cast<const float2*>(f + S * (j * LD + i));
+            fr[u][jj] = has ? v.x : 0.0f;
+            fi[u][jj] = has ? v.y : 0.0f;
+          } else {
+            const double2 v = *reinterpret_cast<const double2*>(f + S * (j * LD + i));
+            fr[u][jj] = has ? v.x : 0.0;
+            fi[u][jj] = has ? v.y : 0.0;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        const bool ok = p + u < q1;
+        const int e = (ok ? p + u : p) - q0;
+        // x_c is read from LDS inside the loop (held in registers across it, it would take 2 * CD)
+        int base = j0;
+        asm volatile("" : "+v"(base));
+        double dr = 0.0, di = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < JH; ++jj) {
+          const int jx = jj < CD - JH ? jj : (h == 0 ? jj : 0);
+          const double2 xc = *reinterpret_cast<const double2*>(&xl[wv][S * (base + jx)]);
+          const double er = static_cast<double>(fr[u][jj]), ei = static_cast<double>(fi[u][jj]);
+          dr += er * xc.x - ei * xc.y;
+          di += er * xc.y + ei * xc.x;
+          acr[jj] += er * sr[u] - ei * si[u];
+          aci[jj] += er * si[u] + ei * sr[u];
+        }
+        dr += __shfl_xor(dr, 32, 64);
+        di += __shfl_xor(di, 32, 64);
+        if (ok && act && h == 0) {
+          double2* pp = reinterpret_cast<double2*>(
+              a.part + S * (static_cast<int64_t>(__builtin_amdgcn_readlane(slotv, e)) * CD + li));
+          *pp = make_double2(dr, di);
+        }
+      }
+    }
+  }
+  // column sums over each half's lanes: lane li of half h ends with column j0 + li
+  double cr[1], ci[1];
+  group_column_sums<32, 1>(li, [&](int k) { return k < JH ? acr[k < JH ? k : 0] : 0.0; }, cr);
+  group_column_sums<32, 1>(li, [&](int k) { return k < JH ? aci[k < JH ? k : 0] : 0.0; }, ci);
+  if (li < (h ? CD - JH : JH)) {
+    double2* y = reinterpret_cast<double2*>(a.bw + S * (off + static_cast<int64_t>(T.y) * CD + j0 + li));
+    *y = make_double2(cr[0], ci[0]);
+  }
+}
+
 // FP32 blocks with columns padded to whole 16-byte row vectors (LD rows): a group of W lanes
 // (W = LD / RPL rounded up to a power of two; lanes past LD / RPL idle) owns a block
 template <bool C, int CD, int LD, int UB>
@@ -1605,13 +1763,16 @@ __device__ __forceinline__ void sweep_fused_body(const LeafArgs& la, const CplAr
     const int t = __builtin_amdgcn_readfirstlane((b - nb_near) * 4 + w);
     if (t < ca.ntgt) {
       if constexpr (SYM) {
-        // upper blocks only (the launcher checks the padded FP32 layout)
+        // one block per pair (the launcher checks the padded FP32 layout): the upper blocks of
+        // real kernels, the lower ones of complex kernels
         if constexpr (R2C)
           coupling_wave_r2_sym<CD, UBC>(t, ca);
         else if constexpr (VC && C)
-          coupling_wave_sym<C, CD, UBC, float, LDV>(t, ca);
+          coupling_wave_lsym<C, CD, UBC, float, LDV>(t, ca);
         else if constexpr (VC)
           coupling_wave_v_sym<C, CD, LDV, UBC>(t, ca);
+        else if constexpr (C)
+          coupling_wave_lsym<C, CD, UBC>(t, ca);
         else
           coupling_wave_sym<C, CD, UBC>(t, ca);
       } else if constexpr (R2C)
