@@ -115,6 +115,7 @@ def library():
         "bmi_h2_matvec_super": (C.c_int, [vp, C.c_int, dp, dp, vp]),
         "bmi_h2_use_graphs": (C.c_int, [vp, C.c_int]),
         "bmi_h2_set_sweep": (C.c_int, [vp, C.c_int]),
+        "bmi_h2_set_overlap": (C.c_int, [vp, C.c_int]),
         "bmi_gmres_start": (C.c_int, [vp, C.c_int, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, vp,
                                        C.POINTER(vp)]),
         "bmi_gmres_poll": (C.c_int, [vp, C.POINTER(C.c_int)]),
@@ -592,6 +593,13 @@ class H2Matrix:
         from them) or "upper_near" (the same for the near-field leaves too)."""
         m = {None: -1, "full": 0, "upper": 1, "upper_near": 2}[mode]
         _check(library().bmi_h2_set_sweep(self._h, m))
+
+    def set_overlap(self, mode):
+        """Single-RHS products: None (timed choice, default), "fused" (near-field leaves and
+        coupling blocks in one launch) or "overlap" (near-field leaves beside the coupling
+        blocks and the backward transformation on a side stream)."""
+        m = {None: -1, "fused": 0, "overlap": 1}[mode]
+        _check(library().bmi_h2_set_overlap(self._h, m))
 
     def get_dense(self):
         """Dense matrix by applying H to the identity (H2Matrix::get_dense, H2Matrix.hpp:254-264)."""

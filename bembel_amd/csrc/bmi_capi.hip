@@ -29,6 +29,28 @@ void set_last_error(const std::string& m) { g_last_error = m; }
 static std::atomic<bool> g_timing{false};
 static std::mutex g_tmu;
 static std::map<std::string, std::pair<double, int64_t>> g_times;
+// event pairs of finished scopes, read when the times are asked for: a scope does not wait for
+// its kernels, so timed launches queue back to back as they do untimed (a wait per scope would
+// leave the device idle before each launch and count the host's launch latency as kernel time)
+struct PendingTime {
+  const char* name;
+  hipEvent_t a, b;
+};
+static std::vector<PendingTime> g_pending;
+// the caller holds g_tmu
+static void resolve_pending(bool keep) {
+  for (const PendingTime& p : g_pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess && keep) {
+      auto& e = g_times[p.name];
+      e.first += ms;
+      e.second += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  g_pending.clear();
+}
 
 static thread_local bool tl_timing_off = false;
 bool timing_enabled() { return g_timing && !tl_timing_off; }
@@ -42,17 +64,9 @@ TimedScope::TimedScope(const char* n, hipStream_t s) : name(n), st(s) {
 TimedScope::~TimedScope() {
   if (!a) return;
   (void)hipEventRecord(b, st);
-  (void)hipEventSynchronize(b);
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, a, b);
-  {
-    std::lock_guard<std::mutex> g(g_tmu);
-    auto& e = g_times[name];
-    e.first += ms;
-    e.second += 1;
-  }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  std::lock_guard<std::mutex> g(g_tmu);
+  g_pending.push_back(PendingTime{name, a, b});
+  if (g_pending.size() >= 4096) resolve_pending(true);  // bounds the events held
 }
 
 static double wall_ms() {
@@ -164,6 +178,10 @@ struct bmi_h2 {
   bool sym = false;
   int sym_fused = 0;  // chosen by the autotuner: FP64 single-RHS sweeps read only the upper blocks
   int sweep_mode = -1;  // -1: timed choice, 0: every coupling block, 1: upper blocks
+  // single-RHS products with the handle's workspace: 1 runs the near-field leaves beside the far
+  // chain on the side stream, 0 both sweeps in one launch after the forward transformation
+  int overlap_mode = -1;  // -1: timed choice, else forced
+  int overlap = 0;
   DevBuf<int2> d_stgt, d_sgtgt;
   // the same for the near field (upper and diagonal leaves, partials added into the rows)
   bool nsym = false;
@@ -341,6 +359,77 @@ static void apply_near(bmi_h2* h, int nrhs, const double* xs, double* ys, hipStr
 
 // time both variants of the two sweeps once (1 RHS) and keep the faster; the variants give
 // identical bits, so the choice affects speed only
+// names the timing scope of the sweep launches made while it lives
+struct SweepScopeName {
+  explicit SweepScopeName(const char* n) { tl_sweep_scope = n; }
+  ~SweepScopeName() { tl_sweep_scope = nullptr; }
+};
+
+// One RHS with the fused sweep kernels: forward transformation (also clears bw), the near-field
+// leaves and the coupling blocks, backward transformation.  overlap = false: both sweeps in one
+// launch after the forward transformation.  overlap = true: after the forward transformation the
+// coupling blocks (with their gather and the backward transformation) run on the side stream and
+// the near-field leaves on `st`, so the two sweeps start together and the tail of the far chain
+// runs beside the near-field leaves.  (The forward transformation stays ahead of the fork: beside
+// the near-field launch its few workgroups wait for free slots, measured 9 -> 167 us.)  The two
+// sweeps write disjoint vectors (ys; bw, then yf), so both forms give the same bits.  false if the
+// sweep kernel is not instantiated for this operator.
+static bool fused_product(bmi_h2* h, const Work& w, const double* xs, double* ys, double* yf, bool reduced,
+                          hipStream_t st, bool overlap) {
+  const bmi_space* s = h->s;
+  const CouplingOperand co = coupling_operand(h, reduced);
+  const int64_t N = s->nsuper;
+  const bool use_sym = h->sym && h->sym_fused && !co.fp32;
+  const SymList* sl = use_sym ? &h->sl : nullptr;
+  const SymList* nsl = use_sym && h->sym_fused == 2 ? &h->nsl : nullptr;
+  auto sweep = [&](int ncl, int ntgt, const SymList* csl, const SymList* nl, hipStream_t ss) {
+    return launch_sweep_fused(h->cplx, co.fp32, h->fused_leaf_variant, h->fused_cpl_variant, ncl, h->dd,
+                              h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, ntgt, h->cd, h->d_tgt.ptr,
+                              h->d_crp.ptr, h->d_csrc.ptr, co.F, h->d_lvl_off.ptr, w.fw[0], w.bw, ss, co.Fc,
+                              co.ncoarse, false, co.ld, csl, nl);
+  };
+  {
+    TimedScope ts("matvec_forward", st);
+    launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
+                   w.fw[0], h->d_lvl_off.ptr, st, w.bw);
+  }
+  hipStream_t fs = st;
+  if (overlap) {
+    if (!h->side_stream) BMI_CHECK_HIP(hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking));
+    if (!h->ev_fork) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    if (!h->ev_join) BMI_CHECK_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    BMI_CHECK_HIP(hipEventRecord(h->ev_fork, st));
+    BMI_CHECK_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
+    fs = h->side_stream;
+  }
+  bool ok;
+  if (!overlap) {
+    ok = sweep(h->ncl_leaf, h->ntgt, sl, nsl, st);
+  } else {
+    const SymList none{};  // no coupling targets in the near-field launch
+    bool okc, okn;
+    {
+      SweepScopeName sn("matvec_sweep_far");
+      okc = sweep(0, h->ntgt, sl, nullptr, fs);
+    }
+    {
+      SweepScopeName sn("matvec_sweep_near");
+      okn = sweep(h->ncl_leaf, 0, sl ? &none : nullptr, nsl, st);
+    }
+    ok = okc && okn;
+  }
+  if (ok) {
+    TimedScope ts("matvec_backward", fs);
+    launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, w.bw,
+                    h->d_lvl_off.ptr, yf, N, 0, fs);
+  }
+  if (overlap) {
+    BMI_CHECK_HIP(hipEventRecord(h->ev_join, fs));
+    BMI_CHECK_HIP(hipStreamWaitEvent(st, h->ev_join, 0));
+  }
+  return ok;
+}
+
 static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) {
   if (h->leaf_variant >= 0 && h->cpl_variant >= 0) return;
   hipEvent_t a, b;
@@ -406,6 +495,21 @@ static void autotune(bmi_h2* h, const double* xs, bool reduced, hipStream_t st) 
       h->fused_leaf_variant = pl;
       h->fused_cpl_variant = pc;
       h->sym_fused = ps;
+      // whole products: both sweeps in one launch, or the near-field leaves beside the far chain
+      h->overlap = 0;
+      if (pl >= 0 && h->overlap_mode != 0) {
+        const Work w = handle_work(h);
+        if (h->overlap_mode > 0) {
+          h->overlap = 1;
+        } else {
+          float tb = 0.f;
+          for (int o = 0; o < 2; ++o) {
+            bool ok = true;
+            const float t = time_it([&] { ok = fused_product(h, w, xs, w.ys, w.yf, reduced, st, o == 1); });
+            if (ok && (o == 0 || t < tb)) tb = t, h->overlap = o;
+          }
+        }
+      }
     }
     if (h->red1) {
       // reduced single-RHS sweep: coupling recomputed on the fly (variants 0, 1; real kernels:
@@ -476,28 +580,9 @@ static void apply_super(bmi_h2* h, const Work& w, int nrhs, const double* xs, do
       return;
     }
   }
-  if (far && nrhs == 1 && h->vd == 1 && h->fused_leaf_variant >= 0 && h->fused_cpl_variant >= 0) {
-    // one RHS: forward transformation (also clears bw), both sweeps in one launch, backward
-    const bmi_space* s = h->s;
-    const CouplingOperand co = coupling_operand(h, reduced);
-    const int64_t N = s->nsuper;
-    {
-      TimedScope ts("matvec_forward", st);
-      launch_forward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, xs, N, 0,
-                     w.fw[0], h->d_lvl_off.ptr, st, w.bw);
-    }
-    const bool use_sym = h->sym && h->sym_fused && !co.fp32;
-    if (launch_sweep_fused(h->cplx, co.fp32, h->fused_leaf_variant, h->fused_cpl_variant, h->ncl_leaf, h->dd,
-                           h->d_drp.ptr, h->d_dsrc.ptr, h->d_dense.ptr, xs, ys, h->ntgt, h->cd, h->d_tgt.ptr,
-                           h->d_crp.ptr, h->d_csrc.ptr, co.F, h->d_lvl_off.ptr, w.fw[0], w.bw, st, co.Fc,
-                           co.ncoarse, false, co.ld, use_sym ? &h->sl : nullptr,
-                           use_sym && h->sym_fused == 2 ? &h->nsl : nullptr)) {
-      TimedScope ts("matvec_backward", st);
-      launch_backward(h->cplx, s->npatch, h->L, h->cd, h->np2, h->dd, 1, h->d_mom[0].ptr, h->d_T.ptr, w.bw,
-                      h->d_lvl_off.ptr, yf, N, 0, st);
-      return;
-    }
-  }
+  if (far && nrhs == 1 && h->vd == 1 && h->fused_leaf_variant >= 0 && h->fused_cpl_variant >= 0 &&
+      fused_product(h, w, xs, ys, yf, reduced, st, w.own && h->overlap == 1))
+    return;
   if (far && !w.own) {
     // one stream: the two chains write disjoint vectors, so the order gives the same bits
     apply_far(h, w, nrhs, xs, yf, reduced, st);
@@ -1737,6 +1822,22 @@ bmi_status bmi_h2_set_sweep(bmi_h2* h, int mode) {
   h->fused_leaf_variant = h->fused_cpl_variant = -1;
   h->fly_variant = -1;
   h->sym_fused = 0;
+  h->overlap = 0;
+  h->drop_graphs();
+  BMI_API_END
+}
+
+bmi_status bmi_h2_set_overlap(bmi_h2* h, int mode) {
+  BMI_API_BEGIN
+  BMI_REQUIRE(h, BMI_INVALID_ARG, "null handle");
+  BMI_REQUIRE(mode >= -1 && mode <= 1, BMI_INVALID_ARG, "overlap mode must be -1, 0 or 1");
+  h->overlap_mode = mode;
+  // choose again at the next product
+  h->leaf_variant = h->cpl_variant = -1;
+  h->fused_leaf_variant = h->fused_cpl_variant = -1;
+  h->fly_variant = -1;
+  h->sym_fused = 0;
+  h->overlap = 0;
   h->drop_graphs();
   BMI_API_END
 }
@@ -1747,6 +1848,7 @@ bmi_status bmi_timing_enable(int enable) {
 }
 bmi_status bmi_timing_reset(void) {
   std::lock_guard<std::mutex> g(bmi::g_tmu);
+  bmi::resolve_pending(false);
   bmi::g_times.clear();
   return BMI_OK;
 }
@@ -1754,6 +1856,7 @@ bmi_status bmi_timing_get(const char* name, double* total_ms, int64_t* launches)
   BMI_API_BEGIN
   BMI_REQUIRE(name && total_ms && launches, BMI_INVALID_ARG, "null argument");
   std::lock_guard<std::mutex> g(bmi::g_tmu);
+  bmi::resolve_pending(true);
   auto it = bmi::g_times.find(name);
   *total_ms = it == bmi::g_times.end() ? 0.0 : it->second.first;
   *launches = it == bmi::g_times.end() ? 0 : it->second.second;

@@ -96,6 +96,8 @@ struct SymList {
   const int64_t* lvl_off = nullptr;
 };
 
+// timing-scope name of the next single-RHS sweep launches on this thread (null: the sweep's own)
+extern thread_local const char* tl_sweep_scope;
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
                         const int* dsrc, const void* dF, const double* xs, double* ys, int ntgt, int cd,
                         const int2* tgt, const int* crp, const int* csrc, const void* cF, const int64_t* lvl_off,

@@ -3009,7 +3009,7 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
   const dim3 grid(static_cast<unsigned>(nbn + nbc));
   bool launched = false;
   {
-    TimedScope ts("matvec_sweep_upper", st);
+    TimedScope ts(tl_sweep_scope ? tl_sweep_scope : "matvec_sweep_upper", st);
 #define BMI_FUSED_S(CC, DDD, FTT, CDD, UN, UC, NFTT)                                                                 \
   if (!launched && cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&               \
       near32 == std::is_same<NFTT, float>::value && leaf_variant == UN - 1 && cpl_variant == UC - 1) {              \
@@ -3062,6 +3062,8 @@ static bool launch_sweep_fused_sym(bool cplx, bool fp32, int leaf_variant, int c
   return true;
 }
 
+thread_local const char* tl_sweep_scope = nullptr;
+
 bool sym_sweep_supported(bool cplx, int dd, int cd) { return dd == 36 && ((cplx && cd == 25) || (!cplx && cd == 16)); }
 
 bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant, int ncl, int dd, const int* drp,
@@ -3081,7 +3083,7 @@ bool launch_sweep_fused(bool cplx, bool fp32, int leaf_variant, int cpl_variant,
   const CplArgs ca{ntgt, 1, tgt, crp, csrc, cF, static_cast<int64_t>(ld) * cd, 0, lvl_off, fw, bw, 0, Fc, ncoarse, ld};
   const int nbn = (ncl + 3) / 4, nbc = (ntgt + 3) / 4;
   const dim3 grid(static_cast<unsigned>(nbn + nbc));
-  TimedScope ts("matvec_sweep", st);
+  TimedScope ts(tl_sweep_scope ? tl_sweep_scope : "matvec_sweep", st);
 #define BMI_FUSED(CC, DDD, FTT, CDD, UN, UC, NFTT)                                                              \
   if (cplx == CC && dd == DDD && fp32 == std::is_same<FTT, float>::value && cd == CDD &&                        \
       near32 == std::is_same<NFTT, float>::value && leaf_variant == UN - 1 && cpl_variant == UC - 1) {          \

@@ -36,6 +36,9 @@ void convert_to_fp32(const double* src, float* dst, size_t n, hipStream_t st) {
 // over k vectors costs one round of loads, not k / 4), then one wave per (j, c) adds the chunk
 // sums in a fixed order (deterministic: the chunk count depends on the sizes only).
 constexpr int DOT_CHUNKS = 64;
+// Hessenberg column and rotations the Givens kernel stages in LDS (48 KB: 1,000 complex steps);
+// later steps of longer cycles read them from memory
+constexpr int GIVENS_STAGE_DOUBLES = 6144;
 constexpr int DOT_JB = 4;
 // chunks per column: 128 for one or two columns; with more columns about 512 workgroups per group
 // of four vectors, each chunk at least 256 rows (a workgroup's reduction then serves several rows
@@ -146,6 +149,52 @@ __global__ void k_subtract(const double* __restrict__ V, int64_t vstride, int nv
   if (C) W[S * gid + 1] = wi;
 }
 
+// W[i, c] -= sum_{j < nv} V_j[i, c] * h[j, c] (as k_subtract), then the chunk sums of
+// ||W[:, c]||^2 (as k_dots_part of W with itself): one pass over W instead of two launches
+template <bool C>
+__global__ __launch_bounds__(256) void k_subtract_norm(const double* __restrict__ V, int64_t vstride, int nv,
+                                                        const double* __restrict__ h, double* W, int64_t n, int nrhs,
+                                                        int nchunk, double* __restrict__ part) {
+  constexpr int S = C ? 2 : 1;
+  __shared__ double red[4];
+  const int chunk = blockIdx.x, c = blockIdx.y;
+  const int64_t per = (n + nchunk - 1) / nchunk;
+  const int64_t r0 = chunk * per, r1 = r0 + per < n ? r0 + per : n;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double ar = 0.0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+    const int64_t gid = static_cast<int64_t>(c) * n + i;
+    double wr = W[S * gid], wi = C ? W[S * gid + 1] : 0.0;
+#pragma unroll 4
+    for (int j = 0; j < nv; ++j) {
+      const double* v = V + j * vstride + S * gid;
+      const double hr = h[S * (j * nrhs + c)];
+      if (C) {
+        const double hi = h[S * (j * nrhs + c) + 1];
+        wr -= v[0] * hr - v[1] * hi;
+        wi -= v[0] * hi + v[1] * hr;
+      } else {
+        wr -= v[0] * hr;
+      }
+    }
+    W[S * gid] = wr;
+    if (C) W[S * gid + 1] = wi;
+    if (C)
+      ar += wr * wr + wi * wi;
+    else
+      ar += wr * wr;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) ar += __shfl_xor(ar, off, 64);
+  if (lane == 0) red[wave] = ar;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t o = static_cast<int64_t>(c) * nchunk + chunk;
+    part[S * o] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (C) part[S * o + 1] = 0.0;
+  }
+}
+
 // Y[i, c] = a_c * X[i, c] + b_c * Y[i, c] with per-column complex coefficients (or real)
 template <bool C>
 __global__ void k_axpby(const double* __restrict__ a, const double* __restrict__ X, const double* __restrict__ b,
@@ -235,6 +284,24 @@ struct Vecs {
       hipLaunchKernelGGL(k_subtract<false>, dim3(nb(n * nrhs)), dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs);
     BMI_CHECK_HIP(hipGetLastError());
   }
+  // W -= V h, then the chunk sums of ||W||^2 per column into `part`; returns the chunk count
+  int subtract_norm(const double* V, int nv, const double* d_h, double* W) {
+    const int nchunk = dot_chunks(n, nrhs);
+    const size_t need = static_cast<size_t>(S()) * nrhs * nchunk;
+    if (part.n < need) {
+      BMI_CHECK_HIP(hipStreamSynchronize(st));  // earlier launches may still use the old buffer
+      part.alloc(std::max(need, std::max(2 * part.n, static_cast<size_t>(S()) * 32 * nrhs * nchunk)));
+    }
+    const dim3 grid(static_cast<unsigned>(nchunk), static_cast<unsigned>(nrhs));
+    if (C)
+      hipLaunchKernelGGL(k_subtract_norm<true>, grid, dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs, nchunk,
+                         part.ptr);
+    else
+      hipLaunchKernelGGL(k_subtract_norm<false>, grid, dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs, nchunk,
+                         part.ptr);
+    BMI_CHECK_HIP(hipGetLastError());
+    return nchunk;
+  }
   void axpby(const double* d_a, const double* X, const double* d_b, double* Y) const {
     if (C)
       hipLaunchKernelGGL(k_axpby<true>, dim3(nb(n * nrhs)), dim3(256), 0, st, d_a, X, d_b, Y, n, nrhs);
@@ -293,36 +360,64 @@ __global__ void k_gm_cycle(const double* __restrict__ nrm, GmresState s, int m, 
 
 // Arnoldi step k: H(:, k) = h1 + h2 (two Gram-Schmidt passes), H(k+1, k) = ||w||, previous
 // rotations, a new rotation, g and the residual estimate |g(k+1)| / ||b||; coefficients 1/||w||
-// for the next basis vector
+// for the next basis vector.  One wave per column: ||w||^2 is the sum of the column's chunk sums
+// in `part` (lane l adds chunks l, l + 64, ... in order, then a fixed shuffle tree, as
+// k_dots_final), the lanes stage h1 + h2 and the earlier rotations in LDS together, and lane 0
+// applies the rotations in order (each rotation needs the previous one's result).
 template <bool C>
-__global__ void k_gm_givens(const double* __restrict__ h1, const double* __restrict__ h2, const double* __restrict__ nrm,
-                            GmresState s, int k, int m, int nrhs, double tol) {
+__global__ __launch_bounds__(64) void k_gm_givens(const double* __restrict__ h1, const double* __restrict__ h2,
+                                                  const double* __restrict__ part, int nchunk, GmresState s, int k,
+                                                  int m, int nrhs, double tol) {
   constexpr int S = C ? 2 : 1;
-  const int c = threadIdx.x + blockIdx.x * blockDim.x;
-  if (c >= nrhs) return;
-  const double hn = sqrt(fmax(nrm[S * c], 0.0));
+  extern __shared__ double stage[];  // (k + 1) column entries, k cs, k sn (S doubles each)
+  const bool staged = S * (3 * k + 1) <= GIVENS_STAGE_DOUBLES;
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
+  double nr2 = 0.0;
+  for (int q = lane; q < nchunk; q += 64) nr2 += part[S * (static_cast<int64_t>(c) * nchunk + q)];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) nr2 += __shfl_xor(nr2, off, 64);
+  const bool act = s.active[c] != 0;
+  double* hst = stage;
+  double* cst = stage + S * (k + 1);
+  double* snt = cst + S * k;
+  const double* __restrict__ cs0 = s.cs + static_cast<int64_t>(c) * m * S;
+  const double* __restrict__ sn0 = s.sn + static_cast<int64_t>(c) * m * S;
+  if (act && staged) {
+    for (int j = lane; j <= k; j += 64) {
+      const int64_t q = S * (static_cast<int64_t>(j) * nrhs + c);
+      hst[S * j] = h1[q] + h2[q];
+      if (C) hst[S * j + 1] = h1[q + 1] + h2[q + 1];
+    }
+    for (int e = lane; e < S * k; e += 64) cst[e] = cs0[e], snt[e] = sn0[e];
+  }
+  __syncthreads();
+  if (lane != 0) return;
+  const double hn = sqrt(fmax(nr2, 0.0));
   s.coef[S * c] = hn > 0 ? 1.0 / hn : 0.0;
   s.coef[S * (nrhs + c)] = 0.0;
   if (C) s.coef[S * c + 1] = 0.0, s.coef[S * (nrhs + c) + 1] = 0.0;
-  if (!s.active[c]) return;
+  if (!act) return;
   double* __restrict__ col = s.H + (static_cast<int64_t>(c) * (m + 1) * m + static_cast<int64_t>(k) * (m + 1)) * S;
-  const double* __restrict__ cs0 = s.cs + static_cast<int64_t>(c) * m * S;
-  const double* __restrict__ sn0 = s.sn + static_cast<int64_t>(c) * m * S;
-  // new Hessenberg column h = h1 + h2 (both Gram-Schmidt passes) with the earlier rotations
-  // applied in order; entry j + 1 is carried in registers from rotation j to rotation j + 1 and
-  // the loads of four rotations are in flight together
+  // new Hessenberg column h = h1 + h2 with the earlier rotations applied in order; entry j + 1 is
+  // carried from rotation j to rotation j + 1
   auto hcol = [&](int j, double& r, double& i) {
-    const int64_t q = S * (static_cast<int64_t>(j) * nrhs + c);
-    r = h1[q] + h2[q];
-    i = C ? h1[q + 1] + h2[q + 1] : 0.0;
+    if (staged) {
+      r = hst[S * j], i = C ? hst[S * j + 1] : 0.0;
+    } else {
+      const int64_t q = S * (static_cast<int64_t>(j) * nrhs + c);
+      r = h1[q] + h2[q];
+      i = C ? h1[q + 1] + h2[q + 1] : 0.0;
+    }
   };
+  const double* __restrict__ csr = staged ? cst : cs0;
+  const double* __restrict__ snr = staged ? snt : sn0;
   double t1r, t1i;
   hcol(0, t1r, t1i);
-#pragma unroll 4
   for (int j = 0; j < k; ++j) {
     double t2r, t2i;
     hcol(j + 1, t2r, t2i);
-    const double cr = cs0[S * j], ci = C ? cs0[S * j + 1] : 0.0, sr = sn0[S * j], si = C ? sn0[S * j + 1] : 0.0;
+    const double cr = csr[S * j], ci = C ? csr[S * j + 1] : 0.0, sr = snr[S * j], si = C ? snr[S * j + 1] : 0.0;
     double ar, ai, br, bi;
     cmul(cr, -ci, t1r, t1i, ar, ai);  // conj(cs) t1 + conj(sn) t2
     cmul(sr, -si, t2r, t2i, br, bi);
@@ -552,14 +647,15 @@ class GmresRun {
       vs_.dots_async(V_.ptr, k_ + 1, W_.ptr, d_h1_.ptr);
       vs_.subtract(V_.ptr, k_ + 1, d_h1_.ptr, W_.ptr);
       vs_.dots_async(V_.ptr, k_ + 1, W_.ptr, d_h2_.ptr);
-      vs_.subtract(V_.ptr, k_ + 1, d_h2_.ptr, W_.ptr);
-      vs_.dots_async(W_.ptr, 1, W_.ptr, d_nrm_.ptr);
+      const int nchunk = vs_.subtract_norm(V_.ptr, k_ + 1, d_h2_.ptr, W_.ptr);
+      const int stage = S_ * (3 * k_ + 1);
+      const size_t lds = stage <= GIVENS_STAGE_DOUBLES ? stage * sizeof(double) : 0;
       if (C_)
-        hipLaunchKernelGGL(k_gm_givens<true>, dim3(sb_), dim3(64), 0, st_, d_h1_.ptr, d_h2_.ptr, d_nrm_.ptr, gs_, k_,
-                           m_, nrhs_, tol_);
+        hipLaunchKernelGGL(k_gm_givens<true>, dim3(nrhs_), dim3(64), lds, st_, d_h1_.ptr, d_h2_.ptr, vs_.part.ptr,
+                           nchunk, gs_, k_, m_, nrhs_, tol_);
       else
-        hipLaunchKernelGGL(k_gm_givens<false>, dim3(sb_), dim3(64), 0, st_, d_h1_.ptr, d_h2_.ptr, d_nrm_.ptr, gs_, k_,
-                           m_, nrhs_, tol_);
+        hipLaunchKernelGGL(k_gm_givens<false>, dim3(nrhs_), dim3(64), lds, st_, d_h1_.ptr, d_h2_.ptr, vs_.part.ptr,
+                           nchunk, gs_, k_, m_, nrhs_, tol_);
       BMI_CHECK_HIP(hipGetLastError());
       vs_.axpby(d_coef_.ptr, W_.ptr, d_coef_.ptr + S_ * nrhs_, V_.ptr + vs_.block() * (k_ + 1));
       ++k_;

@@ -185,6 +185,13 @@ bmi_status bmi_h2_use_graphs(bmi_h2* h, int enable);
    near-field leaves as well (BMI_UNSUPPORTED where a form is not available).  No counterpart in
    the reference (its product is CPU code); exposed so callers can compare the forms. */
 bmi_status bmi_h2_set_sweep(bmi_h2* h, int mode);
+/* Single-RHS products on the handle's workspace: -1 (default) lets the first product time both
+   forms and keep the faster, 0 runs the near-field leaves and coupling blocks in one launch after
+   the forward transformation, 1 runs the near-field leaves on the caller's stream while the
+   coupling blocks and the backward transformation run on a side stream (both forms give the same
+   bits).  No counterpart in the reference; exposed so callers can
+   compare the forms. */
+bmi_status bmi_h2_set_overlap(bmi_h2* h, int mode);
 /* wait for the library's work on `stream` (NULL: the default stream) */
 bmi_status bmi_synchronize(void* stream);
 /* Allocation-free form of bmi_h2_matvec for callers that cannot allocate device memory (code
@@ -334,7 +341,9 @@ bmi_status bmi_server_stats(bmi_server* srv, int64_t out[6]);
 bmi_status bmi_server_destroy(bmi_server* srv);
 
 /* ------------------------------------------------------------------ measurement */
-/* per-kernel timing with HIP events recorded on the stream each kernel is launched on */
+/* per-kernel timing with HIP events recorded on the stream each kernel is launched on; timed
+   launches do not wait for their events (they queue as untimed ones do), bmi_timing_get waits for
+   and reads the events recorded so far */
 bmi_status bmi_timing_enable(int enable);
 bmi_status bmi_timing_reset(void);
 /* total milliseconds and launch count of a kernel family ("nf_duffy", "nf_regular",

@@ -75,7 +75,8 @@ def test_gpu_h2_upper_block_sweep(gpu, kind, kappa, M, P, np_):
     """Single-RHS products that read only the upper coupling blocks, and also only the upper
     near-field leaves (the transposed products formed from them, partials gathered per target),
     equal the oracle like the sweep over every block, for the FP64 and the mixed operator; each
-    form is forced in turn."""
+    form is forced in turn, with both sweeps in one launch and with the near-field leaves running
+    beside the far chain on a side stream."""
     os_, ps, op, H = make(kind, kappa, M, P, np_)
     oh = O.H2(os_, kind, kappa, np_)
     rng = np.random.default_rng(11)
@@ -87,11 +88,18 @@ def test_gpu_h2_upper_block_sweep(gpu, kind, kappa, M, P, np_):
     ys = {}
     for mode in ("full", "upper", "upper_near"):
         H.set_sweep(mode)
-        ys[mode] = H @ x
-        print(mode, "rel", rel(ys[mode], yo))
-        assert rel(ys[mode], yo) <= TOL
-    assert rel(ys["upper"], ys["full"]) <= 1e-14
-    assert rel(ys["upper_near"], ys["full"]) <= 1e-14
+        for ov in ("fused", "overlap"):
+            # near-field leaves in the same launch as the coupling blocks, or beside the far chain
+            H.set_overlap(ov)
+            ys[mode, ov] = H @ x
+            print(mode, ov, "rel", rel(ys[mode, ov], yo))
+            assert rel(ys[mode, ov], yo) <= TOL
+        # the two forms run the same block code on disjoint output vectors
+        print(mode, "overlap == fused bit for bit:", np.array_equal(ys[mode, "overlap"], ys[mode, "fused"]))
+        assert rel(ys[mode, "overlap"], ys[mode, "fused"]) <= 1e-15
+        H.set_overlap(None)
+    assert rel(ys["upper", "fused"], ys["full", "fused"]) <= 1e-14
+    assert rel(ys["upper_near", "fused"], ys["full", "fused"]) <= 1e-14
     H.set_sweep(None)
     assert rel(H @ x, yo) <= TOL
     Hm = B.H2Matrix(ps, op, n_cheb=np_, precision=B.MIXED)

@@ -63,6 +63,7 @@ def test_invalid_arguments_return_status_without_gpu():
     x = (ctypes.c_double * 4)()
     assert lib.bmi_h2_matvec(None, 1, x, 1, x, 1, one, None, B.MEM_HOST, None) == B.INVALID_ARG
     assert lib.bmi_h2_set_sweep(None, 0) == B.INVALID_ARG
+    assert lib.bmi_h2_set_overlap(None, 0) == B.INVALID_ARG
     assert lib.bmi_h2_use_graphs(None, 1) == B.INVALID_ARG
     t = ctypes.c_void_p()
     assert lib.bmi_gmres_start(None, 1, x, x, 1e-8, 30, 10, 1, None, ctypes.byref(t)) == B.INVALID_ARG
