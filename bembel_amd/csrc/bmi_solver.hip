@@ -39,6 +39,8 @@ constexpr int DOT_CHUNKS = 64;
 // Hessenberg column and rotations the Givens kernel stages in LDS (48 KB: 1,000 complex steps);
 // later steps of longer cycles read them from memory
 constexpr int GIVENS_STAGE_DOUBLES = 6144;
+// Gram-Schmidt coefficients the subtraction kernel holds in LDS (48 KB: 3,072 complex vectors)
+constexpr int GS_LDS_DOUBLES = 6144;
 constexpr int DOT_JB = 4;
 // chunks per column: 128 for one or two columns; with more columns about 512 workgroups per group
 // of four vectors, each chunk at least 256 rows (a workgroup's reduction then serves several rows
@@ -149,28 +151,56 @@ __global__ void k_subtract(const double* __restrict__ V, int64_t vstride, int nv
   if (C) W[S * gid + 1] = wi;
 }
 
-// W[i, c] -= sum_{j < nv} V_j[i, c] * h[j, c] (as k_subtract), then the chunk sums of
-// ||W[:, c]||^2 (as k_dots_part of W with itself): one pass over W instead of two launches
-template <bool C>
-__global__ __launch_bounds__(256) void k_subtract_norm(const double* __restrict__ V, int64_t vstride, int nv,
-                                                        const double* __restrict__ h, double* W, int64_t n, int nrhs,
-                                                        int nchunk, double* __restrict__ part) {
+// One Gram-Schmidt subtraction of an Arnoldi step for the rows of one chunk of column c:
+// h[j, c] = the sum of the chunk sums of <V_j, W> in `hpart` (lane l adds chunks l, l + 64, ... in
+// order, then a fixed shuffle tree, as k_dots_final; the chunk-0 workgroup writes h), then
+// W -= sum_{j < nv} V_j h[j, c] (as k_subtract) and, with NORM, the chunk sums of ||W[:, c]||^2
+// (as k_dots_part of W with itself) into `npart`: one launch instead of two or three, and W read
+// once.  `hpart` and `npart` must not overlap (other workgroups still read hpart).
+template <bool C, bool NORM>
+__global__ __launch_bounds__(256) void k_gs_subtract(const double* __restrict__ V, int64_t vstride, int nv,
+                                                      const double* __restrict__ hpart, double* __restrict__ h,
+                                                      double* W, int64_t n, int nrhs, int nchunk,
+                                                      double* __restrict__ npart) {
   constexpr int S = C ? 2 : 1;
+  extern __shared__ double hs[];  // nv coefficients (S doubles each)
   __shared__ double red[4];
   const int chunk = blockIdx.x, c = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = wave; j < nv; j += 4) {
+    const int64_t q = static_cast<int64_t>(j) * nrhs + c;
+    double ar = 0.0, ai = 0.0;
+    for (int k = lane; k < nchunk; k += 64) {
+      ar += hpart[S * (q * nchunk + k)];
+      if (C) ai += hpart[S * (q * nchunk + k) + 1];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      ar += __shfl_xor(ar, off, 64);
+      if (C) ai += __shfl_xor(ai, off, 64);
+    }
+    if (lane == 0) {
+      hs[S * j] = ar;
+      if (C) hs[S * j + 1] = ai;
+      if (chunk == 0) {
+        h[S * q] = ar;
+        if (C) h[S * q + 1] = ai;
+      }
+    }
+  }
+  __syncthreads();
   const int64_t per = (n + nchunk - 1) / nchunk;
   const int64_t r0 = chunk * per, r1 = r0 + per < n ? r0 + per : n;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double ar = 0.0;
+  double nr = 0.0;
   for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const int64_t gid = static_cast<int64_t>(c) * n + i;
     double wr = W[S * gid], wi = C ? W[S * gid + 1] : 0.0;
 #pragma unroll 4
     for (int j = 0; j < nv; ++j) {
       const double* v = V + j * vstride + S * gid;
-      const double hr = h[S * (j * nrhs + c)];
+      const double hr = hs[S * j];
       if (C) {
-        const double hi = h[S * (j * nrhs + c) + 1];
+        const double hi = hs[S * j + 1];
         wr -= v[0] * hr - v[1] * hi;
         wi -= v[0] * hi + v[1] * hr;
       } else {
@@ -179,19 +209,23 @@ __global__ __launch_bounds__(256) void k_subtract_norm(const double* __restrict_
     }
     W[S * gid] = wr;
     if (C) W[S * gid + 1] = wi;
-    if (C)
-      ar += wr * wr + wi * wi;
-    else
-      ar += wr * wr;
+    if (NORM) {
+      if (C)
+        nr += wr * wr + wi * wi;
+      else
+        nr += wr * wr;
+    }
   }
+  if constexpr (NORM) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) ar += __shfl_xor(ar, off, 64);
-  if (lane == 0) red[wave] = ar;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int64_t o = static_cast<int64_t>(c) * nchunk + chunk;
-    part[S * o] = (red[0] + red[1]) + (red[2] + red[3]);
-    if (C) part[S * o + 1] = 0.0;
+    for (int off = 32; off > 0; off >>= 1) nr += __shfl_xor(nr, off, 64);
+    if (lane == 0) red[wave] = nr;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int64_t o = static_cast<int64_t>(c) * nchunk + chunk;
+      npart[S * o] = (red[0] + red[1]) + (red[2] + red[3]);
+      if (C) npart[S * o + 1] = 0.0;
+    }
   }
 }
 
@@ -247,7 +281,8 @@ struct Vecs {
   int64_t n;
   int nrhs;
   hipStream_t st;
-  DevBuf<double> part;  // chunk sums of the inner products
+  DevBuf<double> part;   // chunk sums of the inner products
+  DevBuf<double> npart;  // chunk sums of ||w||^2 of an Arnoldi step (read by the Givens kernel)
   int S() const { return C ? 2 : 1; }
   int64_t block() const { return static_cast<int64_t>(S()) * n * nrhs; }
   void dots(const double* V, int nv, const double* W, double* d_out, std::vector<double>& host) {
@@ -284,21 +319,47 @@ struct Vecs {
       hipLaunchKernelGGL(k_subtract<false>, dim3(nb(n * nrhs)), dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs);
     BMI_CHECK_HIP(hipGetLastError());
   }
-  // W -= V h, then the chunk sums of ||W||^2 per column into `part`; returns the chunk count
-  int subtract_norm(const double* V, int nv, const double* d_h, double* W) {
+  // chunk sums of <V_j, W> into `part` (k_dots_part alone); returns the chunk count
+  int dots_part_async(const double* V, int nv, const double* W) {
     const int nchunk = dot_chunks(n, nrhs);
-    const size_t need = static_cast<size_t>(S()) * nrhs * nchunk;
+    const size_t need = static_cast<size_t>(S()) * nv * nrhs * nchunk;
     if (part.n < need) {
       BMI_CHECK_HIP(hipStreamSynchronize(st));  // earlier launches may still use the old buffer
       part.alloc(std::max(need, std::max(2 * part.n, static_cast<size_t>(S()) * 32 * nrhs * nchunk)));
     }
-    const dim3 grid(static_cast<unsigned>(nchunk), static_cast<unsigned>(nrhs));
+    const dim3 grid(static_cast<unsigned>(nchunk), static_cast<unsigned>(nrhs),
+                    static_cast<unsigned>((nv + DOT_JB - 1) / DOT_JB));
     if (C)
-      hipLaunchKernelGGL(k_subtract_norm<true>, grid, dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs, nchunk,
-                         part.ptr);
+      hipLaunchKernelGGL(k_dots_part<true>, grid, dim3(256), 0, st, V, block(), nv, W, n, nrhs, nchunk, part.ptr);
     else
-      hipLaunchKernelGGL(k_subtract_norm<false>, grid, dim3(256), 0, st, V, block(), nv, d_h, W, n, nrhs, nchunk,
-                         part.ptr);
+      hipLaunchKernelGGL(k_dots_part<false>, grid, dim3(256), 0, st, V, block(), nv, W, n, nrhs, nchunk, part.ptr);
+    BMI_CHECK_HIP(hipGetLastError());
+    return nchunk;
+  }
+  // h = the inner products whose chunk sums dots_part_async left in `part`, then W -= V h; with
+  // norm, the chunk sums of ||W||^2 per column into `npart` (returned chunk count, read by the
+  // Givens kernel)
+  int gs_subtract(const double* V, int nv, double* d_h, double* W, bool norm) {
+    const int nchunk = dot_chunks(n, nrhs);
+    const size_t need = static_cast<size_t>(S()) * nrhs * nchunk;
+    if (norm && npart.n < need) {
+      BMI_CHECK_HIP(hipStreamSynchronize(st));
+      npart.alloc(need);
+    }
+    const dim3 grid(static_cast<unsigned>(nchunk), static_cast<unsigned>(nrhs));
+    const size_t lds = static_cast<size_t>(S()) * nv * sizeof(double);
+    if (C && norm)
+      hipLaunchKernelGGL((k_gs_subtract<true, true>), grid, dim3(256), lds, st, V, block(), nv, part.ptr, d_h, W, n,
+                         nrhs, nchunk, npart.ptr);
+    else if (C)
+      hipLaunchKernelGGL((k_gs_subtract<true, false>), grid, dim3(256), lds, st, V, block(), nv, part.ptr, d_h, W,
+                         n, nrhs, nchunk, nullptr);
+    else if (norm)
+      hipLaunchKernelGGL((k_gs_subtract<false, true>), grid, dim3(256), lds, st, V, block(), nv, part.ptr, d_h, W,
+                         n, nrhs, nchunk, npart.ptr);
+    else
+      hipLaunchKernelGGL((k_gs_subtract<false, false>), grid, dim3(256), lds, st, V, block(), nv, part.ptr, d_h, W,
+                         n, nrhs, nchunk, nullptr);
     BMI_CHECK_HIP(hipGetLastError());
     return nchunk;
   }
@@ -638,38 +699,66 @@ class GmresRun {
     request_flags();
     phase_ = AFTER_CYCLE;
   }
-  // Arnoldi steps from k_ up to the next check; then either wait for the flags or end the cycle
+  // Arnoldi steps up to the next check, then the flags of that check and, queued behind them, the
+  // steps up to the check after (so the device keeps working while the host waits for the
+  // flags); or the end of the cycle.  A queued batch that turns out to follow convergence changes
+  // nothing: the Givens kernel leaves inactive columns as they are (kc, H, g, the step count),
+  // and the new basis vectors are unit vectors or zero, weighted by zero at the end of the cycle.
   void steps() {
+    if (!ahead_) batch();
+    ahead_ = false;
+    if (k_ < m_ && total_ < maxit_) {
+      request_flags();
+      phase_ = AFTER_STEPS;
+      if (k_ < m_ && total_ < maxit_) {
+        batch();
+        ahead_ = true;
+      }
+    } else {
+      end_cycle();
+    }
+  }
+  // Arnoldi steps from k_ up to the next multiple of CHECK (or the end of the cycle)
+  void batch() {
     const double one[2] = {1.0, 0.0};
     do {
       double* Vk = V_.ptr + vs_.block() * k_;
       h2_apply_device(h_, nrhs_, Vk, n_, W_.ptr, n_, one, nullptr, st_, reduced_);
-      vs_.dots_async(V_.ptr, k_ + 1, W_.ptr, d_h1_.ptr);
-      vs_.subtract(V_.ptr, k_ + 1, d_h1_.ptr, W_.ptr);
-      vs_.dots_async(V_.ptr, k_ + 1, W_.ptr, d_h2_.ptr);
-      const int nchunk = vs_.subtract_norm(V_.ptr, k_ + 1, d_h2_.ptr, W_.ptr);
+      int nchunk;
+      const double* npart;
+      if (S_ * (k_ + 1) <= GS_LDS_DOUBLES) {
+        // the subtractions reduce the inner products themselves (coefficients in LDS)
+        vs_.dots_part_async(V_.ptr, k_ + 1, W_.ptr);
+        vs_.gs_subtract(V_.ptr, k_ + 1, d_h1_.ptr, W_.ptr, false);
+        vs_.dots_part_async(V_.ptr, k_ + 1, W_.ptr);
+        nchunk = vs_.gs_subtract(V_.ptr, k_ + 1, d_h2_.ptr, W_.ptr, true);
+        npart = vs_.npart.ptr;
+      } else {
+        // long cycles: separate reductions, coefficients from memory
+        vs_.dots_async(V_.ptr, k_ + 1, W_.ptr, d_h1_.ptr);
+        vs_.subtract(V_.ptr, k_ + 1, d_h1_.ptr, W_.ptr);
+        vs_.dots_async(V_.ptr, k_ + 1, W_.ptr, d_h2_.ptr);
+        vs_.subtract(V_.ptr, k_ + 1, d_h2_.ptr, W_.ptr);
+        nchunk = vs_.dots_part_async(W_.ptr, 1, W_.ptr);  // chunk sums of ||w||^2, column-major
+        npart = vs_.part.ptr;
+      }
       const int stage = S_ * (3 * k_ + 1);
       const size_t lds = stage <= GIVENS_STAGE_DOUBLES ? stage * sizeof(double) : 0;
       if (C_)
-        hipLaunchKernelGGL(k_gm_givens<true>, dim3(nrhs_), dim3(64), lds, st_, d_h1_.ptr, d_h2_.ptr, vs_.part.ptr,
+        hipLaunchKernelGGL(k_gm_givens<true>, dim3(nrhs_), dim3(64), lds, st_, d_h1_.ptr, d_h2_.ptr, npart,
                            nchunk, gs_, k_, m_, nrhs_, tol_);
       else
-        hipLaunchKernelGGL(k_gm_givens<false>, dim3(nrhs_), dim3(64), lds, st_, d_h1_.ptr, d_h2_.ptr, vs_.part.ptr,
+        hipLaunchKernelGGL(k_gm_givens<false>, dim3(nrhs_), dim3(64), lds, st_, d_h1_.ptr, d_h2_.ptr, npart,
                            nchunk, gs_, k_, m_, nrhs_, tol_);
       BMI_CHECK_HIP(hipGetLastError());
       vs_.axpby(d_coef_.ptr, W_.ptr, d_coef_.ptr + S_ * nrhs_, V_.ptr + vs_.block() * (k_ + 1));
       ++k_;
       ++total_;
     } while (k_ < m_ && total_ < maxit_ && k_ % CHECK != 0);
-    if (k_ < m_ && total_ < maxit_) {
-      request_flags();
-      phase_ = AFTER_STEPS;
-    } else {
-      end_cycle();
-    }
   }
   // X += V y, then the next cycle
   void end_cycle() {
+    ahead_ = false;
     if (k_ == 0) {
       phase_ = DONE;
       return;
@@ -703,6 +792,7 @@ class GmresRun {
   hipEvent_t ev_ = nullptr;
   Phase phase_ = CYCLE;
   bool pending_ = false, converged_ = false;
+  bool ahead_ = false;  // a batch of steps is queued past the pending check
   int k_ = 0, total_ = 0;
 };
 
