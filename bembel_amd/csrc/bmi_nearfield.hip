@@ -507,6 +507,18 @@ __device__ __forceinline__ void factor_pass(Acc& acc, const PtK& F, int role, bo
   }
 }
 
+// f(std::integral_constant<int, r>()) for a rotation r that is uniform in the workgroup: the
+// callee's tau and line evaluations then fold the rotation at compile time
+template <class F>
+__device__ __forceinline__ void with_rotation(int r, F&& f) {
+  switch (r) {
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 3: f(std::integral_constant<int, 3>()); break;
+    default: f(std::integral_constant<int, 0>()); break;
+  }
+}
+
 // DuffyTrick::tau (DuffyTrick/tau.hpp:26-42)
 __device__ __forceinline__ void tau(double x, double y, int c, double& u, double& v) {
   switch (c) {
@@ -853,7 +865,7 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
     const FactorShape shp = edge_shape(n, NT, static_cast<int>(sizeof(PtK)), Acc::G);
     const int CI = shp.chunk, S = shp.lanes;
     const int role = lane & (Acc::G - 1);
-    const bool fs1 = (r1 & 1) == 0, fs2 = (r2 & 1) == 0;  // tau(x, y; r) fixes u for r = 0, 2
+    // tau(x, y; r) fixes u for r = 0, 2: the line of a computed point runs along v there
     PtK* tab = reinterpret_cast<PtK*>(lds);
     for (int i0 = 0; i0 < nq; i0 += CI) {
       const int cn = nq - i0 < CI ? nq - i0 : CI;
@@ -909,24 +921,34 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
           b.W *= gw[j1];
           return b;
         });
-        // g = 1: table rows, columns on the line of element 2
-        double L[8 * PD];
-        line_contract<PD>(cp2, fs2, fs2 ? l2x + h * (r2 < 2 ? x : 1 - x) : l2y + h * (r2 < 2 ? x : 1 - x), L);
+        // g = 1: table rows, columns on the line of element 2 (the rotation, uniform in the
+        // workgroup, fixed at compile time: tau and the line evaluation without selects)
         F = tab[(2 + odd) * per + ta];
         F.W *= fb;
-        factor_pass<P, KIND, true>(acc, F, role, act, sub, S, n, prm, [&](int j1) {
-          double u, v;
-          tau(x, xi1 * gx[j1], r2, u, v);
-          return line_point<PD, KIND>(L, fs2, l2x, l2y, h, u, v, gw[j1]);
+        with_rotation(r2, [&](auto R) {
+          constexpr int RC = decltype(R)::value;
+          constexpr bool FS = (RC & 1) == 0;
+          double L[8 * PD];
+          line_contract<PD>(cp2, FS, FS ? l2x + h * (RC < 2 ? x : 1 - x) : l2y + h * (RC < 2 ? x : 1 - x), L);
+          factor_pass<P, KIND, true>(acc, F, role, act, sub, S, n, prm, [&](int j1) {
+            double u, v;
+            tau(x, xi1 * gx[j1], RC, u, v);
+            return line_point<PD, KIND>(L, FS, l2x, l2y, h, u, v, gw[j1]);
+          });
         });
         // g = 2: rows on the line of element 1, table columns
-        line_contract<PD>(cp1, fs1, fs1 ? l1x + h * (r1 < 2 ? x : 1 - x) : l1y + h * (r1 < 2 ? x : 1 - x), L);
         F = tab[(4 + odd) * per + ta];
         F.W *= fb;
-        factor_pass<P, KIND, false>(acc, F, role, act, sub, S, n, prm, [&](int j1) {
-          double u, v;
-          tau(x, xi1 * gx[j1], r1, u, v);
-          return line_point<PD, KIND>(L, fs1, l1x, l1y, h, u, v, gw[j1]);
+        with_rotation(r1, [&](auto R) {
+          constexpr int RC = decltype(R)::value;
+          constexpr bool FS = (RC & 1) == 0;
+          double L[8 * PD];
+          line_contract<PD>(cp1, FS, FS ? l1x + h * (RC < 2 ? x : 1 - x) : l1y + h * (RC < 2 ? x : 1 - x), L);
+          factor_pass<P, KIND, false>(acc, F, role, act, sub, S, n, prm, [&](int j1) {
+            double u, v;
+            tau(x, xi1 * gx[j1], RC, u, v);
+            return line_point<PD, KIND>(L, FS, l1x, l1y, h, u, v, gw[j1]);
+          });
         });
       }
     }
