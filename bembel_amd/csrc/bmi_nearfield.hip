@@ -42,6 +42,33 @@ struct NFParams {
 
 // far-field table record: x0 x1 x2 kappa w s0 s1 pad [ds0 ds1 ds2 dt0 dt1 dt2 pad pad]
 
+// 1 / sqrt(s), s > 0: the hardware estimate (v_rsq_f64) refined by two steps of Newton's
+// iteration for 1/sqrt, y <- y + y (1 - s y^2) / 2, each of which squares the relative error
+// (the residual 1 - s y^2 formed with one rounding by an FMA); within a few ulps of the rounded
+// quotient 1 / sqrt(s), for a third of the instructions of a square root and a division
+__device__ __forceinline__ double rsqrt_newton(double s) {
+  double y = __builtin_amdgcn_rsq(s);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-(s * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+// 1 / x, x != 0: the hardware estimate (v_rcp_f64) refined by two steps of Newton's iteration
+// for the reciprocal, y <- y + y (1 - x y) (residual by an FMA); within an ulp or two of the
+// rounded quotient, for half the instructions of a division
+__device__ __forceinline__ double rcp_newton(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+  }
+  return y;
+}
+
 // evaluation point of the integrand: physical point, weight (kappa*w or w), local coords;
 // Maxwell also keeps the two tangent vectors
 template <int KIND>
@@ -133,16 +160,18 @@ struct BlockAcc {
                                        bool valid) const {
     Eval e;
     const double d0 = a.x0 - b.x0, d1 = a.x1 - b.x1, d2 = a.x2 - b.x2;
-    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double s = d0 * d0 + d1 * d1 + d2 * d2;
+    const double ri = rsqrt_newton(s);  // 1 / r
+    const double r = s * ri;
     e.vi = 0.0, e.qr = 0.0, e.qi = 0.0;
     if constexpr (KIND == BMI_LAPLACE_SL) {
-      e.vr = (a.W * b.W) * (0.25 * M_1_PI) / r;
+      e.vr = (a.W * b.W) * (0.25 * M_1_PI) * ri;
     } else if constexpr (CPLX) {
       // exp(-i kappa r) / (4 pi r), kappa = kre + i kim (Helmholtz/SingleLayerOperator.hpp:115-120)
       double sn, cs;
       sincos(prm.kre * r, &sn, &cs);
       const double damp = prm.kim != 0.0 ? exp(prm.kim * r) : 1.0;
-      const double mag = damp * (a.W * b.W) * (0.25 * M_1_PI) / r;
+      const double mag = damp * (a.W * b.W) * (0.25 * M_1_PI) * ri;
       const double kr = mag * cs, ki = -mag * sn;
       if constexpr (MAXW) {
         // vector part K w w (t_rc(a) . t_cc(b)), divergence part -K w w / (kappa^2 h^2)
@@ -335,7 +364,7 @@ __device__ __forceinline__ Pt<KIND> line_point(const double* L, bool fix_s, doub
       d1[k] += B[a] * L[4 * PD + 4 * a + k];  // across the line (the fixed direction)
       d2[k] += E[a] * L[4 * a + k];           // along the line
     }
-  const double bot = 1.0 / p[3];
+  const double bot = rcp_newton(p[3]);
   const double b2 = bot * bot;
   double X[3], Ds[3], Dt[3];
 #pragma unroll
@@ -401,14 +430,16 @@ template <int KIND>
 __device__ __forceinline__ void kernel_ww(const Pt<KIND>& a, const Pt<KIND>& b, const NFParams& prm, double& kr,
                                           double& ki) {
   const double d0 = a.x0 - b.x0, d1 = a.x1 - b.x1, d2 = a.x2 - b.x2;
-  const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  const double s = d0 * d0 + d1 * d1 + d2 * d2;
+  const double ri = rsqrt_newton(s);  // 1 / r
   if constexpr (KIND == BMI_LAPLACE_SL) {
-    kr = (a.W * b.W) * (0.25 * M_1_PI) / r, ki = 0.0;
+    kr = (a.W * b.W) * (0.25 * M_1_PI) * ri, ki = 0.0;
   } else if constexpr (KIND == BMI_HELMHOLTZ_SL) {
+    const double r = s * ri;
     double sn, cs;
     sincos(prm.kre * r, &sn, &cs);
     const double damp = prm.kim != 0.0 ? exp(prm.kim * r) : 1.0;
-    const double mag = damp * (a.W * b.W) * (0.25 * M_1_PI) / r;
+    const double mag = damp * (a.W * b.W) * (0.25 * M_1_PI) * ri;
     kr = mag * cs, ki = -mag * sn;
   } else {
     kr = dummy_value(prm.dummy_fn, a, b) * a.W * b.W, ki = 0.0;
