@@ -738,7 +738,8 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
     // B_jy(t3) U23(ix,jx), U01 = sum over j1 of K0 B(t2) B(t4)^T + K1 B(t4) B(t2)^T and U23 likewise
     // with K2, K3 (the fourth pair taken in the other order, which the symmetric kernel allows).
     // The rule adds every pair in both orders, so the block is A + A^T, formed when it is written.
-    // S = 4 lanes share a unit (j1 = sub, sub + 4, ...) and add their sums; the unit's sums, basis
+    // S = 4 lanes share a unit (a lane pair of an even wave for the s-lines, one of an odd wave for
+    // the t-lines, each pair taking every second j1) and add their sums; the unit's sums, basis
     // values and outer weight go to LDS, and after each round every thread adds the units'
     // contributions to the one entry of A it owns (no per-lane block in registers).
     constexpr bool CPLX = Acc::CPLX;
@@ -751,9 +752,15 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
     constexpr int UR = 7 + (CPLX ? 4 : 2) * I2;  // record of a unit: sc, B(t1), B(t3), U01, U23
     constexpr int NE = Acc::TOT;               // entries of A (real parts, then imaginary parts)
     static_assert(UPR * 4 == NT && NE <= NT, "one line and at most one block entry per thread");
+    static_assert(NT == 256, "four waves: two lane-pair halves of 32 units each");
     double* ua = lds + UPR * US;
     const int units = nq * n;
-    const int myu = tid / S, sub = tid - (tid / S) * S;
+    // evaluation lanes: even waves sum U01 over the s-lines, odd waves U23 over the t-lines (the
+    // line direction is uniform in a wave and fixed at compile time below); a lane pair holds a
+    // unit and takes every second j1
+    const int wv = tid >> 6, ln = tid & 63;
+    const bool tl = (wv & 1) != 0;
+    const int myu = (wv >> 1) * 32 + (ln >> 1), par = ln & 1;
     // the block entry this thread owns
     const int epart = tid / (I2 * I2), erem = tid - epart * (I2 * I2);
     const int erow = erem / I2, ecol = erem - erow * I2;
@@ -777,33 +784,31 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
       const int i = uc / n, j0 = uc - (uc / n) * n;
       const double xi0 = gx[i / n], xi1 = gx[i % n], wi = gw[i / n] * gw[i % n];
       const double t1 = gx[j0] * (1 - xi0), t3 = t1 + xi0;
-      // lanes sub = 0, 2 sum U01 over the s-lines, sub = 1, 3 sum U23 over the t-lines, each pair
-      // taking every second j1
-      const bool tl = (sub & 1) != 0;
       double ur[I2], ui[CPLX ? I2 : 1];
 #pragma unroll
       for (int k = 0; k < I2; ++k) {
         ur[k] = 0.0;
         if constexpr (CPLX) ui[k] = 0.0;
       }
-      if (act) {
+      auto sums = [&](auto TLC) {
+        constexpr bool TL = decltype(TLC)::value;
 #pragma unroll 1
-        for (int j1 = sub >> 1; j1 < n; j1 += 2) {
+        for (int j1 = par; j1 < n; j1 += 2) {
           const double t2 = gx[j1] * (1 - xi0 * xi1), t4 = t2 + xi0 * xi1;
           const double wj = gw[j1];
           // the lines are read again in every iteration, one at a time (an opaque offset keeps
           // the compiler from holding them in registers across the loop)
-          int off = myu * US + (tl ? 2 * LS : 0);
+          int off = myu * US + (TL ? 2 * LS : 0);
           asm volatile("" : "+v"(off));
           const double* Lq = lds + off;
           // s-lines: (t1,t2)-(t3,t4) and (t1,t4)-(t3,t2); t-lines: (t2,t1)-(t4,t3) and
           // (t4,t1)-(t2,t3) (the second pair in the other order); either way the first line
           // holds the points at (t2, t4) of the first member, the second line those of the second
-          const PtK a0 = line_point<PD, KIND>(Lq, !tl, l1x, l1y, h, tl ? t2 : t1, tl ? t1 : t2, 1.0);
-          const PtK a1 = line_point<PD, KIND>(Lq, !tl, l1x, l1y, h, tl ? t4 : t1, tl ? t1 : t4, tl ? wj : 1.0);
+          const PtK a0 = line_point<PD, KIND>(Lq, !TL, l1x, l1y, h, TL ? t2 : t1, TL ? t1 : t2, 1.0);
+          const PtK a1 = line_point<PD, KIND>(Lq, !TL, l1x, l1y, h, TL ? t4 : t1, TL ? t1 : t4, TL ? wj : 1.0);
           __builtin_amdgcn_sched_barrier(0);
-          const PtK b0 = line_point<PD, KIND>(Lq + LS, !tl, l1x, l1y, h, tl ? t4 : t3, tl ? t3 : t4, wj);
-          const PtK b1 = line_point<PD, KIND>(Lq + LS, !tl, l1x, l1y, h, tl ? t2 : t3, tl ? t3 : t2, tl ? 1.0 : wj);
+          const PtK b0 = line_point<PD, KIND>(Lq + LS, !TL, l1x, l1y, h, TL ? t4 : t3, TL ? t3 : t4, wj);
+          const PtK b1 = line_point<PD, KIND>(Lq + LS, !TL, l1x, l1y, h, TL ? t2 : t3, TL ? t3 : t2, TL ? 1.0 : wj);
           double kar, kai, kbr, kbi;
           kernel_ww<KIND>(a0, b0, prm, kar, kai);
           kernel_ww<KIND>(a1, b1, prm, kbr, kbi);
@@ -819,17 +824,23 @@ __global__ __launch_bounds__(NT) void k_nearfield(const NFPair* __restrict__ pai
               if constexpr (CPLX) ui[p * I + q] += kai * f + kbi * g;
             }
         }
+      };
+      if (act) {
+        if (tl)
+          sums(std::integral_constant<bool, true>());
+        else
+          sums(std::integral_constant<bool, false>());
       }
-      // the two lanes of each half add their sums; lane sub = 0 writes the unit's record with
-      // U01, lane sub = 1 adds U23
+      // the two lanes of a unit add their sums; the first lane of the s-line pair writes the
+      // unit's record with U01, the first lane of the t-line pair adds U23
 #pragma unroll
       for (int k = 0; k < I2; ++k) {
-        ur[k] += __shfl_xor(ur[k], 2, 64);
-        if constexpr (CPLX) ui[k] += __shfl_xor(ui[k], 2, 64);
+        ur[k] += __shfl_xor(ur[k], 1, 64);
+        if constexpr (CPLX) ui[k] += __shfl_xor(ui[k], 1, 64);
       }
-      if (sub < 2) {
+      if (par == 0) {
         double* r = ua + myu * UR;
-        if (sub == 0) {
+        if (!tl) {
           double B1[I], B3[I];
           bernstein_basis<P>(t1, B1);
           bernstein_basis<P>(t3, B3);
